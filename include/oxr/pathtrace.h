@@ -1,0 +1,328 @@
+/*
+ * oxr — MI355X (gfx950) path tracer behind oxideav-render's PathTrace
+ * backend. C ABI of liboxr.so.
+ *
+ * Every entry point returns OXR_OK (0) or a negative status; the text of
+ * the last failure on the calling thread is oxr_last_error(). Inputs are
+ * copied (or consumed) during the call; outputs go into caller buffers.
+ * A handle may move between threads but must not be used by two threads
+ * at once (the reference's `Renderer: Send` + `&mut self` contract,
+ * oxideav-render src/lib.rs:136-167).
+ *
+ * Reference interfaces replaced (oxideav-render 0.0.6):
+ *   oxr_pt_create / oxr_pt_destroy  -> PathTracer::new / Drop
+ *                                      (src/pathtrace.rs:2420-2433)
+ *   oxr_pt_upload_scene             -> the scene half of PathTracer::sync:
+ *                                      TraceScene::new + EmissiveLights::build
+ *                                      (src/pathtrace.rs:2484-2497,
+ *                                       src/trace.rs:320-351,
+ *                                       src/pathtrace.rs:1479-1543)
+ *   oxr_pt_set_camera               -> Camera::resolve result held in State
+ *                                      (src/pathtrace.rs:2488, :2499)
+ *   oxr_pt_set_options              -> the options half of PathTracer::sync
+ *                                      and its radiance_key reset rule
+ *                                      (src/pathtrace.rs:2474-2508, :2402-2418)
+ *   oxr_pt_reset                    -> PathTracer::reset (src/pathtrace.rs:2463)
+ *   oxr_pt_refine                   -> PathTracer::refine (src/pathtrace.rs:2531-2591)
+ *   oxr_pt_samples                  -> PathTracer::samples (src/pathtrace.rs:2511)
+ *   oxr_pt_read_accum               -> the accumulator `accum: Vec<[f64;4]>`
+ *                                      (src/pathtrace.rs:2377)
+ *   oxr_pt_resolve_hdr / _rgba8     -> PathTracer::hdr / image
+ *                                      (src/pathtrace.rs:2687-2734)
+ *   oxr_sobol_owen_4d               -> sobol_owen_4d (src/pathtrace.rs:388-398)
+ *   oxr_trace_closest               -> TraceScene::closest_hit_filtered
+ *                                      (src/trace.rs:405-422)
+ *   oxr_trace_any                   -> TraceScene::occluded_filtered
+ *                                      (src/trace.rs:432-450)
+ *   oxr_bsdf_eval / oxr_bsdf_sample -> Bsdf::new + Bsdf::eval / Bsdf::sample
+ *                                      (src/pathtrace.rs:853-1279)
+ */
+#ifndef OXR_PATHTRACE_H
+#define OXR_PATHTRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------- */
+#define OXR_OK 0
+#define OXR_ERR_INVALID (-1)   /* bad argument (null pointer, size, range)  */
+#define OXR_ERR_DEVICE (-2)    /* HIP runtime failure (Error::Backend)      */
+#define OXR_ERR_STATE (-3)     /* call out of order (e.g. trace before scene)*/
+#define OXR_ERR_MEMORY (-4)    /* device allocation failed / over budget    */
+
+/* ---- scene data contract (a prepared, world-space triangle soup) ---- */
+
+/* Alpha modes (oxideav_mesh3d::AlphaMode). */
+#define OXR_ALPHA_OPAQUE 0u
+#define OXR_ALPHA_MASK 1u
+#define OXR_ALPHA_BLEND 2u
+
+/* Material flags. */
+#define OXR_MAT_DOUBLE_SIDED 1u
+#define OXR_MAT_UNLIT 2u
+
+/* One glTF metallic-roughness material with its KHR extension scalars,
+ * already resolved to the values TraceScene::material_oriented reads when
+ * no texture is bound (src/trace.rs:700-923; PreparedMaterial,
+ * src/prepare.rs:433-476). The library applies the same finite/clamp
+ * rules as material_oriented at upload. */
+typedef struct oxr_material {
+  float base_color[4];          /* factor, linear RGBA                      */
+  float metallic;
+  float roughness;
+  float emissive[3];            /* emissiveFactor x emissive_strength       */
+  uint32_t flags;               /* OXR_MAT_*                                */
+  uint32_t alpha_mode;          /* OXR_ALPHA_*                              */
+  float alpha_cutoff;
+  float ior;                    /* KHR_materials_ior (1.5 default)          */
+  float specular;               /* KHR_materials_specular factor (1)        */
+  float specular_color[3];      /* (1,1,1)                                  */
+  float transmission;           /* KHR_materials_transmission (0)           */
+  float thickness;              /* KHR_materials_volume (0 = thin)          */
+  float attenuation_color[3];   /* (1,1,1)                                  */
+  float attenuation_distance;   /* +inf = none                              */
+  float clearcoat;              /* KHR_materials_clearcoat (0)              */
+  float clearcoat_roughness;
+  float sheen_color[3];         /* KHR_materials_sheen (0,0,0)              */
+  float sheen_roughness;
+  float anisotropy;             /* KHR_materials_anisotropy strength (0)    */
+  float anisotropy_rotation;
+  float iridescence;            /* KHR_materials_iridescence factor (0)     */
+  float iridescence_ior;        /* 1.3                                      */
+  float iridescence_thickness;  /* thicknessMax without a texture (400)    */
+  float diffuse_transmission;   /* KHR_materials_diffuse_transmission (0)   */
+  float diffuse_transmission_color[3];
+  float dispersion;             /* KHR_materials_dispersion (0)             */
+} oxr_material;
+
+/* Punctual light kinds (KHR_lights_punctual; src/prepare.rs:680-720). */
+#define OXR_LIGHT_DIRECTIONAL 0u
+#define OXR_LIGHT_POINT 1u
+#define OXR_LIGHT_SPOT 2u
+
+typedef struct oxr_light {
+  uint32_t kind;
+  float color[3];
+  float intensity;              /* lux (directional) or candela             */
+  float position[3];
+  float direction[3];           /* unit direction the light travels         */
+  float range;                  /* <= 0 or non-finite: infinite             */
+  float inner_cone_angle;       /* radians                                  */
+  float outer_cone_angle;       /* radians                                  */
+} oxr_light;
+
+/* Triangle flags. */
+#define OXR_TRI_HAS_NORMALS 1u
+
+typedef struct oxr_scene_desc {
+  uint32_t num_tris;
+  const float* positions;       /* num_tris x 9: v0,v1,v2 (CCW front)      */
+  const float* normals;         /* num_tris x 9 vertex normals, or NULL     */
+  const uint32_t* tri_flags;    /* num_tris OXR_TRI_*, or NULL (all =
+                                   OXR_TRI_HAS_NORMALS when normals given) */
+  const uint32_t* material_ids; /* num_tris, index into materials          */
+  uint32_t num_materials;
+  const oxr_material* materials;
+  uint32_t num_lights;
+  const oxr_light* lights;      /* punctual lights (PreparedScene::lights)  */
+} oxr_scene_desc;
+
+/* Resolved camera (src/camera.rs:61-91): eye + orthonormal basis + the
+ * half extents of the view plane at unit distance. */
+#define OXR_PROJ_PERSPECTIVE 0u
+#define OXR_PROJ_ORTHOGRAPHIC 1u
+
+typedef struct oxr_camera {
+  float eye[3];
+  float forward[3];
+  float side[3];
+  float up[3];
+  float half_w;
+  float half_h;
+  uint32_t projection;
+} oxr_camera;
+
+/* LightStrategy (src/options.rs:51-64). */
+#define OXR_STRATEGY_MIS 0u
+#define OXR_STRATEGY_LIGHT_ONLY 1u
+#define OXR_STRATEGY_BSDF_ONLY 2u
+
+/* POD mirror of PathTraceOptions (src/options.rs:67-146) plus the
+ * RenderOptions fields the estimator reads: width, height, ambient. */
+typedef struct oxr_pt_options {
+  uint32_t width;
+  uint32_t height;
+  uint32_t samples_per_pixel;   /* 1..=2^20                                  */
+  uint32_t max_bounces;         /* <= 1024                                   */
+  uint32_t rr_start;            /* default 3                                 */
+  float clamp;                  /* 0 = off                                   */
+  uint32_t seed;
+  uint32_t strategy;            /* OXR_STRATEGY_*                            */
+  float ambient;                /* constant sky radiance                     */
+} oxr_pt_options;
+
+/* Display options for the resolve (RenderOptions::background, tone_map,
+ * exposure; src/pathtrace.rs:2593-2734, src/hdr.rs:44-66). */
+#define OXR_TONEMAP_CLAMP 0u
+#define OXR_TONEMAP_REINHARD 1u
+#define OXR_TONEMAP_ACES 2u
+
+typedef struct oxr_display {
+  uint8_t background[4];        /* sRGB bytes + alpha                       */
+  uint32_t tone_map;
+  float exposure;
+} oxr_display;
+
+/* Ray and hit records of the parity entry points (also the device
+ * layout of the ray queues): 32-byte ray, 16-byte hit. */
+typedef struct oxr_ray {
+  float o[3];
+  float t_min;
+  float d[3];
+  float t_max;
+} oxr_ray;
+
+#define OXR_HIT_MISS 0xffffffffu
+#define OXR_HIT_BACK 0x80000000u  /* bit 31 of prim: back-face hit         */
+
+typedef struct oxr_hit {
+  float t;
+  uint32_t prim;                /* global triangle id | OXR_HIT_BACK, or
+                                   OXR_HIT_MISS                              */
+  float u;                      /* barycentric weight of v1                 */
+  float v;                      /* barycentric weight of v2 (v0: 1-u-v)     */
+} oxr_hit;
+
+/* ---- progressive path tracer ---------------------------------------- */
+typedef struct oxr_pt oxr_pt;
+
+const char* oxr_last_error(void);
+
+/* Number of visible HIP devices (0 when none: then every device call
+ * below fails with OXR_ERR_DEVICE). */
+int oxr_device_count(int* out);
+
+int oxr_pt_create(int device, oxr_pt** out);
+int oxr_pt_destroy(oxr_pt* h);
+
+/* Upload a scene: builds the SAH BVH (binned SAH, collapsed to 4-wide
+ * nodes), the emissive-triangle power table and the constant tables, and
+ * keeps everything resident on the device. Resets the accumulation. */
+int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* scene);
+
+/* Camera and options. Each resets the accumulation only when a value the
+ * estimator reads changed (the radiance_key rule); *reset_out (may be
+ * NULL) tells whether it did. */
+int oxr_pt_set_camera(oxr_pt* h, const oxr_camera* cam, int* reset_out);
+int oxr_pt_set_options(oxr_pt* h, const oxr_pt_options* opts, int* reset_out);
+int oxr_pt_reset(oxr_pt* h);
+
+/* Add n samples per pixel, continuing the global sample index; a no-op
+ * before the first upload. refine(a); refine(b) == refine(a+b) bit for
+ * bit. Returns after the device finished. */
+int oxr_pt_refine(oxr_pt* h, uint32_t n);
+int oxr_pt_samples(oxr_pt* h, uint32_t* out);
+
+/* Accumulator readback: accum = W*H*4 doubles (sum r, g, b over covered
+ * samples, covered count), sum_sq = W*H*3 doubles of the squared
+ * radiance (may be NULL). */
+int oxr_pt_read_accum(oxr_pt* h, double* accum, double* sum_sq);
+
+/* Resolve on the device: W*H*4 floats (hdr) or bytes (rgba8). */
+int oxr_pt_resolve_hdr(oxr_pt* h, const oxr_display* disp, float* out);
+int oxr_pt_resolve_rgba8(oxr_pt* h, const oxr_display* disp, uint8_t* out);
+
+/* Counters of the last refine: paths traced, closest-hit rays, shadow
+ * rays, and per-kernel device time (ms) when timing was enabled. */
+typedef struct oxr_pt_stats {
+  uint64_t paths;
+  uint64_t closest_rays;
+  uint64_t shadow_rays;
+  double ms_closest;
+  double ms_anyhit;
+  double ms_shade;
+  double ms_other;
+  uint32_t launches_closest;
+  uint32_t launches_anyhit;
+} oxr_pt_stats;
+int oxr_pt_set_timing(oxr_pt* h, int enable);
+int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
+
+/* ---- parity entry points --------------------------------------------- */
+
+/* out[4*i..4*i+4] = sobol_owen_4d(index[i], pattern_seed[i]) on device. */
+int oxr_sobol_owen_4d(int device, uint32_t n, const uint32_t* index,
+                      const uint32_t* pattern_seed, float* out);
+
+/* Closest hit / occlusion of n host rays against the uploaded scene, all
+ * candidates accepted (no alpha filter). flags bit 0: cull back faces of
+ * single-sided materials (the primary-ray rule). */
+int oxr_trace_closest(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint32_t flags,
+                      oxr_hit* hits);
+int oxr_trace_any(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint8_t* occluded);
+
+/* The same queries on rays already resident in device memory (the bench
+ * path; `stream` is a hipStream_t or NULL for the handle's stream). */
+int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays,
+                             void* d_hits, void* stream);
+
+/* Flattened BVH statistics: node and leaf counts, SAH cost, depth. */
+typedef struct oxr_bvh_info {
+  uint32_t num_nodes;
+  uint32_t num_leaves;
+  uint32_t num_tris;
+  uint32_t max_depth;
+  double sah_cost;
+  double build_ms;
+} oxr_bvh_info;
+int oxr_pt_bvh_info(oxr_pt* h, oxr_bvh_info* out);
+
+/* Copy of the flattened device BVH (for the oracle's traversal count):
+ * nodes = num_nodes * 128 bytes, tris = num_tris * 48 bytes. */
+int oxr_pt_bvh_export(oxr_pt* h, void* nodes, void* tris);
+
+/* Per-point material inputs of Bsdf::new (the MaterialSample fields the
+ * BSDF reads, src/trace.rs:201-267). */
+typedef struct oxr_material_sample {
+  float base_color[3];
+  float metallic;
+  float roughness;
+  float normal[3];
+  float ior;
+  float specular;
+  float specular_color[3];
+  float transmission;
+  float thickness;
+  float clearcoat;
+  float clearcoat_roughness;
+  float clearcoat_normal[3];
+  float sheen_color[3];
+  float sheen_roughness;
+  float anisotropy;
+  float anisotropy_direction[3];
+  float iridescence;
+  float iridescence_ior;
+  float iridescence_thickness;
+  float diffuse_transmission;
+  float diffuse_transmission_color[3];
+} oxr_material_sample;
+
+/* Bsdf::new(ms[i], v[i], ng[i], front[i]) then eval(l[i]) -> f (3), pdf.
+ * Arrays of 3 floats per item. */
+int oxr_bsdf_eval(int device, uint32_t n, const oxr_material_sample* ms,
+                  const float* v, const float* ng, const uint8_t* front,
+                  const float* l, float* f, float* pdf);
+/* ... then sample(u[i]) -> l (3), f (3), pdf; ok[i] = 0 when None. */
+int oxr_bsdf_sample(int device, uint32_t n, const oxr_material_sample* ms,
+                    const float* v, const float* ng, const uint8_t* front,
+                    const float* u, float* l, float* f, float* pdf, uint8_t* ok);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OXR_PATHTRACE_H */

@@ -1,0 +1,313 @@
+"""ctypes front of the oracle (liboxr_cpu.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and the bench's
+cpu_baseline leg use this module, as the checker and the host-CPU
+baseline. Nothing in the product imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from oxr import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+fp = C.POINTER(C.c_float)
+dp = C.POINTER(C.c_double)
+up = C.POINTER(C.c_uint32)
+bp = C.POINTER(C.c_uint8)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboxr_cpu.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.oxrcpu_scene_create.restype = C.c_void_p
+        L.oxrcpu_scene_create.argtypes = [C.POINTER(A.SceneDesc)]
+        L.oxrcpu_scene_destroy.argtypes = [C.c_void_p]
+        L.oxrcpu_num_lights.restype = C.c_uint32
+        L.oxrcpu_num_lights.argtypes = [C.c_void_p]
+        L.oxrcpu_emissive_table.argtypes = [C.c_void_p, up, fp, fp, up]
+        L.oxrcpu_render.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
+                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp, dp]
+        L.oxrcpu_sample_one.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
+                                        C.c_uint32, C.c_uint32, fp]
+        L.oxrcpu_trace_closest.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(A.Ray), C.c_uint32, C.c_int,
+                                           C.POINTER(A.Hit)]
+        L.oxrcpu_trace_any.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(A.Ray), C.c_int, bp]
+        L.oxrcpu_sobol_owen_4d.argtypes = [C.c_uint32, up, up, fp]
+        L.oxrcpu_sobol.restype = C.c_uint32
+        L.oxrcpu_sobol.argtypes = [C.c_uint32, C.c_int]
+        L.oxrcpu_pcg_hash.restype = C.c_uint32
+        L.oxrcpu_pcg_hash.argtypes = [C.c_uint32]
+        L.oxrcpu_sobol_matrices.argtypes = [up]
+        L.oxrcpu_sheen_lut.argtypes = [fp]
+        L.oxrcpu_bsdf_eval.argtypes = [C.c_uint32, C.POINTER(A.MaterialSample), fp, fp, bp, fp, fp, fp]
+        L.oxrcpu_bsdf_sample.argtypes = [C.c_uint32, C.POINTER(A.MaterialSample), fp, fp, bp, fp, fp, fp, fp, bp]
+        L.oxrcpu_bsdf_probs.argtypes = [C.POINTER(A.MaterialSample), fp, fp, C.c_uint8, fp]
+        L.oxrcpu_triangle_solid_angle.restype = C.c_float
+        L.oxrcpu_triangle_solid_angle.argtypes = [fp, fp, fp, fp]
+        L.oxrcpu_sample_spherical_triangle.restype = C.c_int
+        L.oxrcpu_sample_spherical_triangle.argtypes = [fp, fp, fp, fp, C.c_float, C.c_float, fp]
+        L.oxrcpu_iridescent_fresnel.argtypes = [C.c_float, C.c_float, fp, C.c_float, C.c_float, fp]
+        L.oxrcpu_offset_ray_origin.argtypes = [fp, fp, fp]
+        L.oxrcpu_barycentric_of.restype = C.c_int
+        L.oxrcpu_barycentric_of.argtypes = [fp, fp, fp, fp, fp]
+        L.oxrcpu_surface.argtypes = [C.c_void_p, C.c_uint32, fp, fp]
+        L.oxrcpu_resolve_hdr.argtypes = [dp, C.c_uint32, C.c_uint32, C.POINTER(A.Display), fp]
+        L.oxrcpu_resolve_rgba8.argtypes = [dp, C.c_uint32, C.c_uint32, C.POINTER(A.Display), bp]
+        _LIB = L
+    return _LIB
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(fp)
+
+
+def options(width, height, spp=1, max_bounces=8, rr_start=3, clamp=0.0, seed=0, strategy=0, ambient=0.0):
+    o = A.PtOptions()
+    o.width, o.height = width, height
+    o.samples_per_pixel = spp
+    o.max_bounces = max_bounces
+    o.rr_start = rr_start
+    o.clamp = clamp
+    o.seed = seed
+    o.strategy = strategy
+    o.ambient = ambient
+    return o
+
+
+class OracleScene:
+    """The oracle's prepared scene (soup + its own binned-SAH BVH)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self._desc = scene.desc()
+        self.h = lib().oxrcpu_scene_create(C.byref(self._desc))
+        if not self.h:
+            raise RuntimeError("oracle scene creation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.oxrcpu_scene_destroy(self.h)
+            self.h = None
+
+    def render(self, width, height, first, n, opts=None, y0=0, y1=None, threads=0, sum_sq=False, cam=None):
+        """Accumulate samples [first, first+n) into fresh f64 buffers:
+        returns accum (H', W, 4) [and sum_sq (H', W, 3)]."""
+        y1 = height if y1 is None else y1
+        if opts is None:
+            opts = options(width, height, ambient=getattr(self.scene, "ambient", 0.0))
+        cam = cam or self.scene.camera.resolve(width, height)
+        acc = np.zeros((y1 - y0, width, 4), dtype=np.float64)
+        sq = np.zeros((y1 - y0, width, 3), dtype=np.float64) if sum_sq else None
+        rc = lib().oxrcpu_render(self.h, C.byref(cam), C.byref(opts), first, n, y0, y1, threads,
+                                 acc.ctypes.data_as(dp), sq.ctypes.data_as(dp) if sq is not None else dp())
+        if rc != 0:
+            raise RuntimeError("oracle render failed")
+        return (acc, sq) if sum_sq else acc
+
+    def sample_one(self, width, height, x, y, index, opts=None, cam=None):
+        if opts is None:
+            opts = options(width, height, ambient=getattr(self.scene, "ambient", 0.0))
+        cam = cam or self.scene.camera.resolve(width, height)
+        rgb = np.zeros(3, np.float32)
+        cov = lib().oxrcpu_sample_one(self.h, C.byref(cam), C.byref(opts), x, y, index, rgb.ctypes.data_as(fp))
+        return (rgb if cov else None)
+
+    def trace_closest(self, rays, flags=0, brute=False):
+        n = len(rays)
+        hits = (A.Hit * max(n, 1))()
+        lib().oxrcpu_trace_closest(self.h, n, rays, flags, 1 if brute else 0, hits)
+        return hits_to_numpy(hits, n)
+
+    def trace_any(self, rays, brute=False):
+        n = len(rays)
+        occ = np.zeros(max(n, 1), np.uint8)
+        lib().oxrcpu_trace_any(self.h, n, rays, 1 if brute else 0, occ.ctypes.data_as(bp))
+        return occ[:n]
+
+    def emissive_table(self):
+        L = lib()
+        m = L.oxrcpu_num_lights(self.h)
+        tris = np.zeros(max(m, 1), np.uint32)
+        pmf = np.zeros(max(m, 1), np.float32)
+        cdf = np.zeros(max(m, 1), np.float32)
+        lookup = np.zeros(max(self.scene.num_tris, 1), np.uint32)
+        L.oxrcpu_emissive_table(self.h, tris.ctypes.data_as(up), pmf.ctypes.data_as(fp), cdf.ctypes.data_as(fp),
+                                lookup.ctypes.data_as(up))
+        return tris[:m], pmf[:m], cdf[:m], lookup[:self.scene.num_tris]
+
+    def surface(self, g, bary):
+        b, bp_ = _f(bary)
+        out = np.zeros(9, np.float32)
+        lib().oxrcpu_surface(self.h, g, bp_, out.ctypes.data_as(fp))
+        return out[0:3], out[3:6], out[6:9]
+
+
+def rays_array(o, d, t_min=None, t_max=None):
+    """Pack (n,3) origins/directions into an oxr_ray array."""
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    n = len(o)
+    buf = np.zeros((n, 8), np.float32)
+    buf[:, 0:3] = o
+    buf[:, 3] = 0.0 if t_min is None else t_min
+    buf[:, 4:7] = d
+    buf[:, 7] = np.inf if t_max is None else t_max
+    arr = (A.Ray * max(n, 1))()
+    C.memmove(arr, buf.ctypes.data, buf.nbytes)
+    return arr, n
+
+
+class RayBatch:
+    def __init__(self, o, d, t_min=None, t_max=None):
+        self.arr, self.n = rays_array(o, d, t_min, t_max)
+
+    def __len__(self):
+        return self.n
+
+
+def hits_to_numpy(hits, n):
+    raw = np.frombuffer(hits, dtype=np.uint32, count=4 * max(n, 1)).reshape(-1, 4)[:n].copy()
+    return raw  # columns: t bits, prim, u bits, v bits
+
+
+def sobol_owen_4d(idx, seed):
+    idx = np.ascontiguousarray(idx, np.uint32)
+    seed = np.ascontiguousarray(seed, np.uint32)
+    out = np.zeros((len(idx), 4), np.float32)
+    lib().oxrcpu_sobol_owen_4d(len(idx), idx.ctypes.data_as(up), seed.ctypes.data_as(up), out.ctypes.data_as(fp))
+    return out
+
+
+def sobol(index, dim):
+    return lib().oxrcpu_sobol(index, dim)
+
+
+def pcg_hash(x):
+    return lib().oxrcpu_pcg_hash(x)
+
+
+def sobol_matrices():
+    out = np.zeros(128, np.uint32)
+    lib().oxrcpu_sobol_matrices(out.ctypes.data_as(up))
+    return out.reshape(4, 32)
+
+
+def sheen_lut():
+    out = np.zeros(512, np.float32)
+    lib().oxrcpu_sheen_lut(out.ctypes.data_as(fp))
+    return out
+
+
+def _ms_array(ms):
+    if isinstance(ms, A.MaterialSample):
+        ms = [ms]
+    return (A.MaterialSample * len(ms))(*ms)
+
+
+def bsdf_eval(ms, v, ng, front, l):
+    """Vectorised over the rows of l (v, ng, front, ms broadcast)."""
+    l = np.ascontiguousarray(np.asarray(l, np.float32).reshape(-1, 3))
+    n = len(l)
+    if isinstance(ms, A.MaterialSample):
+        ms = [ms] * n
+    v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float32), (n, 3)))
+    ng = np.ascontiguousarray(np.broadcast_to(np.asarray(ng, np.float32), (n, 3)))
+    fr = np.ascontiguousarray(np.broadcast_to(np.asarray(front, np.uint8), (n,)))
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().oxrcpu_bsdf_eval(n, _ms_array(ms), v.ctypes.data_as(fp), ng.ctypes.data_as(fp), fr.ctypes.data_as(bp),
+                           l.ctypes.data_as(fp), f.ctypes.data_as(fp), pdf.ctypes.data_as(fp))
+    return f, pdf
+
+
+def bsdf_sample(ms, v, ng, front, u):
+    u = np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 3))
+    n = len(u)
+    if isinstance(ms, A.MaterialSample):
+        ms = [ms] * n
+    v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float32), (n, 3)))
+    ng = np.ascontiguousarray(np.broadcast_to(np.asarray(ng, np.float32), (n, 3)))
+    fr = np.ascontiguousarray(np.broadcast_to(np.asarray(front, np.uint8), (n,)))
+    l = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    ok = np.zeros(n, np.uint8)
+    lib().oxrcpu_bsdf_sample(n, _ms_array(ms), v.ctypes.data_as(fp), ng.ctypes.data_as(fp), fr.ctypes.data_as(bp),
+                             u.ctypes.data_as(fp), l.ctypes.data_as(fp), f.ctypes.data_as(fp),
+                             pdf.ctypes.data_as(fp), ok.ctypes.data_as(bp))
+    return l, f, pdf, ok.astype(bool)
+
+
+def bsdf_probs(ms, v, ng, front):
+    v, vp = _f(v)
+    ng, ngp = _f(ng)
+    out = np.zeros(6, np.float32)
+    lib().oxrcpu_bsdf_probs(C.byref(ms), vp, ngp, 1 if front else 0, out.ctypes.data_as(fp))
+    return out
+
+
+def triangle_solid_angle(p, a, b, c):
+    args = [_f(x) for x in (p, a, b, c)]
+    return lib().oxrcpu_triangle_solid_angle(*[x[1] for x in args])
+
+
+def sample_spherical_triangle(p, a, b, c, u0, u1):
+    args = [_f(x) for x in (p, a, b, c)]
+    out = np.zeros(3, np.float32)
+    ok = lib().oxrcpu_sample_spherical_triangle(*[x[1] for x in args], u0, u1, out.ctypes.data_as(fp))
+    return out if ok else None
+
+
+def iridescent_fresnel(outside, film, f0, thick, cos1):
+    f0a, f0p = _f(f0)
+    out = np.zeros(3, np.float32)
+    lib().oxrcpu_iridescent_fresnel(outside, film, f0p, thick, cos1, out.ctypes.data_as(fp))
+    return out
+
+
+def offset_ray_origin(p, n):
+    pa, pp = _f(p)
+    na, np_ = _f(n)
+    out = np.zeros(3, np.float32)
+    lib().oxrcpu_offset_ray_origin(pp, np_, out.ctypes.data_as(fp))
+    return out
+
+
+def barycentric_of(q, p0, p1, p2):
+    args = [_f(x) for x in (q, p0, p1, p2)]
+    out = np.zeros(3, np.float32)
+    ok = lib().oxrcpu_barycentric_of(*[x[1] for x in args], out.ctypes.data_as(fp))
+    return out if ok else None
+
+
+def resolve_hdr(accum, samples, background=(0, 0, 0, 255), tone_map=0, exposure=1.0):
+    accum = np.ascontiguousarray(accum, np.float64)
+    npx = accum.size // 4
+    d = A.Display()
+    d.background = (C.c_uint8 * 4)(*background)
+    d.tone_map = tone_map
+    d.exposure = exposure
+    out = np.zeros(accum.shape[:-1] + (4,), np.float32)
+    lib().oxrcpu_resolve_hdr(accum.ctypes.data_as(dp), npx, samples, C.byref(d), out.ctypes.data_as(fp))
+    return out
+
+
+def resolve_rgba8(accum, samples, background=(0, 0, 0, 255), tone_map=0, exposure=1.0):
+    accum = np.ascontiguousarray(accum, np.float64)
+    npx = accum.size // 4
+    d = A.Display()
+    d.background = (C.c_uint8 * 4)(*background)
+    d.tone_map = tone_map
+    d.exposure = exposure
+    out = np.zeros(accum.shape[:-1] + (4,), np.uint8)
+    lib().oxrcpu_resolve_rgba8(accum.ctypes.data_as(dp), npx, samples, C.byref(d), out.ctypes.data_as(bp))
+    return out

@@ -1,0 +1,506 @@
+"""Procedural scenes in the prepared-soup data contract.
+
+A `Scene` is what `PathTracer::sync` hands the trace layer after
+`PreparedScene::build` (oxideav-render src/prepare.rs:1098-1234,
+src/trace.rs:320-351): a world-space, de-indexed triangle soup (CCW
+front), optional per-corner normals, one material index per triangle, the
+material table and the punctual lights. The geometry builders restate the
+reference's test vocabulary (`quad`, `cuboid`, `uv_sphere`,
+src/testscenes.rs:38-148); the scenes are the benchmark workloads:
+
+* `cornell()`        — Cornell box, 36 tris, emissive ceiling quad.
+* `material_sweep()` — 7x7 UV spheres (metallic rows x roughness columns)
+                        on a checker floor, point + spot light, sky 0.3.
+* `glossy_caustic()` — room, glass sphere, small bright emitter, GGX floor.
+* `triangle_soup()`  — seeded random soup (serving mix).
+* small test scenes: furnace box, area-light floor, glass cube, point-lit
+  plane (the reference's own integrator tests, src/pathtrace.rs:3346-3750).
+
+All scene randomness is `pcg_hash` chained from a stated seed.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi as A
+
+F32 = np.float32
+
+
+def _normalize(v):
+    v = np.asarray(v, dtype=F32)
+    n = F32(math.sqrt(float(np.dot(v, v))))
+    return v / n if n > 0 else v
+
+
+class CameraSpec:
+    """A glTF-style perspective/orthographic camera placed by look-at.
+
+    `resolve(width, height)` follows Camera::from_scene_camera
+    (src/camera.rs:208-332): forward, side = forward x up_hint, up = side x
+    forward, half_h = tan(yfov/2), half_w = half_h * aspect (viewport
+    aspect), in f32.
+    """
+
+    def __init__(self, eye, target, up=(0.0, 1.0, 0.0), yfov_deg=40.0, projection=0, ymag=1.0):
+        self.eye = np.asarray(eye, dtype=F32)
+        self.target = np.asarray(target, dtype=F32)
+        self.up_hint = np.asarray(up, dtype=F32)
+        self.yfov = F32(math.radians(yfov_deg))
+        self.projection = projection
+        self.ymag = F32(ymag)
+
+    def resolve(self, width, height):
+        fwd = _normalize(self.target - self.eye)
+        side = _normalize(np.cross(fwd, _normalize(self.up_hint)).astype(F32))
+        up = np.cross(side, fwd).astype(F32)
+        aspect = F32(max(width, 1)) / F32(max(height, 1))
+        cam = A.Camera()
+        cam.eye = A.f3(*self.eye.tolist())
+        cam.forward = A.f3(*fwd.tolist())
+        cam.side = A.f3(*side.tolist())
+        cam.up = A.f3(*up.tolist())
+        if self.projection == A.OXR_PROJ_ORTHOGRAPHIC:
+            cam.half_h = float(self.ymag)
+            cam.half_w = float(F32(self.ymag) * aspect)
+        else:
+            th = F32(math.tan(float(self.yfov * F32(0.5))))
+            cam.half_h = float(th)
+            cam.half_w = float(th * aspect)
+        cam.projection = self.projection
+        return cam
+
+
+class Scene:
+    """Prepared triangle soup + materials + lights + a camera."""
+
+    def __init__(self, name):
+        self.name = name
+        self._pos = []
+        self._nrm = []
+        self._has_n = []
+        self._mid = []
+        self.materials = []
+        self.lights = []
+        self.camera = None
+        self._packed = None
+
+    # -- building ----------------------------------------------------------
+    def add_material(self, m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_tris(self, pos, mat, normals=None):
+        """pos: (n, 9) corners; normals: (n, 9) or None (geometric)."""
+        pos = np.ascontiguousarray(pos, dtype=F32).reshape(-1, 9)
+        self._pos.append(pos)
+        if normals is None:
+            self._nrm.append(np.zeros_like(pos))
+            self._has_n.append(np.zeros(len(pos), dtype=np.uint32))
+        else:
+            self._nrm.append(np.ascontiguousarray(normals, dtype=F32).reshape(-1, 9))
+            self._has_n.append(np.full(len(pos), A.OXR_TRI_HAS_NORMALS, dtype=np.uint32))
+        if np.isscalar(mat):
+            self._mid.append(np.full(len(pos), mat, dtype=np.uint32))
+        else:
+            self._mid.append(np.asarray(mat, dtype=np.uint32))
+        self._packed = None
+
+    def pack(self):
+        if self._packed is None:
+            if self._pos:
+                pos = np.ascontiguousarray(np.concatenate(self._pos), dtype=F32)
+                nrm = np.ascontiguousarray(np.concatenate(self._nrm), dtype=F32)
+                hn = np.ascontiguousarray(np.concatenate(self._has_n), dtype=np.uint32)
+                mid = np.ascontiguousarray(np.concatenate(self._mid), dtype=np.uint32)
+            else:
+                pos = np.zeros((0, 9), F32)
+                nrm = np.zeros((0, 9), F32)
+                hn = np.zeros(0, np.uint32)
+                mid = np.zeros(0, np.uint32)
+            self._packed = (pos, nrm, hn, mid)
+        return self._packed
+
+    @property
+    def num_tris(self):
+        return len(self.pack()[0])
+
+    @property
+    def positions(self):
+        return self.pack()[0]
+
+    @property
+    def material_ids(self):
+        return self.pack()[3]
+
+    def desc(self):
+        """oxr_scene_desc pointing at this scene's arrays (kept alive by
+        the returned object)."""
+        pos, nrm, hn, mid = self.pack()
+        d = A.SceneDesc()
+        d.num_tris = len(pos)
+        fp = C.POINTER(C.c_float)
+        up = C.POINTER(C.c_uint32)
+        d.positions = pos.ctypes.data_as(fp)
+        any_n = bool(hn.any())
+        d.normals = nrm.ctypes.data_as(fp) if any_n else fp()
+        d.tri_flags = hn.ctypes.data_as(up) if any_n else up()
+        d.material_ids = mid.ctypes.data_as(up)
+        mats = (A.Material * max(len(self.materials), 1))(*self.materials)
+        lights = (A.Light * max(len(self.lights), 1))(*self.lights)
+        d.num_materials = len(self.materials)
+        d.materials = C.cast(mats, C.POINTER(A.Material))
+        d.num_lights = len(self.lights)
+        d.lights = C.cast(lights, C.POINTER(A.Light))
+        d._keep = (pos, nrm, hn, mid, mats, lights)
+        return d
+
+    def bounds(self):
+        p = self.positions.reshape(-1, 3)
+        return p.min(axis=0), p.max(axis=0)
+
+
+# ---------------------------------------------------------------------------
+# Geometry builders (src/testscenes.rs:38-148, restated as soups).
+# ---------------------------------------------------------------------------
+
+def quad(p):
+    """Planar quad p0->p1->p2->p3 (CCW from the front): triangles
+    (0,1,2), (0,2,3) with the flat normal of cross(p1-p0, p3-p0)."""
+    p = np.asarray(p, dtype=F32)
+    n = _normalize(np.cross(p[1] - p[0], p[3] - p[0]).astype(F32))
+    tris = np.stack([np.concatenate([p[0], p[1], p[2]]), np.concatenate([p[0], p[2], p[3]])])
+    nrm = np.tile(np.concatenate([n, n, n]), (2, 1))
+    return tris.astype(F32), nrm.astype(F32)
+
+
+def cuboid(mn, mx):
+    """Axis-aligned box with outward faces (12 triangles)."""
+    x0, y0, z0 = mn
+    x1, y1, z1 = mx
+    faces = [
+        [[x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]],
+        [[x1, y0, z0], [x0, y0, z0], [x0, y1, z0], [x1, y1, z0]],
+        [[x1, y0, z1], [x1, y0, z0], [x1, y1, z0], [x1, y1, z1]],
+        [[x0, y0, z0], [x0, y0, z1], [x0, y1, z1], [x0, y1, z0]],
+        [[x0, y1, z1], [x1, y1, z1], [x1, y1, z0], [x0, y1, z0]],
+        [[x0, y0, z0], [x1, y0, z0], [x1, y0, z1], [x0, y0, z1]],
+    ]
+    t, n = zip(*(quad(f) for f in faces))
+    return np.concatenate(t), np.concatenate(n)
+
+
+def uv_sphere(radius, segments, rings):
+    """UV sphere (src/testscenes.rs:106-142): smooth normals; the pole
+    rows keep their zero-area triangles, as the reference's builder does."""
+    segments = max(segments, 3)
+    rings = max(rings, 2)
+    r_ = np.arange(rings + 1, dtype=F32)
+    s_ = np.arange(segments + 1, dtype=F32)
+    v = r_ / F32(rings)
+    u = s_ / F32(segments)
+    theta = (v * F32(math.pi)).astype(F32)
+    phi = (u * F32(2.0) * F32(math.pi)).astype(F32)
+    st, ct = np.sin(theta).astype(F32), np.cos(theta).astype(F32)
+    sp, cp = np.sin(phi).astype(F32), np.cos(phi).astype(F32)
+    n = np.stack([st[:, None] * cp[None, :], np.broadcast_to(ct[:, None], (rings + 1, segments + 1)),
+                  -st[:, None] * sp[None, :]], axis=-1).astype(F32).reshape(-1, 3)
+    pos = (n * F32(radius)).astype(F32)
+    row = segments + 1
+    a = (np.arange(rings)[:, None] * row + np.arange(segments)[None, :]).reshape(-1)
+    b = a + row
+    idx = np.stack([a, b, a + 1, a + 1, b, b + 1], axis=1).reshape(-1, 3)
+    return pos[idx].reshape(-1, 9), n[idx].reshape(-1, 9)
+
+
+def rotate_y(points9, deg, center):
+    """Rotate (n, 9) corners about the Y axis through `center`."""
+    c, s = math.cos(math.radians(deg)), math.sin(math.radians(deg))
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]], dtype=np.float64)
+    p = points9.reshape(-1, 3).astype(np.float64) - np.asarray(center, np.float64)
+    return ((p @ R.T) + np.asarray(center, np.float64)).astype(F32).reshape(-1, 9)
+
+
+def translate(points9, t):
+    return (points9.reshape(-1, 3) + np.asarray(t, dtype=F32)).astype(F32).reshape(-1, 9)
+
+
+def lambert(rgb, **kw):
+    """Lambertian material: dielectric, roughness 1, KHR_materials_specular
+    factor 0 (src/pathtrace.rs:2810-2817)."""
+    return A.material(tuple(rgb) + (1.0,), metallic=0.0, roughness=1.0, specular=0.0, **kw)
+
+
+def pcg_hash(x):
+    x = np.uint32(x)
+    with np.errstate(over="ignore"):
+        state = np.uint32(x * np.uint32(747796405) + np.uint32(2891336453))
+        word = np.uint32(((state >> ((state >> np.uint32(28)) + np.uint32(4))) ^ state) * np.uint32(277803737))
+    return np.uint32((word >> np.uint32(22)) ^ word)
+
+
+def pcg_array(seed, n):
+    """n chained pcg_hash values from `seed` (uint32), as floats in [0,1)."""
+    out = np.empty(n, dtype=np.uint32)
+    x = np.uint32(seed)
+    for i in range(n):
+        x = pcg_hash(x)
+        out[i] = x
+    return out
+
+
+def pcg_uniform(seed, n):
+    """Vectorised: u_i = to_unit(pcg_hash(pcg_hash(seed) ^ i))."""
+    i = np.arange(n, dtype=np.uint32)
+    s = pcg_hash(seed)
+    with np.errstate(over="ignore"):
+        x = i ^ s
+        state = (x * np.uint32(747796405) + np.uint32(2891336453)).astype(np.uint32)
+        word = (((state >> ((state >> np.uint32(28)) + np.uint32(4))) ^ state) * np.uint32(277803737)).astype(np.uint32)
+        h = ((word >> np.uint32(22)) ^ word).astype(np.uint32)
+    return (h >> np.uint32(8)).astype(np.float64) * (1.0 / 16777216.0)
+
+
+# ---------------------------------------------------------------------------
+# Benchmark workloads.
+# ---------------------------------------------------------------------------
+
+def cornell():
+    """Cornell box: the five walls of the unit cube [0,1]^3 (10 tris, open
+    toward +Z), left red (0.63, 0.065, 0.05), right green (0.14, 0.45,
+    0.09), floor/ceiling/back grey 0.73; a short and a tall box rotated
+    -17 / +17 degrees about Y (12 tris each, grey 0.73); a 0.25 x 0.25
+    emissive quad 1 mm below the ceiling facing -Y, radiance (17, 12, 4),
+    single-sided; all Lambert (specular 0). Pinhole camera, yfov 40
+    degrees, on the box axis. 36 triangles."""
+    s = Scene("cornell")
+    white = s.add_material(lambert((0.73, 0.73, 0.73)))
+    red = s.add_material(lambert((0.63, 0.065, 0.05)))
+    green = s.add_material(lambert((0.14, 0.45, 0.09)))
+    light = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(17.0, 12.0, 4.0)))
+    walls = [
+        ([[0, 0, 0], [0, 0, 1], [1, 0, 1], [1, 0, 0]], white),   # floor, faces +Y
+        ([[0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1]], white),   # ceiling, faces -Y
+        ([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], white),   # back, faces +Z
+        ([[0, 0, 0], [0, 1, 0], [0, 1, 1], [0, 0, 1]], red),     # left, faces +X
+        ([[1, 0, 0], [1, 0, 1], [1, 1, 1], [1, 1, 0]], green),   # right, faces -X
+    ]
+    for q, m in walls:
+        t, _ = quad(q)
+        s.add_tris(t, m)
+    tall, _ = cuboid((0.0, 0.0, 0.0), (0.3, 0.6, 0.3))
+    tall = rotate_y(translate(tall, (0.18, 0.0, 0.2)), 17.0, (0.33, 0.0, 0.35))
+    s.add_tris(tall, white)
+    short, _ = cuboid((0.0, 0.0, 0.0), (0.3, 0.3, 0.3))
+    short = rotate_y(translate(short, (0.52, 0.0, 0.5)), -17.0, (0.67, 0.0, 0.65))
+    s.add_tris(short, white)
+    h = 0.125
+    lq, _ = quad([[0.5 - h, 0.999, 0.5 - h], [0.5 + h, 0.999, 0.5 - h],
+                  [0.5 + h, 0.999, 0.5 + h], [0.5 - h, 0.999, 0.5 + h]])
+    s.add_tris(lq, light)
+    dist = 0.5 / math.tan(math.radians(20.0)) + 1.0
+    s.camera = CameraSpec((0.5, 0.5, dist), (0.5, 0.5, 0.0), yfov_deg=40.0)
+    s.ambient = 0.0
+    return s
+
+
+def material_sweep(segments=64, rings=32, floor_half=10):
+    """7x7 UV spheres (radius 0.4, 64x32 segments, the reference builder's
+    4096 triangles each including 128 zero-area pole triangles) on a 1 m
+    grid; row r has metallic r/6, column c roughness 0.05 + 0.95 c/6; base
+    colour (0.9, 0.6, 0.3), default KHR specular. Checker floor of 1 m
+    cells (albedo 0.8 / 0.2, dielectric, roughness 1) as per-cell
+    geometry, 20 m x 20 m. One point light (500 cd) and one spot light
+    (500 cd, outer half-angle 30 deg, inner 20 deg); uniform sky radiance
+    0.3 (RenderOptions::ambient)."""
+    s = Scene("material_sweep")
+    sp, sn = uv_sphere(0.4, segments, rings)
+    for r in range(7):
+        for c in range(7):
+            m = s.add_material(A.material((0.9, 0.6, 0.3, 1.0), metallic=r / 6.0,
+                                          roughness=0.05 + 0.95 * c / 6.0))
+            t = (float(c - 3), 0.4, float(r - 3))
+            s.add_tris(translate(sp, t), m, normals=sn)
+    light_sq = s.add_material(A.material((0.8, 0.8, 0.8, 1.0), metallic=0.0, roughness=1.0))
+    dark_sq = s.add_material(A.material((0.2, 0.2, 0.2, 1.0), metallic=0.0, roughness=1.0))
+    cells = []
+    mats = []
+    for i in range(-floor_half, floor_half):
+        for j in range(-floor_half, floor_half):
+            t, _ = quad([[i, 0, j + 1], [i + 1, 0, j + 1], [i + 1, 0, j], [i, 0, j]])
+            cells.append(t)
+            m = light_sq if (i + j) % 2 == 0 else dark_sq
+            mats += [m, m]
+    s.add_tris(np.concatenate(cells), np.asarray(mats, np.uint32))
+    s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=500.0, position=(2.0, 6.0, 5.0)))
+    d = _normalize(np.array([3.0, -5.0, 3.0], F32))
+    s.lights.append(A.light(A.OXR_LIGHT_SPOT, intensity=500.0, position=(-3.0, 5.0, -3.0),
+                            direction=tuple(d.tolist()), inner=math.radians(20.0), outer=math.radians(30.0)))
+    s.camera = CameraSpec((0.0, 5.0, 9.0), (0.0, 0.3, 0.0), yfov_deg=40.0)
+    s.ambient = 0.3
+    return s
+
+
+def glossy_caustic(segments=64, rings=64):
+    """A 10 m x 10 m x 5 m room (12 tris, inward faces, diffuse 0.7) with
+    a GGX floor (dielectric, roughness 0.1, base 0.8); a 1 m glass sphere
+    (transmission 1, IOR 1.5, volume thickness 1, 64x64 segments = 8192
+    tris) at 1.2 m; a 0.1 m emissive quad (radiance 1000) 5 cm above the
+    sphere facing down, so most of its light reaches the floor through the
+    glass."""
+    s = Scene("glossy_caustic")
+    wall = s.add_material(lambert((0.7, 0.7, 0.7)))
+    floor = s.add_material(A.material((0.8, 0.8, 0.8, 1.0), metallic=0.0, roughness=0.1))
+    glass = s.add_material(A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.0,
+                                      transmission=1.0, thickness=1.0, ior=1.5))
+    emit = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(1000.0, 1000.0, 1000.0)))
+    room, _ = cuboid((-5.0, 0.0, -5.0), (5.0, 5.0, 5.0))
+    room = room.reshape(-1, 3, 3)[:, ::-1, :].reshape(-1, 9).copy()  # inward faces
+    fl = room[10:12]  # the -Y face (floor)
+    s.add_tris(room[:10], wall)
+    s.add_tris(fl, floor)
+    sp, sn = uv_sphere(0.5, segments, rings)
+    s.add_tris(translate(sp, (0.0, 1.2, 0.0)), glass, normals=sn)
+    h = 0.05
+    y = 1.2 + 0.5 + 0.05
+    lq, _ = quad([[-h, y, h], [h, y, h], [h, y, -h], [-h, y, -h]])
+    s.add_tris(lq, emit)
+    s.camera = CameraSpec((0.0, 2.6, 4.6), (0.0, 0.6, 0.0), yfov_deg=40.0)
+    s.ambient = 0.0
+    return s
+
+
+def triangle_soup(seed, n_tris, n_emit=4):
+    """Seeded random soup: n_tris triangles with corners uniform in the unit
+    cube (edge length scaled so the soup stays sparse), random Lambert / GGX
+    materials, and n_emit emissive triangles."""
+    s = Scene("soup")
+    u = pcg_uniform(seed, n_tris * 9 + n_tris * 2 + n_emit * 9).astype(np.float64)
+    k = 0
+    centers = u[k:k + n_tris * 3].reshape(n_tris, 3)
+    k += n_tris * 3
+    off = (u[k:k + n_tris * 6].reshape(n_tris, 6) - 0.5) * (0.6 / max(n_tris, 1) ** (1.0 / 3.0))
+    k += n_tris * 6
+    pos = np.empty((n_tris, 9))
+    pos[:, 0:3] = centers
+    pos[:, 3:6] = centers + off[:, 0:3]
+    pos[:, 6:9] = centers + off[:, 3:6]
+    mats = []
+    for i in range(8):
+        if i % 2 == 0:
+            mats.append(s.add_material(lambert((0.3 + 0.08 * i, 0.5, 0.7 - 0.05 * i))))
+        else:
+            mats.append(s.add_material(A.material((0.8, 0.7, 0.6, 1.0), metallic=0.0, roughness=0.1 + 0.1 * i)))
+    mid = (u[k:k + n_tris] * 8).astype(np.uint32).clip(0, 7)
+    k += n_tris
+    s.add_tris(pos.astype(F32), np.asarray(mats, np.uint32)[mid])
+    em = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(20.0, 20.0, 20.0), double_sided=True))
+    ep = u[k:k + n_emit * 9].reshape(n_emit, 9) * 0.2 + 0.4
+    s.add_tris(ep.astype(F32), em)
+    s.camera = CameraSpec((0.5, 0.5, 2.6), (0.5, 0.5, 0.5), yfov_deg=40.0)
+    s.ambient = 0.0
+    return s
+
+
+# ---------------------------------------------------------------------------
+# The reference's own integrator test scenes (src/pathtrace.rs:3346-3750).
+# ---------------------------------------------------------------------------
+
+def furnace(rho=0.5, le=1.0):
+    """Closed box of double-sided emissive Lambert walls, viewed from inside
+    (src/pathtrace.rs:3348-3359): L = Le / (1 - rho)."""
+    s = Scene("furnace")
+    m = s.add_material(lambert((rho, rho, rho), emissive=(le, le, le), double_sided=True))
+    t, n = cuboid((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    s.add_tris(t, m, normals=n)
+    s.camera = CameraSpec((0.2, 0.1, 0.4), (0.0, -0.3, -1.0), yfov_deg=math.degrees(1.2))
+    s.ambient = 0.0
+    return s
+
+
+def area_light_floor(roughness, metallic):
+    """White floor under a down-facing emissive quad
+    (src/pathtrace.rs:3378-3412)."""
+    s = Scene("area_light")
+    fm = A.material((0.8, 0.8, 0.8, 1.0), metallic=metallic, roughness=roughness)
+    f = s.add_material(fm)
+    lm = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(4.0, 4.0, 4.0)))
+    t, n = quad([[-2, 0, 2], [2, 0, 2], [2, 0, -2], [-2, 0, -2]])
+    s.add_tris(t, f, normals=n)
+    t, n = quad([[-0.5, 1, -0.5], [0.5, 1, -0.5], [0.5, 1, 0.5], [-0.5, 1, 0.5]])
+    s.add_tris(t, lm, normals=n)
+    s.camera = CameraSpec((0.0, 2.5, 3.0), (0.0, 0.0, 0.0), yfov_deg=math.degrees(0.9))
+    s.ambient = 0.0
+    return s
+
+
+def glass_cube(volume):
+    """Emissive backdrop (Le = 1) seen through a glass cube
+    (src/pathtrace.rs:3670-3705)."""
+    s = Scene("glass")
+    kw = dict(transmission=1.0)
+    if volume:
+        kw.update(thickness=1.0, attenuation_distance=1.0, attenuation_color=(0.5, 0.5, 0.5))
+    g = s.add_material(A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.0, **kw))
+    e = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(1.0, 1.0, 1.0)))
+    t, n = cuboid((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    s.add_tris(t, g, normals=n)
+    t, n = quad([[-3, -3, -2], [3, -3, -2], [3, 3, -2], [-3, 3, -2]])
+    s.add_tris(t, e, normals=n)
+    s.camera = CameraSpec((0.0, 0.0, 4.0), (0.0, 0.0, 0.0), yfov_deg=math.degrees(0.15))
+    s.ambient = 0.0
+    return s
+
+
+def lit_plane(rho=1.0, with_point=True):
+    """Lambert plane (ρ) with an optional 4 cd point light 1 m above the
+    origin (src/pathtrace.rs:3717-3742)."""
+    s = Scene("plane")
+    m = s.add_material(lambert((rho, rho, rho)))
+    t, n = quad([[-1, 0, 1], [1, 0, 1], [1, 0, -1], [-1, 0, -1]])
+    s.add_tris(t, m, normals=n)
+    if with_point:
+        s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=4.0, position=(0.0, 1.0, 0.0),
+                                direction=(0.0, -1.0, 0.0)))
+    s.camera = CameraSpec((0.0, 3.0, 0.001), (0.0, 0.0, 0.0), yfov_deg=math.degrees(0.05))
+    s.ambient = 0.0
+    return s
+
+
+WORKLOADS = {
+    "cornell": cornell,
+    "material_sweep": material_sweep,
+    "glossy_caustic": glossy_caustic,
+}
+
+
+def reference_box():
+    """The reference's `testscenes::cornell_box` (src/testscenes.rs:305-385):
+    [-1,1]^3 room open toward +Z, red left / green right / white walls,
+    a tall and a short block, a point light (6 cd, colour (1, .95, .85))
+    under the ceiling; camera at (0, 0, 3.4), yfov 0.75 rad. Materials are
+    `diffuse_material` (default KHR specular)."""
+    s = Scene("reference_box")
+    white = s.add_material(A.material((0.75, 0.75, 0.75, 1.0), metallic=0.0, roughness=1.0))
+    red = s.add_material(A.material((0.65, 0.05, 0.05, 1.0), metallic=0.0, roughness=1.0))
+    green = s.add_material(A.material((0.12, 0.45, 0.15, 1.0), metallic=0.0, roughness=1.0))
+    walls = [
+        ([[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]], white),
+        ([[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1]], white),
+        ([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]], white),
+        ([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]], red),
+        ([[1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]], green),
+    ]
+    for q, m in walls:
+        t, n = quad(q)
+        s.add_tris(t, m, normals=n)
+    t, n = cuboid((-0.6, -1.0, -0.6), (-0.1, 0.2, -0.1))
+    s.add_tris(t, white, normals=n)
+    t, n = cuboid((0.15, -1.0, -0.1), (0.65, -0.4, 0.4))
+    s.add_tris(t, white, normals=n)
+    s.lights.append(A.light(A.OXR_LIGHT_POINT, color=(1.0, 0.95, 0.85), intensity=6.0,
+                            position=(0.0, 0.85, 0.0), direction=(0.0, -1.0, 0.0)))
+    s.camera = CameraSpec((0.0, 0.0, 3.4), (0.0, 0.0, 0.0), yfov_deg=math.degrees(0.75))
+    s.ambient = 0.0
+    return s
