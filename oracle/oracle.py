@@ -61,6 +61,10 @@ def lib():
         L.oxrcpu_surface.argtypes = [C.c_void_p, C.c_uint32, fp, fp]
         L.oxrcpu_resolve_hdr.argtypes = [dp, C.c_uint32, C.c_uint32, C.POINTER(A.Display), fp]
         L.oxrcpu_resolve_rgba8.argtypes = [dp, C.c_uint32, C.c_uint32, C.POINTER(A.Display), bp]
+        L.oxrcpu_record_rays.restype = C.c_uint32
+        L.oxrcpu_record_rays.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
+        L.oxrcpu_bvh4_count.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
         _LIB = L
     return _LIB
 
@@ -143,6 +147,16 @@ class OracleScene:
         L.oxrcpu_emissive_table(self.h, tris.ctypes.data_as(up), pmf.ctypes.data_as(fp), cdf.ctypes.data_as(fp),
                                 lookup.ctypes.data_as(up))
         return tris[:m], pmf[:m], cdf[:m], lookup[:self.scene.num_tris]
+
+    def record_rays(self, width, height, first, n, y0, y1, x_step=1, max_rays=1 << 20, opts=None):
+        """Closest-hit rays the oracle's paths trace (8 floats: o, k, d, t_max)."""
+        if opts is None:
+            opts = options(width, height, ambient=getattr(self.scene, "ambient", 0.0))
+        cam = self.scene.camera.resolve(width, height)
+        out = np.zeros((max_rays, 8), np.float32)
+        m = lib().oxrcpu_record_rays(self.h, C.byref(cam), C.byref(opts), first, n, y0, y1, x_step, max_rays,
+                                     out.ctypes.data_as(fp))
+        return out[:m]
 
     def surface(self, g, bary):
         b, bp_ = _f(bary)
@@ -311,3 +325,12 @@ def resolve_rgba8(accum, samples, background=(0, 0, 0, 255), tone_map=0, exposur
     out = np.zeros(accum.shape[:-1] + (4,), np.uint8)
     lib().oxrcpu_resolve_rgba8(accum.ctypes.data_as(dp), npx, samples, C.byref(d), out.ctypes.data_as(bp))
     return out
+
+
+def bvh4_count(nodes, tris, rays):
+    """(nodes fetched, triangles fetched) per ray on an exported device BVH."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    counts = np.zeros((len(rays), 2), np.uint32)
+    lib().oxrcpu_bvh4_count(nodes.ctypes.data, tris.ctypes.data, len(rays), rays.ctypes.data_as(fp),
+                            counts.ctypes.data_as(up))
+    return counts

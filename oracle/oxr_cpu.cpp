@@ -1620,6 +1620,7 @@ struct Integrator {
   uint32_t width, height;
   float ambient;
   oxr_pt_options pt;
+  std::vector<std::array<float, 8>>* record = nullptr;  // closest-hit rays, when recording
 
   f3 clampc(f3 c) const {
     float m = max3(c);
@@ -1704,6 +1705,7 @@ struct Integrator {
     uint32_t k = 0;
     for (;;) {
       uint32_t ray_id = k << 16;
+      if (record) record->push_back({o[0], o[1], o[2], (float)k, d[0], d[1], d[2], INF_F});
       Hit hit;
       bool got = closest_hit(ts, o, d, 0.0f, INF_F, [&](const Hit& c) { return accept(c, s, ray_id, k == 0); }, hit);
       if (has_medium) {
@@ -2285,6 +2287,101 @@ void oxrcpu_resolve_rgba8(const double* accum, uint32_t npx, uint32_t samples, c
     }
     for (int k = 0; k < 3; ++k) o[k] = linear_to_srgb_byte(r[k]);
     o[3] = (uint8_t)std::round(clampr(r[3], 0.0f, 1.0f) * 255.0f);
+  }
+}
+
+
+// Records the closest-hit rays of samples [first, first+n) of rows
+// [y0, y1) (single thread, row-major, sample-minor) into out (8 floats per
+// ray: o, k, d, t_max), at most max_rays. Returns the number recorded.
+uint32_t oxrcpu_record_rays(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t first,
+                            uint32_t n, uint32_t y0, uint32_t y1, uint32_t x_step, uint32_t max_rays, float* out) {
+  uint32_t W = std::max<uint32_t>(opts->width, 1), H = std::max<uint32_t>(opts->height, 1);
+  std::vector<std::array<float, 8>> rec;
+  Integrator integ{h->s, *cam, W, H, std::isfinite(opts->ambient) ? fmaxr(opts->ambient, 0.0f) : 0.0f, *opts};
+  integ.record = &rec;
+  for (uint32_t y = y0; y < std::min(y1, H); ++y)
+    for (uint32_t x = 0; x < W; x += std::max(x_step, 1u))
+      for (uint32_t si = 0; si < n; ++si) {
+        f3 c;
+        integ.sample(x, y, first + si, c);
+        if (rec.size() >= max_rays) goto done;
+      }
+done:
+  uint32_t m = (uint32_t)std::min<size_t>(rec.size(), max_rays);
+  for (uint32_t i = 0; i < m; ++i)
+    for (int k = 0; k < 8; ++k) out[8 * i + k] = rec[i][k];
+  return m;
+}
+
+// Traversal cost of rays on the device's flattened 4-wide BVH (128-byte
+// nodes: six float4 of child bounds, child refs, leaf counts; 48-byte
+// triangles: v0, gid, v1, flags, v2, alpha), restating the device's
+// closest-hit order: leaves of a node first, inner children near to far.
+// counts[2*i] = nodes fetched, counts[2*i+1] = triangles fetched.
+void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts) {
+  for (uint32_t r = 0; r < n_rays; ++r) {
+    const float* ry = rays + 8ull * r;
+    f3 o = {ry[0], ry[1], ry[2]}, d = {ry[4], ry[5], ry[6]};
+    float tmin = 0.0f, tmax = ry[7];
+    PRay pr = prepare_ray(o, d);
+    f3 inv = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    f3 oi = {o[0] * inv[0], o[1] * inv[1], o[2] * inv[2]};
+    std::vector<uint32_t> stack;
+    uint32_t node = 0, nn = 0, nt = 0;
+    uint32_t best = 0xffffffffu;
+    for (;;) {
+      ++nn;
+      const float* N = nodes + 32ull * node;
+      const uint32_t* NU = reinterpret_cast<const uint32_t*>(N);
+      float tn[4];
+      for (int c = 0; c < 4; ++c) {
+        float t0[3], t1[3];
+        for (int k = 0; k < 3; ++k) {
+          t0[k] = std::fma(N[8 * k + c], inv[k], -oi[k]);
+          t1[k] = std::fma(N[8 * k + 4 + c], inv[k], -oi[k]);
+        }
+        float tnear = tmin, tfar = tmax;
+        for (int k = 0; k < 3; ++k) {
+          bool neg = std::signbit(inv[k]);
+          tnear = std::fmax(tnear, neg ? t1[k] : t0[k]);
+          tfar = std::fmin(tfar, neg ? t0[k] : t1[k]);
+        }
+        bool hit = NU[24 + c] != 0xffffffffu && tnear <= tfar * (1.0f + 1.0f / 65536.0f);
+        tn[c] = hit ? tnear : INF_F;
+      }
+      for (int c = 0; c < 4; ++c) {
+        if (tn[c] != INF_F && NU[28 + c] > 0) {
+          for (uint32_t k = 0; k < NU[28 + c]; ++k) {
+            ++nt;
+            const float* T = tris + 12ull * (NU[24 + c] + k);
+            float v9[9] = {T[0], T[1], T[2], T[4], T[5], T[6], T[8], T[9], T[10]};
+            float t, b1, b2;
+            bool front;
+            if (!intersect_tri(pr, v9, tmin, tmax, t, b1, b2, front)) continue;
+            uint32_t gid = reinterpret_cast<const uint32_t*>(T)[3];
+            if (best != 0xffffffffu && !(t < tmax || (t == tmax && gid < best))) continue;
+            tmax = t;
+            best = gid;
+          }
+          tn[c] = INF_F;
+        }
+      }
+      uint32_t ci[4] = {NU[24], NU[25], NU[26], NU[27]};
+      for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b)
+          if (tn[a] < tn[b]) {
+            std::swap(tn[a], tn[b]);
+            std::swap(ci[a], ci[b]);
+          }
+      for (int c = 0; c < 4; ++c)
+        if (tn[c] != INF_F && tn[c] <= tmax * (1.0f + 1.0f / 65536.0f)) stack.push_back(ci[c]);
+      if (stack.empty()) break;
+      node = stack.back();
+      stack.pop_back();
+    }
+    counts[2 * r] = nn;
+    counts[2 * r + 1] = nt;
   }
 }
 

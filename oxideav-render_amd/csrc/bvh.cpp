@@ -1,0 +1,394 @@
+// Binned-SAH BVH build and 4-wide collapse (see bvh.h).
+#include "bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <limits>
+#include <mutex>
+#include <thread>
+
+namespace oxr {
+namespace {
+
+constexpr int NBINS = 32;
+constexpr float INF = std::numeric_limits<float>::infinity();
+
+struct Box {
+  float lo[3] = {INF, INF, INF};
+  float hi[3] = {-INF, -INF, -INF};
+  void grow(const Box& b) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::fmin(lo[k], b.lo[k]);
+      hi[k] = std::fmax(hi[k], b.hi[k]);
+    }
+  }
+  void grow_pt(const float* p) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::fmin(lo[k], p[k]);
+      hi[k] = std::fmax(hi[k], p[k]);
+    }
+  }
+  double area() const {
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = hi[k] >= lo[k] ? (double)hi[k] - (double)lo[k] : 0.0;
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+  }
+  bool valid() const { return hi[0] >= lo[0]; }
+};
+
+struct Item {
+  Box b;
+  float c[3];
+  uint32_t g;
+};
+
+struct BNode {
+  Box b;
+  uint32_t left = 0, right = 0;  // inner: child node ids
+  uint32_t first = 0, count = 0;  // leaf: item range (count > 0)
+};
+
+struct Builder {
+  std::vector<Item>& items;
+  std::vector<BNode>& nodes;
+  std::atomic<uint32_t> next_node{1};
+  int max_leaf;
+  // Work queue of subtrees for the worker threads.
+  struct Task {
+    uint32_t node, lo, hi;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<Task> queue;
+  int busy = 0;
+  bool done = false;
+
+  Builder(std::vector<Item>& it, std::vector<BNode>& nd, int ml) : items(it), nodes(nd), max_leaf(ml) {}
+
+  // Returns true and (axis, bin) of the best SAH split, or false for a leaf.
+  bool find_split(uint32_t lo, uint32_t hi, const Box& cb, int& axis_out, int& bin_out) {
+    uint32_t cnt = hi - lo;
+    double best = std::numeric_limits<double>::infinity();
+    int best_axis = -1, best_bin = -1;
+    Box bounds;
+    for (uint32_t i = lo; i < hi; ++i) bounds.grow(items[i].b);
+    double parent_area = bounds.area();
+    for (int axis = 0; axis < 3; ++axis) {
+      float ext = cb.hi[axis] - cb.lo[axis];
+      if (!(ext > 0.0f) || !std::isfinite(ext)) continue;
+      Box bb[NBINS];
+      uint32_t bn[NBINS] = {0};
+      float k = (float)NBINS / ext;
+      for (uint32_t i = lo; i < hi; ++i) {
+        int b = (int)((items[i].c[axis] - cb.lo[axis]) * k);
+        b = std::min(std::max(b, 0), NBINS - 1);
+        bb[b].grow(items[i].b);
+        bn[b]++;
+      }
+      double la[NBINS], ra[NBINS];
+      uint32_t ln[NBINS], rn[NBINS];
+      Box acc;
+      uint32_t n = 0;
+      for (int i = 0; i < NBINS; ++i) {
+        acc.grow(bb[i]);
+        n += bn[i];
+        la[i] = acc.area();
+        ln[i] = n;
+      }
+      acc = Box();
+      n = 0;
+      for (int i = NBINS - 1; i >= 0; --i) {
+        acc.grow(bb[i]);
+        n += bn[i];
+        ra[i] = acc.area();
+        rn[i] = n;
+      }
+      for (int i = 0; i < NBINS - 1; ++i) {
+        if (ln[i] == 0 || rn[i + 1] == 0) continue;
+        double c = la[i] * ln[i] + ra[i + 1] * rn[i + 1];
+        if (c < best) {
+          best = c;
+          best_axis = axis;
+          best_bin = i + 1;
+        }
+      }
+    }
+    // SAH leaf test: traversal cost 1, intersection cost 1 per triangle.
+    double leaf_cost = (double)cnt;
+    double split_cost = parent_area > 0.0 ? 1.0 + best / parent_area : 1.0 + (double)cnt;
+    if (cnt <= (uint32_t)max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) return false;
+    axis_out = best_axis;
+    bin_out = best_bin;
+    return true;
+  }
+
+  // Builds the subtree of `node` over items [lo, hi); pushes big
+  // subtrees to the shared queue and recurses on small ones.
+  void build(uint32_t node, uint32_t lo, uint32_t hi) {
+    for (;;) {
+      BNode& nd = nodes[node];
+      Box b, cb;
+      for (uint32_t i = lo; i < hi; ++i) {
+        b.grow(items[i].b);
+        cb.grow_pt(items[i].c);
+      }
+      nd.b = b;
+      uint32_t cnt = hi - lo;
+      int axis = -1, bin = -1;
+      bool split = cnt > 1 && find_split(lo, hi, cb, axis, bin);
+      uint32_t mid = lo;
+      if (split && axis >= 0) {
+        float ext = cb.hi[axis] - cb.lo[axis];
+        float k = (float)NBINS / ext;
+        float clo = cb.lo[axis];
+        auto it = std::partition(items.begin() + lo, items.begin() + hi, [&](const Item& x) {
+          int bb = (int)((x.c[axis] - clo) * k);
+          bb = std::min(std::max(bb, 0), NBINS - 1);
+          return bb < bin;
+        });
+        mid = (uint32_t)(it - items.begin());
+      }
+      if (split && (axis < 0 || mid == lo || mid == hi)) {
+        if (cnt <= (uint32_t)max_leaf) {
+          split = false;
+        } else {
+          // Degenerate centroids: median split along the widest axis.
+          int ax = 0;
+          for (int kk = 1; kk < 3; ++kk)
+            if (cb.hi[kk] - cb.lo[kk] > cb.hi[ax] - cb.lo[ax]) ax = kk;
+          mid = lo + cnt / 2;
+          std::nth_element(items.begin() + lo, items.begin() + mid, items.begin() + hi,
+                           [&](const Item& a, const Item& c) { return a.c[ax] < c.c[ax] || (a.c[ax] == c.c[ax] && a.g < c.g); });
+        }
+      }
+      if (!split) {
+        if (cnt > (uint32_t)max_leaf) {
+          // Can only happen for identical centroids with cnt > max_leaf:
+          // split in the middle of the range.
+          mid = lo + cnt / 2;
+        } else {
+          nd.first = lo;
+          nd.count = cnt;
+          return;
+        }
+      }
+      uint32_t l = next_node.fetch_add(2);
+      nd.left = l;
+      nd.right = l + 1;
+      nd.count = 0;
+      // Hand the larger half to the pool when it is big, keep the other.
+      uint32_t nl = mid - lo, nr = hi - mid;
+      if (nl > nr) {
+        if (nl > 65536) {
+          push({l, lo, mid});
+        } else {
+          build(l, lo, mid);
+        }
+        node = l + 1;
+        lo = mid;
+      } else {
+        if (nr > 65536) {
+          push({l + 1, mid, hi});
+        } else {
+          build(l + 1, mid, hi);
+        }
+        node = l;
+        hi = mid;
+      }
+    }
+  }
+
+  void push(Task t) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      queue.push_back(t);
+    }
+    cv.notify_one();
+  }
+
+  void worker() {
+    for (;;) {
+      Task t;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return !queue.empty() || (busy == 0 && queue.empty()) || done; });
+        if (queue.empty()) {
+          done = true;
+          cv.notify_all();
+          return;
+        }
+        t = queue.back();
+        queue.pop_back();
+        ++busy;
+      }
+      build(t.node, t.lo, t.hi);
+      {
+        std::lock_guard<std::mutex> g(mu);
+        --busy;
+      }
+      cv.notify_all();
+    }
+  }
+};
+
+struct Collapser {
+  const std::vector<BNode>& bn;
+  const std::vector<Item>& items;
+  const float* pos;
+  const uint32_t* flags;
+  const float* alpha;
+  float pad;
+  Bvh4& out;
+
+  double area(uint32_t id) const { return bn[id].b.area(); }
+
+  // Writes the 4-wide node for binary node `id` (an inner node) at out
+  // index `slot`; returns depth of the subtree.
+  uint32_t emit(uint32_t id, uint32_t slot) {
+    uint32_t kids[4];
+    int nk = 0;
+    kids[nk++] = bn[id].left;
+    kids[nk++] = bn[id].right;
+    while (nk < 4) {
+      int best = -1;
+      double ba = -1.0;
+      for (int i = 0; i < nk; ++i) {
+        const BNode& c = bn[kids[i]];
+        if (c.count == 0 && area(kids[i]) > ba) {
+          ba = area(kids[i]);
+          best = i;
+        }
+      }
+      if (best < 0) break;
+      uint32_t c = kids[best];
+      kids[best] = bn[c].left;
+      kids[nk++] = bn[c].right;
+    }
+    Node4 n4;
+    for (int i = 0; i < 4; ++i) {
+      n4.lo_x[i] = n4.lo_y[i] = n4.lo_z[i] = INF;
+      n4.hi_x[i] = n4.hi_y[i] = n4.hi_z[i] = -INF;
+      n4.child[i] = NODE_EMPTY;
+      n4.count[i] = 0;
+    }
+    out.nodes[slot] = n4;
+    uint32_t depth = 1;
+    for (int i = 0; i < nk; ++i) {
+      const BNode& c = bn[kids[i]];
+      Node4& w = out.nodes[slot];
+      w.lo_x[i] = c.b.lo[0] - pad;
+      w.lo_y[i] = c.b.lo[1] - pad;
+      w.lo_z[i] = c.b.lo[2] - pad;
+      w.hi_x[i] = c.b.hi[0] + pad;
+      w.hi_y[i] = c.b.hi[1] + pad;
+      w.hi_z[i] = c.b.hi[2] + pad;
+      if (c.count > 0) {
+        w.child[i] = (uint32_t)out.tris.size();
+        w.count[i] = c.count;
+        for (uint32_t k = 0; k < c.count; ++k) out.tris.push_back(make_tri(items[c.first + k].g));
+        out.num_leaves++;
+        out.sah_cost += c.b.area() * (double)c.count;
+      } else {
+        uint32_t child_slot = (uint32_t)out.nodes.size();
+        out.nodes.emplace_back();
+        out.nodes[slot].child[i] = child_slot;
+        out.nodes[slot].count[i] = 0;
+        out.sah_cost += c.b.area();
+        uint32_t d = emit(kids[i], child_slot);
+        depth = std::max(depth, d + 1);
+      }
+    }
+    return depth;
+  }
+
+  TriRec make_tri(uint32_t g) const {
+    TriRec t;
+    const float* p = pos + 9ull * g;
+    for (int k = 0; k < 3; ++k) {
+      t.v0[k] = p[k];
+      t.v1[k] = p[3 + k];
+      t.v2[k] = p[6 + k];
+    }
+    t.gid = g;
+    t.flags = flags ? flags[g] : 0u;
+    t.alpha = alpha ? alpha[g] : 1.0f;
+    return t;
+  }
+};
+
+}  // namespace
+
+Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float* alpha, int max_leaf, int threads) {
+  auto t0 = std::chrono::steady_clock::now();
+  Bvh4 out;
+  if (n == 0) return out;
+  max_leaf = std::min(std::max(max_leaf, 1), 8);
+  std::vector<Item> items(n);
+  float gmax = 0.0f;
+  for (uint32_t g = 0; g < n; ++g) {
+    Item& it = items[g];
+    const float* p = pos + 9ull * g;
+    for (int k = 0; k < 3; ++k) {
+      it.b.lo[k] = std::fmin(p[k], std::fmin(p[3 + k], p[6 + k]));
+      it.b.hi[k] = std::fmax(p[k], std::fmax(p[3 + k], p[6 + k]));
+      it.c[k] = 0.5f * (it.b.lo[k] + it.b.hi[k]);
+      if (std::isfinite(it.b.lo[k])) gmax = std::fmax(gmax, std::fabs(it.b.lo[k]));
+      if (std::isfinite(it.b.hi[k])) gmax = std::fmax(gmax, std::fabs(it.b.hi[k]));
+    }
+    it.g = g;
+  }
+  // Absolute padding of every box: keeps the slab test conservative
+  // against the rounding of the watertight triangle test.
+  out.pad = std::ldexp(std::fmax(gmax, 1.0f), -20);
+  std::vector<BNode> bn(2ull * n);
+  Builder b(items, bn, max_leaf);
+  int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  nt = std::min(nt, 32);
+  if (n < 200000 || nt == 1) {
+    b.build(0, 0, n);
+  } else {
+    b.push({0, 0, n});
+    std::vector<std::thread> ts;
+    for (int i = 0; i < nt; ++i) ts.emplace_back([&] { b.worker(); });
+    for (auto& t : ts) t.join();
+  }
+  uint32_t used = b.next_node.load();
+  bn.resize(used);
+  out.nodes.reserve(used / 2 + 2);
+  out.tris.reserve(n);
+  Collapser c{bn, items, pos, flags, alpha, out.pad, out};
+  out.nodes.emplace_back();
+  if (bn[0].count > 0) {
+    // The whole scene is one leaf: a root with a single leaf child.
+    Node4 n4;
+    for (int i = 0; i < 4; ++i) {
+      n4.lo_x[i] = n4.lo_y[i] = n4.lo_z[i] = INF;
+      n4.hi_x[i] = n4.hi_y[i] = n4.hi_z[i] = -INF;
+      n4.child[i] = NODE_EMPTY;
+      n4.count[i] = 0;
+    }
+    n4.lo_x[0] = bn[0].b.lo[0] - out.pad;
+    n4.lo_y[0] = bn[0].b.lo[1] - out.pad;
+    n4.lo_z[0] = bn[0].b.lo[2] - out.pad;
+    n4.hi_x[0] = bn[0].b.hi[0] + out.pad;
+    n4.hi_y[0] = bn[0].b.hi[1] + out.pad;
+    n4.hi_z[0] = bn[0].b.hi[2] + out.pad;
+    n4.child[0] = 0;
+    n4.count[0] = bn[0].count;
+    for (uint32_t k = 0; k < bn[0].count; ++k) out.tris.push_back(c.make_tri(items[bn[0].first + k].g));
+    out.nodes[0] = n4;
+    out.num_leaves = 1;
+    out.max_depth = 1;
+  } else {
+    out.max_depth = c.emit(0, 0);
+  }
+  double root_area = bn[0].b.area();
+  if (root_area > 0.0) out.sah_cost /= root_area;
+  out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return out;
+}
+
+}  // namespace oxr

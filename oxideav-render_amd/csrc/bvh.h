@@ -1,0 +1,61 @@
+// Host-side BVH builder of the oxr path tracer: binned SAH (Wald 2007)
+// over the prepared triangle soup, collapsed to 4-wide nodes laid out
+// for coalesced 16-byte fetches on the device.
+//
+// Replaces the BVH of the reference's trace layer (oxideav-render
+// src/trace.rs:320-351, which builds oxideav_mesh3d::Bvh with binned SAH).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace oxr {
+
+// 128-byte wide node: child boxes as six float4 (lo/hi per axis, one
+// lane per child), then child references and leaf triangle counts.
+// count[i] == 0: inner child, child[i] is a node index;
+// count[i]  > 0: leaf of count[i] triangles starting at tris[child[i]];
+// child[i] == EMPTY: unused slot (its box is empty: lo = +inf, hi = -inf).
+struct alignas(16) Node4 {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  uint32_t child[4];
+  uint32_t count[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
+
+// 48-byte triangle record in leaf order: v0, global id; v1, flags; v2,
+// alpha (BLEND coverage).
+struct alignas(16) TriRec {
+  float v0[3];
+  uint32_t gid;
+  float v1[3];
+  uint32_t flags;
+  float v2[3];
+  float alpha;
+};
+static_assert(sizeof(TriRec) == 48, "TriRec is 48 bytes");
+
+// TriRec.flags bits (acceptance rules of Integrator::accept,
+// src/pathtrace.rs:1924-1934, resolved per material).
+constexpr uint32_t TRI_DOUBLE_SIDED = 1u;
+constexpr uint32_t TRI_MASK_REJECT = 2u;  // MASK with alpha < cutoff
+constexpr uint32_t TRI_BLEND = 4u;
+
+constexpr uint32_t NODE_EMPTY = 0xffffffffu;
+
+struct Bvh4 {
+  std::vector<Node4> nodes;
+  std::vector<TriRec> tris;
+  uint32_t num_leaves = 0;
+  uint32_t max_depth = 0;  // levels of 4-wide nodes (root = 1)
+  double sah_cost = 0.0;
+  double build_ms = 0.0;
+  float pad = 0.0f;  // absolute padding applied to every box
+};
+
+// pos: n x 9 floats (v0, v1, v2). flags/alpha: per global triangle.
+// max_leaf: largest leaf (1..8). threads: 0 = hardware concurrency.
+Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float* alpha, int max_leaf, int threads);
+
+}  // namespace oxr

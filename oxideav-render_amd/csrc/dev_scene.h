@@ -1,0 +1,102 @@
+// Device-resident scene, path-state and queue layouts of the oxr path
+// tracer (shared by the kernels and the host runtime).
+#pragma once
+
+#include <stdint.h>
+
+namespace oxr {
+
+// Per-material constants (PreparedMaterial + the untextured reads of
+// TraceScene::material_oriented, oxideav-render src/prepare.rs:623-668,
+// src/trace.rs:724-922), resolved and clamped on the host. 144 bytes.
+struct alignas(16) DevMaterial {
+  float base[4];
+  float metallic, roughness, ior, specular;
+  float emissive[3];
+  uint32_t flags;  // MAT_DS | MAT_UNLIT
+  float spec_color[3];
+  float transmission;
+  float thickness;
+  float att_color[3];
+  float att_dist;
+  float clearcoat, cc_rough, sheen_rough;
+  float sheen[3];
+  float iridescence;
+  float irid_ior, irid_thick, dt, dispersion;
+  float dt_color[3];
+  float pad;
+};
+static_assert(sizeof(DevMaterial) == 144, "DevMaterial is 144 bytes");
+
+constexpr uint32_t MAT_DS = 1u;
+constexpr uint32_t MAT_UNLIT = 2u;
+
+// Punctual light (PreparedLight, src/prepare.rs:693-714) with the
+// spot-cone constants of PreparedLight::sample precomputed. 64 bytes.
+struct alignas(16) DevLight {
+  uint32_t kind;
+  float base[3];  // colour x intensity
+  float pos[3];
+  uint32_t has_range;
+  float dir[3];
+  float range;
+  float spot_scale, spot_offset, pad0, pad1;
+};
+static_assert(sizeof(DevLight) == 64, "DevLight is 64 bytes");
+
+struct Node4;
+struct TriRec;
+
+struct DevScene {
+  const Node4* nodes;
+  const TriRec* tris;
+  const float* pos;         // 9 per global triangle
+  const float* nrm;         // 9 per global triangle (or null)
+  const uint32_t* tri_info; // material id | TI_HAS_NORMALS
+  const DevMaterial* mats;
+  const DevLight* lights;
+  const uint32_t* l_tris;
+  const float* l_pmf;
+  const float* l_cdf;
+  const uint32_t* l_lookup;
+  uint32_t num_tris, num_mats, num_lights, num_emissive;
+  uint32_t has_blend;
+  uint32_t pad[3];
+};
+
+constexpr uint32_t TI_HAS_NORMALS = 0x80000000u;
+constexpr uint32_t TI_MAT_MASK = 0x7fffffffu;
+
+// Camera (src/camera.rs:61-91) + the integrator constants.
+struct DevFrame {
+  float eye[3], forward[3], side[3], up[3];
+  float half_w, half_h;
+  uint32_t projection;
+  uint32_t width, height;
+  uint32_t max_bounces, rr_start, seed, strategy;
+  float clamp, ambient;
+  uint32_t nslots;  // NEE slots per path: punctual lights + 1 area
+};
+
+// Path flags (PathState meta.z >> 16).
+constexpr uint32_t PF_UNCOVERED = 1u;
+constexpr uint32_t PF_PENDING = 2u;
+constexpr uint32_t PF_DONE = 4u;
+
+constexpr uint32_t HIT_MISS = 0xffffffffu;
+constexpr uint32_t HIT_BACK = 0x80000000u;
+constexpr uint32_t NO_SKIP = 0xffffffffu;
+
+// Device control block: queue counters, work counters, error, stats.
+struct Ctrl {
+  uint32_t qcount[2];
+  uint32_t scount;
+  uint32_t error;
+  uint32_t work[4];
+  unsigned long long rays_closest;
+  unsigned long long rays_shadow;
+};
+
+constexpr uint32_t ERR_STACK = 1u;
+
+}  // namespace oxr

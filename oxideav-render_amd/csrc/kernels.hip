@@ -1,0 +1,1088 @@
+// HIP kernels of the oxr wavefront path tracer (gfx950).
+//
+// One refine batch = P paths (NPX pixels x S samples). Per bounce:
+//   k_prep     reset the next queue / shadow queue / work counters
+//   k_closest  persistent closest-hit traversal of the ray queue
+//   k_shade    hit -> emission, material, BSDF, NEE shadow rays, BSDF
+//              sample, roulette; appends next rays and shadow rays
+//   k_anyhit   persistent occlusion traversal of the shadow queue
+// then k_finalize (pending NEE, per-sample radiance) and k_reduce
+// (per-pixel f64 accumulation in sample order).
+//
+// The estimator is oxideav-render's (src/pathtrace.rs:2004-2329,
+// normative spec :1-277); each device function cites its counterpart.
+#include <hip/hip_runtime.h>
+
+#include "bvh.h"
+#include "dev_bsdf.h"
+#include "dev_math.h"
+#include "dev_scene.h"
+#include "runtime.h"
+
+namespace oxr {
+
+__constant__ uint32_t c_sobol[4][32];
+__constant__ float c_sheen[16 * 32];
+__constant__ float c_srgb_th[255];
+
+constexpr int TRACE_TB = 128;
+constexpr int STACK = 32;
+constexpr float SLAB_EPS = 1.0f + 1.0f / 65536.0f;
+
+OXR_D uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append: one atomic per wave; returns this lane's slot.
+OXR_D uint32_t wave_append(uint32_t* counter, bool pred) {
+  uint64_t active = __ballot(1);
+  uint64_t m = __ballot(pred);
+  uint32_t total = (uint32_t)__popcll(m);
+  int leader = __ffsll((unsigned long long)active) - 1;
+  uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if ((int)lane == leader && total) base = atomicAdd(counter, total);
+  base = __shfl(base, leader);
+  uint64_t below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+  return base + (uint32_t)__popcll(below);
+}
+
+// ---------------------------------------------------------------------
+// BVH traversal (TraceScene::closest_hit_filtered / occluded_filtered,
+// src/trace.rs:405-450), 4-wide nodes, per-thread stack: a 32-entry ring
+// in LDS with overflow spilled to a per-thread global buffer whose size
+// the host derives from the tree depth.
+// ---------------------------------------------------------------------
+struct Stack {
+  uint32_t* lds;    // &s[0][tid]; entry i at lds[i * TRACE_TB]
+  uint32_t* spill;  // per-thread overflow
+  uint32_t cap;
+  uint32_t sp, bottom, spn;
+  bool overflow;
+
+  OXR_D void push(uint32_t x) {
+    if (sp == STACK) {
+      if (spn >= cap) {
+        overflow = true;
+        return;
+      }
+      spill[spn++] = lds[(bottom & (STACK - 1)) * TRACE_TB];
+      bottom = (bottom + 1) & (STACK - 1);
+      --sp;
+    }
+    lds[((bottom + sp) & (STACK - 1)) * TRACE_TB] = x;
+    ++sp;
+  }
+  OXR_D bool pop(uint32_t& x) {
+    if (sp > 0) {
+      --sp;
+      x = lds[((bottom + sp) & (STACK - 1)) * TRACE_TB];
+      return true;
+    }
+    if (spn > 0) {
+      x = spill[--spn];
+      return true;
+    }
+    return false;
+  }
+};
+
+// Acceptance of a candidate (Integrator::accept, src/pathtrace.rs:1924-1934)
+// from the per-triangle flags: primary back-face culling of single-sided
+// materials, MASK cutoff, stochastic BLEND coin.
+OXR_D bool accept_tri(uint32_t flags, float alpha, bool front, bool cull, uint32_t path_seed, uint32_t ray_id,
+                      uint32_t gid) {
+  if (cull && !front && !(flags & TRI_DOUBLE_SIDED)) return false;
+  if (flags & TRI_MASK_REJECT) return false;
+  if (flags & TRI_BLEND) return coin(path_seed, ray_id, gid) < alpha;
+  return true;
+}
+
+template <bool ANY>
+OXR_D bool traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, bool cull, uint32_t skip,
+                    uint32_t path_seed, uint32_t ray_id, Stack& st, float& t_out, uint32_t& prim_out, float& u_out,
+                    float& v_out) {
+  V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  V3 oi = mul(o, inv);
+  bool negx = signbit(inv.x), negy = signbit(inv.y), negz = signbit(inv.z);
+  WRay wr = prepare_wray(o, d);
+  uint32_t best = HIT_MISS;
+  float bu = 0.0f, bv = 0.0f;
+  uint32_t node = 0;
+  st.sp = 0;
+  st.bottom = 0;
+  st.spn = 0;
+  st.overflow = false;
+  for (;;) {
+    const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
+    float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+    uint4 cn = reinterpret_cast<const uint4*>(np)[7];
+    float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+    float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    uint32_t chl[4] = {ch.x, ch.y, ch.z, ch.w};
+    uint32_t cnt[4] = {cn.x, cn.y, cn.z, cn.w};
+    float tn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float ax = fmaf(lxa[c], inv.x, -oi.x), bx = fmaf(hxa[c], inv.x, -oi.x);
+      float ay = fmaf(lya[c], inv.y, -oi.y), by = fmaf(hya[c], inv.y, -oi.y);
+      float az = fmaf(lza[c], inv.z, -oi.z), bz = fmaf(hza[c], inv.z, -oi.z);
+      float nx = negx ? bx : ax, fx = negx ? ax : bx;
+      float ny = negy ? by : ay, fy = negy ? ay : by;
+      float nz = negz ? bz : az, fz = negz ? az : bz;
+      float tnear = fmaxf(fmaxf(fmaxf(tmin, nx), ny), nz);
+      float tfar = fminf(fminf(fminf(tmax, fx), fy), fz);
+      bool hit = chl[c] != NODE_EMPTY && tnear <= tfar * SLAB_EPS;
+      tn[c] = hit ? tnear : __builtin_huge_valf();
+    }
+    // Leaves first: test their triangles right away.
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (tn[c] != __builtin_huge_valf() && cnt[c] > 0) {
+        const float4* tp = reinterpret_cast<const float4*>(sc.tris + chl[c]);
+        for (uint32_t k = 0; k < cnt[c]; ++k) {
+          float4 a = tp[3 * k], b = tp[3 * k + 1], e = tp[3 * k + 2];
+          float t, b1, b2;
+          bool front;
+          if (!intersect_tri(wr, f4xyz(a), f4xyz(b), f4xyz(e), tmin, tmax, t, b1, b2, front)) continue;
+          uint32_t gid = __float_as_uint(a.w);
+          uint32_t fl = __float_as_uint(b.w);
+          if (ANY) {
+            if (gid == skip) continue;
+            if (!accept_tri(fl, e.w, front, false, path_seed, ray_id, gid)) continue;
+            prim_out = gid;
+            return true;
+          } else {
+            if (best != HIT_MISS && !(t < tmax || (t == tmax && gid < (best & 0x7fffffffu)))) continue;
+            if (!accept_tri(fl, e.w, front, cull, path_seed, ray_id, gid)) continue;
+            tmax = t;
+            best = gid | (front ? 0u : HIT_BACK);
+            bu = b1;
+            bv = b2;
+          }
+        }
+        tn[c] = __builtin_huge_valf();
+      }
+    }
+    if (!ANY) {
+      // Sort inner children far -> near (push the nearest last).
+      uint32_t ci[4] = {chl[0], chl[1], chl[2], chl[3]};
+#define OXR_CSWAP(i, j)                     \
+  if (tn[i] < tn[j]) {                      \
+    float tt = tn[i];                       \
+    tn[i] = tn[j];                          \
+    tn[j] = tt;                             \
+    uint32_t cc = ci[i];                    \
+    ci[i] = ci[j];                          \
+    ci[j] = cc;                             \
+  }
+      OXR_CSWAP(0, 1) OXR_CSWAP(2, 3) OXR_CSWAP(0, 2) OXR_CSWAP(1, 3) OXR_CSWAP(1, 2)
+#undef OXR_CSWAP
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (tn[c] != __builtin_huge_valf() && tn[c] <= tmax * SLAB_EPS) st.push(ci[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (tn[c] != __builtin_huge_valf()) st.push(chl[c]);
+    }
+    if (st.overflow) break;
+    if (!st.pop(node)) break;
+  }
+  if (ANY) return false;
+  t_out = tmax;
+  prim_out = best;
+  u_out = bu;
+  v_out = bv;
+  return best != HIT_MISS;
+}
+
+// Persistent closest-hit kernel over a ray queue. PATH: rays carry the
+// bounce index k in the t_min slot (t_min = 0); else t_min as given.
+template <bool PATH>
+__global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const float4* __restrict__ rays,
+                                                     const uint32_t* __restrict__ qpid,
+                                                     const uint4* __restrict__ meta, uint32_t fixed_count,
+                                                     const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
+                                                     uint4* __restrict__ hits, uint32_t* spill, uint32_t spill_cap,
+                                                     uint32_t* err) {
+  __shared__ uint32_t s_stack[STACK * TRACE_TB];
+  Stack st;
+  st.lds = s_stack + threadIdx.x;
+  st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
+  st.cap = spill_cap;
+  const uint32_t count = count_ptr ? *count_ptr : fixed_count;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    uint32_t i = base + lane_id();
+    if (i < count) {
+      float4 a = rays[2 * i], b = rays[2 * i + 1];
+      V3 o = f4xyz(a), d = f4xyz(b);
+      float tmin;
+      bool cull;
+      uint32_t path_seed = 0, ray_id = 0;
+      if (PATH) {
+        uint32_t k = __float_as_uint(a.w);
+        tmin = 0.0f;
+        cull = k == 0;
+        ray_id = k << 16;
+        if (sc.has_blend) {
+          uint4 m = meta[qpid[i]];
+          path_seed = hash_mix(m.w, m.y);
+        }
+      } else {
+        tmin = a.w;
+        cull = cull_all != 0;
+      }
+      float t = 0.0f, u = 0.0f, v = 0.0f;
+      uint32_t prim = HIT_MISS;
+      traverse<false>(sc, o, d, tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st, t, prim, u, v);
+      if (st.overflow) {
+        atomicOr(err, ERR_STACK);
+        prim = HIT_MISS;
+      }
+      hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(t), prim, __float_as_uint(u), __float_as_uint(v));
+    }
+  }
+}
+
+// Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
+// sidx[i] = path * nslots + slot; writes vis[slot] = 2 when unoccluded.
+// Else: record = oxr_ray, writes occ[i] (1 = occluded).
+template <bool PATH>
+__global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const float4* __restrict__ rays,
+                                                    const uint32_t* __restrict__ sidx, const uint4* __restrict__ meta,
+                                                    uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr,
+                                                    uint32_t* work, uint8_t* __restrict__ out, uint32_t* spill,
+                                                    uint32_t spill_cap, uint32_t* err) {
+  __shared__ uint32_t s_stack[STACK * TRACE_TB];
+  Stack st;
+  st.lds = s_stack + threadIdx.x;
+  st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
+  st.cap = spill_cap;
+  const uint32_t count = count_ptr ? *count_ptr : fixed_count;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    uint32_t i = base + lane_id();
+    if (i < count) {
+      float4 a = rays[2 * i], b = rays[2 * i + 1];
+      float tmin = 0.0f;
+      uint32_t skip = NO_SKIP, path_seed = 0, ray_id = 0, slot = 0;
+      if (PATH) {
+        skip = __float_as_uint(a.w);
+        slot = sidx[i];
+        if (sc.has_blend) {
+          uint32_t pid = slot / nslots, j = slot - pid * nslots;
+          uint4 m = meta[pid];
+          uint32_t k = (m.z & 0xffffu);
+          path_seed = hash_mix(m.w, m.y);
+          ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (j & 0x7fffu)));
+        }
+      } else {
+        tmin = a.w;
+      }
+      float t, u, v;
+      uint32_t prim;
+      bool occ = traverse<true>(sc, f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st, t, prim, u, v);
+      if (st.overflow) {
+        atomicOr(err, ERR_STACK);
+        occ = true;
+      }
+      if (PATH) {
+        if (!occ) out[slot] = 2;
+      } else {
+        out[i] = occ ? 1 : 0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Camera rays (Sampler::new + pattern 0 + Camera::primary_ray,
+// src/pathtrace.rs:2005-2012, src/camera.rs:489-514).
+// ---------------------------------------------------------------------
+__global__ void k_camera(DevFrame fr, uint32_t P, uint32_t npx, uint32_t px0, uint32_t first_sample,
+                         float4* __restrict__ qray,
+                         uint32_t* __restrict__ qpid, float4* __restrict__ beta, float4* __restrict__ rad,
+                         float4* __restrict__ sigma, uint4* __restrict__ meta, Ctrl* ctrl) {
+  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid == 0) ctrl->qcount[0] = P;
+  if (pid >= P) return;
+  uint32_t s = pid / npx, px = px0 + (pid - s * npx);
+  uint32_t x = px % fr.width, y = px / fr.width;
+  uint32_t index = first_sample + s;
+  uint32_t ps = pixel_seed_of(fr.seed, x, y);
+  F4 p0 = pattern(ps, index, 0);
+  float fx = (float)x + p0.v[0], fy = (float)y + p0.v[1];
+  float ndc_x = (fx / fmaxr((float)fr.width, 1.0f)) * 2.0f - 1.0f;
+  float ndc_y = 1.0f - (fy / fmaxr((float)fr.height, 1.0f)) * 2.0f;
+  V3 side = ld3(fr.side), up = ld3(fr.up), fwd = ld3(fr.forward), eye = ld3(fr.eye);
+  V3 o, d;
+  if (fr.projection == 1u) {
+    o = add(eye, add(scale(side, ndc_x * fr.half_w), scale(up, ndc_y * fr.half_h)));
+    d = fwd;
+  } else {
+    d = normalise(add(add(scale(side, ndc_x * fr.half_w), scale(up, ndc_y * fr.half_h)), fwd));
+    o = eye;
+  }
+  qray[2 * pid] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
+  qray[2 * pid + 1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
+  qpid[pid] = pid;
+  beta[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  sigma[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  meta[pid] = make_uint4(x | (y << 16), index, 0xffu << 16, ps);
+}
+
+__global__ void k_prep(Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
+  if (threadIdx.x == 0) {
+    ctrl->rays_closest += ctrl->qcount[cur];
+    if (count_shadow_prev) ctrl->rays_shadow += ctrl->scount;
+    ctrl->qcount[cur ^ 1u] = 0;
+    ctrl->scount = 0;
+    ctrl->work[0] = 0;
+    ctrl->work[1] = 0;
+    ctrl->work[2] = 0;
+    ctrl->work[3] = 0;
+  }
+}
+
+// Adds the visible NEE contributions of the previous vertex in light
+// order (punctual lights, then the area sample), as the reference adds
+// them to `radiance` during that vertex.
+OXR_D void add_pending(uint32_t pid, uint32_t nslots, const float4* __restrict__ nee, uint8_t* __restrict__ vis,
+                       V3& radiance) {
+  for (uint32_t j = 0; j < nslots; ++j) {
+    size_t s = (size_t)pid * nslots + j;
+    uint8_t v = vis[s];
+    if (v == 2) radiance = add(radiance, f4xyz(nee[s]));
+    if (v) vis[s] = 0;
+  }
+}
+
+OXR_D V3 clampc(V3 c, float cl) {
+  float m = max3(c);
+  if (cl > 0.0f && m > cl) return scale(c, cl / m);
+  return c;
+}
+
+// Light-sampling pdf of emissive triangle g (Integrator::tri_pdf,
+// src/pathtrace.rs:1972-1991); P_env = 0 (no environment map).
+OXR_D float tri_pdf(const DevScene& sc, uint32_t li, uint32_t g, V3 p, V3 q) {
+  const float* tv = sc.pos + 9ull * g;
+  V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
+  float sel = (1.0f - 0.0f) * sc.l_pmf[li];
+  float omega = triangle_solid_angle(p, a, b, c);
+  if (omega >= 1.0e-4f) return sel / omega;
+  V3 cr = cross(sub(b, a), sub(c, a));
+  float len = sqrtf(dot(cr, cr));
+  V3 to = sub(q, p);
+  float d2 = dot(to, to);
+  if (len <= 0.0f || d2 <= 0.0f) return 0.0f;
+  float cos = fabsf(dot(cr, to) / (len * sqrtf(d2)));
+  if (cos <= 0.0f) return 0.0f;
+  return sel * d2 / (0.5f * len * cos);
+}
+
+OXR_D void load_mat(const DevMaterial& m, V3 sn, MatSample& ms) {
+  ms.base = v3(m.base[0], m.base[1], m.base[2]);
+  ms.metallic = m.metallic;
+  ms.roughness = m.roughness;
+  ms.normal = sn;
+  ms.ior = m.ior;
+  ms.specular = m.specular;
+  ms.specular_color = ld3(m.spec_color);
+  ms.transmission = m.transmission;
+  ms.thickness = m.thickness;
+  ms.clearcoat = m.clearcoat;
+  ms.clearcoat_roughness = m.cc_rough;
+  ms.clearcoat_normal = sn;
+  ms.sheen_color = ld3(m.sheen);
+  ms.sheen_roughness = m.sheen_rough;
+  ms.anisotropy = 0.0f;
+  ms.anisotropy_direction = v3(0.0f, 0.0f, 0.0f);
+  ms.iridescence = m.iridescence;
+  ms.iridescence_ior = m.irid_ior;
+  ms.iridescence_thickness = m.irid_thick;
+  ms.diffuse_transmission = m.dt;
+  ms.diffuse_transmission_color = ld3(m.dt_color);
+}
+
+OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_origin(p, ng) : offset_ray_origin(p, neg(ng)); }
+
+// ---------------------------------------------------------------------
+// Shading: one path vertex of Integrator::sample (src/pathtrace.rs:
+// 2027-2322), with shadow rays deferred to k_anyhit.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const float4* __restrict__ qray,
+                                             const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
+                                             float4* __restrict__ nray, uint32_t* __restrict__ npid,
+                                             float4* __restrict__ sray, uint32_t* __restrict__ sidx,
+                                             float4* __restrict__ beta_s, float4* __restrict__ rad_s,
+                                             float4* __restrict__ prev_s, float4* __restrict__ sigma_s,
+                                             uint4* __restrict__ meta_s, float4* __restrict__ nee,
+                                             uint8_t* __restrict__ vis, Ctrl* ctrl, uint32_t cur) {
+  const uint32_t count = ctrl->qcount[cur];
+  uint32_t* ncount = &ctrl->qcount[cur ^ 1u];
+  uint32_t* scount = &ctrl->scount;
+  const uint32_t nslots = fr.nslots;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  // Uniform trip count so every lane reaches the wave-level appends.
+  const uint32_t rounds = (count + stride - 1) / stride;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    uint32_t i = i0 + r * stride;
+    bool valid = i < count;
+    uint32_t pid = valid ? qpid[i] : 0u;
+    bool alive = valid;
+    uint4 meta = valid ? meta_s[pid] : make_uint4(0, 0, 0, 0);
+    uint32_t k = meta.z & 0xffffu;
+    uint32_t hero = (meta.z >> 16) & 0xffu;
+    uint32_t flags = meta.z >> 24;
+    uint32_t ps = meta.w, index = meta.y;
+    float4 b4 = valid ? beta_s[pid] : make_float4(0, 0, 0, 0);
+    V3 beta = f4xyz(b4);
+    float prev_pdf = b4.w;
+    V3 radiance = valid ? f4xyz(rad_s[pid]) : v3(0, 0, 0);
+    float4 sg = valid ? sigma_s[pid] : make_float4(0, 0, 0, 0);
+    V3 prev_pos = valid ? f4xyz(prev_s[pid]) : v3(0, 0, 0);
+    if (valid && (flags & PF_PENDING)) {
+      add_pending(pid, nslots, nee, vis, radiance);
+      flags &= ~PF_PENDING;
+    }
+    uint4 h = valid ? hits[i] : make_uint4(0, HIT_MISS, 0, 0);
+    float4 rd = valid ? qray[2 * i + 1] : make_float4(0, 0, 1, 0);
+    V3 d = f4xyz(rd);
+    bool miss = h.y == HIT_MISS;
+    float t = __uint_as_float(h.x);
+    // Beer-Lambert inside a medium; an escaping ray ends the path.
+    if (alive && sg.w != 0.0f) {
+      if (miss) {
+        alive = false;
+      } else {
+        beta.x *= expf(-sg.x * t);
+        beta.y *= expf(-sg.y * t);
+        beta.z *= expf(-sg.z * t);
+      }
+    }
+    if (alive && miss) {
+      if (k == 0) {
+        flags |= PF_UNCOVERED;
+      } else {
+        V3 le = v3(fr.ambient, fr.ambient, fr.ambient);
+        radiance = add(radiance, clampc(scale(mul(beta, le), 1.0f), fr.clamp));
+      }
+      alive = false;
+    }
+    // Surface + emission.
+    uint32_t g = h.y & 0x7fffffffu;
+    bool front = !(h.y & HIT_BACK);
+    V3 p = v3(0, 0, 0), ngeo = v3(0, 0, 1), sn = v3(0, 0, 1);
+    uint32_t mid = 0;
+    if (alive) {
+      float u = __uint_as_float(h.z), v = __uint_as_float(h.w);
+      V3 bary = v3(1.0f - u - v, u, v);
+      const float* tv = sc.pos + 9ull * g;
+      V3 p0 = ld3(tv), p1 = ld3(tv + 3), p2 = ld3(tv + 6);
+      p = v3(p0.x * bary.x + p1.x * bary.y + p2.x * bary.z, p0.y * bary.x + p1.y * bary.y + p2.y * bary.z,
+             p0.z * bary.x + p1.z * bary.y + p2.z * bary.z);
+      V3 cr = cross(sub(p1, p0), sub(p2, p0));
+      float len = sqrtf(dot(cr, cr));
+      ngeo = len > 0.0f ? v3(cr.x / len, cr.y / len, cr.z / len) : v3(0.0f, 0.0f, 1.0f);
+      uint32_t info = sc.tri_info[g];
+      mid = info & TI_MAT_MASK;
+      sn = ngeo;
+      if (info & TI_HAS_NORMALS) {
+        const float* nv = sc.nrm + 9ull * g;
+        V3 n0 = ld3(nv), n1 = ld3(nv + 3), n2 = ld3(nv + 6);
+        V3 n = normalise(v3(n0.x * bary.x + n1.x * bary.y + n2.x * bary.z, n0.y * bary.x + n1.y * bary.y + n2.y * bary.z,
+                            n0.z * bary.x + n1.z * bary.y + n2.z * bary.z));
+        sn = dot(n, n) > 0.5f ? n : ngeo;
+      }
+    }
+    const DevMaterial* M = sc.mats + mid;
+    if (alive) {
+      V3 em = ld3(M->emissive);
+      bool ds = (M->flags & MAT_DS) != 0;
+      bool unlit = (M->flags & MAT_UNLIT) != 0;
+      if ((front || ds) && !unlit && !is_black(em)) {
+        if (k == 0) {
+          radiance = add(radiance, mul(beta, em));
+        } else {
+          uint32_t li = sc.l_lookup[g];
+          float pl = li == 0xffffffffu ? 0.0f : tri_pdf(sc, li, g, prev_pos, p);
+          float w;
+          if (pl <= 0.0f)
+            w = 1.0f;
+          else if (fr.strategy == 0u)
+            w = power_heuristic(prev_pdf, pl);
+          else if (fr.strategy == 1u)
+            w = 0.0f;
+          else
+            w = 1.0f;
+          if (w > 0.0f) radiance = add(radiance, clampc(scale(mul(beta, em), w), fr.clamp));
+        }
+      }
+      if (unlit) {
+        V3 c = v3(M->base[0], M->base[1], M->base[2]);
+        V3 contrib = mul(beta, c);
+        radiance = add(radiance, k == 0 ? contrib : clampc(contrib, fr.clamp));
+        alive = false;
+      }
+    }
+    if (alive && k >= fr.max_bounces) alive = false;
+    MatSample ms;
+    load_mat(*M, sn, ms);
+    float thickness = M->thickness, transmission = M->transmission;
+    if (alive && M->dispersion > 0.0f && thickness > 0.0f && transmission > 0.0f) {
+      uint32_t c;
+      if (hero != 0xffu) {
+        c = hero;
+      } else {
+        uint32_t path_seed = hash_mix(ps, index);
+        float cf = coin(path_seed, (k << 16) | 0x7fffu, 0xffffffffu) * 3.0f;
+        c = cf > 0.0f ? (uint32_t)cf : 0u;
+        c = c > 2u ? 2u : c;
+        beta = v3(c == 0 ? beta.x * 3.0f : 0.0f, c == 1 ? beta.y * 3.0f : 0.0f, c == 2 ? beta.z * 3.0f : 0.0f);
+        hero = c;
+      }
+      ms.ior = dispersion_ior(ms.ior, M->dispersion, (int)c);
+    }
+    Bsdf bsdf;
+    bsdf.init(ms, neg(d), ngeo, front);
+    if (alive && bsdf.black()) alive = false;
+    V3 ng = bsdf.ng;
+    // NEE: punctual lights, one shadow-ray slot each.
+    bool any_shadow = false;
+    for (uint32_t j = 0; j < sc.num_lights; ++j) {
+      bool emit = false;
+      V3 o2, l, contrib;
+      float tmax = 0.0f;
+      if (alive) {
+        const DevLight& L = sc.lights[j];
+        V3 lrad;
+        bool ok = true;
+        if (L.kind == 0u) {
+          l = neg(ld3(L.dir));
+          tmax = __builtin_huge_valf();
+          lrad = ld3(L.base);
+        } else {
+          V3 to = sub(ld3(L.pos), p);
+          float d2 = dot(to, to);
+          ok = !(isnan(d2) || d2 <= 1.0e-12f);
+          float dist = sqrtf(d2);
+          l = v3(to.x / dist, to.y / dist, to.z / dist);
+          float att = 1.0f / d2;
+          if (L.has_range) {
+            float x = dist / L.range;
+            att *= clampr(1.0f - x * x * x * x, 0.0f, 1.0f);
+          }
+          if (L.kind == 2u) {
+            float cd = dot(ld3(L.dir), neg(l));
+            float a = clampr(cd * L.spot_scale + L.spot_offset, 0.0f, 1.0f);
+            att *= a * a;
+          }
+          ok = ok && att > 0.0f;
+          lrad = v3(L.base[0] * att, L.base[1] * att, L.base[2] * att);
+          tmax = isfinite(dist) ? dist * (1.0f - 1.0e-4f) : __builtin_huge_valf();
+        }
+        if (ok) {
+          V3 f;
+          float pdf_unused;
+          bsdf.eval(l, f, pdf_unused);
+          if (!is_black(f)) {
+            o2 = spawn(p, ng, l);
+            contrib = clampc(mul(mul(beta, f), lrad), fr.clamp);
+            emit = true;
+          }
+        }
+      }
+      uint32_t pos = wave_append(scount, emit);
+      if (emit) {
+        size_t slot = (size_t)pid * nslots + j;
+        sray[2 * pos] = make_float4(o2.x, o2.y, o2.z, __uint_as_float(NO_SKIP));
+        sray[2 * pos + 1] = make_float4(l.x, l.y, l.z, tmax);
+        sidx[pos] = (uint32_t)slot;
+        nee[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+        vis[slot] = 1;
+        any_shadow = true;
+      }
+    }
+    // NEE: one sample of the emissive triangles (src/pathtrace.rs:2173-2279).
+    {
+      bool emit = false;
+      V3 o2, l, contrib;
+      float tmax = 0.0f;
+      uint32_t skip = NO_SKIP;
+      if (alive && fr.strategy != 2u && sc.num_emissive > 0) {
+        F4 pl_u = pattern(ps, index, 2 + 2 * k);
+        float us = pl_u.v[2];
+        // EmissiveLights::pick: partition_point(c <= u).min(len - 1).
+        uint32_t lo = 0, hi = sc.num_emissive;
+        while (lo < hi) {
+          uint32_t mid2 = (lo + hi) >> 1;
+          if (sc.l_cdf[mid2] <= us)
+            lo = mid2 + 1;
+          else
+            hi = mid2;
+        }
+        uint32_t li = lo < sc.num_emissive - 1 ? lo : sc.num_emissive - 1;
+        uint32_t lg = sc.l_tris[li];
+        const float* tv = sc.pos + 9ull * lg;
+        V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
+        bool have = false;
+        V3 q;
+        if (triangle_solid_angle(p, a, b, c) >= 1.0e-4f) {
+          V3 ls;
+          if (sample_spherical_triangle(p, a, b, c, pl_u.v[0], pl_u.v[1], ls)) {
+            V3 n = cross(sub(b, a), sub(c, a));
+            float dn = dot(ls, n);
+            if (dn != 0.0f) {
+              float tt = dot(sub(a, p), n) / dn;
+              if (tt > 0.0f && isfinite(tt)) {
+                q = add(p, scale(ls, tt));
+                have = true;
+              }
+            }
+          }
+        } else {
+          float su = sqrtf(pl_u.v[0]);
+          float w0 = 1.0f - su, w1 = su * (1.0f - pl_u.v[1]), w2 = su * pl_u.v[1];
+          q = v3(a.x * w0 + b.x * w1 + c.x * w2, a.y * w0 + b.y * w1 + c.y * w2, a.z * w0 + b.z * w1 + c.z * w2);
+          have = true;
+        }
+        V3 bary;
+        if (have && barycentric_of(q, a, b, c, bary)) {
+          V3 to = sub(q, p);
+          float dist2 = dot(to, to);
+          V3 cr = cross(sub(b, a), sub(c, a));
+          float len = sqrtf(dot(cr, cr));
+          V3 lng = len > 0.0f ? v3(cr.x / len, cr.y / len, cr.z / len) : v3(0.0f, 0.0f, 1.0f);
+          const DevMaterial& LM = sc.mats[sc.tri_info[lg] & TI_MAT_MASK];
+          if (dist2 > 1.0e-12f) {
+            float dist = sqrtf(dist2);
+            l = scale(to, 1.0f / dist);
+            float cos_l = -dot(lng, l);
+            float pl = tri_pdf(sc, li, lg, p, q);
+            bool lds = (LM.flags & MAT_DS) != 0;
+            if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && pl > 0.0f && isfinite(pl)) {
+              V3 f;
+              float pb;
+              bsdf.eval(l, f, pb);
+              V3 le = (LM.flags & MAT_UNLIT) ? v3(0, 0, 0) : ld3(LM.emissive);
+              if (!is_black(f) && !is_black(le)) {
+                float w = fr.strategy == 0u ? power_heuristic(pl, pb) : 1.0f;
+                o2 = spawn(p, ng, l);
+                tmax = fmaxr(dist * (1.0f - 1.0e-4f), 0.0f);
+                contrib = clampc(scale(mul(mul(beta, f), le), w / pl), fr.clamp);
+                skip = lg;
+                emit = true;
+              }
+            }
+          }
+        }
+      }
+      uint32_t pos = wave_append(scount, emit);
+      if (emit) {
+        size_t slot = (size_t)pid * nslots + (nslots - 1);
+        sray[2 * pos] = make_float4(o2.x, o2.y, o2.z, __uint_as_float(skip));
+        sray[2 * pos + 1] = make_float4(l.x, l.y, l.z, tmax);
+        sidx[pos] = (uint32_t)slot;
+        nee[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+        vis[slot] = 1;
+        any_shadow = true;
+      }
+    }
+    if (any_shadow) flags |= PF_PENDING;
+    // BSDF sampling, medium transitions, roulette (src/pathtrace.rs:2281-2322).
+    bool cont = false;
+    V3 no, nl;
+    if (alive) {
+      F4 pb_u = pattern(ps, index, 1 + 2 * k);
+      V3 l, f;
+      float pdf;
+      if (bsdf.sample(pb_u.v[0], pb_u.v[1], pb_u.v[2], l, f, pdf)) {
+        beta = mul(beta, scale(f, 1.0f / pdf));
+        if (finite3(beta)) {
+          prev_pdf = pdf;
+          prev_pos = p;
+          if (dot(l, ng) < 0.0f && thickness > 0.0f && transmission > 0.0f) {
+            if (front) {
+              float dist = M->att_dist;
+              if (isfinite(dist)) {
+                sg = make_float4(-logf(fmaxr(M->att_color[0], 1.0e-6f)) / dist,
+                                 -logf(fmaxr(M->att_color[1], 1.0e-6f)) / dist,
+                                 -logf(fmaxr(M->att_color[2], 1.0e-6f)) / dist, 1.0f);
+              } else {
+                sg = make_float4(0, 0, 0, 0);
+              }
+            } else {
+              sg = make_float4(0, 0, 0, 0);
+            }
+          }
+          cont = true;
+          if (k + 1 >= fr.rr_start) {
+            float q = clampr(max3(beta), 0.05f, 1.0f);
+            if (q < 1.0f) {
+              if (pb_u.v[3] >= q)
+                cont = false;
+              else
+                beta = scale(beta, 1.0f / q);
+            }
+          }
+          if (cont) {
+            no = spawn(p, ng, l);
+            nl = l;
+          }
+        }
+      }
+    }
+    uint32_t pos = wave_append(ncount, cont);
+    if (cont) {
+      nray[2 * pos] = make_float4(no.x, no.y, no.z, __uint_as_float(k + 1));
+      nray[2 * pos + 1] = make_float4(nl.x, nl.y, nl.z, __builtin_huge_valf());
+      npid[pos] = pid;
+    }
+    if (valid) {
+      if (!cont) flags |= PF_DONE;
+      beta_s[pid] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
+      rad_s[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+      prev_s[pid] = make_float4(prev_pos.x, prev_pos.y, prev_pos.z, 0.0f);
+      sigma_s[pid] = sg;
+      meta_s[pid] = make_uint4(meta.x, meta.y, ((k + (cont ? 1u : 0u)) & 0xffffu) | (hero << 16) | (flags << 24), ps);
+    }
+  }
+}
+
+// Per-sample radiance: the pending NEE of the last vertex, then the
+// finite check of Integrator::sample (src/pathtrace.rs:2324-2328).
+__global__ void k_finalize(uint32_t P, uint32_t nslots, const float4* __restrict__ nee, uint8_t* __restrict__ vis,
+                           const float4* __restrict__ rad_s, const uint4* __restrict__ meta_s,
+                           float4* __restrict__ out) {
+  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= P) return;
+  uint4 meta = meta_s[pid];
+  uint32_t flags = meta.z >> 24;
+  V3 radiance = f4xyz(rad_s[pid]);
+  if (flags & PF_PENDING) add_pending(pid, nslots, nee, vis, radiance);
+  if (!finite3(radiance)) radiance = v3(0.0f, 0.0f, 0.0f);
+  out[pid] = make_float4(radiance.x, radiance.y, radiance.z, (flags & PF_UNCOVERED) ? 0.0f : 1.0f);
+}
+
+// Accumulate S samples per pixel in sample order into the f64
+// accumulator (src/pathtrace.rs:2569-2578) and the squared radiance.
+__global__ void k_reduce(uint32_t npx, uint32_t S, const float4* __restrict__ out, double* __restrict__ accum,
+                         double* __restrict__ sumsq) {
+  uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= npx) return;
+  double a0 = accum[4ull * px], a1 = accum[4ull * px + 1], a2 = accum[4ull * px + 2], a3 = accum[4ull * px + 3];
+  double q0 = sumsq[3ull * px], q1 = sumsq[3ull * px + 1], q2 = sumsq[3ull * px + 2];
+  for (uint32_t s = 0; s < S; ++s) {
+    float4 c = out[(size_t)s * npx + px];
+    if (c.w > 0.0f) {
+      double r = (double)c.x, g = (double)c.y, b = (double)c.z;
+      a0 += r;
+      a1 += g;
+      a2 += b;
+      a3 += 1.0;
+      q0 += r * r;
+      q1 += g * g;
+      q2 += b * b;
+    }
+  }
+  accum[4ull * px] = a0;
+  accum[4ull * px + 1] = a1;
+  accum[4ull * px + 2] = a2;
+  accum[4ull * px + 3] = a3;
+  sumsq[3ull * px] = q0;
+  sumsq[3ull * px + 1] = q1;
+  sumsq[3ull * px + 2] = q2;
+}
+
+__global__ void k_zero_ctrl(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    c->qcount[0] = c->qcount[1] = 0;
+    c->scount = 0;
+    c->error = 0;
+    c->work[0] = c->work[1] = c->work[2] = c->work[3] = 0;
+    c->rays_closest = 0;
+    c->rays_shadow = 0;
+  }
+}
+
+__global__ void k_fill_f64(double* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------
+// Resolve (PathTracer::hdr / image, src/pathtrace.rs:2593-2734;
+// ToneMap::apply, linear_to_srgb_byte, src/hdr.rs:44-129).
+// ---------------------------------------------------------------------
+struct DevDisplay {
+  float bg[4];
+  uint8_t bg_bytes[4];
+  uint32_t tone_map;
+  float exposure;
+};
+
+OXR_D float sanitize(float v) { return isnan(v) ? 0.0f : v; }
+OXR_D float aces(float x) {
+  x = x * 0.6f;
+  float v = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+  return clampr(v, 0.0f, 1.0f);
+}
+OXR_D V3 tone_map(uint32_t tm, V3 rgb) {
+  rgb = v3(fmaxr(sanitize(rgb.x), 0.0f), fmaxr(sanitize(rgb.y), 0.0f), fmaxr(sanitize(rgb.z), 0.0f));
+  if (tm == 1u) {
+    float l = 0.2126f * rgb.x + 0.7152f * rgb.y + 0.0722f * rgb.z;
+    if (l <= 0.0f) return v3(0, 0, 0);
+    float s = 1.0f / (1.0f + l);
+    return v3(fminr(rgb.x * s, 1.0f), fminr(rgb.y * s, 1.0f), fminr(rgb.z * s, 1.0f));
+  }
+  if (tm == 2u) return v3(aces(rgb.x), aces(rgb.y), aces(rgb.z));
+  return v3(fminr(rgb.x, 1.0f), fminr(rgb.y, 1.0f), fminr(rgb.z, 1.0f));
+}
+OXR_D uint8_t srgb_byte(float c) {
+  if (isnan(c)) return 0;
+  int lo = 0, hi = 255;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (c_srgb_th[mid] <= c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (uint8_t)lo;
+}
+
+OXR_D bool resolve_px(const double* a, uint32_t samples, const DevDisplay& dd, bool map, float o[4]) {
+  if (a[3] <= 0.0 || samples == 0) return false;
+  V3 m = v3((float)(a[0] / a[3]), (float)(a[1] / a[3]), (float)(a[2] / a[3]));
+  float f = (float)clampd(a[3] / (double)samples, 0.0, 1.0);
+  V3 c = map ? tone_map(dd.tone_map, scale(m, dd.exposure)) : m;
+  if (f >= 1.0f) {
+    o[0] = c.x;
+    o[1] = c.y;
+    o[2] = c.z;
+    o[3] = 1.0f;
+    return true;
+  }
+  float a_sum = f + (1.0f - f) * dd.bg[3];
+  float cc[3] = {c.x, c.y, c.z};
+  if (a_sum > 0.0f) {
+    for (int k = 0; k < 3; ++k) o[k] = (f * cc[k] + (1.0f - f) * dd.bg[k] * dd.bg[3]) / a_sum;
+    o[3] = a_sum;
+  } else {
+    for (int k = 0; k < 3; ++k) o[k] = f * cc[k] + (1.0f - f) * dd.bg[k];
+    o[3] = 0.0f;
+  }
+  return true;
+}
+
+__global__ void k_resolve_hdr(uint32_t npx, uint32_t samples, const double* __restrict__ accum, DevDisplay dd,
+                              float4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npx) return;
+  float o[4];
+  if (!resolve_px(accum + 4ull * i, samples, dd, false, o)) {
+    out[i] = make_float4(dd.bg[0], dd.bg[1], dd.bg[2], dd.bg[3]);
+    return;
+  }
+  out[i] = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__global__ void k_resolve_rgba8(uint32_t npx, uint32_t samples, const double* __restrict__ accum, DevDisplay dd,
+                                uchar4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npx) return;
+  float o[4];
+  if (!resolve_px(accum + 4ull * i, samples, dd, true, o)) {
+    out[i] = make_uchar4(dd.bg_bytes[0], dd.bg_bytes[1], dd.bg_bytes[2], dd.bg_bytes[3]);
+    return;
+  }
+  float a = clampr(o[3], 0.0f, 1.0f) * 255.0f;
+  out[i] = make_uchar4(srgb_byte(o[0]), srgb_byte(o[1]), srgb_byte(o[2]), (uint8_t)rintf(a));
+}
+
+// ---------------------------------------------------------------------
+// Parity kernels: Sobol' points and the BSDF.
+// ---------------------------------------------------------------------
+__global__ void k_sobol(uint32_t n, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ seed,
+                        float4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  F4 p = sobol_owen_4d(idx[i], seed[i]);
+  out[i] = make_float4(p.v[0], p.v[1], p.v[2], p.v[3]);
+}
+
+OXR_D void ms_from_abi(const float* m, MatSample& ms) {
+  // oxr_material_sample field order (include/oxr/pathtrace.h).
+  ms.base = ld3(m + 0);
+  ms.metallic = m[3];
+  ms.roughness = m[4];
+  ms.normal = ld3(m + 5);
+  ms.ior = m[8];
+  ms.specular = m[9];
+  ms.specular_color = ld3(m + 10);
+  ms.transmission = m[13];
+  ms.thickness = m[14];
+  ms.clearcoat = m[15];
+  ms.clearcoat_roughness = m[16];
+  ms.clearcoat_normal = ld3(m + 17);
+  ms.sheen_color = ld3(m + 20);
+  ms.sheen_roughness = m[23];
+  ms.anisotropy = m[24];
+  ms.anisotropy_direction = ld3(m + 25);
+  ms.iridescence = m[28];
+  ms.iridescence_ior = m[29];
+  ms.iridescence_thickness = m[30];
+  ms.diffuse_transmission = m[31];
+  ms.diffuse_transmission_color = ld3(m + 32);
+}
+
+__global__ void k_bsdf(uint32_t n, int do_sample, const float* __restrict__ ms_abi, const float* __restrict__ v,
+                       const float* __restrict__ ng, const uint8_t* __restrict__ front, const float* __restrict__ in,
+                       float* __restrict__ l_out, float* __restrict__ f_out, float* __restrict__ pdf_out,
+                       uint8_t* __restrict__ ok_out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  MatSample ms;
+  ms_from_abi(ms_abi + 35ull * i, ms);
+  Bsdf b;
+  b.init(ms, ld3(v + 3ull * i), ld3(ng + 3ull * i), front[i] != 0);
+  V3 f = v3(0, 0, 0), l = v3(0, 0, 0);
+  float pdf = 0.0f;
+  if (do_sample) {
+    bool ok = b.sample(in[3 * i], in[3 * i + 1], in[3 * i + 2], l, f, pdf);
+    if (!ok) {
+      l = v3(0, 0, 0);
+      f = v3(0, 0, 0);
+      pdf = 0.0f;
+    }
+    ok_out[i] = ok ? 1 : 0;
+    l_out[3 * i] = l.x;
+    l_out[3 * i + 1] = l.y;
+    l_out[3 * i + 2] = l.z;
+  } else {
+    b.eval(ld3(in + 3ull * i), f, pdf);
+  }
+  f_out[3 * i] = f.x;
+  f_out[3 * i + 1] = f.y;
+  f_out[3 * i + 2] = f.z;
+  pdf_out[i] = pdf;
+}
+
+// ---------------------------------------------------------------------
+// Host-side launchers (declared in runtime.h).
+// ---------------------------------------------------------------------
+namespace launch {
+
+hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, const float* srgb255) {
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sobol), sobol128, sizeof(uint32_t) * 128);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_sheen), sheen512, sizeof(float) * 512);
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_srgb_th), srgb255, sizeof(float) * 255);
+}
+
+int trace_blocks_per_cu(bool any) {
+  int nb = 0;
+  hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_anyhit<true>, TRACE_TB, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_closest<true>, TRACE_TB, 0);
+  if (e != hipSuccess || nb <= 0) nb = 4;
+  return nb;
+}
+
+int trace_block_threads() { return TRACE_TB; }
+int stack_lds_entries() { return STACK; }
+
+void closest(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* qpid,
+             const uint4* meta, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
+             uint4* hits, uint32_t* spill, uint32_t spill_cap, uint32_t* err) {
+  if (path)
+    hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, qpid, meta, fixed_count, count_ptr,
+                       work, cull_all, hits, spill, spill_cap, err);
+  else
+    hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, qpid, meta, fixed_count, count_ptr,
+                       work, cull_all, hits, spill, spill_cap, err);
+}
+
+void anyhit(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* sidx,
+            const uint4* meta, uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work,
+            uint8_t* out, uint32_t* spill, uint32_t spill_cap, uint32_t* err) {
+  if (path)
+    hipLaunchKernelGGL(k_anyhit<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, sidx, meta, nslots, fixed_count,
+                       count_ptr, work, out, spill, spill_cap, err);
+  else
+    hipLaunchKernelGGL(k_anyhit<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, sidx, meta, nslots, fixed_count,
+                       count_ptr, work, out, spill, spill_cap, err);
+}
+
+void camera(hipStream_t s, const DevFrame& fr, uint32_t P, uint32_t npx, uint32_t px0, uint32_t first,
+            const PathBuffers& pb, Ctrl* ctrl) {
+  hipLaunchKernelGGL(k_camera, dim3((P + 255) / 256), dim3(256), 0, s, fr, P, npx, px0, first, pb.qray[0], pb.qpid[0],
+                     pb.beta, pb.rad, pb.sigma, pb.meta, ctrl);
+}
+
+void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
+  hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, ctrl, cur, count_shadow_prev);
+}
+
+void shade(hipStream_t s, int grid, const DevScene& sc, const DevFrame& fr, const PathBuffers& pb, Ctrl* ctrl,
+           uint32_t cur) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, sc, fr, pb.qray[cur], pb.qpid[cur], pb.hits,
+                     pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, pb.prev, pb.sigma, pb.meta,
+                     pb.nee, pb.vis, ctrl, cur);
+}
+
+void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb) {
+  hipLaunchKernelGGL(k_finalize, dim3((P + 255) / 256), dim3(256), 0, s, P, nslots, pb.nee, pb.vis, pb.rad, pb.meta,
+                     pb.out);
+}
+
+void reduce(hipStream_t s, uint32_t npx, uint32_t S, const float4* out, double* accum, double* sumsq) {
+  hipLaunchKernelGGL(k_reduce, dim3((npx + 255) / 256), dim3(256), 0, s, npx, S, out, accum, sumsq);
+}
+
+void zero_ctrl(hipStream_t s, Ctrl* c) { hipLaunchKernelGGL(k_zero_ctrl, dim3(1), dim3(64), 0, s, c); }
+
+void fill_f64(hipStream_t s, double* p, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+void resolve(hipStream_t s, bool rgba8, uint32_t npx, uint32_t samples, const double* accum, const float bg[4],
+             const uint8_t bgb[4], uint32_t tone_map, float exposure, void* out) {
+  DevDisplay dd;
+  for (int k = 0; k < 4; ++k) {
+    dd.bg[k] = bg[k];
+    dd.bg_bytes[k] = bgb[k];
+  }
+  dd.tone_map = tone_map;
+  dd.exposure = exposure;
+  if (rgba8)
+    hipLaunchKernelGGL(k_resolve_rgba8, dim3((npx + 255) / 256), dim3(256), 0, s, npx, samples, accum, dd,
+                       (uchar4*)out);
+  else
+    hipLaunchKernelGGL(k_resolve_hdr, dim3((npx + 255) / 256), dim3(256), 0, s, npx, samples, accum, dd,
+                       (float4*)out);
+}
+
+void sobol(hipStream_t s, uint32_t n, const uint32_t* idx, const uint32_t* seed, float4* out) {
+  hipLaunchKernelGGL(k_sobol, dim3((n + 255) / 256), dim3(256), 0, s, n, idx, seed, out);
+}
+
+void bsdf(hipStream_t s, uint32_t n, int do_sample, const float* ms, const float* v, const float* ng,
+          const uint8_t* front, const float* in, float* l, float* f, float* pdf, uint8_t* ok) {
+  hipLaunchKernelGGL(k_bsdf, dim3((n + 127) / 128), dim3(128), 0, s, n, do_sample, ms, v, ng, front, in, l, f, pdf, ok);
+}
+
+}  // namespace launch
+}  // namespace oxr

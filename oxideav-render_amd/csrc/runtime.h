@@ -1,0 +1,58 @@
+// Internal interface between the host runtime (api.cpp) and the kernel
+// translation unit (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dev_scene.h"
+
+namespace oxr {
+
+// Wavefront buffers for one batch of paths (capacity `cap` paths).
+struct PathBuffers {
+  float4* qray[2] = {nullptr, nullptr};  // 2 float4 per ray: (o, aux), (d, t_max)
+  uint32_t* qpid[2] = {nullptr, nullptr};
+  uint4* hits = nullptr;
+  float4* sray = nullptr;  // shadow rays: (o, skip gid), (d, t_max)
+  uint32_t* sidx = nullptr;
+  float4* beta = nullptr;  // (beta, prev_pdf)
+  float4* rad = nullptr;
+  float4* prev = nullptr;
+  float4* sigma = nullptr;  // (sigma, in-medium flag)
+  uint4* meta = nullptr;    // (x | y << 16, sample, k | hero << 16 | flags << 24, pixel seed)
+  float4* nee = nullptr;    // cap * nslots contributions
+  uint8_t* vis = nullptr;   // cap * nslots: 0 none, 1 pending, 2 visible
+  float4* out = nullptr;    // per-sample radiance + covered flag
+  uint32_t cap = 0;
+  uint32_t nslots = 0;
+};
+
+namespace launch {
+hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, const float* srgb255);
+int trace_blocks_per_cu(bool any);
+int trace_block_threads();
+int stack_lds_entries();
+void closest(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* qpid,
+             const uint4* meta, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
+             uint4* hits, uint32_t* spill, uint32_t spill_cap, uint32_t* err);
+void anyhit(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* sidx,
+            const uint4* meta, uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work,
+            uint8_t* out, uint32_t* spill, uint32_t spill_cap, uint32_t* err);
+void camera(hipStream_t s, const DevFrame& fr, uint32_t P, uint32_t npx, uint32_t px0, uint32_t first,
+            const PathBuffers& pb, Ctrl* ctrl);
+void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev);
+void shade(hipStream_t s, int grid, const DevScene& sc, const DevFrame& fr, const PathBuffers& pb, Ctrl* ctrl,
+           uint32_t cur);
+void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb);
+void reduce(hipStream_t s, uint32_t npx, uint32_t S, const float4* out, double* accum, double* sumsq);
+void fill_f64(hipStream_t s, double* p, size_t n);
+void zero_ctrl(hipStream_t s, Ctrl* c);
+void resolve(hipStream_t s, bool rgba8, uint32_t npx, uint32_t samples, const double* accum, const float bg[4],
+             const uint8_t bgb[4], uint32_t tone_map, float exposure, void* out);
+void sobol(hipStream_t s, uint32_t n, const uint32_t* idx, const uint32_t* seed, float4* out);
+void bsdf(hipStream_t s, uint32_t n, int do_sample, const float* ms, const float* v, const float* ng,
+          const uint8_t* front, const float* in, float* l, float* f, float* pdf, uint8_t* ok);
+}  // namespace launch
+
+}  // namespace oxr
