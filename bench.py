@@ -71,7 +71,7 @@ def cpu_baseline(scene, cfg, ambient, target_s=12.0):
     t = time.perf_counter()
     os_.render(W, H, 0, 1, opts, y0=y0, y1=y0 + rows, threads=threads)
     dt1 = time.perf_counter() - t
-    spp = int(max(1, min(64, target_s / max(dt1, 1e-3))))
+    spp = int(max(1, min(512, target_s / max(dt1, 1e-3))))
     t = time.perf_counter()
     os_.render(W, H, 1, spp, opts, y0=y0, y1=y0 + rows, threads=threads)
     dt = time.perf_counter() - t
