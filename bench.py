@@ -81,6 +81,23 @@ def cpu_baseline(scene, cfg, ambient, target_s=12.0):
                       f"({paths} paths, {dt:.1f} s, band-parallel f64 accumulate)"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size (HIP's copy kernel), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,7 +107,11 @@ def main():
     ap.add_argument("--workload", default="material_sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
               "the path shards by sample range, see DESIGN.md", file=sys.stderr)
@@ -155,8 +176,13 @@ def main():
         avg_ms = tot["ms_closest"] / tot["launches_closest"]
         bytes_per_launch = bpr * tot["closest_rays"] / tot["launches_closest"]
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        prof = os.path.join(ROOT, "profiles", f"pathtrace_{a.workload}_{W}.json")
+        if os.path.exists(prof):
+            traffic = json.load(open(prof)).get("hbm_bytes_per_dispatch")
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                           "algorithmic_bytes_per_launch": bytes_per_launch,
                            "kernel": "k_closest<true>", "avg_launch_ms": avg_ms,
                            "bytes_per_ray": bpr, "nodes_per_ray": n_node, "tris_per_ray": n_tri,
                            "rays_sampled": nr}
