@@ -36,6 +36,8 @@ def workload(name):
         return scenes.cornell(), dict(width=256, height=256, spp=64, bounces=8)
     if name == "glossy_caustic":
         return scenes.glossy_caustic(), dict(width=1024, height=1024, spp=64, bounces=8)
+    if name == "terrain":
+        return scenes.terrain(), dict(width=1920, height=1080, spp=1024, bounces=8)
     raise SystemExit(f"unknown workload {name}")
 
 

@@ -252,6 +252,59 @@ typedef struct oxr_pt_stats {
 int oxr_pt_set_timing(oxr_pt* h, int enable);
 int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
 
+/* ---- serving: continuous batching of render jobs ---------------------
+ *
+ * No counterpart in the reference, which renders one scene per call. A
+ * server keeps scenes (BVH, tables) and job accumulators resident on one
+ * device. Jobs are admitted FIFO while their accumulator fits the memory
+ * budget. Each oxr_srv_step runs one wavefront round whose ray queues mix
+ * the paths of every active job in shared launches. A job's result is
+ * bit-identical to rendering it alone with oxr_pt_refine(spp): each
+ * sample depends only on (scene, options, pixel, sample index), and each
+ * pixel accumulates its samples in index order. */
+typedef struct oxr_srv oxr_srv;
+
+#define OXR_JOB_QUEUED 0u
+#define OXR_JOB_ACTIVE 1u
+#define OXR_JOB_DONE 2u
+
+typedef struct oxr_job_status {
+  uint32_t state;               /* OXR_JOB_*                                */
+  uint32_t samples;             /* samples accumulated so far               */
+  uint32_t target;              /* options.samples_per_pixel                */
+  uint32_t rounds;              /* rounds the job took part in              */
+  double t_submit_ms;           /* host clock (ms since server creation)    */
+  double t_admit_ms;
+  double t_done_ms;
+} oxr_job_status;
+
+typedef struct oxr_srv_stats {
+  uint64_t rounds;
+  uint64_t paths;
+  uint64_t jobs_done;
+  uint32_t active;
+  uint32_t queued;
+  uint64_t bytes_resident;      /* scenes + accumulators + wavefront buffers */
+  double busy_ms;               /* wall time inside oxr_srv_step            */
+} oxr_srv_stats;
+
+/* mem_budget: bytes the server may keep resident for accumulators
+ * (0 = 1/4 of the device). */
+int oxr_srv_create(int device, uint64_t mem_budget, oxr_srv** out);
+int oxr_srv_destroy(oxr_srv* s);
+int oxr_srv_add_scene(oxr_srv* s, const oxr_scene_desc* scene, uint32_t* scene_id);
+int oxr_srv_submit(oxr_srv* s, uint32_t scene_id, const oxr_camera* cam, const oxr_pt_options* opts,
+                   uint32_t* job_id);
+/* Admit what fits, then run one round over the active jobs (no-op when
+ * none). active/done (may be NULL): jobs active after the round, jobs that
+ * finished in it. */
+int oxr_srv_step(oxr_srv* s, uint32_t* active, uint32_t* done);
+int oxr_srv_job_status(oxr_srv* s, uint32_t job_id, oxr_job_status* out);
+int oxr_srv_read_accum(oxr_srv* s, uint32_t job_id, double* accum, double* sum_sq);
+/* Frees a finished (or queued) job's accumulator and slot. */
+int oxr_srv_release(oxr_srv* s, uint32_t job_id);
+int oxr_srv_stats_get(oxr_srv* s, oxr_srv_stats* out);
+
 /* ---- parity entry points --------------------------------------------- */
 
 /* out[4*i..4*i+4] = sobol_owen_4d(index[i], pattern_seed[i]) on device. */

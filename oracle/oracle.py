@@ -56,6 +56,19 @@ def lib():
         L.oxrcpu_sample_spherical_triangle.argtypes = [fp, fp, fp, fp, C.c_float, C.c_float, fp]
         L.oxrcpu_iridescent_fresnel.argtypes = [C.c_float, C.c_float, fp, C.c_float, C.c_float, fp]
         L.oxrcpu_offset_ray_origin.argtypes = [fp, fp, fp]
+        gc = C.POINTER(A.GuideConfig)
+        L.oxrcpu_guide_resolutions.argtypes = [gc, up, bp]
+        L.oxrcpu_bf16_round.restype = C.c_float
+        L.oxrcpu_bf16_round.argtypes = [C.c_float]
+        L.oxrcpu_guide_corners.argtypes = [gc, C.c_uint32, fp, up, fp]
+        L.oxrcpu_guide_forward.argtypes = [gc, fp, fp, C.c_uint32, fp, C.c_int, fp, fp]
+        L.oxrcpu_guide_lobes.argtypes = [C.c_uint32, fp, fp]
+        L.oxrcpu_guide_pdf.argtypes = [C.c_uint32, fp, fp, dp]
+        L.oxrcpu_guide_sample.argtypes = [C.c_uint32, fp, fp, fp]
+        L.oxrcpu_guide_loss_grad.restype = C.c_double
+        L.oxrcpu_guide_loss_grad.argtypes = [gc, fp, fp, C.c_uint32, fp, fp, fp, C.c_int, dp, dp]
+        L.oxrcpu_guide_adam.argtypes = [C.c_uint64, fp, fp, fp, fp, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_int]
         L.oxrcpu_barycentric_of.restype = C.c_int
         L.oxrcpu_barycentric_of.argtypes = [fp, fp, fp, fp, fp]
         L.oxrcpu_surface.argtypes = [C.c_void_p, C.c_uint32, fp, fp]
@@ -334,3 +347,88 @@ def bvh4_count(nodes, tris, rays):
     lib().oxrcpu_bvh4_count(nodes.ctypes.data, tris.ctypes.data, len(rays), rays.ctypes.data_as(fp),
                             counts.ctypes.data_as(up))
     return counts
+
+
+# ---------------------------------------------------------------------------
+# Learned path guide (guide_cpu.cpp).
+# ---------------------------------------------------------------------------
+GUIDE_NIN = A.GUIDE_LEVELS * A.GUIDE_FEATURES
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def guide_num_grid(cfg):
+    return A.GUIDE_LEVELS * (1 << cfg.log2_table) * A.GUIDE_FEATURES
+
+
+def guide_resolutions(cfg):
+    res = np.zeros(A.GUIDE_LEVELS, np.uint32)
+    dense = np.zeros(A.GUIDE_LEVELS, np.uint8)
+    lib().oxrcpu_guide_resolutions(C.byref(cfg), res.ctypes.data_as(up), dense.ctypes.data_as(bp))
+    return res, dense.astype(bool)
+
+
+def bf16_round(x):
+    x = _f32(x)
+    return np.array([lib().oxrcpu_bf16_round(float(v)) for v in x.ravel()], np.float32).reshape(x.shape)
+
+
+def guide_corners(cfg, pos):
+    pos = _f32(pos).reshape(-1, 3)
+    n = len(pos)
+    idx = np.zeros((n, A.GUIDE_LEVELS, 8), np.uint32)
+    w = np.zeros((n, A.GUIDE_LEVELS, 8), np.float32)
+    lib().oxrcpu_guide_corners(C.byref(cfg), n, pos.ctypes.data_as(fp), idx.ctypes.data_as(up), w.ctypes.data_as(fp))
+    return idx, w
+
+
+def guide_forward(cfg, grid, mlp, pos, mode=1):
+    grid, mlp, pos = _f32(grid), _f32(mlp), _f32(pos).reshape(-1, 3)
+    n = len(pos)
+    feat = np.zeros((n, GUIDE_NIN), np.float32)
+    raw = np.zeros((n, A.GUIDE_OUTPUTS), np.float32)
+    lib().oxrcpu_guide_forward(C.byref(cfg), grid.ctypes.data_as(fp), mlp.ctypes.data_as(fp), n,
+                               pos.ctypes.data_as(fp), mode, feat.ctypes.data_as(fp), raw.ctypes.data_as(fp))
+    return feat, raw
+
+
+def guide_lobes(raw):
+    raw = _f32(raw).reshape(-1, A.GUIDE_OUTPUTS)
+    out = np.zeros((len(raw), A.GUIDE_LOBES, 5), np.float32)
+    lib().oxrcpu_guide_lobes(len(raw), raw.ctypes.data_as(fp), out.ctypes.data_as(fp))
+    return out
+
+
+def guide_pdf(lobes, dirs):
+    lobes, dirs = _f32(lobes), _f32(dirs).reshape(-1, 3)
+    out = np.zeros(len(dirs), np.float64)
+    lib().oxrcpu_guide_pdf(len(dirs), lobes.ctypes.data_as(fp), dirs.ctypes.data_as(fp), out.ctypes.data_as(dp))
+    return out
+
+
+def guide_sample(lobes, u):
+    lobes, u = _f32(lobes), _f32(u).reshape(-1, 3)
+    out = np.zeros((len(u), 3), np.float32)
+    lib().oxrcpu_guide_sample(len(u), lobes.ctypes.data_as(fp), u.ctypes.data_as(fp), out.ctypes.data_as(fp))
+    return out
+
+
+def guide_loss_grad(cfg, grid, mlp, pos, dirs, weight, mode=1, grads=True):
+    grid, mlp = _f32(grid), _f32(mlp)
+    pos, dirs, weight = _f32(pos).reshape(-1, 3), _f32(dirs).reshape(-1, 3), _f32(weight).ravel()
+    gg = np.zeros(grid.size, np.float64) if grads else None
+    gm = np.zeros(mlp.size, np.float64) if grads else None
+    loss = lib().oxrcpu_guide_loss_grad(C.byref(cfg), grid.ctypes.data_as(fp), mlp.ctypes.data_as(fp), len(pos),
+                                        pos.ctypes.data_as(fp), dirs.ctypes.data_as(fp), weight.ctypes.data_as(fp),
+                                        mode, gg.ctypes.data_as(dp) if grads else dp(),
+                                        gm.ctypes.data_as(dp) if grads else dp())
+    return loss, gg, gm
+
+
+def guide_adam(p, m, v, g, t, lr, b1, b2, eps, lazy):
+    """In place on float32 arrays p, m, v."""
+    g = _f32(g)
+    lib().oxrcpu_guide_adam(p.size, p.ctypes.data_as(fp), m.ctypes.data_as(fp), v.ctypes.data_as(fp),
+                            g.ctypes.data_as(fp), t, lr, b1, b2, eps, int(lazy))

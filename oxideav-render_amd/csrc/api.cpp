@@ -11,12 +11,17 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <chrono>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "bvh.h"
+#include "dev_guide.h"
 #include "dev_scene.h"
+#include "guide_rt.h"
+#include "oxr/guide.h"
 #include "oxr/pathtrace.h"
 #include "runtime.h"
 
@@ -264,14 +269,9 @@ bool prepare_light(const oxr_light& l, DevLight& d) {
 }  // namespace
 
 // =====================================================================
-// The tracer.
+// Scenes resident on the device.
 // =====================================================================
-struct oxr_pt {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int num_cu = 0;
-  // Scene.
-  bool has_scene = false;
+struct SceneBuf {
   DevScene sc{};
   Node4* d_nodes = nullptr;
   TriRec* d_tris = nullptr;
@@ -286,49 +286,12 @@ struct oxr_pt {
   uint32_t* d_llookup = nullptr;
   oxr_bvh_info bvh{};
   uint32_t spill_cap = 4;
-  // Frame.
-  bool has_cam = false, has_opts = false;
-  oxr_camera cam{};
-  oxr_pt_options opts{};
-  // Accumulator.
-  double* d_accum = nullptr;
-  double* d_sumsq = nullptr;
-  size_t npx_alloc = 0;
-  uint32_t samples = 0;
-  // Wavefront buffers.
-  PathBuffers pb{};
-  Ctrl* d_ctrl = nullptr;
-  uint32_t* d_spill = nullptr;
-  size_t spill_alloc = 0;
-  int grid_closest = 0, grid_any = 0;
-  uint32_t path_budget = 1u << 21;
-  // Scratch for the parity entry points.
-  void* d_tmp = nullptr;
-  size_t tmp_alloc = 0;
-  // Stats / timing.
-  bool timing = false;
-  oxr_pt_stats stats{};
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  struct Span {
-    int cat;
-    hipEvent_t a, b;
-  };
-  std::vector<Span> spans;
+  size_t bytes = 0;
+  bool valid = false;
+  float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};  // finite-vertex bounds
 
-  ~oxr_pt() {
-    if (device >= 0) (void)hipSetDevice(device);
-    free_scene();
-    free_paths();
-    dfree(d_accum);
-    dfree(d_sumsq);
-    dfree(d_ctrl);
-    if (d_tmp) (void)hipFree(d_tmp);
-    for (auto e : ev_pool) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-
-  void free_scene() {
+  ~SceneBuf() { release(); }
+  void release() {
     dfree(d_nodes);
     dfree(d_tris);
     dfree(d_pos);
@@ -340,324 +303,58 @@ struct oxr_pt {
     dfree(d_lpmf);
     dfree(d_lcdf);
     dfree(d_llookup);
-    has_scene = false;
+    valid = false;
+    bytes = 0;
   }
-
-  void free_paths() {
-    for (int i = 0; i < 2; ++i) {
-      dfree(pb.qray[i]);
-      dfree(pb.qpid[i]);
-    }
-    dfree(pb.hits);
-    dfree(pb.sray);
-    dfree(pb.sidx);
-    dfree(pb.beta);
-    dfree(pb.rad);
-    dfree(pb.prev);
-    dfree(pb.sigma);
-    dfree(pb.meta);
-    dfree(pb.nee);
-    dfree(pb.vis);
-    dfree(pb.out);
-    pb.cap = 0;
-    pb.nslots = 0;
-  }
-
-  int tmp(size_t bytes, void** out) {
-    if (bytes > tmp_alloc) {
-      if (d_tmp) (void)hipFree(d_tmp);
-      d_tmp = nullptr;
-      tmp_alloc = 0;
-      hipError_t e = hipMalloc(&d_tmp, bytes);
-      if (e != hipSuccess) return fail(OXR_ERR_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-      tmp_alloc = bytes;
-    }
-    *out = d_tmp;
-    return OXR_OK;
-  }
-
-  int ensure_paths(uint32_t cap, uint32_t nslots) {
-    if (pb.cap >= cap && pb.nslots == nslots) return OXR_OK;
-    free_paths();
-    size_t c = cap, cs = (size_t)cap * nslots;
-    int rc;
-    for (int i = 0; i < 2; ++i) {
-      if ((rc = dalloc(&pb.qray[i], 2 * c))) return rc;
-      if ((rc = dalloc(&pb.qpid[i], c))) return rc;
-    }
-    if ((rc = dalloc(&pb.hits, c))) return rc;
-    if ((rc = dalloc(&pb.sray, 2 * cs))) return rc;
-    if ((rc = dalloc(&pb.sidx, cs))) return rc;
-    if ((rc = dalloc(&pb.beta, c))) return rc;
-    if ((rc = dalloc(&pb.rad, c))) return rc;
-    if ((rc = dalloc(&pb.prev, c))) return rc;
-    if ((rc = dalloc(&pb.sigma, c))) return rc;
-    if ((rc = dalloc(&pb.meta, c))) return rc;
-    if ((rc = dalloc(&pb.nee, cs))) return rc;
-    if ((rc = dalloc(&pb.vis, cs))) return rc;
-    if ((rc = dalloc(&pb.out, c))) return rc;
-    HIPC(hipMemsetAsync(pb.vis, 0, cs, stream));
-    pb.cap = cap;
-    pb.nslots = nslots;
-    return OXR_OK;
-  }
-
-  int ensure_spill() {
-    int g = std::max(grid_closest, grid_any);
-    size_t need = (size_t)g * launch::trace_block_threads() * spill_cap;
-    if (need <= spill_alloc && d_spill) return OXR_OK;
-    dfree(d_spill);
-    spill_alloc = 0;
-    int rc = dalloc(&d_spill, need);
-    if (rc) return rc;
-    spill_alloc = need;
-    return OXR_OK;
-  }
-
-  int ensure_accum() {
-    size_t npx = (size_t)std::max(opts.width, 1u) * std::max(opts.height, 1u);
-    if (npx != npx_alloc) {
-      dfree(d_accum);
-      dfree(d_sumsq);
-      npx_alloc = 0;
-      int rc;
-      if ((rc = dalloc(&d_accum, 4 * npx))) return rc;
-      if ((rc = dalloc(&d_sumsq, 3 * npx))) return rc;
-      npx_alloc = npx;
-    }
-    return OXR_OK;
-  }
-
-  int reset() {
-    samples = 0;
-    if (npx_alloc) {
-      launch::fill_f64(stream, d_accum, 4 * npx_alloc);
-      launch::fill_f64(stream, d_sumsq, 3 * npx_alloc);
-      HIPC(hipGetLastError());
-    }
-    return OXR_OK;
-  }
-
-  void span_begin(int cat, hipEvent_t& a) {
-    if (!timing) return;
-    a = next_event();
-    (void)hipEventRecord(a, stream);
-    (void)cat;
-  }
-  void span_end(int cat, hipEvent_t a) {
-    if (!timing) return;
-    hipEvent_t b = next_event();
-    (void)hipEventRecord(b, stream);
-    spans.push_back({cat, a, b});
-  }
-  hipEvent_t next_event() {
-    if (ev_used == ev_pool.size()) {
-      hipEvent_t e;
-      (void)hipEventCreate(&e);
-      ev_pool.push_back(e);
-    }
-    return ev_pool[ev_used++];
-  }
-
-  DevFrame frame() const {
-    DevFrame f;
-    for (int k = 0; k < 3; ++k) {
-      f.eye[k] = cam.eye[k];
-      f.forward[k] = cam.forward[k];
-      f.side[k] = cam.side[k];
-      f.up[k] = cam.up[k];
-    }
-    f.half_w = cam.half_w;
-    f.half_h = cam.half_h;
-    f.projection = cam.projection;
-    f.width = std::max(opts.width, 1u);
-    f.height = std::max(opts.height, 1u);
-    f.max_bounces = std::min(opts.max_bounces, 65534u);
-    f.rr_start = opts.rr_start;
-    f.seed = opts.seed;
-    f.strategy = opts.strategy <= 2u ? opts.strategy : 0u;
-    f.clamp = std::isfinite(opts.clamp) ? opts.clamp : 0.0f;
-    f.ambient = std::isfinite(opts.ambient) ? std::fmax(opts.ambient, 0.0f) : 0.0f;
-    f.nslots = sc.num_lights + 1;
-    return f;
-  }
-
-  int refine(uint32_t n);
+  int upload(const oxr_scene_desc* d);
 };
 
-int oxr_pt::refine(uint32_t n) {
-  if (!has_scene || !has_cam || !has_opts) return OXR_OK;  // no-op before sync
-  if (n == 0) return OXR_OK;
-  n = std::min(n, std::numeric_limits<uint32_t>::max() - samples);
-  if (n == 0) return OXR_OK;
-  int rc;
-  HIPC(hipSetDevice(device));
-  if ((rc = ensure_accum())) return rc;
-  const uint32_t W = std::max(opts.width, 1u), H = std::max(opts.height, 1u);
-  const uint64_t npx = (uint64_t)W * H;
-  if (npx > (1ull << 31)) return fail(OXR_ERR_INVALID, "image too large (%llu pixels)", (unsigned long long)npx);
-  DevFrame fr = frame();
-  const uint32_t nslots = fr.nslots;
-  // Batch = pixel range x sample range, at most path_budget paths.
-  uint32_t npx_b = (uint32_t)std::min<uint64_t>(npx, path_budget);
-  uint32_t S_b = npx_b < npx ? 1u : std::max(1u, std::min(n, (uint32_t)(path_budget / npx)));
-  uint32_t cap = npx_b * S_b;
-  if ((rc = ensure_paths(cap, nslots))) return rc;
-  if ((rc = ensure_spill())) return rc;
-  launch::zero_ctrl(stream, d_ctrl);
-  spans.clear();
-  ev_used = 0;
-  const int grid_shade = (int)std::min<uint64_t>((cap + 255) / 256, (uint64_t)num_cu * 8);
-  uint64_t paths = 0;
-  for (uint32_t s0 = 0; s0 < n; s0 += S_b) {
-    uint32_t S = std::min(S_b, n - s0);
-    for (uint64_t px0 = 0; px0 < npx; px0 += npx_b) {
-      uint32_t npx_c = (uint32_t)std::min<uint64_t>(npx_b, npx - px0);
-      uint32_t P = npx_c * S;
-      paths += P;
-      hipEvent_t ea{};
-      span_begin(3, ea);
-      launch::camera(stream, fr, P, npx_c, (uint32_t)px0, samples + s0, pb, d_ctrl);
-      span_end(3, ea);
-      uint32_t cur = 0;
-      for (uint32_t k = 0; k <= fr.max_bounces; ++k) {
-        launch::prep(stream, d_ctrl, cur, k > 0 ? 1u : 0u);
-        hipEvent_t e1{};
-        span_begin(0, e1);
-        launch::closest(stream, grid_closest, sc, true, pb.qray[cur], pb.qpid[cur], pb.meta, 0, &d_ctrl->qcount[cur],
-                        &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error);
-        span_end(0, e1);
-        hipEvent_t e2{};
-        span_begin(2, e2);
-        launch::shade(stream, grid_shade, sc, fr, pb, d_ctrl, cur);
-        span_end(2, e2);
-        hipEvent_t e3{};
-        span_begin(1, e3);
-        launch::anyhit(stream, grid_any, sc, true, pb.sray, pb.sidx, pb.meta, nslots, 0, &d_ctrl->scount,
-                       &d_ctrl->work[1], pb.vis, d_spill, spill_cap, &d_ctrl->error);
-        span_end(1, e3);
-        cur ^= 1u;
-        // Long paths: stop launching once every path has ended.
-        if (k >= 8 && (k % 4) == 0) {
-          uint32_t left = 0;
-          HIPC(hipMemcpyAsync(&left, &d_ctrl->qcount[cur], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-          HIPC(hipStreamSynchronize(stream));
-          if (left == 0) break;
-        }
-      }
-      launch::prep(stream, d_ctrl, cur, 1u);
-      hipEvent_t e4{};
-      span_begin(3, e4);
-      launch::finalize(stream, P, nslots, pb);
-      launch::reduce(stream, npx_c, S, pb.out, d_accum + 4 * px0, d_sumsq + 3 * px0);
-      span_end(3, e4);
-    }
-  }
-  HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(stream));
-  Ctrl c{};
-  HIPC(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
-  if (c.error) return fail(OXR_ERR_DEVICE, "traversal stack overflow (BVH depth %u)", bvh.max_depth);
-  stats.paths = paths;
-  stats.closest_rays = c.rays_closest;
-  stats.shadow_rays = c.rays_shadow;
-  stats.ms_closest = stats.ms_anyhit = stats.ms_shade = stats.ms_other = 0.0;
-  stats.launches_closest = stats.launches_anyhit = 0;
-  for (auto& s : spans) {
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, s.a, s.b);
-    if (s.cat == 0) {
-      stats.ms_closest += ms;
-      stats.launches_closest++;
-    } else if (s.cat == 1) {
-      stats.ms_anyhit += ms;
-      stats.launches_anyhit++;
-    } else if (s.cat == 2) {
-      stats.ms_shade += ms;
-    } else {
-      stats.ms_other += ms;
-    }
-  }
-  samples += n;
-  return OXR_OK;
-}
-
-// =====================================================================
-// C ABI.
-// =====================================================================
-extern "C" {
-
-const char* oxr_last_error(void) { return g_err.c_str(); }
-
-int oxr_device_count(int* out) {
-  if (!out) return fail(OXR_ERR_INVALID, "null out");
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
-  *out = n;
-  return OXR_OK;
-}
-
-int oxr_pt_create(int device, oxr_pt** out) {
-  if (!out) return fail(OXR_ERR_INVALID, "null out");
-  *out = nullptr;
-  int rc = ensure_device(device);
-  if (rc) return rc;
-  oxr_pt* h = new oxr_pt();
-  h->device = device;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
-    delete h;
-    return fail(OXR_ERR_DEVICE, "device %d: stream creation failed", device);
-  }
-  h->num_cu = prop.multiProcessorCount;
-  h->grid_closest = h->num_cu * launch::trace_blocks_per_cu(false);
-  h->grid_any = h->num_cu * launch::trace_blocks_per_cu(true);
-  if (const char* b = std::getenv("OXR_PATH_BUDGET")) {
-    long v = std::atol(b);
-    if (v >= 4096 && v <= (1l << 26)) h->path_budget = (uint32_t)v;
-  }
-  if ((rc = dalloc(&h->d_ctrl, 1))) {
-    delete h;
-    return rc;
-  }
-  *out = h;
-  return OXR_OK;
-}
-
-int oxr_pt_destroy(oxr_pt* h) {
-  delete h;
-  return OXR_OK;
-}
-
-int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* d) {
-  if (!h || !d) return fail(OXR_ERR_INVALID, "null argument");
+int SceneBuf::upload(const oxr_scene_desc* d) {
+  if (!d) return fail(OXR_ERR_INVALID, "null scene");
   if (d->num_tris > 0 && (!d->positions || !d->material_ids))
     return fail(OXR_ERR_INVALID, "positions and material_ids are required");
   if (d->num_tris >= (1u << 31)) return fail(OXR_ERR_INVALID, "too many triangles (%u)", d->num_tris);
   if (d->num_materials > 0 && !d->materials) return fail(OXR_ERR_INVALID, "null materials");
   if (d->num_lights > 0 && !d->lights) return fail(OXR_ERR_INVALID, "null lights");
-  HIPC(hipSetDevice(h->device));
-  HIPC(hipStreamSynchronize(h->stream));
-  h->free_scene();
+  release();
   const uint32_t n = d->num_tris;
+  {
+    float l[3] = {INF_F, INF_F, INF_F}, h[3] = {-INF_F, -INF_F, -INF_F};
+    for (size_t k = 0; k < (size_t)n * 9; ++k) {
+      float v = d->positions[k];
+      if (!std::isfinite(v)) continue;
+      l[k % 3] = std::min(l[k % 3], v);
+      h[k % 3] = std::max(h[k % 3], v);
+    }
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = l[a] <= h[a] ? l[a] : 0.0f;
+      hi[a] = l[a] <= h[a] ? h[a] : 0.0f;
+    }
+  }
   // Materials (+ the fallback for out-of-range ids).
   std::vector<DevMaterial> mats;
+  mats.reserve(d->num_materials + 1);
   for (uint32_t i = 0; i < d->num_materials; ++i) mats.push_back(prepare_material(d->materials[i]));
   const uint32_t fallback = (uint32_t)mats.size();
-  mats.push_back(prepare_material(fallback_material()));
+  const oxr_material fb = fallback_material();
+  mats.push_back(prepare_material(fb));
+  auto src_of = [&](uint32_t mid) -> const oxr_material& { return mid < d->num_materials ? d->materials[mid] : fb; };
   std::vector<uint32_t> info(std::max(n, 1u), 0), tflags(std::max(n, 1u), 0);
   std::vector<float> alpha(std::max(n, 1u), 1.0f);
-  std::vector<oxr_material> src(d->materials, d->materials + d->num_materials);
-  src.push_back(fallback_material());
+  bool has_blend = false;
   for (uint32_t g = 0; g < n; ++g) {
     uint32_t mid = d->material_ids[g];
     if (mid >= d->num_materials) mid = fallback;
     bool has_n = d->normals && (d->tri_flags ? (d->tri_flags[g] & OXR_TRI_HAS_NORMALS) != 0 : true);
     info[g] = mid | (has_n ? TI_HAS_NORMALS : 0u);
     const DevMaterial& m = mats[mid];
+    const oxr_material& sm = src_of(mid);
     uint32_t f = (m.flags & MAT_DS) ? TRI_DOUBLE_SIDED : 0u;
-    if (src[mid].alpha_mode == OXR_ALPHA_MASK && !(m.base[3] >= src[mid].alpha_cutoff)) f |= TRI_MASK_REJECT;
-    if (src[mid].alpha_mode == OXR_ALPHA_BLEND) f |= TRI_BLEND;
+    if (sm.alpha_mode == OXR_ALPHA_MASK && !(m.base[3] >= sm.alpha_cutoff)) f |= TRI_MASK_REJECT;
+    if (sm.alpha_mode == OXR_ALPHA_BLEND) {
+      f |= TRI_BLEND;
+      has_blend = true;
+    }
     tflags[g] = f;
     alpha[g] = m.base[3];
   }
@@ -701,8 +398,8 @@ int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* d) {
   }
   if (!cdf.empty()) cdf.back() = 1.0f;
   // BVH over the soup.
-  Bvh4 bvh = build_bvh4(d->positions, n, tflags.data(), alpha.data(), 4, 0);
-  if (bvh.nodes.empty()) {
+  Bvh4 bvh4 = build_bvh4(d->positions, n, tflags.data(), alpha.data(), 4, 0);
+  if (bvh4.nodes.empty()) {
     Node4 root;
     for (int i = 0; i < 4; ++i) {
       root.lo_x[i] = root.lo_y[i] = root.lo_z[i] = INF_F;
@@ -710,62 +407,924 @@ int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* d) {
       root.child[i] = NODE_EMPTY;
       root.count[i] = 0;
     }
-    bvh.nodes.push_back(root);
+    bvh4.nodes.push_back(root);
   }
   int rc;
   auto up = [&](auto** dst, const auto* srcp, size_t count) -> int {
     int r = dalloc(dst, count);
     if (r) return r;
+    bytes += count * sizeof(**dst);
     if (count) {
       hipError_t e = hipMemcpy(*dst, srcp, count * sizeof(**dst), hipMemcpyHostToDevice);
       if (e != hipSuccess) return fail(OXR_ERR_DEVICE, "upload: %s", hipGetErrorString(e));
     }
     return OXR_OK;
   };
-  if ((rc = up(&h->d_nodes, bvh.nodes.data(), bvh.nodes.size()))) return rc;
-  if ((rc = up(&h->d_tris, bvh.tris.data(), bvh.tris.size()))) return rc;
-  if ((rc = up(&h->d_pos, d->positions, 9ull * n))) return rc;
+  if ((rc = up(&d_nodes, bvh4.nodes.data(), bvh4.nodes.size()))) return rc;
+  if ((rc = up(&d_tris, bvh4.tris.data(), bvh4.tris.size()))) return rc;
+  if ((rc = up(&d_pos, d->positions, 9ull * n))) return rc;
   if (d->normals) {
-    if ((rc = up(&h->d_nrm, d->normals, 9ull * n))) return rc;
+    if ((rc = up(&d_nrm, d->normals, 9ull * n))) return rc;
   }
-  if ((rc = up(&h->d_info, info.data(), info.size()))) return rc;
-  if ((rc = up(&h->d_mats, mats.data(), mats.size()))) return rc;
-  if ((rc = up(&h->d_lights, lights.data(), lights.size()))) return rc;
-  if ((rc = up(&h->d_ltris, ltris.data(), ltris.size()))) return rc;
-  if ((rc = up(&h->d_lpmf, pmf.data(), pmf.size()))) return rc;
-  if ((rc = up(&h->d_lcdf, cdf.data(), cdf.size()))) return rc;
-  if ((rc = up(&h->d_llookup, lookup.data(), lookup.size()))) return rc;
-  DevScene& sc = h->sc;
-  sc.nodes = h->d_nodes;
-  sc.tris = h->d_tris;
-  sc.pos = h->d_pos;
-  sc.nrm = h->d_nrm;
-  sc.tri_info = h->d_info;
-  sc.mats = h->d_mats;
-  sc.lights = h->d_lights;
-  sc.l_tris = h->d_ltris;
-  sc.l_pmf = h->d_lpmf;
-  sc.l_cdf = h->d_lcdf;
-  sc.l_lookup = h->d_llookup;
+  if ((rc = up(&d_info, info.data(), info.size()))) return rc;
+  if ((rc = up(&d_mats, mats.data(), mats.size()))) return rc;
+  if ((rc = up(&d_lights, lights.data(), lights.size()))) return rc;
+  if ((rc = up(&d_ltris, ltris.data(), ltris.size()))) return rc;
+  if ((rc = up(&d_lpmf, pmf.data(), pmf.size()))) return rc;
+  if ((rc = up(&d_lcdf, cdf.data(), cdf.size()))) return rc;
+  if ((rc = up(&d_llookup, lookup.data(), lookup.size()))) return rc;
+  sc.nodes = d_nodes;
+  sc.tris = d_tris;
+  sc.pos = d_pos;
+  sc.nrm = d_nrm;
+  sc.tri_info = d_info;
+  sc.mats = d_mats;
+  sc.lights = d_lights;
+  sc.l_tris = d_ltris;
+  sc.l_pmf = d_lpmf;
+  sc.l_cdf = d_lcdf;
+  sc.l_lookup = d_llookup;
   sc.num_tris = n;
   sc.num_mats = (uint32_t)mats.size();
   sc.num_lights = (uint32_t)lights.size();
   sc.num_emissive = (uint32_t)ltris.size();
-  sc.has_blend = 0;
-  for (uint32_t g = 0; g < n; ++g)
-    if (tflags[g] & TRI_BLEND) sc.has_blend = 1;
-  h->bvh.num_nodes = (uint32_t)bvh.nodes.size();
-  h->bvh.num_leaves = bvh.num_leaves;
-  h->bvh.num_tris = (uint32_t)bvh.tris.size();
-  h->bvh.max_depth = bvh.max_depth;
-  h->bvh.sah_cost = bvh.sah_cost;
-  h->bvh.build_ms = bvh.build_ms;
+  sc.has_blend = has_blend ? 1u : 0u;
+  bvh.num_nodes = (uint32_t)bvh4.nodes.size();
+  bvh.num_leaves = bvh4.num_leaves;
+  bvh.num_tris = (uint32_t)bvh4.tris.size();
+  bvh.max_depth = bvh4.max_depth;
+  bvh.sah_cost = bvh4.sah_cost;
+  bvh.build_ms = bvh4.build_ms;
   // Stack bound: each level leaves at most 3 siblings behind.
-  int lds = launch::stack_lds_entries();
-  h->spill_cap = (uint32_t)std::max(4, 4 * (int)bvh.max_depth + 8 - lds);
-  h->spill_alloc = 0;
-  dfree(h->d_spill);
-  h->has_scene = true;
+  spill_cap = (uint32_t)std::max(4, 4 * (int)bvh4.max_depth + 8 - launch::stack_lds_entries());
+  valid = true;
+  return OXR_OK;
+}
+
+DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const DevScene& sc) {
+  DevFrame f;
+  for (int k = 0; k < 3; ++k) {
+    f.eye[k] = cam.eye[k];
+    f.forward[k] = cam.forward[k];
+    f.side[k] = cam.side[k];
+    f.up[k] = cam.up[k];
+  }
+  f.half_w = cam.half_w;
+  f.half_h = cam.half_h;
+  f.projection = cam.projection;
+  f.width = std::max(opts.width, 1u);
+  f.height = std::max(opts.height, 1u);
+  f.max_bounces = std::min(opts.max_bounces, 65534u);
+  f.rr_start = opts.rr_start;
+  f.seed = opts.seed;
+  f.strategy = opts.strategy <= 2u ? opts.strategy : 0u;
+  f.clamp = std::isfinite(opts.clamp) ? opts.clamp : 0.0f;
+  f.ambient = std::isfinite(opts.ambient) ? std::fmax(opts.ambient, 0.0f) : 0.0f;
+  f.nslots = sc.num_lights + 1;
+  return f;
+}
+
+// What one batch of a guided tracer needs: the network, the shading
+// kernel's view (lobes per queue entry, training records) and the
+// training-sample buffer.
+struct GuideBatch {
+  DevGuideNet net;
+  DevGuideUse use;
+  float4* samp;
+  uint32_t* count;
+  uint32_t samp_cap;
+};
+
+// =====================================================================
+// The wavefront engine: path buffers, control block, traversal spill,
+// and the batch loop shared by the tracer and the server.
+// =====================================================================
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  PathBuffers pb{};
+  Ctrl* d_ctrl = nullptr;
+  uint32_t* d_spill = nullptr;
+  size_t spill_alloc = 0;
+  uint32_t spill_cap = 4;
+  int grid_closest = 0, grid_any = 0;
+  uint32_t path_budget = 1u << 24;
+  int shade_waves = 1;
+  void* d_tmp = nullptr;
+  size_t tmp_alloc = 0;
+  bool timing = false;
+  oxr_pt_stats stats{};
+  double guide_infer_ms = 0.0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Span {
+    int cat;
+    hipEvent_t a, b;
+  };
+  std::vector<Span> spans;
+
+  // Target of one segment's reduction into a job accumulator.
+  struct Reduce {
+    uint32_t path0, npx, S;
+    double* accum;
+    double* sumsq;
+  };
+
+  ~Engine() {
+    if (device >= 0) (void)hipSetDevice(device);
+    free_paths();
+    dfree(d_ctrl);
+    dfree(d_spill);
+    if (d_tmp) (void)hipFree(d_tmp);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int init(int dev) {
+    int rc = ensure_device(dev);
+    if (rc) return rc;
+    device = dev;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess || hipStreamCreate(&stream) != hipSuccess)
+      return fail(OXR_ERR_DEVICE, "device %d: stream creation failed", dev);
+    num_cu = prop.multiProcessorCount;
+    grid_closest = num_cu * launch::trace_blocks_per_cu(false);
+    grid_any = num_cu * launch::trace_blocks_per_cu(true);
+    if (const char* b = std::getenv("OXR_PATH_BUDGET")) {
+      long v = std::atol(b);
+      if (v >= 4096 && v <= (1l << 26)) path_budget = (uint32_t)v;
+    }
+    if (const char* w = std::getenv("OXR_SHADE_WAVES")) {
+      int v = std::atoi(w);
+      if (v >= 1 && v <= 3) shade_waves = v;
+    }
+    // Keep the wavefront buffers (~400 B per path) under 1/16 of the device.
+    size_t cap_by_mem = prop.totalGlobalMem / 16 / 400;
+    if (path_budget > cap_by_mem) path_budget = (uint32_t)std::max<size_t>(cap_by_mem, 1u << 16);
+    return dalloc(&d_ctrl, 1);
+  }
+
+  void free_paths() {
+    for (int i = 0; i < 2; ++i) {
+      dfree(pb.qray[i]);
+      dfree(pb.qpid[i]);
+    }
+    dfree(pb.hits);
+    dfree(pb.sray);
+    dfree(pb.sidx);
+    dfree(pb.beta);
+    dfree(pb.rad);
+    dfree(pb.prev);
+    dfree(pb.sigma);
+    dfree(pb.meta);
+    dfree(pb.nee);
+    dfree(pb.vis);
+    dfree(pb.out);
+    dfree(pb.pjob);
+    pb.cap = 0;
+    pb.nslots = 0;
+  }
+
+  size_t path_bytes() const {
+    size_t c = pb.cap, cs = (size_t)pb.cap * pb.nslots;
+    return c * (2 * 32 + 2 * 4 + 16 + 5 * 16 + 16 + 4) + cs * (32 + 4 + 16 + 1);
+  }
+
+  int ensure_paths(uint32_t cap, uint32_t nslots) {
+    if (pb.cap >= cap && pb.nslots == nslots) return OXR_OK;
+    free_paths();
+    size_t c = cap, cs = (size_t)cap * nslots;
+    int rc;
+    for (int i = 0; i < 2; ++i) {
+      if ((rc = dalloc(&pb.qray[i], 2 * c))) return rc;
+      if ((rc = dalloc(&pb.qpid[i], c))) return rc;
+    }
+    if ((rc = dalloc(&pb.hits, c))) return rc;
+    if ((rc = dalloc(&pb.sray, 2 * cs))) return rc;
+    if ((rc = dalloc(&pb.sidx, cs))) return rc;
+    if ((rc = dalloc(&pb.beta, c))) return rc;
+    if ((rc = dalloc(&pb.rad, c))) return rc;
+    if ((rc = dalloc(&pb.prev, c))) return rc;
+    if ((rc = dalloc(&pb.sigma, c))) return rc;
+    if ((rc = dalloc(&pb.meta, c))) return rc;
+    if ((rc = dalloc(&pb.nee, cs))) return rc;
+    if ((rc = dalloc(&pb.vis, cs))) return rc;
+    if ((rc = dalloc(&pb.out, c))) return rc;
+    if ((rc = dalloc(&pb.pjob, c))) return rc;
+    HIPC(hipMemsetAsync(pb.vis, 0, cs, stream));
+    pb.cap = cap;
+    pb.nslots = nslots;
+    return OXR_OK;
+  }
+
+  int ensure_spill(uint32_t cap) {
+    if (cap > spill_cap) spill_cap = cap;
+    int g = std::max(grid_closest, grid_any);
+    size_t need = (size_t)g * launch::trace_block_threads() * spill_cap;
+    if (need <= spill_alloc && d_spill) return OXR_OK;
+    dfree(d_spill);
+    spill_alloc = 0;
+    int rc = dalloc(&d_spill, need);
+    if (rc) return rc;
+    spill_alloc = need;
+    return OXR_OK;
+  }
+
+  int tmp(size_t bytes, void** out) {
+    if (bytes > tmp_alloc) {
+      if (d_tmp) (void)hipFree(d_tmp);
+      d_tmp = nullptr;
+      tmp_alloc = 0;
+      hipError_t e = hipMalloc(&d_tmp, bytes);
+      if (e != hipSuccess) return fail(OXR_ERR_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+      tmp_alloc = bytes;
+    }
+    *out = d_tmp;
+    return OXR_OK;
+  }
+
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  void span_begin(hipEvent_t& a) {
+    if (!timing) return;
+    a = next_event();
+    (void)hipEventRecord(a, stream);
+  }
+  void span_end(int cat, hipEvent_t a) {
+    if (!timing) return;
+    hipEvent_t b = next_event();
+    (void)hipEventRecord(b, stream);
+    spans.push_back({cat, a, b});
+  }
+
+  void begin_stats() {
+    spans.clear();
+    ev_used = 0;
+    stats = oxr_pt_stats{};
+    launch::zero_ctrl(stream, d_ctrl);
+  }
+
+  // One batch: paths of `segs` (P in total) through the whole bounce
+  // loop, then per-segment reduction into the job accumulators.
+  void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
+                 const std::vector<Reduce>& red, const DevScene& sc_any, int& rc, const GuideBatch* gb = nullptr) {
+    const int grid_shade = (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
+    hipEvent_t ea{};
+    span_begin(ea);
+    launch::camera(stream, d_jobs, segs, P, pb, d_ctrl);
+    span_end(3, ea);
+    uint32_t cur = 0;
+    for (uint32_t k = 0; k <= max_bounces; ++k) {
+      launch::prep(stream, d_ctrl, cur, k > 0 ? 1u : 0u);
+      hipEvent_t e1{};
+      span_begin(e1);
+      launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
+                      &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error);
+      span_end(0, e1);
+      if (gb) {
+        hipEvent_t eg{};
+        span_begin(eg);
+        glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur], P, nullptr, pb.qray[cur], pb.hits,
+                       const_cast<GuideLobes*>(gb->use.lobes), nullptr);
+        span_end(4, eg);
+      }
+      hipEvent_t e2{};
+      span_begin(e2);
+      launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr);
+      span_end(2, e2);
+      hipEvent_t e3{};
+      span_begin(e3);
+      launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, pb.sray, pb.sidx, pb.meta, nslots, 0,
+                     &d_ctrl->scount, &d_ctrl->work[1], pb.vis, d_spill, spill_cap, &d_ctrl->error);
+      span_end(1, e3);
+      cur ^= 1u;
+      // Long paths: stop launching once every path has ended.
+      if (k >= 8 && (k % 4) == 0) {
+        uint32_t left = 0;
+        if (hipMemcpyAsync(&left, &d_ctrl->qcount[cur], sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+          rc = fail(OXR_ERR_DEVICE, "queue readback failed");
+          return;
+        }
+        if (left == 0) break;
+      }
+    }
+    launch::prep(stream, d_ctrl, cur, 1u);
+    hipEvent_t e4{};
+    span_begin(e4);
+    launch::finalize(stream, P, nslots, pb);
+    for (const Reduce& r : red) launch::reduce(stream, r.npx, r.S, pb.out + r.path0, r.accum, r.sumsq);
+    if (gb && gb->use.rec) launch::guide_collect(stream, P, gb->use, pb, gb->samp, gb->count, gb->samp_cap);
+    span_end(3, e4);
+    stats.paths += P;
+  }
+
+  // Sync, check the device error flag, fold the counters and event times.
+  int end_stats() {
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(stream));
+    Ctrl c{};
+    HIPC(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    if (c.error) return fail(OXR_ERR_DEVICE, "traversal stack overflow");
+    stats.closest_rays = c.rays_closest;
+    stats.shadow_rays = c.rays_shadow;
+    for (auto& s : spans) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, s.a, s.b);
+      if (s.cat == 0) {
+        stats.ms_closest += ms;
+        stats.launches_closest++;
+      } else if (s.cat == 1) {
+        stats.ms_anyhit += ms;
+        stats.launches_anyhit++;
+      } else if (s.cat == 2) {
+        stats.ms_shade += ms;
+      } else if (s.cat == 4) {
+        guide_infer_ms += ms;
+      } else {
+        stats.ms_other += ms;
+      }
+    }
+    return OXR_OK;
+  }
+};
+
+// =====================================================================
+// The learned path guide: parameters, Adam state and training scratch on
+// one device (include/oxr/guide.h).
+// =====================================================================
+namespace {
+
+uint32_t pcg_u32(uint32_t x) {
+  uint32_t state = x * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28) + 4u)) ^ state) * 277803737u;
+  return (word >> 22) ^ word;
+}
+float pcg_unit(uint32_t seed, uint64_t i) {
+  uint32_t h = pcg_u32(seed ^ pcg_u32((uint32_t)i ^ pcg_u32((uint32_t)(i >> 32) + 0x9e3779b9u)));
+  return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+}  // namespace
+
+struct oxr_guide {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  oxr_guide_config cfg{};
+  DevGuideNet net{};
+  uint64_t n_grid = 0, n_total = 0;
+  float* d_params = nullptr;  // [grid | mlp]
+  float* d_m = nullptr;
+  float* d_v = nullptr;
+  float* d_grad = nullptr;
+  uint4* d_frag = nullptr;
+  uint32_t t = 0;  // Adam steps
+  float4* d_samp = nullptr;
+  uint32_t samp_cap = 0;
+  uint32_t* d_count = nullptr;
+  float* d_scal = nullptr;  // inv weight sum, loss
+  uint16_t* d_xs = nullptr;
+  uint16_t* d_ds = nullptr;
+  uint32_t act_cap = 0;  // samples
+  float4* d_part = nullptr;
+  float* d_bpart = nullptr;
+  uint32_t part_splits = 0;
+  GuideLobes* d_lobes = nullptr;
+  uint32_t lobes_cap = 0;
+  float4* d_rec = nullptr;
+  size_t rec_cap = 0;  // float4s
+  oxr_guide_stats st{};
+  bool box_set = false;
+
+  ~oxr_guide() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    dfree(d_params);
+    dfree(d_m);
+    dfree(d_v);
+    dfree(d_grad);
+    dfree(d_frag);
+    dfree(d_samp);
+    dfree(d_count);
+    dfree(d_scal);
+    dfree(d_xs);
+    dfree(d_ds);
+    dfree(d_part);
+    dfree(d_bpart);
+    dfree(d_lobes);
+    dfree(d_rec);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void set_box(const float lo[3], const float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+      float e = hi[a] - lo[a];
+      net.lo[a] = lo[a];
+      net.inv[a] = e > 0.0f ? 1.0f / e : 0.0f;
+    }
+    box_set = true;
+  }
+
+  int init(int dev, const oxr_guide_config& c) {
+    int rc = ensure_device(dev);
+    if (rc) return rc;
+    device = dev;
+    cfg = c;
+    HIPC(hipStreamCreate(&stream));
+    const uint32_t T = 1u << cfg.log2_table;
+    n_grid = (uint64_t)OXR_GUIDE_LEVELS * T * OXR_GUIDE_FEATURES;
+    n_total = n_grid + OXR_GUIDE_MLP_PARAMS;
+    double b = std::exp((std::log((double)cfg.max_res) - std::log((double)cfg.base_res)) / (OXR_GUIDE_LEVELS - 1));
+    net.dense_mask = 0;
+    for (int l = 0; l < OXR_GUIDE_LEVELS; ++l) {
+      net.res[l] = (uint32_t)std::floor((double)cfg.base_res * std::pow(b, l) + 1e-6);
+      double v = (double)net.res[l] + 1.0;
+      if (v * v * v <= (double)T) net.dense_mask |= 1u << l;
+    }
+    net.tmask = T - 1u;
+    net.log2_table = cfg.log2_table;
+    bool empty = true;
+    for (int a = 0; a < 3; ++a) empty = empty && !(cfg.aabb_max[a] > cfg.aabb_min[a]);
+    if (!empty) set_box(cfg.aabb_min, cfg.aabb_max);
+    if ((rc = dalloc(&d_params, n_total)) || (rc = dalloc(&d_m, n_total)) || (rc = dalloc(&d_v, n_total)) ||
+        (rc = dalloc(&d_grad, n_total)) || (rc = dalloc(&d_frag, (size_t)NFRAG * 64)) || (rc = dalloc(&d_count, 4)) ||
+        (rc = dalloc(&d_scal, 4)))
+      return rc;
+    // Initialisation: grid U(-1e-4, 1e-4); hidden weights Glorot-uniform;
+    // last layer x0.1 (broad, near-uniform lobes at first); biases 0.
+    std::vector<float> h(n_total, 0.0f);
+    for (uint64_t i = 0; i < n_grid; ++i) h[i] = (pcg_unit(cfg.seed, i) * 2.0f - 1.0f) * 1.0e-4f;
+    float* mlp = h.data() + n_grid;
+    const int shapes[4][3] = {{G_OW1, 64, 32}, {G_OW2, 64, 64}, {G_OW3, 64, 64}, {G_OW4, 48, 64}};
+    for (int L = 0; L < 4; ++L) {
+      float lim = std::sqrt(6.0f / (float)(shapes[L][1] + shapes[L][2])) * (L == 3 ? 0.1f : 1.0f);
+      for (int k = 0; k < shapes[L][1] * shapes[L][2]; ++k)
+        mlp[shapes[L][0] + k] = (pcg_unit(cfg.seed + 7919u * (L + 1), (uint64_t)k) * 2.0f - 1.0f) * lim;
+    }
+    HIPC(hipMemcpy(d_params, h.data(), n_total * sizeof(float), hipMemcpyHostToDevice));
+    return reset_state();
+  }
+
+  int reset_state() {
+    glaunch::fill(stream, d_m, n_total);
+    glaunch::fill(stream, d_v, n_total);
+    glaunch::fill(stream, d_grad, n_total);
+    t = 0;
+    net.grid = d_params;
+    net.mlp = d_params + n_grid;
+    net.frag = d_frag;
+    glaunch::pack(stream, net.mlp, d_frag);
+    HIPC(hipGetLastError());
+    return OXR_OK;
+  }
+
+  int ensure_samples(uint32_t n) {
+    if (n <= samp_cap) return OXR_OK;
+    dfree(d_samp);
+    samp_cap = 0;
+    int rc = dalloc(&d_samp, 2 * (size_t)n);
+    if (rc) return rc;
+    samp_cap = n;
+    return OXR_OK;
+  }
+
+  // Scratch of one gradient evaluation over n samples (n % 32 == 0).
+  int ensure_train(uint32_t n, uint32_t nsplit) {
+    int rc;
+    if (n > act_cap) {
+      dfree(d_xs);
+      dfree(d_ds);
+      act_cap = 0;
+      if ((rc = dalloc(&d_xs, (size_t)(n / 16) * glaunch::xs_group())) ||
+          (rc = dalloc(&d_ds, (size_t)(n / 16) * glaunch::ds_group())))
+        return rc;
+      act_cap = n;
+    }
+    if (nsplit > part_splits) {
+      dfree(d_part);
+      dfree(d_bpart);
+      part_splits = 0;
+      if ((rc = dalloc(&d_part, (size_t)nsplit * glaunch::wtiles() * 64)) ||
+          (rc = dalloc(&d_bpart, (size_t)nsplit * glaunch::wtiles() * 64)))
+        return rc;
+      part_splits = nsplit;
+    }
+    return OXR_OK;
+  }
+
+  static uint32_t splits_for(uint32_t n) { return std::min<uint32_t>(64u, std::max<uint32_t>(1u, n / 32u / 32u)); }
+
+  // Gradient of samples [samp, samp + n) into d_grad (grid part must be
+  // zero on entry).
+  int gradient(hipStream_t s, const float4* samp, uint32_t n) {
+    uint32_t ns = splits_for(n);
+    int rc = ensure_train(n, ns);
+    if (rc) return rc;
+    glaunch::train_grad(s, net, n, samp, d_scal, d_grad, d_grad + n_grid, d_xs, d_ds, d_part, d_bpart, ns,
+                        d_scal + 1);
+    HIPC(hipGetLastError());
+    return OXR_OK;
+  }
+
+  int step(hipStream_t s, const float4* samp, uint32_t n) {
+    int rc = gradient(s, samp, n);
+    if (rc) return rc;
+    ++t;
+    float c1 = 1.0f - std::pow(cfg.beta1, (float)t), c2 = 1.0f - std::pow(cfg.beta2, (float)t);
+    glaunch::adam(s, n_grid, n_total, d_params, d_m, d_v, d_grad, cfg.lr, cfg.beta1, cfg.beta2, cfg.eps, c1, c2);
+    glaunch::pack(s, net.mlp, d_frag);
+    HIPC(hipGetLastError());
+    st.steps++;
+    st.samples += n;
+    return OXR_OK;
+  }
+
+  // Host batch -> device samples (padded with zero-weight samples to a
+  // multiple of 32); returns the padded count.
+  int upload_batch(uint32_t n, const float* pos, const float* dir, const float* w, uint32_t* padded) {
+    uint32_t np = (n + 31u) & ~31u;
+    int rc = ensure_samples(np);
+    if (rc) return rc;
+    std::vector<float4> h(2 * (size_t)np, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (uint32_t i = 0; i < n; ++i) {
+      h[2 * i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w ? w[i] : 0.0f);
+      h[2 * i + 1] = make_float4(dir ? dir[3 * i] : 0.0f, dir ? dir[3 * i + 1] : 0.0f, dir ? dir[3 * i + 2] : 1.0f,
+                                 0.0f);
+    }
+    for (uint32_t i = n; i < np; ++i) h[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    HIPC(hipMemcpyAsync(d_samp, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice, stream));
+    *padded = np;
+    return OXR_OK;
+  }
+
+  int ensure_lobes(uint32_t n) {
+    if (n <= lobes_cap) return OXR_OK;
+    dfree(d_lobes);
+    lobes_cap = 0;
+    int rc = dalloc(&d_lobes, n);
+    if (rc) return rc;
+    lobes_cap = n;
+    return OXR_OK;
+  }
+
+  int ensure_rec(size_t n4) {
+    if (n4 <= rec_cap) return OXR_OK;
+    dfree(d_rec);
+    rec_cap = 0;
+    int rc = dalloc(&d_rec, n4);
+    if (rc) return rc;
+    rec_cap = n4;
+    return OXR_OK;
+  }
+};
+
+// =====================================================================
+// The tracer.
+// =====================================================================
+struct oxr_pt {
+  Engine eng;
+  SceneBuf scene;
+  bool has_cam = false, has_opts = false;
+  oxr_camera cam{};
+  oxr_pt_options opts{};
+  double* d_accum = nullptr;
+  double* d_sumsq = nullptr;
+  size_t npx_alloc = 0;
+  uint32_t samples = 0;
+  DevJob* d_job = nullptr;
+  oxr_guide* guide = nullptr;  // not owned
+  bool guide_train = false;
+
+  ~oxr_pt() {
+    (void)hipSetDevice(eng.device);
+    (void)hipStreamSynchronize(eng.stream);
+    dfree(d_accum);
+    dfree(d_sumsq);
+    dfree(d_job);
+  }
+
+  int ensure_accum() {
+    size_t npx = (size_t)std::max(opts.width, 1u) * std::max(opts.height, 1u);
+    if (npx != npx_alloc) {
+      dfree(d_accum);
+      dfree(d_sumsq);
+      npx_alloc = 0;
+      int rc;
+      if ((rc = dalloc(&d_accum, 4 * npx))) return rc;
+      if ((rc = dalloc(&d_sumsq, 3 * npx))) return rc;
+      npx_alloc = npx;
+    }
+    return OXR_OK;
+  }
+
+  int reset() {
+    samples = 0;
+    if (npx_alloc) {
+      launch::fill_f64(eng.stream, d_accum, 4 * npx_alloc);
+      launch::fill_f64(eng.stream, d_sumsq, 3 * npx_alloc);
+      HIPC(hipGetLastError());
+    }
+    return OXR_OK;
+  }
+
+  int refine(uint32_t n);
+  int train_guide();
+};
+
+int oxr_pt::refine(uint32_t n) {
+  if (!scene.valid || !has_cam || !has_opts) return OXR_OK;  // no-op before sync
+  if (n == 0) return OXR_OK;
+  n = std::min(n, std::numeric_limits<uint32_t>::max() - samples);
+  if (n == 0) return OXR_OK;
+  int rc;
+  HIPC(hipSetDevice(eng.device));
+  if ((rc = ensure_accum())) return rc;
+  const uint32_t W = std::max(opts.width, 1u), H = std::max(opts.height, 1u);
+  const uint64_t npx = (uint64_t)W * H;
+  if (npx > (1ull << 31)) return fail(OXR_ERR_INVALID, "image too large (%llu pixels)", (unsigned long long)npx);
+  DevJob job;
+  job.sc = scene.sc;
+  job.fr = make_frame(cam, opts, scene.sc);
+  job.accum = d_accum;
+  job.sumsq = d_sumsq;
+  HIPC(hipMemcpyAsync(d_job, &job, sizeof(DevJob), hipMemcpyHostToDevice, eng.stream));
+  const uint32_t nslots = job.fr.nslots;
+  // Batch = pixel range x sample range, at most path_budget paths (a
+  // guided batch also holds 160 bytes of lobes per path: at most 4M).
+  const uint32_t budget = guide ? std::min<uint32_t>(eng.path_budget, 1u << 22) : eng.path_budget;
+  uint32_t npx_b = (uint32_t)std::min<uint64_t>(npx, budget);
+  uint32_t S_b = npx_b < npx ? 1u : std::max(1u, std::min(n, (uint32_t)(budget / npx)));
+  if ((rc = eng.ensure_paths(npx_b * S_b, nslots))) return rc;
+  if ((rc = eng.ensure_spill(scene.spill_cap))) return rc;
+  GuideBatch gbuf{};
+  const GuideBatch* gb = nullptr;
+  size_t rec_floats = 0;
+  if (guide) {
+    if (!guide->box_set) guide->set_box(scene.lo, scene.hi);
+    const uint32_t cap = npx_b * S_b;
+    if ((rc = guide->ensure_lobes(cap))) return rc;
+    if (guide_train) {
+      size_t nsel = ((size_t)cap + (1ull << guide->cfg.record_shift) - 1) >> guide->cfg.record_shift;
+      rec_floats = nsel * G_KREC * 16;
+      if ((rc = guide->ensure_rec(nsel * G_KREC * 4))) return rc;
+      if ((rc = guide->ensure_samples(std::max<uint32_t>(1u << 20, guide->cfg.batch)))) return rc;
+      HIPC(hipMemsetAsync(guide->d_count, 0, sizeof(uint32_t), eng.stream));
+    }
+    gbuf.net = guide->net;
+    gbuf.use = DevGuideUse{guide->d_lobes, guide_train ? guide->d_rec : nullptr, guide->cfg.mix,
+                           guide->cfg.record_shift};
+    gbuf.samp = guide->d_samp;
+    gbuf.count = guide->d_count;
+    gbuf.samp_cap = guide->samp_cap;
+    gb = &gbuf;
+  }
+  eng.begin_stats();
+  eng.guide_infer_ms = 0.0;
+  for (uint32_t s0 = 0; s0 < n; s0 += S_b) {
+    uint32_t S = std::min(S_b, n - s0);
+    for (uint64_t px0 = 0; px0 < npx; px0 += npx_b) {
+      uint32_t npx_c = (uint32_t)std::min<uint64_t>(npx_b, npx - px0);
+      uint32_t P = npx_c * S;
+      SegTable segs{};
+      segs.n = 1;
+      segs.s[0] = Seg{0u, 0u, npx_c, (uint32_t)px0, samples + s0, S};
+      std::vector<Engine::Reduce> red{{0u, npx_c, S, d_accum + 4 * px0, d_sumsq + 3 * px0}};
+      rc = OXR_OK;
+      if (gb && gb->use.rec) glaunch::fill(eng.stream, reinterpret_cast<float*>(guide->d_rec), rec_floats);
+      eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, rc, gb);
+      if (rc) return rc;
+    }
+  }
+  if ((rc = eng.end_stats())) return rc;
+  samples += n;
+  if (guide) {
+    guide->st.infer_ms += eng.guide_infer_ms;
+    if (guide_train && (rc = train_guide())) return rc;
+  }
+  return OXR_OK;
+}
+
+// Online training after a refine pass: cfg.steps Adam steps over windows
+// of the samples the pass collected.
+int oxr_pt::train_guide() {
+  uint32_t count = 0;
+  HIPC(hipMemcpy(&count, guide->d_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  guide->st.collected += count;
+  uint32_t n = std::min(count, guide->samp_cap) & ~31u;
+  if (n < 32) return OXR_OK;
+  uint32_t B = std::min<uint32_t>(n, std::max<uint32_t>(32u, guide->cfg.batch & ~31u));
+  hipEvent_t a{}, b{};
+  if (eng.timing) {
+    a = eng.next_event();
+    b = eng.next_event();
+    (void)hipEventRecord(a, eng.stream);
+  }
+  for (uint32_t k = 0; k < guide->cfg.steps; ++k) {
+    uint64_t start = ((uint64_t)k * B) % n;
+    if (start + B > n) start = 0;
+    int rc = guide->step(eng.stream, guide->d_samp + 2 * start, B);
+    if (rc) return rc;
+  }
+  if (eng.timing) (void)hipEventRecord(b, eng.stream);
+  HIPC(hipStreamSynchronize(eng.stream));
+  float loss = 0.0f;
+  HIPC(hipMemcpy(&loss, guide->d_scal + 1, sizeof(float), hipMemcpyDeviceToHost));
+  guide->st.last_loss = loss;
+  if (eng.timing) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    guide->st.train_ms += ms;
+  }
+  return OXR_OK;
+}
+
+// =====================================================================
+// The server: resident scenes, FIFO admission by accumulator footprint,
+// continuous batching of the active jobs' paths into shared rounds.
+// =====================================================================
+struct oxr_srv {
+  Engine eng;
+  std::vector<std::unique_ptr<SceneBuf>> scenes;
+  struct Job {
+    uint32_t id, scene;
+    oxr_camera cam;
+    oxr_pt_options opts;
+    uint32_t state = OXR_JOB_QUEUED;
+    uint32_t samples = 0, rounds = 0, slot = 0xffffffffu;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    double* d_accum = nullptr;
+    double* d_sumsq = nullptr;
+    size_t npx = 0;
+  };
+  std::vector<Job> jobs;
+  std::vector<uint32_t> queue;   // queued job ids (FIFO)
+  std::vector<uint32_t> active;  // active job ids
+  uint64_t mem_budget = 0, mem_used = 0;
+  DevJob* d_jobs = nullptr;      // device job table, MAX_SEGS slots
+  std::vector<uint32_t> free_slots;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  oxr_srv_stats st{};
+  uint32_t rr = 0;  // round-robin start for fairness when over budget
+
+  double now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  ~oxr_srv() {
+    (void)hipSetDevice(eng.device);
+    (void)hipStreamSynchronize(eng.stream);
+    for (Job& j : jobs) {
+      dfree(j.d_accum);
+      dfree(j.d_sumsq);
+    }
+    dfree(d_jobs);
+  }
+
+  int admit() {
+    while (!queue.empty() && !free_slots.empty()) {
+      Job& j = jobs[queue.front()];
+      size_t need = j.npx * 7 * sizeof(double);
+      if (mem_used + need > mem_budget && !active.empty()) break;  // wait for memory
+      int rc;
+      if ((rc = dalloc(&j.d_accum, 4 * j.npx)) || (rc = dalloc(&j.d_sumsq, 3 * j.npx))) {
+        dfree(j.d_accum);
+        dfree(j.d_sumsq);
+        if (active.empty()) return rc;
+        break;
+      }
+      launch::fill_f64(eng.stream, j.d_accum, 4 * j.npx);
+      launch::fill_f64(eng.stream, j.d_sumsq, 3 * j.npx);
+      mem_used += need;
+      j.slot = free_slots.back();
+      free_slots.pop_back();
+      DevJob dj;
+      dj.sc = scenes[j.scene]->sc;
+      dj.fr = make_frame(j.cam, j.opts, dj.sc);
+      dj.accum = j.d_accum;
+      dj.sumsq = j.d_sumsq;
+      HIPC(hipMemcpyAsync(d_jobs + j.slot, &dj, sizeof(DevJob), hipMemcpyHostToDevice, eng.stream));
+      j.state = OXR_JOB_ACTIVE;
+      j.t_admit = now_ms();
+      active.push_back(j.id);
+      queue.erase(queue.begin());
+    }
+    return OXR_OK;
+  }
+
+  int step(uint32_t* n_active, uint32_t* n_done) {
+    double t_start = now_ms();
+    int rc;
+    HIPC(hipSetDevice(eng.device));
+    if ((rc = admit())) return rc;
+    uint32_t done = 0;
+    if (!active.empty()) {
+      // Share the path budget: every active job contributes whole sample
+      // passes over its pixels (at least one), round-robin from `rr`.
+      SegTable segs{};
+      std::vector<Engine::Reduce> red;
+      std::vector<std::pair<uint32_t, uint32_t>> took;  // (job id, samples)
+      uint32_t P = 0, nslots = 1, max_b = 0, spill = 4;
+      size_t na = active.size();
+      uint32_t budget = eng.path_budget;
+      for (size_t q = 0; q < na && segs.n < (uint32_t)MAX_SEGS; ++q) {
+        Job& j = jobs[active[(rr + q) % na]];
+        uint32_t left = j.opts.samples_per_pixel - j.samples;
+        uint32_t npx = (uint32_t)j.npx;
+        uint32_t fair = std::max<uint32_t>(1u, (uint32_t)(budget / std::max<size_t>(na, 1) / std::max(npx, 1u)));
+        uint32_t S = std::min(left, fair);
+        if (P > 0 && (uint64_t)P + (uint64_t)npx * S > budget) {
+          S = (uint32_t)((budget - P) / std::max(npx, 1u));
+          if (S == 0) continue;
+        }
+        segs.s[segs.n++] = Seg{j.slot, P, npx, 0u, j.samples, S};
+        red.push_back({P, npx, S, j.d_accum, j.d_sumsq});
+        took.push_back({j.id, S});
+        P += npx * S;
+        const DevScene& sc = scenes[j.scene]->sc;
+        nslots = std::max(nslots, sc.num_lights + 1);
+        max_b = std::max(max_b, std::min(j.opts.max_bounces, 65534u));
+        spill = std::max(spill, scenes[j.scene]->spill_cap);
+      }
+      rr = na ? (rr + 1) % (uint32_t)na : 0;
+      if ((rc = eng.ensure_paths(std::max(P, 1u), nslots))) return rc;
+      if ((rc = eng.ensure_spill(spill))) return rc;
+      eng.begin_stats();
+      rc = OXR_OK;
+      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, rc);
+      if (rc) return rc;
+      if ((rc = eng.end_stats())) return rc;
+      st.rounds++;
+      st.paths += P;
+      double t = now_ms();
+      for (auto& [id, S] : took) {
+        Job& j = jobs[id];
+        j.samples += S;
+        j.rounds++;
+        if (j.samples >= j.opts.samples_per_pixel) {
+          j.state = OXR_JOB_DONE;
+          j.t_done = t;
+          active.erase(std::find(active.begin(), active.end(), id));
+          free_slots.push_back(j.slot);
+          ++done;
+          st.jobs_done++;
+        }
+      }
+    }
+    st.busy_ms += now_ms() - t_start;
+    if (n_active) *n_active = (uint32_t)active.size();
+    if (n_done) *n_done = done;
+    return OXR_OK;
+  }
+};
+
+// =====================================================================
+// C ABI.
+// =====================================================================
+extern "C" {
+
+const char* oxr_last_error(void) { return g_err.c_str(); }
+
+int oxr_device_count(int* out) {
+  if (!out) return fail(OXR_ERR_INVALID, "null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *out = n;
+  return OXR_OK;
+}
+
+int oxr_pt_create(int device, oxr_pt** out) {
+  if (!out) return fail(OXR_ERR_INVALID, "null out");
+  *out = nullptr;
+  oxr_pt* h = new oxr_pt();
+  int rc = h->eng.init(device);
+  if (!rc) rc = dalloc(&h->d_job, 1);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return OXR_OK;
+}
+
+int oxr_pt_destroy(oxr_pt* h) {
+  delete h;
+  return OXR_OK;
+}
+
+int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* d) {
+  if (!h || !d) return fail(OXR_ERR_INVALID, "null argument");
+  HIPC(hipSetDevice(h->eng.device));
+  HIPC(hipStreamSynchronize(h->eng.stream));
+  int rc = h->scene.upload(d);
+  if (rc) return rc;
   return h->reset();
 }
 
@@ -775,7 +1334,10 @@ int oxr_pt_set_camera(oxr_pt* h, const oxr_camera* cam, int* reset_out) {
   h->cam = *cam;
   h->has_cam = true;
   if (reset_out) *reset_out = changed ? 1 : 0;
-  if (changed) return h->reset();
+  if (changed) {
+    HIPC(hipSetDevice(h->eng.device));
+    return h->reset();
+  }
   return OXR_OK;
 }
 
@@ -790,7 +1352,7 @@ int oxr_pt_set_options(oxr_pt* h, const oxr_pt_options* o, int* reset_out) {
   h->has_opts = true;
   if (reset_out) *reset_out = changed ? 1 : 0;
   if (changed) {
-    HIPC(hipSetDevice(h->device));
+    HIPC(hipSetDevice(h->eng.device));
     int rc = h->ensure_accum();
     if (rc) return rc;
     return h->reset();
@@ -800,7 +1362,7 @@ int oxr_pt_set_options(oxr_pt* h, const oxr_pt_options* o, int* reset_out) {
 
 int oxr_pt_reset(oxr_pt* h) {
   if (!h) return fail(OXR_ERR_INVALID, "null handle");
-  HIPC(hipSetDevice(h->device));
+  HIPC(hipSetDevice(h->eng.device));
   return h->reset();
 }
 
@@ -823,8 +1385,8 @@ int oxr_pt_read_accum(oxr_pt* h, double* accum, double* sum_sq) {
     if (sum_sq) std::memset(sum_sq, 0, npx * 3 * sizeof(double));
     return OXR_OK;
   }
-  HIPC(hipSetDevice(h->device));
-  HIPC(hipStreamSynchronize(h->stream));
+  HIPC(hipSetDevice(h->eng.device));
+  HIPC(hipStreamSynchronize(h->eng.stream));
   HIPC(hipMemcpy(accum, h->d_accum, h->npx_alloc * 4 * sizeof(double), hipMemcpyDeviceToHost));
   if (sum_sq) HIPC(hipMemcpy(sum_sq, h->d_sumsq, h->npx_alloc * 3 * sizeof(double), hipMemcpyDeviceToHost));
   return OXR_OK;
@@ -836,17 +1398,17 @@ static int resolve_common(oxr_pt* h, const oxr_display* disp, bool rgba8, void* 
   float bg[4];
   for (int k = 0; k < 3; ++k) bg[k] = srgb_to_linear((float)disp->background[k] / 255.0f);
   bg[3] = (float)disp->background[3] / 255.0f;
-  HIPC(hipSetDevice(h->device));
+  HIPC(hipSetDevice(h->eng.device));
   int rc;
   if ((rc = h->ensure_accum())) return rc;
   void* d_out;
   size_t bytes = npx * (rgba8 ? 4 : 16);
-  if ((rc = h->tmp(bytes, &d_out))) return rc;
-  launch::resolve(h->stream, rgba8, (uint32_t)npx, h->samples, h->d_accum, bg, disp->background, disp->tone_map,
-                  disp->exposure, d_out);
+  if ((rc = h->eng.tmp(bytes, &d_out))) return rc;
+  launch::resolve(h->eng.stream, rgba8, (uint32_t)npx, h->samples, h->d_accum, bg, disp->background,
+                  disp->tone_map, disp->exposure, d_out);
   HIPC(hipGetLastError());
-  HIPC(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, h->stream));
-  HIPC(hipStreamSynchronize(h->stream));
+  HIPC(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, h->eng.stream));
+  HIPC(hipStreamSynchronize(h->eng.stream));
   return OXR_OK;
 }
 
@@ -857,31 +1419,31 @@ int oxr_pt_resolve_rgba8(oxr_pt* h, const oxr_display* disp, uint8_t* out) {
 
 int oxr_pt_set_timing(oxr_pt* h, int enable) {
   if (!h) return fail(OXR_ERR_INVALID, "null handle");
-  h->timing = enable != 0;
+  h->eng.timing = enable != 0;
   return OXR_OK;
 }
 
 int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out) {
   if (!h || !out) return fail(OXR_ERR_INVALID, "null argument");
-  *out = h->stats;
+  *out = h->eng.stats;
   return OXR_OK;
 }
 
 int oxr_pt_bvh_info(oxr_pt* h, oxr_bvh_info* out) {
   if (!h || !out) return fail(OXR_ERR_INVALID, "null argument");
-  if (!h->has_scene) return fail(OXR_ERR_STATE, "no scene uploaded");
-  *out = h->bvh;
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
+  *out = h->scene.bvh;
   return OXR_OK;
 }
 
 int oxr_pt_bvh_export(oxr_pt* h, void* nodes, void* tris) {
   if (!h || !nodes || !tris) return fail(OXR_ERR_INVALID, "null argument");
-  if (!h->has_scene) return fail(OXR_ERR_STATE, "no scene uploaded");
-  HIPC(hipSetDevice(h->device));
-  HIPC(hipStreamSynchronize(h->stream));
-  HIPC(hipMemcpy(nodes, h->d_nodes, (size_t)h->bvh.num_nodes * sizeof(Node4), hipMemcpyDeviceToHost));
-  if (h->bvh.num_tris)
-    HIPC(hipMemcpy(tris, h->d_tris, (size_t)h->bvh.num_tris * sizeof(TriRec), hipMemcpyDeviceToHost));
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
+  HIPC(hipSetDevice(h->eng.device));
+  HIPC(hipStreamSynchronize(h->eng.stream));
+  HIPC(hipMemcpy(nodes, h->scene.d_nodes, (size_t)h->scene.bvh.num_nodes * sizeof(Node4), hipMemcpyDeviceToHost));
+  if (h->scene.bvh.num_tris)
+    HIPC(hipMemcpy(tris, h->scene.d_tris, (size_t)h->scene.bvh.num_tris * sizeof(TriRec), hipMemcpyDeviceToHost));
   return OXR_OK;
 }
 
@@ -914,30 +1476,31 @@ int oxr_sobol_owen_4d(int device, uint32_t n, const uint32_t* index, const uint3
 
 static int trace_host(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint32_t flags, oxr_hit* hits, uint8_t* occ) {
   if (!h || (n && !rays)) return fail(OXR_ERR_INVALID, "null argument");
-  if (!h->has_scene) return fail(OXR_ERR_STATE, "no scene uploaded");
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
   if (n == 0) return OXR_OK;
-  HIPC(hipSetDevice(h->device));
+  Engine& e = h->eng;
+  HIPC(hipSetDevice(e.device));
   int rc;
-  if ((rc = h->ensure_spill())) return rc;
+  if ((rc = e.ensure_spill(h->scene.spill_cap))) return rc;
   size_t rbytes = (size_t)n * 32, obytes = occ ? (size_t)n : (size_t)n * 16;
   size_t off = (rbytes + 255) & ~(size_t)255;
   void* buf;
-  if ((rc = h->tmp(off + obytes + 256, &buf))) return rc;
+  if ((rc = e.tmp(off + obytes + 256, &buf))) return rc;
   float4* d_rays = (float4*)buf;
   void* d_out = (char*)buf + off;
-  HIPC(hipMemcpyAsync(d_rays, rays, rbytes, hipMemcpyHostToDevice, h->stream));
-  launch::zero_ctrl(h->stream, h->d_ctrl);
+  HIPC(hipMemcpyAsync(d_rays, rays, rbytes, hipMemcpyHostToDevice, e.stream));
+  launch::zero_ctrl(e.stream, e.d_ctrl);
   if (occ)
-    launch::anyhit(h->stream, h->grid_any, h->sc, false, d_rays, nullptr, nullptr, 1, n, nullptr, &h->d_ctrl->work[1],
-                   (uint8_t*)d_out, h->d_spill, h->spill_cap, &h->d_ctrl->error);
+    launch::anyhit(e.stream, e.grid_any, h->scene.sc, nullptr, nullptr, false, d_rays, nullptr, nullptr, 1, n, nullptr,
+                   &e.d_ctrl->work[1], (uint8_t*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   else
-    launch::closest(h->stream, h->grid_closest, h->sc, false, d_rays, nullptr, nullptr, n, nullptr, &h->d_ctrl->work[0],
-                    flags & 1u, (uint4*)d_out, h->d_spill, h->spill_cap, &h->d_ctrl->error);
+    launch::closest(e.stream, e.grid_closest, h->scene.sc, nullptr, nullptr, false, d_rays, nullptr, nullptr, n,
+                    nullptr, &e.d_ctrl->work[0], flags & 1u, (uint4*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
-  HIPC(hipMemcpyAsync(occ ? (void*)occ : (void*)hits, d_out, obytes, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipMemcpyAsync(occ ? (void*)occ : (void*)hits, d_out, obytes, hipMemcpyDeviceToHost, e.stream));
   Ctrl c{};
-  HIPC(hipMemcpyAsync(&c, h->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
-  HIPC(hipStreamSynchronize(h->stream));
+  HIPC(hipMemcpyAsync(&c, e.d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, e.stream));
+  HIPC(hipStreamSynchronize(e.stream));
   if (c.error) return fail(OXR_ERR_DEVICE, "traversal stack overflow");
   return OXR_OK;
 }
@@ -954,15 +1517,16 @@ int oxr_trace_any(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint8_t* occluded)
 
 int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays, void* d_hits, void* stream) {
   if (!h || (n && (!d_rays || !d_hits))) return fail(OXR_ERR_INVALID, "null argument");
-  if (!h->has_scene) return fail(OXR_ERR_STATE, "no scene uploaded");
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
   if (n == 0) return OXR_OK;
-  HIPC(hipSetDevice(h->device));
+  Engine& e = h->eng;
+  HIPC(hipSetDevice(e.device));
   int rc;
-  if ((rc = h->ensure_spill())) return rc;
-  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  launch::zero_ctrl(s, h->d_ctrl);
-  launch::closest(s, h->grid_closest, h->sc, false, (const float4*)d_rays, nullptr, nullptr, n, nullptr,
-                  &h->d_ctrl->work[0], 0, (uint4*)d_hits, h->d_spill, h->spill_cap, &h->d_ctrl->error);
+  if ((rc = e.ensure_spill(h->scene.spill_cap))) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : e.stream;
+  launch::zero_ctrl(s, e.d_ctrl);
+  launch::closest(s, e.grid_closest, h->scene.sc, nullptr, nullptr, false, (const float4*)d_rays, nullptr, nullptr, n,
+                  nullptr, &e.d_ctrl->work[0], 0, (uint4*)d_hits, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
   return OXR_OK;
 }
@@ -1020,6 +1584,343 @@ int oxr_bsdf_eval(int device, uint32_t n, const oxr_material_sample* ms, const f
 int oxr_bsdf_sample(int device, uint32_t n, const oxr_material_sample* ms, const float* v, const float* ng,
                     const uint8_t* front, const float* u, float* l, float* f, float* pdf, uint8_t* ok) {
   return bsdf_common(device, n, 1, ms, v, ng, front, u, l, f, pdf, ok);
+}
+
+// ---- serving ----------------------------------------------------------
+
+int oxr_srv_create(int device, uint64_t mem_budget, oxr_srv** out) {
+  if (!out) return fail(OXR_ERR_INVALID, "null out");
+  *out = nullptr;
+  oxr_srv* s = new oxr_srv();
+  int rc = s->eng.init(device);
+  if (!rc) rc = dalloc(&s->d_jobs, MAX_SEGS);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = 1ull << 34;
+  s->mem_budget = mem_budget ? mem_budget : tot / 4;
+  for (int i = MAX_SEGS - 1; i >= 0; --i) s->free_slots.push_back((uint32_t)i);
+  *out = s;
+  return OXR_OK;
+}
+
+int oxr_srv_destroy(oxr_srv* s) {
+  delete s;
+  return OXR_OK;
+}
+
+int oxr_srv_add_scene(oxr_srv* s, const oxr_scene_desc* scene, uint32_t* scene_id) {
+  if (!s || !scene || !scene_id) return fail(OXR_ERR_INVALID, "null argument");
+  HIPC(hipSetDevice(s->eng.device));
+  auto sb = std::make_unique<SceneBuf>();
+  int rc = sb->upload(scene);
+  if (rc) return rc;
+  s->scenes.push_back(std::move(sb));
+  *scene_id = (uint32_t)s->scenes.size() - 1;
+  return OXR_OK;
+}
+
+int oxr_srv_submit(oxr_srv* s, uint32_t scene_id, const oxr_camera* cam, const oxr_pt_options* opts,
+                   uint32_t* job_id) {
+  if (!s || !cam || !opts || !job_id) return fail(OXR_ERR_INVALID, "null argument");
+  if (scene_id >= s->scenes.size()) return fail(OXR_ERR_INVALID, "unknown scene %u", scene_id);
+  if (opts->samples_per_pixel == 0) return fail(OXR_ERR_INVALID, "samples_per_pixel must be >= 1");
+  uint64_t npx = (uint64_t)std::max(opts->width, 1u) * std::max(opts->height, 1u);
+  if (npx > s->eng.path_budget) return fail(OXR_ERR_INVALID, "job larger than one round (%llu pixels)",
+                                            (unsigned long long)npx);
+  oxr_srv::Job j;
+  j.id = (uint32_t)s->jobs.size();
+  j.scene = scene_id;
+  j.cam = *cam;
+  j.opts = *opts;
+  j.npx = (size_t)npx;
+  j.t_submit = s->now_ms();
+  s->jobs.push_back(j);
+  s->queue.push_back(j.id);
+  *job_id = j.id;
+  return OXR_OK;
+}
+
+int oxr_srv_step(oxr_srv* s, uint32_t* active, uint32_t* done) {
+  if (!s) return fail(OXR_ERR_INVALID, "null server");
+  return s->step(active, done);
+}
+
+int oxr_srv_job_status(oxr_srv* s, uint32_t job_id, oxr_job_status* out) {
+  if (!s || !out) return fail(OXR_ERR_INVALID, "null argument");
+  if (job_id >= s->jobs.size()) return fail(OXR_ERR_INVALID, "unknown job %u", job_id);
+  const oxr_srv::Job& j = s->jobs[job_id];
+  out->state = j.state;
+  out->samples = j.samples;
+  out->target = j.opts.samples_per_pixel;
+  out->rounds = j.rounds;
+  out->t_submit_ms = j.t_submit;
+  out->t_admit_ms = j.t_admit;
+  out->t_done_ms = j.t_done;
+  return OXR_OK;
+}
+
+int oxr_srv_read_accum(oxr_srv* s, uint32_t job_id, double* accum, double* sum_sq) {
+  if (!s || !accum) return fail(OXR_ERR_INVALID, "null argument");
+  if (job_id >= s->jobs.size()) return fail(OXR_ERR_INVALID, "unknown job %u", job_id);
+  const oxr_srv::Job& j = s->jobs[job_id];
+  if (!j.d_accum) return fail(OXR_ERR_STATE, "job %u has no accumulator (queued or released)", job_id);
+  HIPC(hipSetDevice(s->eng.device));
+  HIPC(hipStreamSynchronize(s->eng.stream));
+  HIPC(hipMemcpy(accum, j.d_accum, j.npx * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  if (sum_sq) HIPC(hipMemcpy(sum_sq, j.d_sumsq, j.npx * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return OXR_OK;
+}
+
+int oxr_srv_release(oxr_srv* s, uint32_t job_id) {
+  if (!s) return fail(OXR_ERR_INVALID, "null server");
+  if (job_id >= s->jobs.size()) return fail(OXR_ERR_INVALID, "unknown job %u", job_id);
+  oxr_srv::Job& j = s->jobs[job_id];
+  if (j.state == OXR_JOB_ACTIVE) return fail(OXR_ERR_STATE, "job %u is still rendering", job_id);
+  HIPC(hipSetDevice(s->eng.device));
+  HIPC(hipStreamSynchronize(s->eng.stream));
+  if (j.d_accum) s->mem_used -= j.npx * 7 * sizeof(double);
+  dfree(j.d_accum);
+  dfree(j.d_sumsq);
+  auto q = std::find(s->queue.begin(), s->queue.end(), job_id);
+  if (q != s->queue.end()) s->queue.erase(q);
+  return OXR_OK;
+}
+
+int oxr_srv_stats_get(oxr_srv* s, oxr_srv_stats* out) {
+  if (!s || !out) return fail(OXR_ERR_INVALID, "null argument");
+  *out = s->st;
+  out->active = (uint32_t)s->active.size();
+  out->queued = (uint32_t)s->queue.size();
+  uint64_t b = s->mem_used + s->eng.path_bytes();
+  for (auto& sc : s->scenes) b += sc->bytes;
+  out->bytes_resident = b;
+  return OXR_OK;
+}
+
+// ---------------------------------------------------------------------
+// Learned path guide (include/oxr/guide.h).
+// ---------------------------------------------------------------------
+int oxr_guide_default_config(oxr_guide_config* c) {
+  if (!c) return fail(OXR_ERR_INVALID, "null config");
+  *c = oxr_guide_config{};
+  c->log2_table = 19;
+  c->base_res = 16;
+  c->max_res = 2048;
+  c->lr = 5.0e-3f;
+  c->beta1 = 0.9f;
+  c->beta2 = 0.99f;
+  c->eps = 1.0e-15f;
+  c->mix = 0.5f;
+  c->batch = 65536;
+  c->steps = 4;
+  c->record_shift = 3;
+  return OXR_OK;
+}
+
+int oxr_guide_create(int device, const oxr_guide_config* c, oxr_guide** out) {
+  if (!c || !out) return fail(OXR_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (c->log2_table < 10 || c->log2_table > 22) return fail(OXR_ERR_INVALID, "log2_table must be in 10..22");
+  if (c->base_res < 2 || c->max_res < c->base_res || c->max_res > (1u << 16))
+    return fail(OXR_ERR_INVALID, "need 2 <= base_res <= max_res <= 65536");
+  if (!(c->mix >= 0.0f && c->mix <= 1.0f)) return fail(OXR_ERR_INVALID, "mix must be in [0, 1]");
+  if (!(c->lr > 0.0f) || !std::isfinite(c->lr)) return fail(OXR_ERR_INVALID, "lr must be finite and > 0");
+  if (!(c->beta1 >= 0.0f && c->beta1 < 1.0f && c->beta2 >= 0.0f && c->beta2 < 1.0f) || !(c->eps >= 0.0f))
+    return fail(OXR_ERR_INVALID, "Adam betas must be in [0, 1) and eps >= 0");
+  if (c->batch < 32 || c->record_shift > 16) return fail(OXR_ERR_INVALID, "batch >= 32 and record_shift <= 16");
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(c->aabb_min[a]) || !std::isfinite(c->aabb_max[a]))
+      return fail(OXR_ERR_INVALID, "aabb must be finite");
+  std::unique_ptr<oxr_guide> g(new oxr_guide());
+  int rc = g->init(device, *c);
+  if (rc) return rc;
+  *out = g.release();
+  return OXR_OK;
+}
+
+int oxr_guide_destroy(oxr_guide* g) {
+  delete g;
+  return OXR_OK;
+}
+
+int oxr_guide_num_params(oxr_guide* g, uint64_t* n_grid, uint64_t* n_mlp) {
+  if (!g) return fail(OXR_ERR_INVALID, "null guide");
+  if (n_grid) *n_grid = g->n_grid;
+  if (n_mlp) *n_mlp = OXR_GUIDE_MLP_PARAMS;
+  return OXR_OK;
+}
+
+int oxr_guide_get_params(oxr_guide* g, float* grid, float* mlp) {
+  if (!g) return fail(OXR_ERR_INVALID, "null guide");
+  HIPC(hipSetDevice(g->device));
+  HIPC(hipDeviceSynchronize());
+  if (grid) HIPC(hipMemcpy(grid, g->d_params, g->n_grid * sizeof(float), hipMemcpyDeviceToHost));
+  if (mlp) HIPC(hipMemcpy(mlp, g->d_params + g->n_grid, OXR_GUIDE_MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+  return OXR_OK;
+}
+
+int oxr_guide_set_params(oxr_guide* g, const float* grid, const float* mlp) {
+  if (!g) return fail(OXR_ERR_INVALID, "null guide");
+  HIPC(hipSetDevice(g->device));
+  HIPC(hipDeviceSynchronize());
+  if (grid) HIPC(hipMemcpy(g->d_params, grid, g->n_grid * sizeof(float), hipMemcpyHostToDevice));
+  if (mlp) HIPC(hipMemcpy(g->d_params + g->n_grid, mlp, OXR_GUIDE_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+  int rc = g->reset_state();
+  if (rc) return rc;
+  HIPC(hipStreamSynchronize(g->stream));
+  return OXR_OK;
+}
+
+namespace {
+// Lobes of n host positions into g->d_lobes (and raw outputs into *d_raw).
+int guide_infer_host(oxr_guide* g, uint32_t n, const float* pos, float** d_raw) {
+  if (!g->box_set) return fail(OXR_ERR_STATE, "guide has no position box (set aabb or attach it to a tracer)");
+  int rc;
+  float* d_pos = nullptr;
+  if ((rc = g->ensure_lobes(n)) || (rc = dalloc(&d_pos, 3 * (size_t)n))) return rc;
+  if (d_raw && (rc = dalloc(d_raw, (size_t)OXR_GUIDE_OUTPUTS * n))) {
+    dfree(d_pos);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(d_pos, pos, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) {
+    glaunch::infer(g->stream, g->net, n, nullptr, n, d_pos, nullptr, nullptr, g->d_lobes, d_raw ? *d_raw : nullptr);
+    e = hipStreamSynchronize(g->stream);
+  }
+  dfree(d_pos);
+  if (e != hipSuccess) return fail(OXR_ERR_DEVICE, "guide inference: %s", hipGetErrorString(e));
+  return OXR_OK;
+}
+}  // namespace
+
+int oxr_guide_infer(oxr_guide* g, uint32_t n, const float* pos, float* raw, float* lobes) {
+  if (!g || (n && !pos)) return fail(OXR_ERR_INVALID, "null argument");
+  if (n == 0) return OXR_OK;
+  HIPC(hipSetDevice(g->device));
+  float* d_raw = nullptr;
+  int rc = guide_infer_host(g, n, pos, raw ? &d_raw : nullptr);
+  if (rc) return rc;
+  if (raw) {
+    hipError_t e = hipMemcpy(raw, d_raw, (size_t)OXR_GUIDE_OUTPUTS * n * sizeof(float), hipMemcpyDeviceToHost);
+    dfree(d_raw);
+    if (e != hipSuccess) return fail(OXR_ERR_DEVICE, "copy: %s", hipGetErrorString(e));
+  }
+  if (lobes) {
+    std::vector<GuideLobes> h(n);
+    HIPC(hipMemcpy(h.data(), g->d_lobes, n * sizeof(GuideLobes), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i)
+      for (int k = 0; k < OXR_GUIDE_LOBES; ++k) {
+        float* o = lobes + ((size_t)i * OXR_GUIDE_LOBES + k) * 5;
+        o[0] = h[i].mk[k].x;
+        o[1] = h[i].mk[k].y;
+        o[2] = h[i].mk[k].z;
+        o[3] = h[i].mk[k].w;
+        o[4] = h[i].pi[k];
+      }
+  }
+  return OXR_OK;
+}
+
+namespace {
+int guide_pdf_sample(oxr_guide* g, uint32_t n, const float* pos, const float* dir, const float* u, float* dir_out,
+                     float* pdf_out) {
+  HIPC(hipSetDevice(g->device));
+  int rc = guide_infer_host(g, n, pos, nullptr);
+  if (rc) return rc;
+  float *d_in = nullptr, *d_dir = nullptr, *d_pdf = nullptr;
+  if ((rc = dalloc(&d_in, 3 * (size_t)n)) || (rc = dalloc(&d_dir, 3 * (size_t)n)) || (rc = dalloc(&d_pdf, n))) {
+    dfree(d_in);
+    dfree(d_dir);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(d_in, u ? u : dir, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    glaunch::pdf_sample(g->stream, n, g->d_lobes, u ? nullptr : d_in, u ? d_in : nullptr, d_dir, d_pdf);
+    e = hipStreamSynchronize(g->stream);
+  }
+  if (e == hipSuccess && dir_out) e = hipMemcpy(dir_out, d_dir, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(pdf_out, d_pdf, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+  dfree(d_in);
+  dfree(d_dir);
+  dfree(d_pdf);
+  if (e != hipSuccess) return fail(OXR_ERR_DEVICE, "guide pdf/sample: %s", hipGetErrorString(e));
+  return OXR_OK;
+}
+}  // namespace
+
+int oxr_guide_pdf(oxr_guide* g, uint32_t n, const float* pos, const float* dir, float* pdf) {
+  if (!g || (n && (!pos || !dir || !pdf))) return fail(OXR_ERR_INVALID, "null argument");
+  if (n == 0) return OXR_OK;
+  return guide_pdf_sample(g, n, pos, dir, nullptr, nullptr, pdf);
+}
+
+int oxr_guide_sample(oxr_guide* g, uint32_t n, const float* pos, const float* u, float* dir_out, float* pdf_out) {
+  if (!g || (n && (!pos || !u || !dir_out || !pdf_out))) return fail(OXR_ERR_INVALID, "null argument");
+  if (n == 0) return OXR_OK;
+  return guide_pdf_sample(g, n, pos, nullptr, u, dir_out, pdf_out);
+}
+
+int oxr_guide_gradient(oxr_guide* g, uint32_t n, const float* pos, const float* dir, const float* weight,
+                       float* grad_grid, float* grad_mlp, double* loss) {
+  if (!g || (n && (!pos || !dir || !weight))) return fail(OXR_ERR_INVALID, "null argument");
+  if (!g->box_set) return fail(OXR_ERR_STATE, "guide has no position box");
+  HIPC(hipSetDevice(g->device));
+  HIPC(hipDeviceSynchronize());
+  uint32_t np = 0;
+  int rc = g->upload_batch(n, pos, dir, weight, &np);
+  if (rc) return rc;
+  glaunch::fill(g->stream, g->d_grad, g->n_total);
+  if (np && (rc = g->gradient(g->stream, g->d_samp, np))) return rc;
+  HIPC(hipStreamSynchronize(g->stream));
+  if (grad_grid) HIPC(hipMemcpy(grad_grid, g->d_grad, g->n_grid * sizeof(float), hipMemcpyDeviceToHost));
+  if (grad_mlp)
+    HIPC(hipMemcpy(grad_mlp, g->d_grad + g->n_grid, OXR_GUIDE_MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) {
+    float l = 0.0f;
+    if (np) HIPC(hipMemcpy(&l, g->d_scal + 1, sizeof(float), hipMemcpyDeviceToHost));
+    *loss = l;
+  }
+  glaunch::fill(g->stream, g->d_grad, g->n_total);  // Adam expects a zero grid gradient
+  HIPC(hipStreamSynchronize(g->stream));
+  return OXR_OK;
+}
+
+int oxr_guide_train(oxr_guide* g, uint32_t n, const float* pos, const float* dir, const float* weight,
+                    uint32_t steps, double* loss) {
+  if (!g || (n && (!pos || !dir || !weight))) return fail(OXR_ERR_INVALID, "null argument");
+  if (!g->box_set) return fail(OXR_ERR_STATE, "guide has no position box");
+  HIPC(hipSetDevice(g->device));
+  HIPC(hipDeviceSynchronize());
+  uint32_t np = 0;
+  int rc = g->upload_batch(n, pos, dir, weight, &np);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < steps && np; ++k)
+    if ((rc = g->step(g->stream, g->d_samp, np))) return rc;
+  HIPC(hipStreamSynchronize(g->stream));
+  float l = 0.0f;
+  if (np && steps) HIPC(hipMemcpy(&l, g->d_scal + 1, sizeof(float), hipMemcpyDeviceToHost));
+  g->st.last_loss = l;
+  if (loss) *loss = l;
+  return OXR_OK;
+}
+
+int oxr_guide_stats_get(oxr_guide* g, oxr_guide_stats* out) {
+  if (!g || !out) return fail(OXR_ERR_INVALID, "null argument");
+  *out = g->st;
+  return OXR_OK;
+}
+
+int oxr_pt_set_guide(oxr_pt* h, oxr_guide* g, int train) {
+  if (!h) return fail(OXR_ERR_INVALID, "null handle");
+  if (g && g->device != h->eng.device) return fail(OXR_ERR_INVALID, "guide and tracer live on different devices");
+  HIPC(hipSetDevice(h->eng.device));
+  if (g) HIPC(hipStreamSynchronize(g->stream));
+  h->guide = g;
+  h->guide_train = g && train != 0;
+  return h->reset();
 }
 
 }  // extern "C"

@@ -89,19 +89,17 @@ OXR_D uint32_t nested_uniform_scramble(uint32_t x, uint32_t seed) {
 }
 OXR_D float to_unit(uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); }
 
-// Sobol' generator matrices of dimensions 0-3 (Joe-Kuo), 512 bytes in
-// constant memory; filled by the host from the same recurrence.
-extern __constant__ uint32_t c_sobol[4][32];
+// Sobol' generator matrices of dimensions 1-3 (Joe-Kuo) as byte tables:
+// c_sobol_tab[d-1][b][v] = XOR of the columns 8b..8b+7 selected by the
+// bits of v. The matrix-vector product over GF(2) is then four
+// independent lookups per dimension (12 KB, built by the host from the
+// same recurrence); dimension 0 (van der Corput) is the bit reversal.
+extern __constant__ uint32_t c_sobol_tab[3][4][256];
 
 OXR_D uint32_t sobol(uint32_t index, int dim) {
-  uint32_t r = 0;
-  // Walk the set bits of the index (<= 32 steps, usually ~log2(index)).
-  while (index != 0) {
-    int k = __builtin_ctz(index);
-    r ^= c_sobol[dim][k];
-    index &= index - 1u;
-  }
-  return r;
+  if (dim == 0) return __builtin_bitreverse32(index);
+  const uint32_t(*t)[256] = c_sobol_tab[dim - 1];
+  return t[0][index & 255u] ^ t[1][(index >> 8) & 255u] ^ t[2][(index >> 16) & 255u] ^ t[3][index >> 24];
 }
 
 struct F4 {

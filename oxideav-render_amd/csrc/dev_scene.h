@@ -87,6 +87,29 @@ constexpr uint32_t HIT_MISS = 0xffffffffu;
 constexpr uint32_t HIT_BACK = 0x80000000u;
 constexpr uint32_t NO_SKIP = 0xffffffffu;
 
+// One render job of a wavefront batch: its scene, camera/options and
+// accumulator. A batch mixes paths of several jobs (continuous batching);
+// every path carries its job index.
+struct DevJob {
+  DevScene sc;
+  DevFrame fr;
+  double* accum;
+  double* sumsq;
+};
+
+// A batch segment: paths [path0, path0 + npx * S) of job `job` cover
+// pixels [px0, px0 + npx) x samples [first, first + S), pixel-major
+// within a sample.
+struct Seg {
+  uint32_t job, path0, npx, px0, first, S;
+};
+
+constexpr int MAX_SEGS = 64;
+struct SegTable {
+  uint32_t n;
+  Seg s[MAX_SEGS];
+};
+
 // Device control block: queue counters, work counters, error, stats.
 struct Ctrl {
   uint32_t qcount[2];

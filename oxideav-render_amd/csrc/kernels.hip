@@ -15,13 +15,14 @@
 
 #include "bvh.h"
 #include "dev_bsdf.h"
+#include "dev_guide.h"
 #include "dev_math.h"
 #include "dev_scene.h"
 #include "runtime.h"
 
 namespace oxr {
 
-__constant__ uint32_t c_sobol[4][32];
+__constant__ uint32_t c_sobol_tab[3][4][256];
 __constant__ float c_sheen[16 * 32];
 __constant__ float c_srgb_th[255];
 
@@ -96,23 +97,50 @@ OXR_D bool accept_tri(uint32_t flags, float alpha, bool front, bool cull, uint32
   return true;
 }
 
+// One ray's traversal state. step() visits one node (its leaves are
+// tested right away) and returns true once the ray is finished.
 template <bool ANY>
-OXR_D bool traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, bool cull, uint32_t skip,
-                    uint32_t path_seed, uint32_t ray_id, Stack& st, float& t_out, uint32_t& prim_out, float& u_out,
-                    float& v_out) {
-  V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  V3 oi = mul(o, inv);
-  bool negx = signbit(inv.x), negy = signbit(inv.y), negz = signbit(inv.z);
-  WRay wr = prepare_wray(o, d);
-  uint32_t best = HIT_MISS;
-  float bu = 0.0f, bv = 0.0f;
-  uint32_t node = 0;
-  st.sp = 0;
-  st.bottom = 0;
-  st.spn = 0;
-  st.overflow = false;
-  for (;;) {
-    const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
+struct Trav {
+  const Node4* nodes;
+  const TriRec* tris;
+  V3 inv, oi;
+  WRay wr;
+  float tmin, tmax;
+  bool negx, negy, negz, cull;
+  uint32_t skip, path_seed, ray_id;
+  uint32_t best;
+  float bu, bv;
+  uint32_t node;
+  bool hit_any;
+
+  OXR_D void setup(const DevScene& sc, V3 o, V3 d, float tmin_, float tmax_, bool cull_, uint32_t skip_, uint32_t seed_,
+                   uint32_t rid_, Stack& st) {
+    nodes = sc.nodes;
+    tris = sc.tris;
+    inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    oi = mul(o, inv);
+    negx = signbit(inv.x);
+    negy = signbit(inv.y);
+    negz = signbit(inv.z);
+    wr = prepare_wray(o, d);
+    tmin = tmin_;
+    tmax = tmax_;
+    cull = cull_;
+    skip = skip_;
+    path_seed = seed_;
+    ray_id = rid_;
+    best = HIT_MISS;
+    bu = bv = 0.0f;
+    node = 0;
+    hit_any = false;
+    st.sp = 0;
+    st.bottom = 0;
+    st.spn = 0;
+    st.overflow = false;
+  }
+
+  OXR_D bool step(Stack& st) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + node);
     float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     uint4 ch = reinterpret_cast<const uint4*>(np)[6];
     uint4 cn = reinterpret_cast<const uint4*>(np)[7];
@@ -135,11 +163,10 @@ OXR_D bool traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, bool
       bool hit = chl[c] != NODE_EMPTY && tnear <= tfar * SLAB_EPS;
       tn[c] = hit ? tnear : __builtin_huge_valf();
     }
-    // Leaves first: test their triangles right away.
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (tn[c] != __builtin_huge_valf() && cnt[c] > 0) {
-        const float4* tp = reinterpret_cast<const float4*>(sc.tris + chl[c]);
+        const float4* tp = reinterpret_cast<const float4*>(tris + chl[c]);
         for (uint32_t k = 0; k < cnt[c]; ++k) {
           float4 a = tp[3 * k], b = tp[3 * k + 1], e = tp[3 * k + 2];
           float t, b1, b2;
@@ -150,7 +177,7 @@ OXR_D bool traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, bool
           if (ANY) {
             if (gid == skip) continue;
             if (!accept_tri(fl, e.w, front, false, path_seed, ray_id, gid)) continue;
-            prim_out = gid;
+            hit_any = true;
             return true;
           } else {
             if (best != HIT_MISS && !(t < tmax || (t == tmax && gid < (best & 0x7fffffffu)))) continue;
@@ -165,16 +192,15 @@ OXR_D bool traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, bool
       }
     }
     if (!ANY) {
-      // Sort inner children far -> near (push the nearest last).
       uint32_t ci[4] = {chl[0], chl[1], chl[2], chl[3]};
-#define OXR_CSWAP(i, j)                     \
-  if (tn[i] < tn[j]) {                      \
-    float tt = tn[i];                       \
-    tn[i] = tn[j];                          \
-    tn[j] = tt;                             \
-    uint32_t cc = ci[i];                    \
-    ci[i] = ci[j];                          \
-    ci[j] = cc;                             \
+#define OXR_CSWAP(i, j)  \
+  if (tn[i] < tn[j]) {   \
+    float tt = tn[i];    \
+    tn[i] = tn[j];       \
+    tn[j] = tt;          \
+    uint32_t cc = ci[i]; \
+    ci[i] = ci[j];       \
+    ci[j] = cc;          \
   }
       OXR_CSWAP(0, 1) OXR_CSWAP(2, 3) OXR_CSWAP(0, 2) OXR_CSWAP(1, 3) OXR_CSWAP(1, 2)
 #undef OXR_CSWAP
@@ -186,21 +212,64 @@ OXR_D bool traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, bool
       for (int c = 0; c < 4; ++c)
         if (tn[c] != __builtin_huge_valf()) st.push(chl[c]);
     }
-    if (st.overflow) break;
-    if (!st.pop(node)) break;
+    if (st.overflow) return true;
+    return !st.pop(node);
   }
-  if (ANY) return false;
-  t_out = tmax;
-  prim_out = best;
-  u_out = bu;
-  v_out = bv;
-  return best != HIT_MISS;
+};
+
+// Persistent traversal loop shared by the closest-hit and occlusion
+// kernels. Lanes whose ray finished are refilled from the shared counter
+// once REFILL of them are idle (one atomic per refill); the first ray of
+// every lane is assigned by wave index, so a launch whose rays fit in one
+// round makes no atomic at all.
+constexpr uint32_t REFILL = 16;
+
+template <bool ANY, class Setup, class Finish>
+OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, Finish finish) {
+  const uint32_t lane = lane_id();
+  const uint32_t nwaves = gridDim.x * (TRACE_TB / 64);
+  const uint32_t wave = blockIdx.x * (TRACE_TB / 64) + threadIdx.x / 64;
+  Trav<ANY> tr;
+  uint32_t ri = wave * 64u + lane;
+  bool active = ri < count;
+  bool exhausted = nwaves * 64u >= count;
+  if (active) setup(tr, ri);
+  for (;;) {
+    if (active) {
+      if (tr.step(st)) {
+        finish(tr, ri);
+        active = false;
+      }
+    }
+    uint64_t freem = __ballot(!active);
+    uint32_t nfree = (uint32_t)__popcll(freem);
+    if (exhausted) {
+      if (nfree == 64u) break;
+      continue;
+    }
+    if (nfree >= REFILL) {
+      int leader = __ffsll((unsigned long long)freem) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(work, nfree) + nwaves * 64u;
+      base = __shfl(base, leader);
+      if (base + nfree >= count) exhausted = true;
+      if (!active) {
+        uint64_t below = lane == 0 ? 0ull : (freem & ((~0ull) >> (64 - lane)));
+        ri = base + (uint32_t)__popcll(below);
+        if (ri < count) {
+          setup(tr, ri);
+          active = true;
+        }
+      }
+    }
+  }
 }
 
 // Persistent closest-hit kernel over a ray queue. PATH: rays carry the
 // bounce index k in the t_min slot (t_min = 0); else t_min as given.
 template <bool PATH>
-__global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const float4* __restrict__ rays,
+__global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                      const uint32_t* __restrict__ qpid,
                                                      const uint4* __restrict__ meta, uint32_t fixed_count,
                                                      const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
@@ -212,48 +281,47 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const float4*
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    uint32_t i = base + lane_id();
-    if (i < count) {
-      float4 a = rays[2 * i], b = rays[2 * i + 1];
-      V3 o = f4xyz(a), d = f4xyz(b);
-      float tmin;
-      bool cull;
-      uint32_t path_seed = 0, ray_id = 0;
-      if (PATH) {
-        uint32_t k = __float_as_uint(a.w);
-        tmin = 0.0f;
-        cull = k == 0;
-        ray_id = k << 16;
-        if (sc.has_blend) {
-          uint4 m = meta[qpid[i]];
-          path_seed = hash_mix(m.w, m.y);
-        }
-      } else {
-        tmin = a.w;
-        cull = cull_all != 0;
+  auto setup = [&](Trav<false>& tr, uint32_t i) {
+    float4 a = rays[2 * i], b = rays[2 * i + 1];
+    uint32_t path_seed = 0, ray_id = 0;
+    float tmin;
+    bool cull;
+    if (PATH) {
+      uint32_t k = __float_as_uint(a.w);
+      uint32_t pid = qpid[i];
+      const DevScene& js = jobs[pjob[pid]].sc;
+      tmin = 0.0f;
+      cull = k == 0;
+      ray_id = k << 16;
+      if (js.has_blend) {
+        uint4 m = meta[pid];
+        path_seed = hash_mix(m.w, m.y);
       }
-      float t = 0.0f, u = 0.0f, v = 0.0f;
-      uint32_t prim = HIT_MISS;
-      traverse<false>(sc, o, d, tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st, t, prim, u, v);
-      if (st.overflow) {
-        atomicOr(err, ERR_STACK);
-        prim = HIT_MISS;
-      }
-      hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(t), prim, __float_as_uint(u), __float_as_uint(v));
+      tr.setup(js, f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
+    } else {
+      tmin = a.w;
+      cull = cull_all != 0;
+      tr.setup(sc, f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
     }
-  }
+  };
+  auto finish = [&](Trav<false>& tr, uint32_t i) {
+    uint32_t prim = tr.best;
+    if (st.overflow) {
+      atomicOr(err, ERR_STACK);
+      prim = HIT_MISS;
+    }
+    hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(tr.tmax), prim, __float_as_uint(tr.bu),
+                         __float_as_uint(tr.bv));
+  };
+  trace_loop<false>(count, work, st, setup, finish);
 }
 
 // Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
 // sidx[i] = path * nslots + slot; writes vis[slot] = 2 when unoccluded.
 // Else: record = oxr_ray, writes occ[i] (1 = occluded).
 template <bool PATH>
-__global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const float4* __restrict__ rays,
+__global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                     const uint32_t* __restrict__ sidx, const uint4* __restrict__ meta,
                                                     uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr,
                                                     uint32_t* work, uint8_t* __restrict__ out, uint32_t* spill,
@@ -264,59 +332,60 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const float4* 
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    uint32_t i = base + lane_id();
-    if (i < count) {
-      float4 a = rays[2 * i], b = rays[2 * i + 1];
-      float tmin = 0.0f;
-      uint32_t skip = NO_SKIP, path_seed = 0, ray_id = 0, slot = 0;
-      if (PATH) {
-        skip = __float_as_uint(a.w);
-        slot = sidx[i];
-        if (sc.has_blend) {
-          uint32_t pid = slot / nslots, j = slot - pid * nslots;
-          uint4 m = meta[pid];
-          uint32_t k = (m.z & 0xffffu);
-          path_seed = hash_mix(m.w, m.y);
-          ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (j & 0x7fffu)));
-        }
-      } else {
-        tmin = a.w;
+  auto setup = [&](Trav<true>& tr, uint32_t i) {
+    float4 a = rays[2 * i], b = rays[2 * i + 1];
+    float tmin = 0.0f;
+    uint32_t skip = NO_SKIP, path_seed = 0, ray_id = 0;
+    if (PATH) {
+      skip = __float_as_uint(a.w);
+      uint32_t slot = sidx[i];
+      uint32_t pid = slot / nslots, j = slot - pid * nslots;
+      const DevScene& js = jobs[pjob[pid]].sc;
+      if (js.has_blend) {
+        uint4 m = meta[pid];
+        uint32_t k = (m.z & 0xffffu);
+        path_seed = hash_mix(m.w, m.y);
+        ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (j & 0x7fffu)));
       }
-      float t, u, v;
-      uint32_t prim;
-      bool occ = traverse<true>(sc, f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st, t, prim, u, v);
-      if (st.overflow) {
-        atomicOr(err, ERR_STACK);
-        occ = true;
-      }
-      if (PATH) {
-        if (!occ) out[slot] = 2;
-      } else {
-        out[i] = occ ? 1 : 0;
-      }
+      tr.setup(js, f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
+    } else {
+      tmin = a.w;
+      tr.setup(sc, f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
     }
-  }
+  };
+  auto finish = [&](Trav<true>& tr, uint32_t i) {
+    bool occ = tr.hit_any;
+    if (st.overflow) {
+      atomicOr(err, ERR_STACK);
+      occ = true;
+    }
+    if (PATH) {
+      if (!occ) out[sidx[i]] = 2;
+    } else {
+      out[i] = occ ? 1 : 0;
+    }
+  };
+  trace_loop<true>(count, work, st, setup, finish);
 }
 
 // ---------------------------------------------------------------------
 // Camera rays (Sampler::new + pattern 0 + Camera::primary_ray,
 // src/pathtrace.rs:2005-2012, src/camera.rs:489-514).
 // ---------------------------------------------------------------------
-__global__ void k_camera(DevFrame fr, uint32_t P, uint32_t npx, uint32_t px0, uint32_t first_sample,
-                         float4* __restrict__ qray,
+__global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_t P, float4* __restrict__ qray,
                          uint32_t* __restrict__ qpid, float4* __restrict__ beta, float4* __restrict__ rad,
-                         float4* __restrict__ sigma, uint4* __restrict__ meta, Ctrl* ctrl) {
+                         float4* __restrict__ sigma, uint4* __restrict__ meta, uint32_t* __restrict__ pjob, Ctrl* ctrl) {
   uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid == 0) ctrl->qcount[0] = P;
   if (pid >= P) return;
-  uint32_t s = pid / npx, px = px0 + (pid - s * npx);
+  uint32_t si = 0;
+  while (si + 1 < segs.n && segs.s[si + 1].path0 <= pid) ++si;
+  const Seg sg = segs.s[si];
+  const DevFrame& fr = jobs[sg.job].fr;
+  uint32_t local = pid - sg.path0;
+  uint32_t s = local / sg.npx, px = sg.px0 + (local - s * sg.npx);
   uint32_t x = px % fr.width, y = px / fr.width;
-  uint32_t index = first_sample + s;
+  uint32_t index = sg.first + s;
   uint32_t ps = pixel_seed_of(fr.seed, x, y);
   F4 p0 = pattern(ps, index, 0);
   float fx = (float)x + p0.v[0], fy = (float)y + p0.v[1];
@@ -338,6 +407,7 @@ __global__ void k_camera(DevFrame fr, uint32_t P, uint32_t npx, uint32_t px0, ui
   rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   sigma[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   meta[pid] = make_uint4(x | (y << 16), index, 0xffu << 16, ps);
+  pjob[pid] = sg.job;
 }
 
 __global__ void k_prep(Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
@@ -420,18 +490,21 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // Shading: one path vertex of Integrator::sample (src/pathtrace.rs:
 // 2027-2322), with shadow rays deferred to k_anyhit.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const float4* __restrict__ qray,
+template <int WAVES, bool GUIDE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
+                                             const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
                                              float4* __restrict__ nray, uint32_t* __restrict__ npid,
                                              float4* __restrict__ sray, uint32_t* __restrict__ sidx,
                                              float4* __restrict__ beta_s, float4* __restrict__ rad_s,
                                              float4* __restrict__ prev_s, float4* __restrict__ sigma_s,
                                              uint4* __restrict__ meta_s, float4* __restrict__ nee,
-                                             uint8_t* __restrict__ vis, Ctrl* ctrl, uint32_t cur) {
+                                             uint8_t* __restrict__ vis, Ctrl* ctrl, uint32_t cur,
+                                             DevGuideUse gu) {
   const uint32_t count = ctrl->qcount[cur];
+  const uint32_t rec_mask = GUIDE ? (1u << gu.shift) - 1u : 0u;
   uint32_t* ncount = &ctrl->qcount[cur ^ 1u];
   uint32_t* scount = &ctrl->scount;
-  const uint32_t nslots = fr.nslots;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   // Uniform trip count so every lane reaches the wave-level appends.
@@ -441,6 +514,9 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
     bool valid = i < count;
     uint32_t pid = valid ? qpid[i] : 0u;
     bool alive = valid;
+    const DevJob& job = jobs[valid ? pjob[pid] : 0u];
+    const DevScene& sc = job.sc;
+    const DevFrame& fr = job.fr;
     uint4 meta = valid ? meta_s[pid] : make_uint4(0, 0, 0, 0);
     uint32_t k = meta.z & 0xffffu;
     uint32_t hero = (meta.z >> 16) & 0xffu;
@@ -456,6 +532,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
       add_pending(pid, nslots, nee, vis, radiance);
       flags &= ~PF_PENDING;
     }
+    // Guide training record of the previous vertex: the path radiance now
+    // holds everything that vertex added itself (emission, NEE).
+    if (GUIDE && gu.rec && valid && k >= 1 && k - 1 < (uint32_t)G_KREC && (pid & rec_mask) == 0u)
+      gu.rec[((size_t)(pid >> gu.shift) * G_KREC + (k - 1)) * 4 + 3] =
+          make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
     uint4 h = valid ? hits[i] : make_uint4(0, HIT_MISS, 0, 0);
     float4 rd = valid ? qray[2 * i + 1] : make_float4(0, 0, 1, 0);
     V3 d = f4xyz(rd);
@@ -558,6 +639,15 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
     bsdf.init(ms, neg(d), ngeo, front);
     if (alive && bsdf.black()) alive = false;
     V3 ng = bsdf.ng;
+    // One-sample MIS with the guide at eligible vertices: no transmission,
+    // not a near-mirror metal (the BSDF sampler alone serves those).
+    float gmix = 0.0f;
+    const GuideLobes* GL = nullptr;
+    if (GUIDE) {
+      GL = gu.lobes + i;
+      if (alive && transmission == 0.0f && M->dt == 0.0f && !(M->metallic >= 0.99f && M->roughness < 0.2f))
+        gmix = gu.mix;
+    }
     // NEE: punctual lights, one shadow-ray slot each.
     bool any_shadow = false;
     for (uint32_t j = 0; j < sc.num_lights; ++j) {
@@ -677,6 +767,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
               bsdf.eval(l, f, pb);
               V3 le = (LM.flags & MAT_UNLIT) ? v3(0, 0, 0) : ld3(LM.emissive);
               if (!is_black(f) && !is_black(le)) {
+                if (GUIDE && gmix > 0.0f) pb = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pb;
                 float w = fr.strategy == 0u ? power_heuristic(pl, pb) : 1.0f;
                 o2 = spawn(p, ng, l);
                 tmax = fmaxr(dist * (1.0f - 1.0e-4f), 0.0f);
@@ -707,7 +798,27 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
       F4 pb_u = pattern(ps, index, 1 + 2 * k);
       V3 l, f;
       float pdf;
-      if (bsdf.sample(pb_u.v[0], pb_u.v[1], pb_u.v[2], l, f, pdf)) {
+      bool ok;
+      if (GUIDE && gmix > 0.0f) {
+        // Direction from the guide with probability gmix, else from the
+        // BSDF; weighted by the mixture pdf of both.
+        F4 gq = pattern(ps, index, G_PATTERN + k);
+        float pbs;
+        if (gq.v[0] < gmix) {
+          l = guide_sample(GL, gq.v[1], gq.v[2], gq.v[3]);
+          bsdf.eval(l, f, pbs);
+          ok = !is_black(f);
+        } else {
+          ok = bsdf.sample(pb_u.v[0], pb_u.v[1], pb_u.v[2], l, f, pbs);
+        }
+        if (ok) {
+          pdf = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pbs;
+          ok = pdf > 0.0f && isfinite(pdf);
+        }
+      } else {
+        ok = bsdf.sample(pb_u.v[0], pb_u.v[1], pb_u.v[2], l, f, pdf);
+      }
+      if (ok) {
         beta = mul(beta, scale(f, 1.0f / pdf));
         if (finite3(beta)) {
           prev_pdf = pdf;
@@ -743,6 +854,12 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
         }
       }
     }
+    if (GUIDE && gu.rec && cont && gmix > 0.0f && k < (uint32_t)G_KREC && (pid & rec_mask) == 0u) {
+      float4* r = gu.rec + ((size_t)(pid >> gu.shift) * G_KREC + k) * 4;
+      r[0] = make_float4(p.x, p.y, p.z, prev_pdf);
+      r[1] = make_float4(nl.x, nl.y, nl.z, 1.0f);
+      r[2] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+    }
     uint32_t pos = wave_append(ncount, cont);
     if (cont) {
       nray[2 * pos] = make_float4(no.x, no.y, no.z, __uint_as_float(k + 1));
@@ -756,6 +873,41 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, DevFrame fr, const f
       prev_s[pid] = make_float4(prev_pos.x, prev_pos.y, prev_pos.z, 0.0f);
       sigma_s[pid] = sg;
       meta_s[pid] = make_uint4(meta.x, meta.y, ((k + (cont ? 1u : 0u)) & 0xffffu) | (hero << 16) | (flags << 24), ps);
+    }
+  }
+}
+
+// Guide training samples of the recorded paths, once their radiance is
+// final: incident radiance estimate L_i = (L_final - L_mark) / beta per
+// channel, target = luminance(max(L_i, 0)), weight = target / pdf.
+__global__ void k_guide_collect(uint32_t P, uint32_t nsel, uint32_t shift, const float4* __restrict__ rec,
+                                const float4* __restrict__ out, float4* __restrict__ samp, uint32_t* count,
+                                uint32_t cap) {
+  uint32_t sp = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t pid = sp << shift;
+  bool live = sp < nsel && pid < P;
+  float4 fin = live ? out[pid] : make_float4(0, 0, 0, 0);
+  for (int k = 0; k < G_KREC; ++k) {
+    bool emit = false;
+    float4 r0, r1;
+    float w = 0.0f;
+    if (live) {
+      const float4* r = rec + ((size_t)sp * G_KREC + k) * 4;
+      r1 = r[1];
+      if (r1.w == 1.0f) {
+        r0 = r[0];
+        float4 b = r[2], lm = r[3];
+        float li[3] = {b.x > 0.0f ? (fin.x - lm.x) / b.x : 0.0f, b.y > 0.0f ? (fin.y - lm.y) / b.y : 0.0f,
+                       b.z > 0.0f ? (fin.z - lm.z) / b.z : 0.0f};
+        float t = luminance(v3(fmaxr(li[0], 0.0f), fmaxr(li[1], 0.0f), fmaxr(li[2], 0.0f)));
+        w = t / r0.w;
+        emit = isfinite(w) && w > 0.0f;
+      }
+    }
+    uint32_t pos = wave_append(count, emit);
+    if (emit && pos < cap) {
+      samp[2ull * pos] = make_float4(r0.x, r0.y, r0.z, w);
+      samp[2ull * pos + 1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
     }
   }
 }
@@ -985,7 +1137,16 @@ __global__ void k_bsdf(uint32_t n, int do_sample, const float* __restrict__ ms_a
 namespace launch {
 
 hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, const float* srgb255) {
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sobol), sobol128, sizeof(uint32_t) * 128);
+  static uint32_t tab[3][4][256];
+  for (int d = 1; d < 4; ++d)
+    for (int by = 0; by < 4; ++by)
+      for (int v = 0; v < 256; ++v) {
+        uint32_t x = 0;
+        for (int j = 0; j < 8; ++j)
+          if ((v >> j) & 1) x ^= sobol128[d * 32 + by * 8 + j];
+        tab[d - 1][by][v] = x;
+      }
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sobol_tab), tab, sizeof(tab));
   if (e != hipSuccess) return e;
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_sheen), sheen512, sizeof(float) * 512);
   if (e != hipSuccess) return e;
@@ -1003,43 +1164,62 @@ int trace_blocks_per_cu(bool any) {
 int trace_block_threads() { return TRACE_TB; }
 int stack_lds_entries() { return STACK; }
 
-void closest(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* qpid,
-             const uint4* meta, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
-             uint4* hits, uint32_t* spill, uint32_t spill_cap, uint32_t* err) {
+void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
+             const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
+             const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
+             uint32_t spill_cap, uint32_t* err) {
   if (path)
-    hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, qpid, meta, fixed_count, count_ptr,
-                       work, cull_all, hits, spill, spill_cap, err);
+    hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta, fixed_count,
+                       count_ptr, work, cull_all, hits, spill, spill_cap, err);
   else
-    hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, qpid, meta, fixed_count, count_ptr,
-                       work, cull_all, hits, spill, spill_cap, err);
+    hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta, fixed_count,
+                       count_ptr, work, cull_all, hits, spill, spill_cap, err);
 }
 
-void anyhit(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* sidx,
-            const uint4* meta, uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work,
-            uint8_t* out, uint32_t* spill, uint32_t spill_cap, uint32_t* err) {
+void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
+            const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
+            const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,
+            uint32_t* err) {
   if (path)
-    hipLaunchKernelGGL(k_anyhit<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, sidx, meta, nslots, fixed_count,
-                       count_ptr, work, out, spill, spill_cap, err);
+    hipLaunchKernelGGL(k_anyhit<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots,
+                       fixed_count, count_ptr, work, out, spill, spill_cap, err);
   else
-    hipLaunchKernelGGL(k_anyhit<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, rays, sidx, meta, nslots, fixed_count,
-                       count_ptr, work, out, spill, spill_cap, err);
+    hipLaunchKernelGGL(k_anyhit<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots,
+                       fixed_count, count_ptr, work, out, spill, spill_cap, err);
 }
 
-void camera(hipStream_t s, const DevFrame& fr, uint32_t P, uint32_t npx, uint32_t px0, uint32_t first,
-            const PathBuffers& pb, Ctrl* ctrl) {
-  hipLaunchKernelGGL(k_camera, dim3((P + 255) / 256), dim3(256), 0, s, fr, P, npx, px0, first, pb.qray[0], pb.qpid[0],
-                     pb.beta, pb.rad, pb.sigma, pb.meta, ctrl);
+void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl) {
+  hipLaunchKernelGGL(k_camera, dim3((P + 255) / 256), dim3(256), 0, s, jobs, segs, P, pb.qray[0], pb.qpid[0], pb.beta,
+                     pb.rad, pb.sigma, pb.meta, pb.pjob, ctrl);
 }
 
 void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
   hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, ctrl, cur, count_shadow_prev);
 }
 
-void shade(hipStream_t s, int grid, const DevScene& sc, const DevFrame& fr, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, sc, fr, pb.qray[cur], pb.qpid[cur], pb.hits,
-                     pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, pb.prev, pb.sigma, pb.meta,
-                     pb.nee, pb.vis, ctrl, cur);
+void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
+           uint32_t cur, const DevGuideUse* guide) {
+  DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u};
+#define OXR_SHADE(W, G)                                                                                          \
+  hipLaunchKernelGGL((k_shade<W, G>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],             \
+                     pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
+                     pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
+  if (guide)
+    OXR_SHADE(1, true);
+  else if (waves >= 3)
+    OXR_SHADE(3, false);
+  else if (waves == 2)
+    OXR_SHADE(2, false);
+  else
+    OXR_SHADE(1, false);
+#undef OXR_SHADE
+}
+
+void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
+                   uint32_t* count, uint32_t cap) {
+  uint32_t nsel = (uint32_t)(((uint64_t)P + (1ull << gu.shift) - 1ull) >> gu.shift);
+  hipLaunchKernelGGL(k_guide_collect, dim3((nsel + 255) / 256), dim3(256), 0, s, P, nsel, gu.shift, gu.rec, pb.out,
+                     samp, count, cap);
 }
 
 void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb) {

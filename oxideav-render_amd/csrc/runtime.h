@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dev_guide.h"
 #include "dev_scene.h"
 
 namespace oxr {
@@ -24,6 +25,7 @@ struct PathBuffers {
   float4* nee = nullptr;    // cap * nslots contributions
   uint8_t* vis = nullptr;   // cap * nslots: 0 none, 1 pending, 2 visible
   float4* out = nullptr;    // per-sample radiance + covered flag
+  uint32_t* pjob = nullptr; // job index per path
   uint32_t cap = 0;
   uint32_t nslots = 0;
 };
@@ -33,17 +35,23 @@ hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, con
 int trace_blocks_per_cu(bool any);
 int trace_block_threads();
 int stack_lds_entries();
-void closest(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* qpid,
-             const uint4* meta, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
-             uint4* hits, uint32_t* spill, uint32_t spill_cap, uint32_t* err);
-void anyhit(hipStream_t s, int grid, const DevScene& sc, bool path, const float4* rays, const uint32_t* sidx,
-            const uint4* meta, uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work,
-            uint8_t* out, uint32_t* spill, uint32_t spill_cap, uint32_t* err);
-void camera(hipStream_t s, const DevFrame& fr, uint32_t P, uint32_t npx, uint32_t px0, uint32_t first,
-            const PathBuffers& pb, Ctrl* ctrl);
+void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
+             const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
+             const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
+             uint32_t spill_cap, uint32_t* err);
+void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
+            const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
+            const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,
+            uint32_t* err);
+void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl);
 void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev);
-void shade(hipStream_t s, int grid, const DevScene& sc, const DevFrame& fr, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur);
+// guide: null for the plain estimator.
+void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
+           uint32_t cur, const DevGuideUse* guide);
+// Training samples (2 float4 each) of the guide-recorded paths of a batch,
+// appended at *count (entries past cap are dropped).
+void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
+                   uint32_t* count, uint32_t cap);
 void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb);
 void reduce(hipStream_t s, uint32_t npx, uint32_t S, const float4* out, double* accum, double* sumsq);
 void fill_f64(hipStream_t s, double* p, size_t n);

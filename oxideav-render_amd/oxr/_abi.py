@@ -224,6 +224,57 @@ class BvhInfo(C.Structure):
     ]
 
 
+class JobStatus(C.Structure):
+    """oxr_job_status."""
+    _fields_ = [("state", C.c_uint32), ("samples", C.c_uint32), ("target", C.c_uint32), ("rounds", C.c_uint32),
+                ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+
+class SrvStats(C.Structure):
+    """oxr_srv_stats."""
+    _fields_ = [("rounds", C.c_uint64), ("paths", C.c_uint64), ("jobs_done", C.c_uint64), ("active", C.c_uint32),
+                ("queued", C.c_uint32), ("bytes_resident", C.c_uint64), ("busy_ms", C.c_double)]
+
+
+OXR_JOB_QUEUED = 0
+OXR_JOB_ACTIVE = 1
+OXR_JOB_DONE = 2
+
+
+GUIDE_LEVELS = 16
+GUIDE_FEATURES = 2
+GUIDE_HIDDEN = 64
+GUIDE_OUTPUTS = 48
+GUIDE_LOBES = 8
+GUIDE_MLP_PARAMS = 64 * 32 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 48 * 64 + 48
+
+
+class GuideConfig(C.Structure):
+    """oxr_guide_config."""
+    _fields_ = [("log2_table", C.c_uint32), ("base_res", C.c_uint32), ("max_res", C.c_uint32), ("seed", C.c_uint32),
+                ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("mix", C.c_float), ("batch", C.c_uint32),
+                ("steps", C.c_uint32), ("record_shift", C.c_uint32)]
+
+
+class GuideStats(C.Structure):
+    """oxr_guide_stats."""
+    _fields_ = [("steps", C.c_uint64), ("samples", C.c_uint64), ("collected", C.c_uint64), ("last_loss", C.c_float),
+                ("pad", C.c_float), ("train_ms", C.c_double), ("infer_ms", C.c_double)]
+
+
+def guide_config(log2_table=19, base_res=16, max_res=2048, seed=0, aabb_min=(0.0, 0.0, 0.0), aabb_max=(0.0, 0.0, 0.0),
+                 lr=5e-3, beta1=0.9, beta2=0.99, eps=1e-15, mix=0.5, batch=65536, steps=4, record_shift=3):
+    """oxr_guide_default_config with overrides."""
+    c = GuideConfig()
+    c.log2_table, c.base_res, c.max_res, c.seed = log2_table, base_res, max_res, seed
+    c.aabb_min = f3(*aabb_min)
+    c.aabb_max = f3(*aabb_max)
+    c.lr, c.beta1, c.beta2, c.eps, c.mix = lr, beta1, beta2, eps, mix
+    c.batch, c.steps, c.record_shift = batch, steps, record_shift
+    return c
+
+
 class MaterialSample(C.Structure):
     _fields_ = [
         ("base_color", f3),

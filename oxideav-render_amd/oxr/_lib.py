@@ -57,6 +57,28 @@ def lib():
         "oxr_pt_bvh_export": (C.c_int, [vp, vp, vp]),
         "oxr_bsdf_eval": (C.c_int, [C.c_int, u32, C.POINTER(A.MaterialSample), fp, fp, bp, fp, fp, fp]),
         "oxr_bsdf_sample": (C.c_int, [C.c_int, u32, C.POINTER(A.MaterialSample), fp, fp, bp, fp, fp, fp, fp, bp]),
+        "oxr_srv_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
+        "oxr_srv_destroy": (C.c_int, [vp]),
+        "oxr_srv_add_scene": (C.c_int, [vp, C.POINTER(A.SceneDesc), up]),
+        "oxr_srv_submit": (C.c_int, [vp, u32, C.POINTER(A.Camera), C.POINTER(A.PtOptions), up]),
+        "oxr_srv_step": (C.c_int, [vp, up, up]),
+        "oxr_srv_job_status": (C.c_int, [vp, u32, C.POINTER(A.JobStatus)]),
+        "oxr_srv_read_accum": (C.c_int, [vp, u32, dp, dp]),
+        "oxr_srv_release": (C.c_int, [vp, u32]),
+        "oxr_srv_stats_get": (C.c_int, [vp, C.POINTER(A.SrvStats)]),
+        "oxr_guide_default_config": (C.c_int, [C.POINTER(A.GuideConfig)]),
+        "oxr_guide_create": (C.c_int, [C.c_int, C.POINTER(A.GuideConfig), C.POINTER(vp)]),
+        "oxr_guide_destroy": (C.c_int, [vp]),
+        "oxr_guide_num_params": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "oxr_guide_get_params": (C.c_int, [vp, fp, fp]),
+        "oxr_guide_set_params": (C.c_int, [vp, fp, fp]),
+        "oxr_guide_infer": (C.c_int, [vp, u32, fp, fp, fp]),
+        "oxr_guide_pdf": (C.c_int, [vp, u32, fp, fp, fp]),
+        "oxr_guide_sample": (C.c_int, [vp, u32, fp, fp, fp, fp]),
+        "oxr_guide_gradient": (C.c_int, [vp, u32, fp, fp, fp, fp, fp, dp]),
+        "oxr_guide_train": (C.c_int, [vp, u32, fp, fp, fp, u32, dp]),
+        "oxr_guide_stats_get": (C.c_int, [vp, C.POINTER(A.GuideStats)]),
+        "oxr_pt_set_guide": (C.c_int, [vp, vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -72,6 +94,11 @@ EXPORTED = [
     "oxr_pt_read_accum", "oxr_pt_resolve_hdr", "oxr_pt_resolve_rgba8", "oxr_pt_set_timing", "oxr_pt_stats_get",
     "oxr_sobol_owen_4d", "oxr_trace_closest", "oxr_trace_any", "oxr_trace_closest_device", "oxr_pt_bvh_info",
     "oxr_pt_bvh_export", "oxr_bsdf_eval", "oxr_bsdf_sample",
+    "oxr_srv_create", "oxr_srv_destroy", "oxr_srv_add_scene", "oxr_srv_submit", "oxr_srv_step",
+    "oxr_srv_job_status", "oxr_srv_read_accum", "oxr_srv_release", "oxr_srv_stats_get",
+    "oxr_guide_default_config", "oxr_guide_create", "oxr_guide_destroy", "oxr_guide_num_params",
+    "oxr_guide_get_params", "oxr_guide_set_params", "oxr_guide_infer", "oxr_guide_pdf", "oxr_guide_sample",
+    "oxr_guide_gradient", "oxr_guide_train", "oxr_guide_stats_get", "oxr_pt_set_guide",
 ]
 
 

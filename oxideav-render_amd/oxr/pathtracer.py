@@ -162,6 +162,14 @@ class PathTracer:
                                            out.ctypes.data_as(C.POINTER(C.c_uint8))), "resolve_rgba8")
         return out
 
+    # -- learned path guide ----------------------------------------------------
+    def set_guide(self, guide, train=True):
+        """Attach an oxr.guide.Guide (None detaches); resets the
+        accumulation. With train=True every refine pass also trains it."""
+        self._guide = guide  # keep it alive while attached
+        check(self._L.oxr_pt_set_guide(self._h, guide.handle if guide is not None else None, 1 if train else 0),
+              "set_guide")
+
     # -- instrumentation -----------------------------------------------------
     def set_timing(self, enable=True):
         check(self._L.oxr_pt_set_timing(self._h, 1 if enable else 0), "set_timing")

@@ -26,6 +26,12 @@ import numpy as np
 from . import _abi as A
 
 F32 = np.float32
+MATERIAL_DTYPE = np.dtype(A.Material)
+
+
+def material_array(n, template):
+    """n copies of the A.Material `template` as a structured array."""
+    return np.repeat(np.frombuffer(bytes(template), dtype=MATERIAL_DTYPE), n)
 
 
 def _normalize(v):
@@ -89,7 +95,31 @@ class Scene:
     # -- building ----------------------------------------------------------
     def add_material(self, m):
         self.materials.append(m)
-        return len(self.materials) - 1
+        return self.num_materials - 1
+
+    def add_material_array(self, arr):
+        """Append a numpy array of dtype MATERIAL_DTYPE; returns the index
+        of its first entry."""
+        first = self.num_materials
+        self.materials.append(np.ascontiguousarray(arr, dtype=MATERIAL_DTYPE))
+        return first
+
+    @property
+    def num_materials(self):
+        return sum(len(m) if isinstance(m, np.ndarray) else 1 for m in self.materials)
+
+    def material_table(self):
+        """All materials as one MATERIAL_DTYPE array."""
+        parts = [m if isinstance(m, np.ndarray) else np.frombuffer(bytes(m), dtype=MATERIAL_DTYPE)
+                 for m in self.materials]
+        if not parts:
+            return np.zeros(0, MATERIAL_DTYPE)
+        return np.ascontiguousarray(np.concatenate(parts))
+
+    def material(self, i):
+        """Material i as an A.Material."""
+        t = self.material_table()
+        return A.Material.from_buffer_copy(t[i:i + 1].tobytes())
 
     def add_tris(self, pos, mat, normals=None):
         """pos: (n, 9) corners; normals: (n, 9) or None (geometric)."""
@@ -147,10 +177,14 @@ class Scene:
         d.normals = nrm.ctypes.data_as(fp) if any_n else fp()
         d.tri_flags = hn.ctypes.data_as(up) if any_n else up()
         d.material_ids = mid.ctypes.data_as(up)
-        mats = (A.Material * max(len(self.materials), 1))(*self.materials)
+        mats = self.material_table()
+        if len(mats) == 0:
+            mats = np.zeros(1, MATERIAL_DTYPE)
+            d.num_materials = 0
+        else:
+            d.num_materials = len(mats)
         lights = (A.Light * max(len(self.lights), 1))(*self.lights)
-        d.num_materials = len(self.materials)
-        d.materials = C.cast(mats, C.POINTER(A.Material))
+        d.materials = mats.ctypes.data_as(C.POINTER(A.Material))
         d.num_lights = len(self.lights)
         d.lights = C.cast(lights, C.POINTER(A.Light))
         d._keep = (pos, nrm, hn, mid, mats, lights)
@@ -504,3 +538,149 @@ def reference_box():
     s.camera = CameraSpec((0.0, 0.0, 3.4), (0.0, 0.0, 0.0), yfov_deg=math.degrees(0.75))
     s.ambient = 0.0
     return s
+
+
+# ---------------------------------------------------------------------------
+# Large-scene throughput: displaced terrain + emitters + instanced spheres.
+# ---------------------------------------------------------------------------
+
+def _hash2(ix, iz, seed):
+    """pcg-based lattice hash -> [0, 1) (uint32 arithmetic, vectorised)."""
+    with np.errstate(over="ignore"):
+        x = (ix.astype(np.uint32) * np.uint32(73856093)) ^ (iz.astype(np.uint32) * np.uint32(19349663)) ^ np.uint32(seed)
+        state = (x * np.uint32(747796405) + np.uint32(2891336453)).astype(np.uint32)
+        word = (((state >> ((state >> np.uint32(28)) + np.uint32(4))) ^ state) * np.uint32(277803737)).astype(np.uint32)
+        h = ((word >> np.uint32(22)) ^ word).astype(np.uint32)
+    return (h >> np.uint32(8)).astype(np.float64) * (1.0 / 16777216.0)
+
+
+def fbm(x, z, seed, octaves=6, lacunarity=2.0, gain=0.5, base_freq=4.0):
+    """Value-noise fBm in [-1, 1]-ish (smoothstep-interpolated lattice)."""
+    out = np.zeros_like(x, dtype=np.float64)
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for o in range(octaves):
+        fx, fz = x * freq, z * freq
+        ix, iz = np.floor(fx), np.floor(fz)
+        tx, tz = fx - ix, fz - iz
+        tx = tx * tx * (3 - 2 * tx)
+        tz = tz * tz * (3 - 2 * tz)
+        ix = ix.astype(np.int64)
+        iz = iz.astype(np.int64)
+        s = seed + 1013 * o
+        a = _hash2(ix, iz, s)
+        b = _hash2(ix + 1, iz, s)
+        c = _hash2(ix, iz + 1, s)
+        d = _hash2(ix + 1, iz + 1, s)
+        v = (a * (1 - tx) + b * tx) * (1 - tz) + (c * (1 - tx) + d * tx) * tz
+        out += amp * (2.0 * v - 1.0)
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return out / norm
+
+
+def icosphere1():
+    """Subdivision-1 icosphere: 80 triangles on the unit sphere (CCW out),
+    with smooth (radial) corner normals."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    tris = []
+    for a, b, c in f:
+        A, B, Cc = v[a], v[b], v[c]
+        ab = (A + B) / np.linalg.norm(A + B)
+        bc = (B + Cc) / np.linalg.norm(B + Cc)
+        ca = (Cc + A) / np.linalg.norm(Cc + A)
+        tris += [[A, ab, ca], [B, bc, ab], [Cc, ca, bc], [ab, bc, ca]]
+    tris = np.array(tris, np.float64)  # (80, 3, 3)
+    # Orient CCW outward.
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    flip = (n * tris.mean(axis=1)).sum(1) < 0
+    tris[flip] = tris[flip][:, ::-1]
+    return tris
+
+
+def terrain(seed=7, grid=2048, n_emit=10000, n_spheres=50000, amplitude=0.1):
+    """Procedural displaced terrain over [0, 1]^2: a grid x grid heightfield
+    (2·grid² tris, (grid+1)² vertices) with 6-octave fBm heights of the
+    given amplitude; n_emit emissive triangles (double-sided, radiance
+    log-uniform 1..100, edge ~4 mm) scattered uniformly in the volume above
+    it; n_spheres instanced 80-tri icospheres (radius 2-6 mm, resting on the
+    surface) whose materials are drawn 25 % each from thin transmission
+    (IOR 1.5), clearcoat, sheen and GGX metal (roughness U(0.05, 0.5)).
+    Defaults give 8,388,608 + 10,000 + 4,000,000 = 12,398,608 tris."""
+    s = Scene("terrain")
+    g = grid
+    xs = np.linspace(0.0, 1.0, g + 1)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    H = amplitude * fbm(X, Z, seed)
+    P = np.stack([X, H, Z], -1).astype(np.float32)  # (g+1, g+1, 3)
+    p00 = P[:-1, :-1].reshape(-1, 3)
+    p10 = P[:-1, 1:].reshape(-1, 3)
+    p01 = P[1:, :-1].reshape(-1, 3)
+    p11 = P[1:, 1:].reshape(-1, 3)
+    # CCW seen from +Y: (p00, p01, p11), (p00, p11, p10).
+    t1 = np.concatenate([p00, p01, p11], 1)
+    t2 = np.concatenate([p00, p11, p10], 1)
+    tris = np.empty((2 * len(t1), 9), np.float32)
+    tris[0::2] = t1
+    tris[1::2] = t2
+    ground = s.add_material(A.material((0.45, 0.5, 0.35, 1.0), metallic=0.0, roughness=0.8))
+    s.add_tris(tris, ground)
+    del tris, t1, t2
+    u = pcg_uniform(seed * 7919 + 1, n_emit * 9)
+    ctr = np.stack([u[:n_emit], 0.05 + 0.25 * u[n_emit:2 * n_emit], u[2 * n_emit:3 * n_emit]], 1)
+    off = (u[3 * n_emit:9 * n_emit].reshape(n_emit, 2, 3) - 0.5) * 0.008
+    em = np.concatenate([ctr, ctr + off[:, 0], ctr + off[:, 1]], 1).astype(np.float32)
+    ue = pcg_uniform(seed * 7919 + 2, n_emit)
+    le = np.exp(np.log(1.0) + ue * (np.log(100.0) - np.log(1.0)))
+    # One material per emitter radiance (log-uniform 1..100).
+    emat = material_array(n_emit, lambert((0.0, 0.0, 0.0), double_sided=True))
+    emat["emissive"] = np.repeat(le[:, None], 3, 1).astype(np.float32)
+    mids = (s.add_material_array(emat) + np.arange(n_emit)).astype(np.uint32)
+    s.add_tris(em, mids)
+    ico = icosphere1()  # (80, 3, 3) unit
+    us = pcg_uniform(seed * 7919 + 3, n_spheres * 6)
+    cx = us[:n_spheres]
+    cz = us[n_spheres:2 * n_spheres]
+    r = 0.002 + 0.004 * us[2 * n_spheres:3 * n_spheres]
+    cy = amplitude * fbm(cx, cz, seed) + r
+    kind = (us[3 * n_spheres:4 * n_spheres] * 4).astype(int).clip(0, 3)
+    p1 = us[4 * n_spheres:5 * n_spheres]
+    p2 = us[5 * n_spheres:6 * n_spheres]
+    sm = material_array(n_spheres, A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.5))
+    t0, t1, t2, t3 = (kind == 0), (kind == 1), (kind == 2), (kind == 3)
+    bc = np.ones((n_spheres, 4), np.float32)
+    bc[t0, :3] = (0.95, 0.97, 1.0)
+    bc[t1, 0], bc[t1, 1], bc[t1, 2] = 0.2 + 0.6 * p1[t1], 0.2, 0.6 * p2[t1]
+    bc[t2, 0], bc[t2, 1], bc[t2, 2] = 0.3, 0.1 + 0.3 * p2[t2], 0.1
+    bc[t3, 0], bc[t3, 1], bc[t3, 2] = 0.95, 0.64 + 0.3 * p2[t3], 0.54
+    sm["base_color"] = bc
+    rough = np.full(n_spheres, 0.5, np.float32)
+    rough[t0] = 0.05 + 0.2 * p1[t0]
+    rough[t1] = 0.6
+    rough[t2] = 0.8
+    rough[t3] = 0.05 + 0.45 * p1[t3]
+    sm["roughness"] = rough
+    sm["metallic"] = np.where(t3, 1.0, 0.0).astype(np.float32)
+    sm["transmission"] = np.where(t0, 1.0, 0.0).astype(np.float32)
+    sm["clearcoat"] = np.where(t1, 1.0, 0.0).astype(np.float32)
+    sm["clearcoat_roughness"] = np.where(t1, 0.1, 0.0).astype(np.float32)
+    sh = np.zeros((n_spheres, 3), np.float32)
+    sh[t2, 0], sh[t2, 1], sh[t2, 2] = 0.9, 0.7 * p1[t2], 0.7
+    sm["sheen_color"] = sh
+    sm["sheen_roughness"] = np.where(t2, 0.3 + 0.5 * p2, 0.0).astype(np.float32)
+    mids = (s.add_material_array(sm) + np.arange(n_spheres)).astype(np.uint32)
+    pos = (ico[None] * r[:, None, None, None] + np.stack([cx, cy, cz], 1)[:, None, None, :]).astype(np.float32)
+    nrm = np.broadcast_to(ico[None], pos.shape).astype(np.float32)
+    s.add_tris(pos.reshape(-1, 9), np.repeat(mids, 80), normals=nrm.reshape(-1, 9))
+    s.camera = CameraSpec((0.5, 0.45, 1.25), (0.5, 0.0, 0.45), yfov_deg=45.0)
+    s.ambient = 0.0
+    return s
+
+
+WORKLOADS["terrain"] = terrain

@@ -1,0 +1,222 @@
+"""The learned path guide on the GPU against its CPU oracle
+(oracle/guide_cpu.cpp): MFMA inference, mixture pdf and sampling, the
+loss gradient (hash grid + MLP), the Adam step; then, inside the path
+tracer, that a guide with mix 0 changes nothing, that guided renders
+converge to the plain estimator's image, and what training gains."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oxr import _abi as A
+
+pytestmark = pytest.mark.gpu
+
+
+def small_cfg(**kw):
+    base = dict(log2_table=14, base_res=4, max_res=256, aabb_min=(-1.0, -1.0, -1.0), aabb_max=(1.0, 1.0, 1.0),
+                seed=3)
+    base.update(kw)
+    return A.guide_config(**base)
+
+
+def rand_params(cfg, seed=1, grid_scale=0.5):
+    rng = np.random.default_rng(seed)
+    grid = rng.uniform(-grid_scale, grid_scale, O.guide_num_grid(cfg)).astype(np.float32)
+    mlp = np.zeros(A.GUIDE_MLP_PARAMS, np.float32)
+    off = 0
+    for nout, nin in [(64, 32), (64, 64), (64, 64), (48, 64)]:
+        lim = math.sqrt(6.0 / (nin + nout))
+        mlp[off:off + nout * nin] = rng.uniform(-lim, lim, nout * nin)
+        off += nout * nin
+        mlp[off:off + nout] = rng.uniform(-0.1, 0.1, nout)
+        off += nout
+    return grid, mlp
+
+
+def unit(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v, axis=-1, keepdims=True)).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def guide_and_params():
+    from oxr.guide import Guide
+    cfg = small_cfg()
+    g = Guide(0, config=cfg)
+    grid, mlp = rand_params(cfg)
+    g.set_params(grid, mlp)
+    return cfg, g, grid, mlp
+
+
+def test_params_roundtrip(guide_and_params):
+    cfg, g, grid, mlp = guide_and_params
+    gg, gm = g.params()
+    assert np.array_equal(gg, grid) and np.array_equal(gm, mlp)
+
+
+def test_inference_matches_oracle(guide_and_params):
+    cfg, g, grid, mlp = guide_and_params
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-1.1, 1.1, (4099, 3)).astype(np.float32)  # ragged: not a multiple of 16
+    raw, lobes = g.infer(pos)
+    _, want = O.guide_forward(cfg, grid, mlp, pos, mode=1)
+    err = np.abs(raw - want)
+    scale = np.abs(want).max()
+    # Different f32 summation order inside the MFMA; rare bf16 rounding
+    # flips of an activation move an output by ~2^-8 of its scale.
+    assert np.median(err) <= 1e-5 * scale
+    assert err.max() <= 2e-2 * scale
+    want_lobes = O.guide_lobes(raw)
+    np.testing.assert_allclose(lobes, want_lobes, rtol=1e-4, atol=1e-5)
+
+
+def test_pdf_and_sampling_match_oracle(guide_and_params):
+    cfg, g, grid, mlp = guide_and_params
+    rng = np.random.default_rng(5)
+    n = 2048
+    pos = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    _, lobes = g.infer(pos)
+    dirs = unit(rng.normal(size=(n, 3)))
+    pdf = g.pdf(pos, dirs)
+    np.testing.assert_allclose(pdf, O.guide_pdf(lobes, dirs), rtol=2e-4, atol=1e-7)
+    u = rng.random((n, 3)).astype(np.float32)
+    d, p = g.sample(pos, u)
+    want = O.guide_sample(lobes, u)
+    np.testing.assert_allclose(d, want, atol=2e-4)
+    np.testing.assert_allclose(p, O.guide_pdf(lobes, d), rtol=2e-4, atol=1e-7)
+
+
+def test_gradient_matches_oracle(guide_and_params):
+    cfg, g, grid, mlp = guide_and_params
+    rng = np.random.default_rng(6)
+    n = 1000  # padded to 1024 with zero-weight samples on the device
+    pos = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    dirs = unit(rng.normal(size=(n, 3)))
+    w = rng.uniform(0.0, 2.0, n).astype(np.float32)
+    loss, gg, gm = g.gradient(pos, dirs, w)
+    l_o, gg_o, gm_o = O.guide_loss_grad(cfg, grid, mlp, pos, dirs, w, mode=1)
+    assert abs(loss - l_o) <= 1e-4 * abs(l_o)
+    assert np.linalg.norm(gm - gm_o) <= 2e-2 * np.linalg.norm(gm_o)
+    assert np.linalg.norm(gg - gg_o) <= 2e-2 * np.linalg.norm(gg_o)
+    # The last layer carries only one rounding: tight.
+    nl = 64 * 32 + 64 + 2 * (64 * 64 + 64)
+    assert np.linalg.norm(gm[nl:] - gm_o[nl:]) <= 2e-3 * np.linalg.norm(gm_o[nl:])
+    # Entries the batch never touches stay exactly zero.
+    assert np.array_equal(gg == 0, gg_o == 0)
+    # A second call sees a clean gradient buffer.
+    loss2, gg2, gm2 = g.gradient(pos, dirs, w)
+    assert np.allclose(gg2, gg, rtol=1e-5, atol=1e-9) and np.allclose(gm2, gm, rtol=1e-5, atol=1e-9)
+
+
+def test_adam_step_matches_oracle():
+    from oxr.guide import Guide
+    cfg = small_cfg(lr=1e-3, eps=1e-8)
+    g = Guide(0, config=cfg)
+    grid, mlp = rand_params(cfg, seed=11)
+    g.set_params(grid, mlp)
+    rng = np.random.default_rng(7)
+    n = 512
+    pos = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    dirs = unit(rng.normal(size=(n, 3)))
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    _, gg, gm = O.guide_loss_grad(cfg, grid, mlp, pos, dirs, w, mode=1)
+    g.train(pos, dirs, w, steps=1)
+    ng, nm = g.params()
+    # Oracle Adam on the oracle gradient; compare where the gradient is
+    # well above the device/oracle difference (first step: +-lr).
+    for p, gr, lazy, new in ((grid.copy(), gg, True, ng), (mlp.copy(), gm, False, nm)):
+        m = np.zeros_like(p)
+        v = np.zeros_like(p)
+        O.guide_adam(p, m, v, gr.astype(np.float32), 1, cfg.lr, cfg.beta1, cfg.beta2, cfg.eps, lazy)
+        sig = np.abs(gr) > 1e-3 * np.abs(gr).max()
+        assert sig.sum() > 100
+        np.testing.assert_allclose(new[sig], p[sig], atol=2e-6)
+        if lazy:
+            untouched = gr == 0
+            assert np.array_equal(new[untouched], p[untouched])
+
+
+def test_training_learns_a_direction_field():
+    """Synthetic target: at x, radiance arrives from the direction of a
+    fixed point; the trained mixture concentrates on it."""
+    from oxr.guide import Guide
+    cfg = small_cfg(lr=1e-2, seed=5)
+    g = Guide(0, config=cfg)
+    rng = np.random.default_rng(8)
+    src = np.array([0.3, 0.9, -0.2], np.float32)
+    n = 8192
+    pos = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    to = unit(src - pos)
+    # Directions: a tight cone around `to` (weight 1) plus uniform ones
+    # (weight 0: no radiance).
+    jitter = unit(to + 0.05 * rng.normal(size=(n, 3)))
+    uni = unit(rng.normal(size=(n, 3)))
+    P = np.concatenate([pos, pos])
+    D = np.concatenate([jitter, uni])
+    W = np.concatenate([np.ones(n), np.zeros(n)]).astype(np.float32)
+    test = rng.uniform(-0.8, 0.8, (256, 3)).astype(np.float32)
+    before = g.pdf(test, unit(src - test))
+    l0 = g.train(P, D, W, steps=1)
+    l1 = g.train(P, D, W, steps=300)
+    after = g.pdf(test, unit(src - test))
+    assert l1 < l0 - 2.0
+    assert np.median(after) > 20.0 * np.median(before)
+    assert np.median(after) > 5.0  # a uniform sphere is 1 / (4 pi) = 0.08
+
+
+# ---------------------------------------------------------------------------
+# Inside the tracer.
+# ---------------------------------------------------------------------------
+def _render(scene, W, H, spp, passes, guide=None, train=True, bounces=6, seed=0):
+    import oxr
+    pt = oxr.PathTracer(0)
+    o = oxr.RenderOptions(width=W, height=H, ambient=scene.ambient)
+    o.path_trace.samples_per_pixel = spp * passes
+    o.path_trace.max_bounces = bounces
+    o.path_trace.seed = seed
+    pt.sync(scene, o)
+    if guide is not None:
+        pt.set_guide(guide, train=train)
+    for _ in range(passes):
+        pt.refine(spp)
+    acc, sq = pt.accum(with_sum_sq=True)
+    pt.close()
+    return acc, sq
+
+
+def test_mix_zero_guide_is_the_plain_estimator():
+    from oxr import scenes as S
+    from oxr.guide import Guide
+    sc = S.glossy_caustic(segments=24, rings=16)
+    plain, psq = _render(sc, 48, 40, 4, 2)
+    g = Guide(0, mix=0.0, log2_table=16, steps=2, batch=4096)
+    guided, gsq = _render(sc, 48, 40, 4, 2, guide=g)
+    assert np.array_equal(plain, guided) and np.array_equal(psq, gsq)
+    assert g.stats()["steps"] == 0  # mix 0: no eligible vertex, no records
+
+
+def _mean_sigma(acc, sq):
+    n = acc[..., 3:4]
+    mean = acc[..., :3] / n
+    var = np.maximum(sq / n - mean ** 2, 0.0) / np.maximum(n - 1, 1)
+    return mean, var
+
+
+def test_guided_render_converges_to_plain():
+    from oxr import scenes as S
+    from oxr.guide import Guide
+    sc = S.glossy_caustic(segments=32, rings=24)
+    W = H = 64
+    plain_acc, plain_sq = _render(sc, W, H, 64, 8, seed=1)
+    g = Guide(0, log2_table=18, batch=16384, steps=8, record_shift=0)
+    guided_acc, guided_sq = _render(sc, W, H, 64, 8, guide=g, seed=2)
+    assert g.stats()["steps"] == 64 and g.stats()["collected"] > 0
+    mp, vp = _mean_sigma(plain_acc, plain_sq)
+    mg, vg = _mean_sigma(guided_acc, guided_sq)
+    z = (mg - mp) / np.sqrt(vp + vg + 1e-30)
+    # Unbiased: image means agree, per-pixel errors look Gaussian.
+    tot = (mg.mean() - mp.mean()) / math.sqrt((vp.sum() + vg.sum()) / mp.size ** 2)
+    assert abs(tot) < 4.0, tot
+    assert np.mean(np.abs(z) > 4.0) < 0.005
