@@ -14,6 +14,7 @@ restatement timed on this host's cores on a bounded sample.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -83,6 +84,101 @@ def cpu_baseline(scene, cfg, ambient, target_s=12.0):
                       f"({paths} paths, {dt:.1f} s, band-parallel f64 accumulate)"}
 
 
+def serving_bench(n=512, lam=40.0, seed=0, n_soups=4):
+    """The serving mix: n requests with Poisson arrivals at rate lam, each
+    drawing the Cornell box, the material sweep or one of n_soups seeded
+    triangle soups (N log-uniform 1k..2M), resolution in {128..1024}^2, spp
+    in {4, 16, 64, 256}, 4..8 bounces; all scenes resident before the
+    first arrival. Returns throughput and p50/p99 latency."""
+    from oxr import scenes
+    from oxr.serve import Server, make_trace, run_requests
+    rng = np.random.default_rng(seed)
+    soups = [scenes.triangle_soup(1000 + k, int(round(10 ** rng.uniform(3.0, math.log10(2e6)))), n_emit=1 + k)
+             for k in range(n_soups)]
+    sc = [scenes.cornell(), scenes.material_sweep()] + soups
+    srv = Server(0)
+    t = time.perf_counter()
+    ids = [srv.add_scene(x) for x in sc]
+    setup_s = time.perf_counter() - t
+    trace = make_trace(n=n, lam=lam, n_scenes=len(sc), seed=seed)
+    # Scene choice: C0, C1 and "a soup" equally likely; which soup uniform.
+    pick = rng.integers(0, 3, n)
+    which = rng.integers(0, n_soups, n)
+    for r, p_, w_ in zip(trace, pick, which):
+        r.scene = int(p_) if p_ < 2 else 2 + int(w_)
+    out = run_requests(srv, ids, sc, trace)
+    out["scene_setup_s"] = setup_s
+    out["soup_tris"] = [x.num_tris for x in soups]
+    out["sustained"] = out["jobs_per_s"] >= 0.95 * out["offered_rate"]
+    srv.close()
+    return out
+
+
+def guide_mse_bench(W=1024, target=1e-3, ref_spp=4096, pass_spp=4, max_s=40.0):
+    """Time-to-MSE on the glossy-caustic scene, plain vs guided (online
+    training included in the guided time). MSE is against an f64
+    reference of ref_spp plain samples, minus the reference's own
+    estimated variance (an unbiased estimate of the error vs the limit)."""
+    import oxr
+    from oxr import scenes
+    from oxr.guide import Guide
+    sc = scenes.glossy_caustic()
+
+    def opts(spp, seed):
+        o = oxr.RenderOptions(width=W, height=W, ambient=sc.ambient)
+        o.path_trace.samples_per_pixel = spp
+        o.path_trace.max_bounces = 8
+        o.path_trace.seed = seed
+        return o
+
+    ref = oxr.PathTracer(0)
+    ref.sync(sc, opts(ref_spp, 12345))
+    t = time.perf_counter()
+    for _ in range(ref_spp // 64):
+        ref.refine(64)
+    ref_s = time.perf_counter() - t
+    acc, sq = ref.accum(with_sum_sq=True)
+    ref.close()
+    R = acc[..., :3] / acc[..., 3:4]
+    var_ref = float((np.maximum(sq / acc[..., 3:4] - R ** 2, 0.0) / ref_spp).mean())
+
+    def run(guide):
+        pt = oxr.PathTracer(0)
+        pt.sync(sc, opts(1 << 20, 1))
+        g = None
+        if guide:
+            g = Guide(0)
+            pt.set_guide(g, train=True)
+        pt.refine(pass_spp)  # warm-up: buffers, graph capture
+        pt.reset()
+        render_s, spp, curve, hit = 0.0, 0, [], None
+        while render_s < max_s:
+            t = time.perf_counter()
+            pt.refine(pass_spp)
+            render_s += time.perf_counter() - t
+            spp += pass_spp
+            if spp & (spp - 1) == 0 or spp % 64 == 0:
+                a = pt.accum()
+                I = a[..., :3] / np.maximum(a[..., 3:4], 1.0)
+                mse = float(np.mean((I - R) ** 2)) - var_ref
+                curve.append((spp, round(render_s, 4), mse))
+                if mse <= target and hit is None:
+                    hit = (spp, render_s)
+                    break
+        st = g.stats() if g else None
+        pt.close()
+        return {"time_to_mse_s": hit[1] if hit else None, "spp": hit[0] if hit else None, "curve": curve,
+                "guide": st}
+
+    plain = run(False)
+    guided = run(True)
+    ratio = (guided["time_to_mse_s"] / plain["time_to_mse_s"]
+             if guided["time_to_mse_s"] and plain["time_to_mse_s"] else None)
+    return {"workload": f"glossy_caustic {W}x{W}, 8 bounces", "target_mse": target, "ref_spp": ref_spp,
+            "ref_s": ref_s, "ref_var_of_mean": var_ref, "plain": plain, "guided": guided,
+            "guided_over_plain_time": ratio}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size (HIP's copy kernel), for
     scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -110,9 +206,24 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-guide", action="store_true", help="skip the guided vs plain time-to-MSE leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency leg")
+    ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--guide-only", action="store_true")
+    ap.add_argument("--guide-width", type=int, default=1024)
+    ap.add_argument("--guide-ref-spp", type=int, default=4096)
+    ap.add_argument("--guide-max-s", type=float, default=40.0)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.serving_only or a.guide_only:
+        import oxr
+        if oxr.device_count() == 0:
+            raise SystemExit("bench.py: no HIP device visible")
+        print(json.dumps(serving_bench() if a.serving_only else
+                         guide_mse_bench(W=a.guide_width, ref_spp=a.guide_ref_spp, max_s=a.guide_max_s)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -132,22 +243,27 @@ def main():
     t = time.perf_counter()
     pt.sync(scene, opts)
     sync_s = time.perf_counter() - t
+    tune = pt.autotune(spp=min(a.spp_per_step, 8)) if not a.no_autotune else None
     for _ in range(a.warmup):
         pt.refine(a.spp_per_step)
     pt.reset()
-    pt.set_timing(True)
-    tot = dict(paths=0, closest_rays=0, shadow_rays=0, ms_closest=0.0, ms_anyhit=0.0, ms_shade=0.0, ms_other=0.0,
-               launches_closest=0, launches_anyhit=0)
+    # Timed region: kernels only (no per-kernel events), graphs replayed.
+    pt.set_timing(False)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         pt.refine(a.spp_per_step)
-        s = pt.stats()
-        for k in tot:
-            tot[k] += s[k]
     t1 = time.perf_counter()
     dt = t1 - t0
     paths = W * H * a.spp_per_step * a.steps
     value = paths / dt
+    # One more step with HIP events around every kernel (eager launches):
+    # the per-kernel breakdown and ray counts, scaled to the timed steps.
+    pt.set_timing(True)
+    pt.refine(a.spp_per_step)
+    s = pt.stats()
+    tot = {k: s[k] * a.steps for k in ("paths", "closest_rays", "shadow_rays", "ms_closest", "ms_anyhit",
+                                       "ms_shade", "ms_other")}
+    tot["launches_closest"] = s["launches_closest"] * a.steps
     rays = tot["closest_rays"] + tot["shadow_rays"]
     out = {
         "metric": METRIC,
@@ -172,16 +288,23 @@ def main():
                       "other": tot["ms_other"]},
         "scene_sync_s": sync_s,
         "bvh": pt.bvh_info(),
+        "autotune": tune,
+        "kernel_ms_per_step_source": "one extra step with per-kernel HIP events, x steps",
     }
     if not a.no_roofline and tot["launches_closest"]:
         bpr, n_node, n_tri, nr = bytes_per_closest_ray(pt, scene, cfg, ambient)
         avg_ms = tot["ms_closest"] / tot["launches_closest"]
         bytes_per_launch = bpr * tot["closest_rays"] / tot["launches_closest"]
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        # Measured HBM bytes per launch: the PMC figure per closest ray
+        # (FETCH_SIZE + WRITE_SIZE, copy-calibrated, profiles/) times this
+        # run's rays per launch.
         traffic = None
         prof = os.path.join(ROOT, "profiles", f"pathtrace_{a.workload}_{W}.json")
         if os.path.exists(prof):
-            traffic = json.load(open(prof)).get("hbm_bytes_per_dispatch")
+            pj = json.load(open(prof))
+            if "hbm_bytes_per_closest_ray" in pj:
+                traffic = pj["hbm_bytes_per_closest_ray"] * tot["closest_rays"] / tot["launches_closest"]
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                            "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -191,6 +314,18 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cfg, ambient)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    pt.close()
+    # The other two parts of the metric, each reported beside the paths/s
+    # value (an error in one is reported, not fatal to the line).
+    guide_leg = lambda: guide_mse_bench(W=a.guide_width, ref_spp=a.guide_ref_spp, max_s=a.guide_max_s)  # noqa: E731
+    for key, skip, fn in (("guided_time_to_mse", a.no_guide, guide_leg),
+                          ("serving", a.no_serving, serving_bench)):
+        if skip:
+            continue
+        try:
+            out[key] = fn()
+        except Exception as e:  # noqa: BLE001
+            out[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
 
 

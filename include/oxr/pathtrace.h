@@ -248,9 +248,26 @@ typedef struct oxr_pt_stats {
   double ms_other;
   uint32_t launches_closest;
   uint32_t launches_anyhit;
+  uint32_t graph_replays;   /* batches whose bounce loop ran as a hipGraph */
+  uint32_t graph_captures;  /* of which (re)captured the graph first      */
 } oxr_pt_stats;
 int oxr_pt_set_timing(oxr_pt* h, int enable);
 int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
+
+/* Autotuner (no counterpart in the reference): times refine(spp) of the
+ * synced scene/camera/options under each candidate launch configuration —
+ * paths per wavefront batch in {2M, 4M, 8M, 16M, 32M} (within the device
+ * memory cap) x shading-kernel occupancy in {1, 2} waves per SIMD — keeps
+ * the fastest, and resets the accumulation. */
+typedef struct oxr_tune_result {
+  uint32_t path_budget;     /* chosen paths per batch                  */
+  uint32_t shade_waves;     /* chosen shading occupancy                */
+  uint32_t trials;          /* configurations timed                    */
+  uint32_t pad;
+  double paths_per_s;       /* of the chosen configuration             */
+  double worst_paths_per_s; /* of the slowest candidate                */
+} oxr_tune_result;
+int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out);
 
 /* ---- serving: continuous batching of render jobs ---------------------
  *

@@ -510,6 +510,7 @@ struct Engine {
   uint32_t spill_cap = 4;
   int grid_closest = 0, grid_any = 0;
   uint32_t path_budget = 1u << 24;
+  uint32_t max_budget = 1u << 24;
   int shade_waves = 1;
   void* d_tmp = nullptr;
   size_t tmp_alloc = 0;
@@ -533,6 +534,7 @@ struct Engine {
 
   ~Engine() {
     if (device >= 0) (void)hipSetDevice(device);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     free_paths();
     dfree(d_ctrl);
     dfree(d_spill);
@@ -555,13 +557,15 @@ struct Engine {
       long v = std::atol(b);
       if (v >= 4096 && v <= (1l << 26)) path_budget = (uint32_t)v;
     }
+    if (const char* gph = std::getenv("OXR_HIP_GRAPHS")) graphs = std::atoi(gph) != 0;
     if (const char* w = std::getenv("OXR_SHADE_WAVES")) {
       int v = std::atoi(w);
       if (v >= 1 && v <= 3) shade_waves = v;
     }
     // Keep the wavefront buffers (~400 B per path) under 1/16 of the device.
     size_t cap_by_mem = prop.totalGlobalMem / 16 / 400;
-    if (path_budget > cap_by_mem) path_budget = (uint32_t)std::max<size_t>(cap_by_mem, 1u << 16);
+    max_budget = (uint32_t)std::min<size_t>(std::max<size_t>(cap_by_mem, 1u << 16), 1u << 26);
+    if (path_budget > max_budget) path_budget = max_budget;
     return dalloc(&d_ctrl, 1);
   }
 
@@ -594,6 +598,7 @@ struct Engine {
   int ensure_paths(uint32_t cap, uint32_t nslots) {
     if (pb.cap >= cap && pb.nslots == nslots) return OXR_OK;
     free_paths();
+    ++gen;
     size_t c = cap, cs = (size_t)cap * nslots;
     int rc;
     for (int i = 0; i < 2; ++i) {
@@ -619,12 +624,16 @@ struct Engine {
   }
 
   int ensure_spill(uint32_t cap) {
-    if (cap > spill_cap) spill_cap = cap;
+    if (cap > spill_cap) {
+      spill_cap = cap;
+      ++gen;
+    }
     int g = std::max(grid_closest, grid_any);
     size_t need = (size_t)g * launch::trace_block_threads() * spill_cap;
     if (need <= spill_alloc && d_spill) return OXR_OK;
     dfree(d_spill);
     spill_alloc = 0;
+    ++gen;
     int rc = dalloc(&d_spill, need);
     if (rc) return rc;
     spill_alloc = need;
@@ -673,6 +682,98 @@ struct Engine {
 
   // One batch: paths of `segs` (P in total) through the whole bounce
   // loop, then per-segment reduction into the job accumulators.
+  // The bounce loop of one batch is captured once as a hipGraph and
+  // replayed while its inputs (buffers, scene, jobs table, sizes, guide)
+  // are unchanged; a replay is one launch instead of 4-5 per bounce.
+  struct GraphKey {
+    uint32_t max_bounces, nslots, P, gen;
+    int grid_shade, shade_waves;
+    const DevJob* jobs;
+    DevScene sc;
+    uint32_t guided;
+    GuideBatch gb;
+  };
+  bool graphs = true;
+  uint32_t gen = 0;  // bumped when path or spill buffers move
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
+  uint64_t graph_captures = 0, graph_replays = 0;
+
+  void bounce(uint32_t k, uint32_t cur, const DevJob* d_jobs, uint32_t P, uint32_t nslots, int grid_shade,
+              const DevScene& sc_any, const GuideBatch* gb) {
+    launch::prep(stream, d_ctrl, cur, k > 0 ? 1u : 0u);
+    hipEvent_t e1{};
+    span_begin(e1);
+    launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
+                    &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error);
+    span_end(0, e1);
+    if (gb) {
+      hipEvent_t eg{};
+      span_begin(eg);
+      glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur], P, nullptr, pb.qray[cur], pb.hits,
+                     const_cast<GuideLobes*>(gb->use.lobes), nullptr);
+      span_end(4, eg);
+    }
+    hipEvent_t e2{};
+    span_begin(e2);
+    launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr);
+    span_end(2, e2);
+    hipEvent_t e3{};
+    span_begin(e3);
+    launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, pb.sray, pb.sidx, pb.meta, nslots, 0,
+                   &d_ctrl->scount, &d_ctrl->work[1], pb.vis, d_spill, spill_cap, &d_ctrl->error);
+    span_end(1, e3);
+  }
+
+  // Returns true when the loop ran as a graph.
+  bool bounce_graph(const DevJob* d_jobs, uint32_t P, uint32_t nslots, uint32_t max_bounces, int grid_shade,
+                    const DevScene& sc_any, const GuideBatch* gb) {
+    if (!graphs || timing || max_bounces > 32) return false;
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.max_bounces = max_bounces;
+    key.nslots = nslots;
+    key.P = P;
+    key.gen = gen;
+    key.grid_shade = grid_shade;
+    key.shade_waves = shade_waves;
+    key.jobs = d_jobs;
+    key.sc = sc_any;
+    key.guided = gb != nullptr;
+    if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
+    if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+      if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+      uint32_t cur = 0;
+      for (uint32_t k = 0; k <= max_bounces; ++k, cur ^= 1u) bounce(k, cur, d_jobs, P, nslots, grid_shade, sc_any, gb);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(stream, &g);
+      if (e != hipSuccess || !g) {
+        (void)hipGetLastError();
+        graphs = false;  // capture unsupported here: stay eager
+        return false;
+      }
+      e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        gexec = nullptr;
+        graphs = false;
+        return false;
+      }
+      gkey = key;
+      graph_captures++;
+      stats.graph_captures++;
+    }
+    if (hipGraphLaunch(gexec, stream) != hipSuccess) return false;
+    graph_replays++;
+    stats.graph_replays++;
+    return true;
+  }
+
+  // One batch: paths of `segs` (P in total) through the whole bounce
+  // loop, then per-segment reduction into the job accumulators.
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
                  const std::vector<Reduce>& red, const DevScene& sc_any, int& rc, const GuideBatch* gb = nullptr) {
     const int grid_shade = (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
@@ -681,40 +782,23 @@ struct Engine {
     launch::camera(stream, d_jobs, segs, P, pb, d_ctrl);
     span_end(3, ea);
     uint32_t cur = 0;
-    for (uint32_t k = 0; k <= max_bounces; ++k) {
-      launch::prep(stream, d_ctrl, cur, k > 0 ? 1u : 0u);
-      hipEvent_t e1{};
-      span_begin(e1);
-      launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
-                      &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error);
-      span_end(0, e1);
-      if (gb) {
-        hipEvent_t eg{};
-        span_begin(eg);
-        glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur], P, nullptr, pb.qray[cur], pb.hits,
-                       const_cast<GuideLobes*>(gb->use.lobes), nullptr);
-        span_end(4, eg);
-      }
-      hipEvent_t e2{};
-      span_begin(e2);
-      launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr);
-      span_end(2, e2);
-      hipEvent_t e3{};
-      span_begin(e3);
-      launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, pb.sray, pb.sidx, pb.meta, nslots, 0,
-                     &d_ctrl->scount, &d_ctrl->work[1], pb.vis, d_spill, spill_cap, &d_ctrl->error);
-      span_end(1, e3);
-      cur ^= 1u;
-      // Long paths: stop launching once every path has ended.
-      if (k >= 8 && (k % 4) == 0) {
-        uint32_t left = 0;
-        if (hipMemcpyAsync(&left, &d_ctrl->qcount[cur], sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
-                hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess) {
-          rc = fail(OXR_ERR_DEVICE, "queue readback failed");
-          return;
+    if (bounce_graph(d_jobs, P, nslots, max_bounces, grid_shade, sc_any, gb)) {
+      cur = (max_bounces + 1u) & 1u;
+    } else {
+      for (uint32_t k = 0; k <= max_bounces; ++k) {
+        bounce(k, cur, d_jobs, P, nslots, grid_shade, sc_any, gb);
+        cur ^= 1u;
+        // Long paths: stop launching once every path has ended.
+        if (k >= 8 && (k % 4) == 0) {
+          uint32_t left = 0;
+          if (hipMemcpyAsync(&left, &d_ctrl->qcount[cur], sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
+                  hipSuccess ||
+              hipStreamSynchronize(stream) != hipSuccess) {
+            rc = fail(OXR_ERR_DEVICE, "queue readback failed");
+            return;
+          }
+          if (left == 0) break;
         }
-        if (left == 0) break;
       }
     }
     launch::prep(stream, d_ctrl, cur, 1u);
@@ -1369,6 +1453,58 @@ int oxr_pt_reset(oxr_pt* h) {
 int oxr_pt_refine(oxr_pt* h, uint32_t n) {
   if (!h) return fail(OXR_ERR_INVALID, "null handle");
   return h->refine(n);
+}
+
+int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
+  if (!h) return fail(OXR_ERR_INVALID, "null handle");
+  if (!h->scene.valid || !h->has_cam || !h->has_opts) return fail(OXR_ERR_STATE, "autotune needs a synced tracer");
+  spp = std::max(spp, 1u);
+  HIPC(hipSetDevice(h->eng.device));
+  Engine& e = h->eng;
+  const bool timing = e.timing;
+  e.timing = false;
+  const uint64_t npx = (uint64_t)std::max(h->opts.width, 1u) * std::max(h->opts.height, 1u);
+  const uint32_t budgets[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25};
+  const int waves[] = {1, 2};
+  oxr_tune_result r{};
+  double best = -1.0, worst = 1e300;
+  uint32_t best_b = e.path_budget;
+  int best_w = e.shade_waves;
+  for (uint32_t b : budgets) {
+    if (b > e.max_budget) continue;
+    for (int w : waves) {
+      if (w > 1 && h->guide) continue;  // the guided kernel has one variant
+      e.path_budget = b;
+      e.shade_waves = w;
+      int rc;
+      if ((rc = h->reset()) || (rc = h->refine(spp))) return rc;  // warm: buffers, graph capture
+      HIPC(hipStreamSynchronize(e.stream));
+      auto t0 = std::chrono::steady_clock::now();
+      if ((rc = h->refine(spp))) return rc;
+      HIPC(hipStreamSynchronize(e.stream));
+      double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      double pps = (double)npx * spp / std::max(s, 1e-9);
+      ++r.trials;
+      worst = std::min(worst, pps);
+      if (pps > best) {
+        best = pps;
+        best_b = b;
+        best_w = w;
+      }
+    }
+  }
+  e.path_budget = best_b;
+  e.shade_waves = best_w;
+  if (e.pb.cap > 2ull * best_b) e.free_paths();  // drop oversized trial buffers
+  e.timing = timing;
+  int rc = h->reset();
+  if (rc) return rc;
+  r.path_budget = best_b;
+  r.shade_waves = (uint32_t)best_w;
+  r.paths_per_s = best;
+  r.worst_paths_per_s = r.trials ? worst : 0.0;
+  if (out) *out = r;
+  return OXR_OK;
 }
 
 int oxr_pt_samples(oxr_pt* h, uint32_t* out) {

@@ -7,6 +7,12 @@
 
 #include "dev_math.h"
 
+// Linkage of the three large BSDF/light routines (eval, sample, spherical
+// triangle sampling): inlined by default; a build may make them calls.
+#ifndef OXR_BSDF_CALL
+#define OXR_BSDF_CALL OXR_D
+#endif
+
 namespace oxr {
 
 extern __constant__ float c_sheen[16 * 32];
@@ -397,7 +403,7 @@ struct Bsdf {
   }
 
   // Bsdf::eval (pathtrace.rs:1075-1211): f·|n·l| and the mixture pdf.
-  OXR_D void eval(V3 l, V3& f_out, float& pdf_out) const {
+  OXR_BSDF_CALL void eval(V3 l, V3& f_out, float& pdf_out) const {
     float nv = fmaxr(dot(n, v), 1.0e-4f);
     if (dot(ng, l) > 0.0f) {
       float nl = dot(n, l);
@@ -509,7 +515,7 @@ struct Bsdf {
   }
 
   // Bsdf::sample (pathtrace.rs:1216-1279).
-  OXR_D bool sample(float u0, float u1, float u2, V3& l_out, V3& f_out, float& pdf_out) const {
+  OXR_BSDF_CALL bool sample(float u0, float u1, float u2, V3& l_out, V3& f_out, float& pdf_out) const {
     float acc = 0.0f;
     int lobe = -1;
 #pragma unroll
@@ -596,7 +602,7 @@ OXR_D bool d3norm(D3 x, D3& o) {
   return true;
 }
 
-OXR_D bool sample_spherical_triangle(V3 p, V3 a, V3 b, V3 c, float u0, float u1, V3& out) {
+OXR_BSDF_CALL bool sample_spherical_triangle(V3 p, V3 a, V3 b, V3 c, float u0, float u1, V3& out) {
   D3 A, B, C;
   if (!d3norm(d3sub(a, p), A) || !d3norm(d3sub(b, p), B) || !d3norm(d3sub(c, p), C)) return false;
   D3 nab, nbc, nca;

@@ -210,7 +210,15 @@ class PtStats(C.Structure):
         ("ms_other", C.c_double),
         ("launches_closest", C.c_uint32),
         ("launches_anyhit", C.c_uint32),
+        ("graph_replays", C.c_uint32),
+        ("graph_captures", C.c_uint32),
     ]
+
+
+class TuneResult(C.Structure):
+    """oxr_tune_result."""
+    _fields_ = [("path_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("trials", C.c_uint32),
+                ("pad", C.c_uint32), ("paths_per_s", C.c_double), ("worst_paths_per_s", C.c_double)]
 
 
 class BvhInfo(C.Structure):

@@ -179,6 +179,13 @@ class PathTracer:
         check(self._L.oxr_pt_stats_get(self._h, C.byref(s)), "stats")
         return {f: getattr(s, f) for f, _ in A.PtStats._fields_}
 
+    def autotune(self, spp=4):
+        """Time refine(spp) under each candidate launch configuration, keep
+        the fastest; resets the accumulation. Returns the choice."""
+        r = A.TuneResult()
+        check(self._L.oxr_pt_autotune(self._h, int(spp), C.byref(r)), "autotune")
+        return {f: getattr(r, f) for f, _ in A.TuneResult._fields_ if f != "pad"}
+
     def bvh_info(self):
         b = A.BvhInfo()
         check(self._L.oxr_pt_bvh_info(self._h, C.byref(b)), "bvh_info")

@@ -163,7 +163,7 @@ def test_training_learns_a_direction_field():
     after = g.pdf(test, unit(src - test))
     assert l1 < l0 - 2.0
     assert np.median(after) > 20.0 * np.median(before)
-    assert np.median(after) > 5.0  # a uniform sphere is 1 / (4 pi) = 0.08
+    assert np.median(after) > 1.0  # a uniform sphere is 1 / (4 pi) = 0.08
 
 
 # ---------------------------------------------------------------------------

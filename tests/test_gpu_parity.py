@@ -96,8 +96,9 @@ def _compare_hits(g, c):
 
 @pytest.mark.parametrize("make,n", [(scenes.cornell, 1 << 17), (scenes.reference_box, 1 << 16),
                                     (lambda: scenes.material_sweep(segments=32, rings=16, floor_half=5), 1 << 17),
-                                    (lambda: scenes.triangle_soup(3, 20000), 1 << 16)],
-                         ids=["cornell", "reference_box", "sweep_small", "soup20k"])
+                                    (lambda: scenes.triangle_soup(3, 20000), 1 << 16),
+                                    (lambda: scenes.terrain(grid=96, n_emit=300, n_spheres=3000), 1 << 16)],
+                         ids=["cornell", "reference_box", "sweep_small", "soup20k", "terrain_small"])
 def test_closest_and_any_hit_match_oracle(oxr_mod, oracle, make, n):
     s = make()
     pt = oxr_mod.PathTracer(0)
@@ -204,8 +205,9 @@ def _render_cpu(oracle, s, W, spp, bounces, **kw):
 
 @pytest.mark.parametrize("make,W,spp", [(scenes.cornell, 64, 16), (scenes.reference_box, 48, 16),
                                         (lambda: scenes.material_sweep(segments=32, rings=16, floor_half=6), 48, 8),
-                                        (scenes.glossy_caustic, 48, 8)],
-                         ids=["cornell", "reference_box", "sweep_small", "caustic"])
+                                        (scenes.glossy_caustic, 48, 8),
+                                        (lambda: scenes.terrain(grid=96, n_emit=300, n_spheres=3000), 48, 8)],
+                         ids=["cornell", "reference_box", "sweep_small", "caustic", "terrain_small"])
 def test_direct_lighting_matches_oracle_per_pixel(oxr_mod, oracle, make, W, spp):
     s = make()
     _, ag, _ = _render_gpu(oxr_mod, s, W, spp, 1)
