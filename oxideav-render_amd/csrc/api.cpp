@@ -700,14 +700,14 @@ struct Engine {
   uint64_t graph_captures = 0, graph_replays = 0;
 
   void bounce(uint32_t k, uint32_t cur, const DevJob* d_jobs, uint32_t P, uint32_t nslots, int grid_shade,
-              const DevScene& sc_any, const GuideBatch* gb) {
+              const DevScene& sc_any, bool multi, const GuideBatch* gb) {
     launch::prep(stream, d_ctrl, cur, k > 0 ? 1u : 0u);
     hipEvent_t e1{};
     span_begin(e1);
-    launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
+    launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, multi, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
                     &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error);
     span_end(0, e1);
-    if (gb) {
+    if (gb && k < (uint32_t)G_KREC) {  // the guide serves the first G_KREC bounces
       hipEvent_t eg{};
       span_begin(eg);
       glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur], P, nullptr, pb.qray[cur], pb.hits,
@@ -720,14 +720,14 @@ struct Engine {
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);
-    launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, pb.sray, pb.sidx, pb.meta, nslots, 0,
+    launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray, pb.sidx, pb.meta, nslots, 0,
                    &d_ctrl->scount, &d_ctrl->work[1], pb.vis, d_spill, spill_cap, &d_ctrl->error);
     span_end(1, e3);
   }
 
   // Returns true when the loop ran as a graph.
   bool bounce_graph(const DevJob* d_jobs, uint32_t P, uint32_t nslots, uint32_t max_bounces, int grid_shade,
-                    const DevScene& sc_any, const GuideBatch* gb) {
+                    const DevScene& sc_any, bool multi, const GuideBatch* gb) {
     if (!graphs || timing || max_bounces > 32) return false;
     GraphKey key;
     std::memset(&key, 0, sizeof(key));
@@ -739,14 +739,15 @@ struct Engine {
     key.shade_waves = shade_waves;
     key.jobs = d_jobs;
     key.sc = sc_any;
-    key.guided = gb != nullptr;
+    key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
     if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
       if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
       uint32_t cur = 0;
-      for (uint32_t k = 0; k <= max_bounces; ++k, cur ^= 1u) bounce(k, cur, d_jobs, P, nslots, grid_shade, sc_any, gb);
+      for (uint32_t k = 0; k <= max_bounces; ++k, cur ^= 1u)
+        bounce(k, cur, d_jobs, P, nslots, grid_shade, sc_any, multi, gb);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(stream, &g);
       if (e != hipSuccess || !g) {
@@ -774,19 +775,22 @@ struct Engine {
 
   // One batch: paths of `segs` (P in total) through the whole bounce
   // loop, then per-segment reduction into the job accumulators.
+  // multi: the batch mixes scenes (each path's scene from its job); else
+  // every path traces sc_any.
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
-                 const std::vector<Reduce>& red, const DevScene& sc_any, int& rc, const GuideBatch* gb = nullptr) {
+                 const std::vector<Reduce>& red, const DevScene& sc_any, bool multi, int& rc,
+                 const GuideBatch* gb = nullptr) {
     const int grid_shade = (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
     hipEvent_t ea{};
     span_begin(ea);
     launch::camera(stream, d_jobs, segs, P, pb, d_ctrl);
     span_end(3, ea);
     uint32_t cur = 0;
-    if (bounce_graph(d_jobs, P, nslots, max_bounces, grid_shade, sc_any, gb)) {
+    if (bounce_graph(d_jobs, P, nslots, max_bounces, grid_shade, sc_any, multi, gb)) {
       cur = (max_bounces + 1u) & 1u;
     } else {
       for (uint32_t k = 0; k <= max_bounces; ++k) {
-        bounce(k, cur, d_jobs, P, nslots, grid_shade, sc_any, gb);
+        bounce(k, cur, d_jobs, P, nslots, grid_shade, sc_any, multi, gb);
         cur ^= 1u;
         // Long paths: stop launching once every path has ended.
         if (k >= 8 && (k % 4) == 0) {
@@ -1183,7 +1187,7 @@ int oxr_pt::refine(uint32_t n) {
       std::vector<Engine::Reduce> red{{0u, npx_c, S, d_accum + 4 * px0, d_sumsq + 3 * px0}};
       rc = OXR_OK;
       if (gb && gb->use.rec) glaunch::fill(eng.stream, reinterpret_cast<float*>(guide->d_rec), rec_floats);
-      eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, rc, gb);
+      eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, false, rc, gb);
       if (rc) return rc;
     }
   }
@@ -1342,7 +1346,9 @@ struct oxr_srv {
       if ((rc = eng.ensure_spill(spill))) return rc;
       eng.begin_stats();
       rc = OXR_OK;
-      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, rc);
+      bool multi = false;
+      for (auto& [id, S_] : took) multi = multi || jobs[id].scene != jobs[took[0].first].scene;
+      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, multi, rc);
       if (rc) return rc;
       if ((rc = eng.end_stats())) return rc;
       st.rounds++;
@@ -1627,10 +1633,10 @@ static int trace_host(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint32_t flags
   HIPC(hipMemcpyAsync(d_rays, rays, rbytes, hipMemcpyHostToDevice, e.stream));
   launch::zero_ctrl(e.stream, e.d_ctrl);
   if (occ)
-    launch::anyhit(e.stream, e.grid_any, h->scene.sc, nullptr, nullptr, false, d_rays, nullptr, nullptr, 1, n, nullptr,
+    launch::anyhit(e.stream, e.grid_any, h->scene.sc, nullptr, nullptr, false, false, d_rays, nullptr, nullptr, 1, n, nullptr,
                    &e.d_ctrl->work[1], (uint8_t*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   else
-    launch::closest(e.stream, e.grid_closest, h->scene.sc, nullptr, nullptr, false, d_rays, nullptr, nullptr, n,
+    launch::closest(e.stream, e.grid_closest, h->scene.sc, nullptr, nullptr, false, false, d_rays, nullptr, nullptr, n,
                     nullptr, &e.d_ctrl->work[0], flags & 1u, (uint4*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(occ ? (void*)occ : (void*)hits, d_out, obytes, hipMemcpyDeviceToHost, e.stream));
@@ -1661,7 +1667,7 @@ int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays, void* d_
   if ((rc = e.ensure_spill(h->scene.spill_cap))) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : e.stream;
   launch::zero_ctrl(s, e.d_ctrl);
-  launch::closest(s, e.grid_closest, h->scene.sc, nullptr, nullptr, false, (const float4*)d_rays, nullptr, nullptr, n,
+  launch::closest(s, e.grid_closest, h->scene.sc, nullptr, nullptr, false, false, (const float4*)d_rays, nullptr, nullptr, n,
                   nullptr, &e.d_ctrl->work[0], 0, (uint4*)d_hits, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
   return OXR_OK;

@@ -69,22 +69,24 @@ struct Builder {
   Builder(std::vector<Item>& it, std::vector<BNode>& nd, int ml) : items(it), nodes(nd), max_leaf(ml) {}
 
   // Returns true and (axis, bin) of the best SAH split, or false for a leaf.
-  bool find_split(uint32_t lo, uint32_t hi, const Box& cb, int& axis_out, int& bin_out) {
+  bool find_split(uint32_t lo, uint32_t hi, const Box& nodes_bounds, const Box& cb, int& nb_out, int& axis_out,
+                  int& bin_out) {
     uint32_t cnt = hi - lo;
     double best = std::numeric_limits<double>::infinity();
     int best_axis = -1, best_bin = -1;
-    Box bounds;
-    for (uint32_t i = lo; i < hi; ++i) bounds.grow(items[i].b);
+    const Box& bounds = nodes_bounds;
     double parent_area = bounds.area();
+    // Bins: as many as items up to NBINS (more bins than items only costs).
+    const int nb = (int)std::min<uint32_t>(NBINS, std::max<uint32_t>(4u, cnt));
     for (int axis = 0; axis < 3; ++axis) {
       float ext = cb.hi[axis] - cb.lo[axis];
       if (!(ext > 0.0f) || !std::isfinite(ext)) continue;
       Box bb[NBINS];
       uint32_t bn[NBINS] = {0};
-      float k = (float)NBINS / ext;
+      float k = (float)nb / ext;
       for (uint32_t i = lo; i < hi; ++i) {
         int b = (int)((items[i].c[axis] - cb.lo[axis]) * k);
-        b = std::min(std::max(b, 0), NBINS - 1);
+        b = std::min(std::max(b, 0), nb - 1);
         bb[b].grow(items[i].b);
         bn[b]++;
       }
@@ -92,7 +94,7 @@ struct Builder {
       uint32_t ln[NBINS], rn[NBINS];
       Box acc;
       uint32_t n = 0;
-      for (int i = 0; i < NBINS; ++i) {
+      for (int i = 0; i < nb; ++i) {
         acc.grow(bb[i]);
         n += bn[i];
         la[i] = acc.area();
@@ -100,13 +102,13 @@ struct Builder {
       }
       acc = Box();
       n = 0;
-      for (int i = NBINS - 1; i >= 0; --i) {
+      for (int i = nb - 1; i >= 0; --i) {
         acc.grow(bb[i]);
         n += bn[i];
         ra[i] = acc.area();
         rn[i] = n;
       }
-      for (int i = 0; i < NBINS - 1; ++i) {
+      for (int i = 0; i < nb - 1; ++i) {
         if (ln[i] == 0 || rn[i + 1] == 0) continue;
         double c = la[i] * ln[i] + ra[i + 1] * rn[i + 1];
         if (c < best) {
@@ -122,6 +124,7 @@ struct Builder {
     if (cnt <= (uint32_t)max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) return false;
     axis_out = best_axis;
     bin_out = best_bin;
+    nb_out = nb;
     return true;
   }
 
@@ -137,16 +140,16 @@ struct Builder {
       }
       nd.b = b;
       uint32_t cnt = hi - lo;
-      int axis = -1, bin = -1;
-      bool split = cnt > 1 && find_split(lo, hi, cb, axis, bin);
+      int axis = -1, bin = -1, nbins = NBINS;
+      bool split = cnt > 1 && find_split(lo, hi, b, cb, nbins, axis, bin);
       uint32_t mid = lo;
       if (split && axis >= 0) {
         float ext = cb.hi[axis] - cb.lo[axis];
-        float k = (float)NBINS / ext;
+        float k = (float)nbins / ext;
         float clo = cb.lo[axis];
         auto it = std::partition(items.begin() + lo, items.begin() + hi, [&](const Item& x) {
           int bb = (int)((x.c[axis] - clo) * k);
-          bb = std::min(std::max(bb, 0), NBINS - 1);
+          bb = std::min(std::max(bb, 0), nbins - 1);
           return bb < bin;
         });
         mid = (uint32_t)(it - items.begin());
@@ -348,7 +351,14 @@ Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
   nt = std::min(nt, 32);
   if (n < 200000 || nt == 1) {
+    // One thread: build() still hands subtrees of more than 65536 items to
+    // the queue; drain it here.
     b.build(0, 0, n);
+    while (!b.queue.empty()) {
+      Builder::Task t = b.queue.back();
+      b.queue.pop_back();
+      b.build(t.node, t.lo, t.hi);
+    }
   } else {
     b.push({0, 0, n});
     std::vector<std::thread> ts;

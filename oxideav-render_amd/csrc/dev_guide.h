@@ -58,7 +58,8 @@ struct GuideLobes {
   float pi[G_LOBES];
 };
 
-// What the shading kernel needs of an attached guide for one bounce.
+// What the shading kernel needs of an attached guide for one bounce. The
+// guide serves bounces k < G_KREC (inference runs only for those).
 // Training records: for paths with pid % 2^shift == 0, vertex k < G_KREC
 // keeps 4 float4 at rec[((pid >> shift) * G_KREC + k) * 4]:
 //   (position, mixture pdf), (direction, 1 = valid), (throughput after

@@ -13,6 +13,8 @@
 // normative spec :1-277); each device function cites its counterpart.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bvh.h"
 #include "dev_bsdf.h"
 #include "dev_guide.h"
@@ -101,8 +103,6 @@ OXR_D bool accept_tri(uint32_t flags, float alpha, bool front, bool cull, uint32
 // tested right away) and returns true once the ray is finished.
 template <bool ANY>
 struct Trav {
-  const Node4* nodes;
-  const TriRec* tris;
   V3 inv, oi;
   WRay wr;
   float tmin, tmax;
@@ -113,10 +113,8 @@ struct Trav {
   uint32_t node;
   bool hit_any;
 
-  OXR_D void setup(const DevScene& sc, V3 o, V3 d, float tmin_, float tmax_, bool cull_, uint32_t skip_, uint32_t seed_,
-                   uint32_t rid_, Stack& st) {
-    nodes = sc.nodes;
-    tris = sc.tris;
+  OXR_D void setup(V3 o, V3 d, float tmin_, float tmax_, bool cull_, uint32_t skip_, uint32_t seed_, uint32_t rid_,
+                   Stack& st) {
     inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     oi = mul(o, inv);
     negx = signbit(inv.x);
@@ -139,7 +137,9 @@ struct Trav {
     st.overflow = false;
   }
 
-  OXR_D bool step(Stack& st) {
+  // nodes / tris: the scene's BVH (uniform across the wave when every
+  // ray of the launch belongs to one scene, so they stay in SGPRs).
+  OXR_D bool step(Stack& st, const Node4* __restrict__ nodes, const TriRec* __restrict__ tris) {
     const float4* np = reinterpret_cast<const float4*>(nodes + node);
     float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     uint4 ch = reinterpret_cast<const uint4*>(np)[6];
@@ -222,10 +222,13 @@ struct Trav {
 // once REFILL of them are idle (one atomic per refill); the first ray of
 // every lane is assigned by wave index, so a launch whose rays fit in one
 // round makes no atomic at all.
-constexpr uint32_t REFILL = 16;
+#ifndef OXR_REFILL
+#define OXR_REFILL 16
+#endif
+constexpr uint32_t REFILL = OXR_REFILL;
 
-template <bool ANY, class Setup, class Finish>
-OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, Finish finish) {
+template <bool ANY, class Setup, class Step, class Finish>
+OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, Step step, Finish finish) {
   const uint32_t lane = lane_id();
   const uint32_t nwaves = gridDim.x * (TRACE_TB / 64);
   const uint32_t wave = blockIdx.x * (TRACE_TB / 64) + threadIdx.x / 64;
@@ -236,7 +239,7 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, Fi
   if (active) setup(tr, ri);
   for (;;) {
     if (active) {
-      if (tr.step(st)) {
+      if (step(tr)) {
         finish(tr, ri);
         active = false;
       }
@@ -267,7 +270,9 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, Fi
 
 // Persistent closest-hit kernel over a ray queue. PATH: rays carry the
 // bounce index k in the t_min slot (t_min = 0); else t_min as given.
-template <bool PATH>
+// MULTI: the launch mixes scenes (server rounds); each ray finds its scene
+// through its path's job. Otherwise every ray traces `sc`.
+template <bool PATH, bool MULTI>
 __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob* __restrict__ jobs,
                                                      const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                      const uint32_t* __restrict__ qpid,
@@ -281,6 +286,8 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
+  const Node4* ray_nodes = sc.nodes;
+  const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<false>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     uint32_t path_seed = 0, ray_id = 0;
@@ -289,20 +296,28 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
     if (PATH) {
       uint32_t k = __float_as_uint(a.w);
       uint32_t pid = qpid[i];
-      const DevScene& js = jobs[pjob[pid]].sc;
+      bool blend = sc.has_blend;
+      if (MULTI) {
+        const DevScene& js = jobs[pjob[pid]].sc;
+        ray_nodes = js.nodes;
+        ray_tris = js.tris;
+        blend = js.has_blend;
+      }
       tmin = 0.0f;
       cull = k == 0;
       ray_id = k << 16;
-      if (js.has_blend) {
+      if (blend) {
         uint4 m = meta[pid];
         path_seed = hash_mix(m.w, m.y);
       }
-      tr.setup(js, f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
     } else {
       tmin = a.w;
       cull = cull_all != 0;
-      tr.setup(sc, f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
     }
+    tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
+  };
+  auto step = [&](Trav<false>& tr) {
+    return MULTI ? tr.step(st, ray_nodes, ray_tris) : tr.step(st, sc.nodes, sc.tris);
   };
   auto finish = [&](Trav<false>& tr, uint32_t i) {
     uint32_t prim = tr.best;
@@ -313,13 +328,13 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
     hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(tr.tmax), prim, __float_as_uint(tr.bu),
                          __float_as_uint(tr.bv));
   };
-  trace_loop<false>(count, work, st, setup, finish);
+  trace_loop<false>(count, work, st, setup, step, finish);
 }
 
 // Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
 // sidx[i] = path * nslots + slot; writes vis[slot] = 2 when unoccluded.
-// Else: record = oxr_ray, writes occ[i] (1 = occluded).
-template <bool PATH>
+// Else: record = oxr_ray, writes occ[i] (1 = occluded). MULTI as k_closest.
+template <bool PATH, bool MULTI>
 __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                     const uint32_t* __restrict__ sidx, const uint4* __restrict__ meta,
@@ -332,6 +347,8 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
+  const Node4* ray_nodes = sc.nodes;
+  const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<true>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     float tmin = 0.0f;
@@ -340,18 +357,26 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
       skip = __float_as_uint(a.w);
       uint32_t slot = sidx[i];
       uint32_t pid = slot / nslots, j = slot - pid * nslots;
-      const DevScene& js = jobs[pjob[pid]].sc;
-      if (js.has_blend) {
+      bool blend = sc.has_blend;
+      if (MULTI) {
+        const DevScene& js = jobs[pjob[pid]].sc;
+        ray_nodes = js.nodes;
+        ray_tris = js.tris;
+        blend = js.has_blend;
+      }
+      if (blend) {
         uint4 m = meta[pid];
         uint32_t k = (m.z & 0xffffu);
         path_seed = hash_mix(m.w, m.y);
         ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (j & 0x7fffu)));
       }
-      tr.setup(js, f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
     } else {
       tmin = a.w;
-      tr.setup(sc, f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
     }
+    tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
+  };
+  auto step = [&](Trav<true>& tr) {
+    return MULTI ? tr.step(st, ray_nodes, ray_tris) : tr.step(st, sc.nodes, sc.tris);
   };
   auto finish = [&](Trav<true>& tr, uint32_t i) {
     bool occ = tr.hit_any;
@@ -365,7 +390,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
       out[i] = occ ? 1 : 0;
     }
   };
-  trace_loop<true>(count, work, st, setup, finish);
+  trace_loop<true>(count, work, st, setup, step, finish);
 }
 
 // ---------------------------------------------------------------------
@@ -639,13 +664,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     bsdf.init(ms, neg(d), ngeo, front);
     if (alive && bsdf.black()) alive = false;
     V3 ng = bsdf.ng;
-    // One-sample MIS with the guide at eligible vertices: no transmission,
-    // not a near-mirror metal (the BSDF sampler alone serves those).
+    // One-sample MIS with the guide at eligible vertices: the first G_KREC
+    // bounces, no transmission, not a near-mirror metal (the BSDF sampler
+    // alone serves those).
     float gmix = 0.0f;
     const GuideLobes* GL = nullptr;
     if (GUIDE) {
       GL = gu.lobes + i;
-      if (alive && transmission == 0.0f && M->dt == 0.0f && !(M->metallic >= 0.99f && M->roughness < 0.2f))
+      if (alive && k < (uint32_t)G_KREC && transmission == 0.0f && M->dt == 0.0f &&
+          !(M->metallic >= 0.99f && M->roughness < 0.2f))
         gmix = gu.mix;
     }
     // NEE: punctual lights, one shadow-ray slot each.
@@ -1154,38 +1181,49 @@ hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, con
 }
 
 int trace_blocks_per_cu(bool any) {
-  int nb = 0;
-  hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_anyhit<true>, TRACE_TB, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_closest<true>, TRACE_TB, 0);
-  if (e != hipSuccess || nb <= 0) nb = 4;
-  return nb;
+  int nb = 0, nb2 = 0;
+  hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_anyhit<true, false>, TRACE_TB, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_closest<true, false>, TRACE_TB, 0);
+  hipError_t e2 = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_anyhit<true, true>, TRACE_TB, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_closest<true, true>, TRACE_TB, 0);
+  // The persistent grid must fit on the device for either variant.
+  if (e != hipSuccess || e2 != hipSuccess || nb <= 0 || nb2 <= 0) return 4;
+  return std::min(nb, nb2);
 }
 
 int trace_block_threads() { return TRACE_TB; }
 int stack_lds_entries() { return STACK; }
 
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
-             const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
+             bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
              uint32_t spill_cap, uint32_t* err) {
-  if (path)
-    hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta, fixed_count,
-                       count_ptr, work, cull_all, hits, spill, spill_cap, err);
+#define OXR_CLOSEST(P, M)                                                                                        \
+  hipLaunchKernelGGL((k_closest<P, M>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta,         \
+                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err)
+  if (!path)
+    OXR_CLOSEST(false, false);
+  else if (multi)
+    OXR_CLOSEST(true, true);
   else
-    hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta, fixed_count,
-                       count_ptr, work, cull_all, hits, spill, spill_cap, err);
+    OXR_CLOSEST(true, false);
+#undef OXR_CLOSEST
 }
 
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
-            const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
-            const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,
-            uint32_t* err) {
-  if (path)
-    hipLaunchKernelGGL(k_anyhit<true>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots,
-                       fixed_count, count_ptr, work, out, spill, spill_cap, err);
+            bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots,
+            uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill,
+            uint32_t spill_cap, uint32_t* err) {
+#define OXR_ANYHIT(P, M)                                                                                         \
+  hipLaunchKernelGGL((k_anyhit<P, M>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots, \
+                     fixed_count, count_ptr, work, out, spill, spill_cap, err)
+  if (!path)
+    OXR_ANYHIT(false, false);
+  else if (multi)
+    OXR_ANYHIT(true, true);
   else
-    hipLaunchKernelGGL(k_anyhit<false>, dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots,
-                       fixed_count, count_ptr, work, out, spill, spill_cap, err);
+    OXR_ANYHIT(true, false);
+#undef OXR_ANYHIT
 }
 
 void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl) {

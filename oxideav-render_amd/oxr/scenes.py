@@ -398,7 +398,8 @@ def glossy_caustic(segments=64, rings=64):
     s.add_tris(translate(sp, (0.0, 1.2, 0.0)), glass, normals=sn)
     h = 0.05
     y = 1.2 + 0.5 + 0.05
-    lq, _ = quad([[-h, y, h], [h, y, h], [h, y, -h], [-h, y, -h]])
+    # Front (emitting) face down, into the sphere: cross(p1 - p0, p3 - p0) = -Y.
+    lq, _ = quad([[-h, y, -h], [h, y, -h], [h, y, h], [-h, y, h]])
     s.add_tris(lq, emit)
     s.camera = CameraSpec((0.0, 2.6, 4.6), (0.0, 0.6, 0.0), yfov_deg=40.0)
     s.ambient = 0.0
