@@ -7,6 +7,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <thread>
 
@@ -66,7 +67,109 @@ struct Builder {
   int busy = 0;
   bool done = false;
 
-  Builder(std::vector<Item>& it, std::vector<BNode>& nd, int ml) : items(it), nodes(nd), max_leaf(ml) {}
+  Builder(std::vector<Item>& it, std::vector<BNode>& nd, int ml, int threads)
+      : items(it), nodes(nd), max_leaf(ml), par(threads) {}
+
+  // Nodes of at least BIG items (the top levels, which would otherwise run
+  // on one thread) split their passes over `par` threads; the item order
+  // they produce does not depend on the thread count.
+  static constexpr uint32_t BIG = 1u << 20;
+  int par = 1;
+  std::vector<Item> tmp;  // scatter target of the big-node partitions
+
+  template <class F>
+  void par_for(uint32_t lo, uint32_t hi, F fn) {
+    int T = std::max(1, par);
+    std::vector<std::thread> ts;
+    for (int t = 0; t < T; ++t) {
+      uint32_t a = lo + (uint32_t)((uint64_t)(hi - lo) * t / T), b = lo + (uint32_t)((uint64_t)(hi - lo) * (t + 1) / T);
+      if (T == 1) {
+        fn(t, a, b);
+        return;
+      }
+      ts.emplace_back([=, &fn] { fn(t, a, b); });
+    }
+    for (auto& th : ts) th.join();
+  }
+
+  void bounds(uint32_t lo, uint32_t hi, Box& b, Box& cb) {
+    if (hi - lo < BIG) {
+      for (uint32_t i = lo; i < hi; ++i) {
+        b.grow(items[i].b);
+        cb.grow_pt(items[i].c);
+      }
+      return;
+    }
+    std::vector<Box> pb(par), pc(par);
+    par_for(lo, hi, [&](int t, uint32_t a, uint32_t e) {
+      for (uint32_t i = a; i < e; ++i) {
+        pb[t].grow(items[i].b);
+        pc[t].grow_pt(items[i].c);
+      }
+    });
+    for (int t = 0; t < par; ++t) {
+      b.grow(pb[t]);
+      cb.grow(pc[t]);
+    }
+  }
+
+  // Per-axis bins of items [lo, hi): boxes and counts.
+  void bin(uint32_t lo, uint32_t hi, const Box& cb, int nb, Box bb[3][NBINS], uint32_t bn[3][NBINS]) {
+    auto pass = [&](uint32_t a, uint32_t e, Box lb[3][NBINS], uint32_t ln[3][NBINS]) {
+      for (int axis = 0; axis < 3; ++axis) {
+        float ext = cb.hi[axis] - cb.lo[axis];
+        if (!(ext > 0.0f) || !std::isfinite(ext)) continue;
+        float k = (float)nb / ext;
+        for (uint32_t i = a; i < e; ++i) {
+          int b = (int)((items[i].c[axis] - cb.lo[axis]) * k);
+          b = std::min(std::max(b, 0), nb - 1);
+          lb[axis][b].grow(items[i].b);
+          ln[axis][b]++;
+        }
+      }
+    };
+    if (hi - lo < BIG) {
+      pass(lo, hi, bb, bn);
+      return;
+    }
+    struct Part {
+      Box b[3][NBINS];
+      uint32_t n[3][NBINS] = {};
+    };
+    std::vector<Part> parts(par);
+    par_for(lo, hi, [&](int t, uint32_t a, uint32_t e) { pass(a, e, parts[t].b, parts[t].n); });
+    for (int t = 0; t < par; ++t)
+      for (int axis = 0; axis < 3; ++axis)
+        for (int i = 0; i < nb; ++i) {
+          bb[axis][i].grow(parts[t].b[axis][i]);
+          bn[axis][i] += parts[t].n[axis][i];
+        }
+  }
+
+  // Stable partition of a big node by the predicate, through `tmp`.
+  template <class Pred>
+  uint32_t big_partition(uint32_t lo, uint32_t hi, Pred left) {
+    std::vector<uint32_t> nl(par, 0), nr(par, 0);
+    par_for(lo, hi, [&](int t, uint32_t a, uint32_t e) {
+      for (uint32_t i = a; i < e; ++i) (left(items[i]) ? nl[t] : nr[t])++;
+    });
+    uint32_t total_l = 0;
+    for (int t = 0; t < par; ++t) total_l += nl[t];
+    std::vector<uint32_t> ol(par), orr(par);
+    uint32_t accl = lo, accr = lo + total_l;
+    for (int t = 0; t < par; ++t) {
+      ol[t] = accl;
+      orr[t] = accr;
+      accl += nl[t];
+      accr += nr[t];
+    }
+    par_for(lo, hi, [&](int t, uint32_t a, uint32_t e) {
+      uint32_t pl = ol[t], pr = orr[t];
+      for (uint32_t i = a; i < e; ++i) tmp[left(items[i]) ? pl++ : pr++] = items[i];
+    });
+    par_for(lo, hi, [&](int, uint32_t a, uint32_t e) { std::copy(tmp.begin() + a, tmp.begin() + e, items.begin() + a); });
+    return lo + total_l;
+  }
 
   // Returns true and (axis, bin) of the best SAH split, or false for a leaf.
   bool find_split(uint32_t lo, uint32_t hi, const Box& nodes_bounds, const Box& cb, int& nb_out, int& axis_out,
@@ -78,17 +181,33 @@ struct Builder {
     double parent_area = bounds.area();
     // Bins: as many as items up to NBINS (more bins than items only costs).
     const int nb = (int)std::min<uint32_t>(NBINS, std::max<uint32_t>(4u, cnt));
+    // Big nodes: all three axes binned in one parallel pass.
+    std::unique_ptr<Box[]> big_b;
+    std::unique_ptr<uint32_t[]> big_n;
+    if (cnt >= BIG) {
+      big_b.reset(new Box[3 * NBINS]);
+      big_n.reset(new uint32_t[3 * NBINS]());
+      bin(lo, hi, cb, nb, reinterpret_cast<Box(*)[NBINS]>(big_b.get()),
+          reinterpret_cast<uint32_t(*)[NBINS]>(big_n.get()));
+    }
     for (int axis = 0; axis < 3; ++axis) {
       float ext = cb.hi[axis] - cb.lo[axis];
       if (!(ext > 0.0f) || !std::isfinite(ext)) continue;
       Box bb[NBINS];
       uint32_t bn[NBINS] = {0};
-      float k = (float)nb / ext;
-      for (uint32_t i = lo; i < hi; ++i) {
-        int b = (int)((items[i].c[axis] - cb.lo[axis]) * k);
-        b = std::min(std::max(b, 0), nb - 1);
-        bb[b].grow(items[i].b);
-        bn[b]++;
+      if (cnt >= BIG) {
+        for (int i = 0; i < nb; ++i) {
+          bb[i] = big_b[axis * NBINS + i];
+          bn[i] = big_n[axis * NBINS + i];
+        }
+      } else {
+        float k = (float)nb / ext;
+        for (uint32_t i = lo; i < hi; ++i) {
+          int b = (int)((items[i].c[axis] - cb.lo[axis]) * k);
+          b = std::min(std::max(b, 0), nb - 1);
+          bb[b].grow(items[i].b);
+          bn[b]++;
+        }
       }
       double la[NBINS], ra[NBINS];
       uint32_t ln[NBINS], rn[NBINS];
@@ -134,10 +253,7 @@ struct Builder {
     for (;;) {
       BNode& nd = nodes[node];
       Box b, cb;
-      for (uint32_t i = lo; i < hi; ++i) {
-        b.grow(items[i].b);
-        cb.grow_pt(items[i].c);
-      }
+      bounds(lo, hi, b, cb);
       nd.b = b;
       uint32_t cnt = hi - lo;
       int axis = -1, bin = -1, nbins = NBINS;
@@ -147,12 +263,17 @@ struct Builder {
         float ext = cb.hi[axis] - cb.lo[axis];
         float k = (float)nbins / ext;
         float clo = cb.lo[axis];
-        auto it = std::partition(items.begin() + lo, items.begin() + hi, [&](const Item& x) {
+        auto left = [&](const Item& x) {
           int bb = (int)((x.c[axis] - clo) * k);
           bb = std::min(std::max(bb, 0), nbins - 1);
           return bb < bin;
-        });
-        mid = (uint32_t)(it - items.begin());
+        };
+        if (cnt >= BIG) {
+          mid = big_partition(lo, hi, left);
+        } else {
+          auto it = std::partition(items.begin() + lo, items.begin() + hi, left);
+          mid = (uint32_t)(it - items.begin());
+        }
       }
       if (split && (axis < 0 || mid == lo || mid == hi)) {
         if (cnt <= (uint32_t)max_leaf) {
@@ -347,9 +468,10 @@ Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float
   // against the rounding of the watertight triangle test.
   out.pad = std::ldexp(std::fmax(gmax, 1.0f), -20);
   std::vector<BNode> bn(2ull * n);
-  Builder b(items, bn, max_leaf);
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
   nt = std::min(nt, 32);
+  Builder b(items, bn, max_leaf, nt);
+  if (n >= Builder::BIG) b.tmp.resize(n);
   if (n < 200000 || nt == 1) {
     // One thread: build() still hands subtrees of more than 65536 items to
     // the queue; drain it here.

@@ -2,7 +2,8 @@
 small harness: every triangle lands in exactly one leaf, every leaf box
 holds its triangles, and the tree is the same with 1 and 8 threads —
 including sizes where large subtrees are handed to the work queue
-(more than 65536 triangles per side) below the multi-thread cut-off."""
+(more than 65536 triangles per side) below the multi-thread cut-off, and
+nodes big enough (2^20 items) to split their passes over threads."""
 import os
 import subprocess
 
@@ -68,7 +69,7 @@ def _soup(n, seed):
     return np.concatenate([c, c + off[:, :3], c + off[:, 3:]], 1).astype(np.float32)
 
 
-@pytest.mark.parametrize("n", [1000, 150_000, 260_000])
+@pytest.mark.parametrize("n", [1000, 150_000, 260_000, 1_200_000])
 def test_every_triangle_in_one_leaf_any_thread_count(harness, n):
     d, exe = harness
     pos = _soup(n, n)
@@ -83,5 +84,7 @@ def test_every_triangle_in_one_leaf_any_thread_count(harness, n):
         gids = np.fromfile(gf, dtype=np.uint32)
         assert ntris == n and inleaf == n and bad == 0
         assert np.array_equal(np.sort(gids), np.arange(n, dtype=np.uint32))
-        outs.append((nodes, depth))
+        outs.append((nodes, depth, gids))
     assert outs[0][1] < 40
+    # Same tree for any thread count (leaf order included).
+    assert outs[0][0] == outs[1][0] and np.array_equal(outs[0][2], outs[1][2])
