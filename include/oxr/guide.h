@@ -79,6 +79,7 @@ typedef struct oxr_guide_config {
   uint32_t batch;        /* samples per online training step (65536) */
   uint32_t steps;        /* online training steps after each refine pass (4) */
   uint32_t record_shift; /* online samples come from paths with id % 2^shift == 0 (3) */
+  uint32_t max_steps;    /* online training stops after this many Adam steps; 0 = never (0) */
 } oxr_guide_config;
 
 typedef struct oxr_guide_stats {

@@ -1155,11 +1155,15 @@ int oxr_pt::refine(uint32_t n) {
   GuideBatch gbuf{};
   const GuideBatch* gb = nullptr;
   size_t rec_floats = 0;
+  // Online training until the guide's step budget is spent; after that
+  // the guide only samples (no records, no collection).
+  const bool train_now =
+      guide && guide_train && (guide->cfg.max_steps == 0 || guide->st.steps < guide->cfg.max_steps);
   if (guide) {
     if (!guide->box_set) guide->set_box(scene.lo, scene.hi);
     const uint32_t cap = npx_b * S_b;
     if ((rc = guide->ensure_lobes(cap))) return rc;
-    if (guide_train) {
+    if (train_now) {
       size_t nsel = ((size_t)cap + (1ull << guide->cfg.record_shift) - 1) >> guide->cfg.record_shift;
       rec_floats = nsel * G_KREC * 16;
       if ((rc = guide->ensure_rec(nsel * G_KREC * 4))) return rc;
@@ -1167,7 +1171,7 @@ int oxr_pt::refine(uint32_t n) {
       HIPC(hipMemsetAsync(guide->d_count, 0, sizeof(uint32_t), eng.stream));
     }
     gbuf.net = guide->net;
-    gbuf.use = DevGuideUse{guide->d_lobes, guide_train ? guide->d_rec : nullptr, guide->cfg.mix,
+    gbuf.use = DevGuideUse{guide->d_lobes, train_now ? guide->d_rec : nullptr, guide->cfg.mix,
                            guide->cfg.record_shift};
     gbuf.samp = guide->d_samp;
     gbuf.count = guide->d_count;
@@ -1195,7 +1199,7 @@ int oxr_pt::refine(uint32_t n) {
   samples += n;
   if (guide) {
     guide->st.infer_ms += eng.guide_infer_ms;
-    if (guide_train && (rc = train_guide())) return rc;
+    if (train_now && (rc = train_guide())) return rc;
   }
   return OXR_OK;
 }
@@ -1215,7 +1219,9 @@ int oxr_pt::train_guide() {
     b = eng.next_event();
     (void)hipEventRecord(a, eng.stream);
   }
-  for (uint32_t k = 0; k < guide->cfg.steps; ++k) {
+  uint32_t steps = guide->cfg.steps;
+  if (guide->cfg.max_steps) steps = (uint32_t)std::min<uint64_t>(steps, guide->cfg.max_steps - guide->st.steps);
+  for (uint32_t k = 0; k < steps; ++k) {
     uint64_t start = ((uint64_t)k * B) % n;
     if (start + B > n) start = 0;
     int rc = guide->step(eng.stream, guide->d_samp + 2 * start, B);

@@ -96,3 +96,26 @@ def test_product_never_imports_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h", "Makefile")):
                 txt = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in re.sub(r"#.*|//.*", "", txt).lower().replace("oracle_", ""), f
+
+
+def test_guide_struct_layouts_match_header():
+    from oxr import _abi as A
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "oxr/guide.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(oxr_guide_config), sizeof(oxr_guide_stats), sizeof(oxr_tune_result),
+         offsetof(oxr_guide_config, max_steps), offsetof(oxr_guide_stats, infer_ms));
+  return 0;
+}
+'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "t.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(td, "t")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out == [C.sizeof(A.GuideConfig), C.sizeof(A.GuideStats), C.sizeof(A.TuneResult),
+                   A.GuideConfig.max_steps.offset, A.GuideStats.infer_ms.offset]
