@@ -257,13 +257,14 @@ int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
 /* Autotuner (no counterpart in the reference): times refine(spp) of the
  * synced scene/camera/options under each candidate launch configuration —
  * paths per wavefront batch in {2M, 4M, 8M, 16M, 32M} (within the device
- * memory cap) x shading-kernel occupancy in {1, 2} waves per SIMD — keeps
- * the fastest, and resets the accumulation. */
+ * memory cap) x shading-kernel occupancy in {1, 2} waves per SIMD x the
+ * traversal queue whole or split into 8 per-XCD segments — keeps the
+ * fastest, and resets the accumulation. */
 typedef struct oxr_tune_result {
   uint32_t path_budget;     /* chosen paths per batch                  */
   uint32_t shade_waves;     /* chosen shading occupancy                */
   uint32_t trials;          /* configurations timed                    */
-  uint32_t pad;
+  uint32_t xcd_segments;    /* chosen traversal queue split (1 or 8)   */
   double paths_per_s;       /* of the chosen configuration             */
   double worst_paths_per_s; /* of the slowest candidate                */
 } oxr_tune_result;

@@ -512,6 +512,7 @@ struct Engine {
   uint32_t path_budget = 1u << 24;
   uint32_t max_budget = 1u << 24;
   int shade_waves = 1;
+  uint32_t xcd_segments = 1;  // persistent traversal queue split per XCD (1 or 8)
   void* d_tmp = nullptr;
   size_t tmp_alloc = 0;
   bool timing = false;
@@ -558,6 +559,7 @@ struct Engine {
       if (v >= 4096 && v <= (1l << 26)) path_budget = (uint32_t)v;
     }
     if (const char* gph = std::getenv("OXR_HIP_GRAPHS")) graphs = std::atoi(gph) != 0;
+    if (const char* x = std::getenv("OXR_XCD")) xcd_segments = std::atoi(x) >= 8 ? 8u : 1u;
     if (const char* w = std::getenv("OXR_SHADE_WAVES")) {
       int v = std::atoi(w);
       if (v >= 1 && v <= 3) shade_waves = v;
@@ -688,6 +690,7 @@ struct Engine {
   struct GraphKey {
     uint32_t max_bounces, nslots, P, gen;
     int grid_shade, shade_waves;
+    uint32_t nseg;
     const DevJob* jobs;
     DevScene sc;
     uint32_t guided;
@@ -705,7 +708,8 @@ struct Engine {
     hipEvent_t e1{};
     span_begin(e1);
     launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, multi, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
-                    &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error);
+                    &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
+                    xcd_segments);
     span_end(0, e1);
     if (gb && k < (uint32_t)G_KREC) {  // the guide serves the first G_KREC bounces
       hipEvent_t eg{};
@@ -721,7 +725,8 @@ struct Engine {
     hipEvent_t e3{};
     span_begin(e3);
     launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray, pb.sidx, pb.meta, nslots, 0,
-                   &d_ctrl->scount, &d_ctrl->work[1], pb.vis, d_spill, spill_cap, &d_ctrl->error);
+                   &d_ctrl->scount, &d_ctrl->work[8], pb.vis, d_spill, spill_cap, &d_ctrl->error,
+                   xcd_segments);
     span_end(1, e3);
   }
 
@@ -737,6 +742,7 @@ struct Engine {
     key.gen = gen;
     key.grid_shade = grid_shade;
     key.shade_waves = shade_waves;
+    key.nseg = xcd_segments;
     key.jobs = d_jobs;
     key.sc = sc_any;
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
@@ -1478,16 +1484,19 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
   const uint64_t npx = (uint64_t)std::max(h->opts.width, 1u) * std::max(h->opts.height, 1u);
   const uint32_t budgets[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25};
   const int waves[] = {1, 2};
+  const uint32_t segs[] = {1u, 8u};
   oxr_tune_result r{};
   double best = -1.0, worst = 1e300;
-  uint32_t best_b = e.path_budget;
+  uint32_t best_b = e.path_budget, best_x = e.xcd_segments;
   int best_w = e.shade_waves;
   for (uint32_t b : budgets) {
     if (b > e.max_budget) continue;
-    for (int w : waves) {
+    for (int w : waves)
+    for (uint32_t x : segs) {
       if (w > 1 && h->guide) continue;  // the guided kernel has one variant
       e.path_budget = b;
       e.shade_waves = w;
+      e.xcd_segments = x;
       int rc;
       if ((rc = h->reset()) || (rc = h->refine(spp))) return rc;  // warm: buffers, graph capture
       HIPC(hipStreamSynchronize(e.stream));
@@ -1502,17 +1511,20 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
         best = pps;
         best_b = b;
         best_w = w;
+        best_x = x;
       }
     }
   }
   e.path_budget = best_b;
   e.shade_waves = best_w;
+  e.xcd_segments = best_x;
   if (e.pb.cap > 2ull * best_b) e.free_paths();  // drop oversized trial buffers
   e.timing = timing;
   int rc = h->reset();
   if (rc) return rc;
   r.path_budget = best_b;
   r.shade_waves = (uint32_t)best_w;
+  r.xcd_segments = best_x;
   r.paths_per_s = best;
   r.worst_paths_per_s = r.trials ? worst : 0.0;
   if (out) *out = r;
@@ -1640,7 +1652,7 @@ static int trace_host(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint32_t flags
   launch::zero_ctrl(e.stream, e.d_ctrl);
   if (occ)
     launch::anyhit(e.stream, e.grid_any, h->scene.sc, nullptr, nullptr, false, false, d_rays, nullptr, nullptr, 1, n, nullptr,
-                   &e.d_ctrl->work[1], (uint8_t*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
+                   &e.d_ctrl->work[8], (uint8_t*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   else
     launch::closest(e.stream, e.grid_closest, h->scene.sc, nullptr, nullptr, false, false, d_rays, nullptr, nullptr, n,
                     nullptr, &e.d_ctrl->work[0], flags & 1u, (uint4*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);

@@ -115,7 +115,7 @@ struct Ctrl {
   uint32_t qcount[2];
   uint32_t scount;
   uint32_t error;
-  uint32_t work[4];
+  uint32_t work[16];  // traversal work counters: closest [0, 8), any-hit [8, 16), one per XCD segment
   unsigned long long rays_closest;
   unsigned long long rays_shadow;
 };

@@ -227,15 +227,25 @@ struct Trav {
 #endif
 constexpr uint32_t REFILL = OXR_REFILL;
 
+// nseg > 1: the queue is split into nseg contiguous segments and block b
+// works on segment b % nseg. Workgroups are placed on the XCDs round-robin,
+// so with nseg = 8 each XCD traces its own run of neighbouring rays and
+// keeps their BVH nodes in its own L2. `work` holds one counter per segment;
+// gridDim.x is a multiple of nseg.
 template <bool ANY, class Setup, class Step, class Finish>
-OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, Step step, Finish finish) {
+OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, Setup setup, Step step,
+                      Finish finish) {
   const uint32_t lane = lane_id();
-  const uint32_t nwaves = gridDim.x * (TRACE_TB / 64);
-  const uint32_t wave = blockIdx.x * (TRACE_TB / 64) + threadIdx.x / 64;
+  const uint32_t seg = blockIdx.x % nseg;
+  const uint32_t nwaves = (gridDim.x / nseg) * (TRACE_TB / 64);
+  const uint32_t wave = (blockIdx.x / nseg) * (TRACE_TB / 64) + threadIdx.x / 64;
+  const uint32_t s0 = (uint32_t)((uint64_t)count * seg / nseg);
+  const uint32_t s1 = (uint32_t)((uint64_t)count * (seg + 1) / nseg);
+  uint32_t* wk = work + seg;
   Trav<ANY> tr;
-  uint32_t ri = wave * 64u + lane;
-  bool active = ri < count;
-  bool exhausted = nwaves * 64u >= count;
+  uint32_t ri = s0 + wave * 64u + lane;
+  bool active = ri < s1;
+  bool exhausted = (uint64_t)s0 + (uint64_t)nwaves * 64u >= s1;
   if (active) setup(tr, ri);
   for (;;) {
     if (active) {
@@ -253,13 +263,13 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, Stack& st, Setup setup, St
     if (nfree >= REFILL) {
       int leader = __ffsll((unsigned long long)freem) - 1;
       uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(work, nfree) + nwaves * 64u;
+      if ((int)lane == leader) base = atomicAdd(wk, nfree) + nwaves * 64u + s0;
       base = __shfl(base, leader);
-      if (base + nfree >= count) exhausted = true;
+      if ((uint64_t)base + nfree >= s1) exhausted = true;
       if (!active) {
         uint64_t below = lane == 0 ? 0ull : (freem & ((~0ull) >> (64 - lane)));
         ri = base + (uint32_t)__popcll(below);
-        if (ri < count) {
+        if (ri < s1) {
           setup(tr, ri);
           active = true;
         }
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
                                                      const uint4* __restrict__ meta, uint32_t fixed_count,
                                                      const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
                                                      uint4* __restrict__ hits, uint32_t* spill, uint32_t spill_cap,
-                                                     uint32_t* err) {
+                                                     uint32_t* err, uint32_t nseg) {
   __shared__ uint32_t s_stack[STACK * TRACE_TB];
   Stack st;
   st.lds = s_stack + threadIdx.x;
@@ -328,7 +338,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
     hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(tr.tmax), prim, __float_as_uint(tr.bu),
                          __float_as_uint(tr.bv));
   };
-  trace_loop<false>(count, work, st, setup, step, finish);
+  trace_loop<false>(count, work, nseg, st, setup, step, finish);
 }
 
 // Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
@@ -340,7 +350,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
                                                     const uint32_t* __restrict__ sidx, const uint4* __restrict__ meta,
                                                     uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr,
                                                     uint32_t* work, uint8_t* __restrict__ out, uint32_t* spill,
-                                                    uint32_t spill_cap, uint32_t* err) {
+                                                    uint32_t spill_cap, uint32_t* err, uint32_t nseg) {
   __shared__ uint32_t s_stack[STACK * TRACE_TB];
   Stack st;
   st.lds = s_stack + threadIdx.x;
@@ -390,7 +400,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
       out[i] = occ ? 1 : 0;
     }
   };
-  trace_loop<true>(count, work, st, setup, step, finish);
+  trace_loop<true>(count, work, nseg, st, setup, step, finish);
 }
 
 // ---------------------------------------------------------------------
@@ -441,11 +451,8 @@ __global__ void k_prep(Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
     if (count_shadow_prev) ctrl->rays_shadow += ctrl->scount;
     ctrl->qcount[cur ^ 1u] = 0;
     ctrl->scount = 0;
-    ctrl->work[0] = 0;
-    ctrl->work[1] = 0;
-    ctrl->work[2] = 0;
-    ctrl->work[3] = 0;
   }
+  if (threadIdx.x < 16) ctrl->work[threadIdx.x] = 0;
 }
 
 // Adds the visible NEE contributions of the previous vertex in light
@@ -989,10 +996,10 @@ __global__ void k_zero_ctrl(Ctrl* c) {
     c->qcount[0] = c->qcount[1] = 0;
     c->scount = 0;
     c->error = 0;
-    c->work[0] = c->work[1] = c->work[2] = c->work[3] = 0;
     c->rays_closest = 0;
     c->rays_shadow = 0;
   }
+  if (threadIdx.x < 16) c->work[threadIdx.x] = 0;
 }
 
 __global__ void k_fill_f64(double* p, size_t n) {
@@ -1197,10 +1204,11 @@ int stack_lds_entries() { return STACK; }
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
-             uint32_t spill_cap, uint32_t* err) {
+             uint32_t spill_cap, uint32_t* err, uint32_t nseg) {
+  if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
 #define OXR_CLOSEST(P, M)                                                                                        \
   hipLaunchKernelGGL((k_closest<P, M>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta,         \
-                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err)
+                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err, nseg)
   if (!path)
     OXR_CLOSEST(false, false);
   else if (multi)
@@ -1213,10 +1221,11 @@ void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, co
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots,
             uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill,
-            uint32_t spill_cap, uint32_t* err) {
+            uint32_t spill_cap, uint32_t* err, uint32_t nseg) {
+  if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
 #define OXR_ANYHIT(P, M)                                                                                         \
   hipLaunchKernelGGL((k_anyhit<P, M>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots, \
-                     fixed_count, count_ptr, work, out, spill, spill_cap, err)
+                     fixed_count, count_ptr, work, out, spill, spill_cap, err, nseg)
   if (!path)
     OXR_ANYHIT(false, false);
   else if (multi)

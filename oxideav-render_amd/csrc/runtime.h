@@ -35,17 +35,20 @@ hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, con
 int trace_blocks_per_cu(bool any);
 int trace_block_threads();
 int stack_lds_entries();
+// work: nseg counters (closest: Ctrl::work, any-hit: Ctrl::work + 8);
+// nseg = 8 splits the queue into per-XCD segments (1 when the grid is not
+// a multiple of it).
 // path: rays of the wavefront (else host rays of the parity entry points);
 // multi: the launch mixes scenes (each ray's scene from its job), else all
 // rays trace `sc`.
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
-             uint32_t spill_cap, uint32_t* err);
+             uint32_t spill_cap, uint32_t* err, uint32_t nseg = 1);
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
             const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,
-            uint32_t* err);
+            uint32_t* err, uint32_t nseg = 1);
 void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl);
 void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev);
 // guide: null for the plain estimator.

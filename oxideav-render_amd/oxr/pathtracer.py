@@ -184,7 +184,7 @@ class PathTracer:
         the fastest; resets the accumulation. Returns the choice."""
         r = A.TuneResult()
         check(self._L.oxr_pt_autotune(self._h, int(spp), C.byref(r)), "autotune")
-        return {f: getattr(r, f) for f, _ in A.TuneResult._fields_ if f != "pad"}
+        return {f: getattr(r, f) for f, _ in A.TuneResult._fields_}
 
     def bvh_info(self):
         b = A.BvhInfo()

@@ -69,9 +69,24 @@ def test_autotune_keeps_results():
     sc = S.material_sweep(segments=16, rings=8, floor_half=6)
     pt = _tracer(sc, 256, 192, 8, bounces=4)
     r = pt.autotune(spp=2)
-    assert r["trials"] >= 4 and r["paths_per_s"] >= r["worst_paths_per_s"] > 0
+    assert r["trials"] >= 8 and r["paths_per_s"] >= r["worst_paths_per_s"] > 0
+    assert r["xcd_segments"] in (1, 8)
     assert pt.samples() == 0
     pt.refine(8)
     ref = _tracer(sc, 256, 192, 8, bounces=4)
     ref.refine(8)
     assert np.array_equal(pt.accum(), ref.accum())
+
+
+def test_xcd_split_traversal_equals_whole_queue(monkeypatch):
+    """The per-XCD queue split changes only which workgroup traces a ray."""
+    from oxr import scenes as S
+    sc = S.material_sweep(segments=16, rings=8, floor_half=6)
+    out = []
+    for x in ("1", "8"):
+        monkeypatch.setenv("OXR_XCD", x)
+        pt = _tracer(sc, 160, 120, 8, bounces=6)
+        pt.refine(8)
+        out.append(pt.accum(with_sum_sq=True))
+        pt.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
