@@ -114,11 +114,18 @@ def serving_bench(n=512, lam=40.0, seed=0, n_soups=4):
     return out
 
 
-def guide_mse_bench(W=1024, target=1e-3, ref_spp=4096, pass_spp=4, max_s=40.0):
+def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=32, max_s=60.0):
     """Time-to-MSE on the glossy-caustic scene, plain vs guided (online
-    training included in the guided time). MSE is against an f64
-    reference of ref_spp plain samples, minus the reference's own
-    estimated variance (an unbiased estimate of the error vs the limit)."""
+    training included in the guided time).
+
+    Both estimators are unbiased, so the expected MSE of the image after N
+    samples per pixel is the mean per-pixel variance of that image:
+    MSE(N) = sum_k sigma_k^2 * s_k / N^2 over the passes k (s_k samples of
+    single-sample variance sigma_k^2, measured from the f64 sums and sums of
+    squares of each pass). The time to MSE `target` is the wall time of the
+    passes until MSE(N) <= target, continued at the last passes' variance
+    and time per sample when the run ends first. The two final images are
+    compared pixel by pixel (z-scores) as the unbiasedness check."""
     import oxr
     from oxr import scenes
     from oxr.guide import Guide
@@ -131,52 +138,71 @@ def guide_mse_bench(W=1024, target=1e-3, ref_spp=4096, pass_spp=4, max_s=40.0):
         o.path_trace.seed = seed
         return o
 
-    ref = oxr.PathTracer(0)
-    ref.sync(sc, opts(ref_spp, 12345))
-    t = time.perf_counter()
-    for _ in range(ref_spp // 64):
-        ref.refine(64)
-    ref_s = time.perf_counter() - t
-    acc, sq = ref.accum(with_sum_sq=True)
-    ref.close()
-    R = acc[..., :3] / acc[..., 3:4]
-    var_ref = float((np.maximum(sq / acc[..., 3:4] - R ** 2, 0.0) / ref_spp).mean())
-
     def run(guide):
         pt = oxr.PathTracer(0)
-        pt.sync(sc, opts(1 << 20, 1))
+        pt.sync(sc, opts(1 << 20, 1 if guide else 2))
         g = None
         if guide:
             g = Guide(0)
             pt.set_guide(g, train=True)
-        pt.refine(pass_spp)  # warm-up: buffers, graph capture
+        pt.refine(pass_spp)  # warm-up: buffers, graph capture (the guide trains on it too)
         pt.reset()
-        render_s, spp, curve, hit = 0.0, 0, [], None
-        while render_s < max_s:
+        if g:  # start from an untrained guide: the training is part of the measured time
+            pt.set_guide(None)
+            g.close()
+            g = Guide(0)
+            pt.set_guide(g, train=True)
+        prev_a = np.zeros((W, W, 4))
+        prev_s = np.zeros((W, W, 3))
+        rows, t_tot = [], 0.0
+        for k in range(passes):
             t = time.perf_counter()
             pt.refine(pass_spp)
-            render_s += time.perf_counter() - t
-            spp += pass_spp
-            if spp & (spp - 1) == 0 or spp % 64 == 0:
-                a = pt.accum()
-                I = a[..., :3] / np.maximum(a[..., 3:4], 1.0)
-                mse = float(np.mean((I - R) ** 2)) - var_ref
-                curve.append((spp, round(render_s, 4), mse))
-                if mse <= target and hit is None:
-                    hit = (spp, render_s)
-                    break
+            dt = time.perf_counter() - t
+            t_tot += dt
+            a, sq = pt.accum(with_sum_sq=True)
+            da, ds = a - prev_a, sq - prev_s
+            prev_a, prev_s = a, sq
+            n = np.maximum(da[..., 3:4], 1.0)
+            m = da[..., :3] / n
+            var = float((np.maximum(ds / n - m * m, 0.0) * n / np.maximum(n - 1.0, 1.0)).mean())
+            rows.append((dt, var))
+            if t_tot > max_s:
+                break
+        n = np.maximum(prev_a[..., 3:4], 1.0)
+        img = prev_a[..., :3] / n
+        var_img = np.maximum(prev_s / n - img * img, 0.0) / n
         st = g.stats() if g else None
         pt.close()
-        return {"time_to_mse_s": hit[1] if hit else None, "spp": hit[0] if hit else None, "curve": curve,
-                "guide": st}
+        # Expected MSE after each pass; time to the target.
+        V, N, T, hit = 0.0, 0, 0.0, None
+        curve = []
+        for dt, var in rows:
+            V += var * pass_spp
+            N += pass_spp
+            T += dt
+            curve.append((N, round(T, 4), V / N ** 2))
+            if hit is None and V / N ** 2 <= target:
+                hit = (N, T, "measured")
+        tail = rows[len(rows) // 2:]
+        var_ss = float(np.mean([v for _, v in tail]))
+        t_spp = float(np.mean([dt for dt, _ in tail])) / pass_spp
+        if hit is None:
+            # V + var_ss * (M - N) <= target * M^2  ->  smallest M.
+            a_, b_, c_ = target, -var_ss, -(V - var_ss * N)
+            M = (-b_ + math.sqrt(b_ * b_ - 4 * a_ * c_)) / (2 * a_)
+            hit = (M, T + (M - N) * t_spp, "extrapolated")
+        return {"time_to_mse_s": hit[1], "spp": hit[0], "how": hit[2], "steady_var_per_sample": var_ss,
+                "s_per_spp": t_spp, "passes": len(rows), "curve": curve[::4], "guide": st}, img, var_img
 
-    plain = run(False)
-    guided = run(True)
-    ratio = (guided["time_to_mse_s"] / plain["time_to_mse_s"]
-             if guided["time_to_mse_s"] and plain["time_to_mse_s"] else None)
-    return {"workload": f"glossy_caustic {W}x{W}, 8 bounces", "target_mse": target, "ref_spp": ref_spp,
-            "ref_s": ref_s, "ref_var_of_mean": var_ref, "plain": plain, "guided": guided,
-            "guided_over_plain_time": ratio}
+    plain, pi, pv = run(False)
+    guided, gi, gv = run(True)
+    z = (gi - pi) / np.sqrt(pv + gv + 1e-30)
+    return {"workload": f"glossy_caustic {W}x{W}, 8 bounces, {pass_spp} spp per pass", "target_mse": target,
+            "plain": plain, "guided": guided,
+            "guided_over_plain_time": guided["time_to_mse_s"] / plain["time_to_mse_s"],
+            "unbiased_check": {"mean_abs_z": float(np.abs(z).mean()), "frac_abs_z_gt_4": float((np.abs(z) > 4).mean()),
+                               "image_mean_plain": float(pi.mean()), "image_mean_guided": float(gi.mean())}}
 
 
 def copy_calibration(mib):
@@ -212,8 +238,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--guide-only", action="store_true")
     ap.add_argument("--guide-width", type=int, default=1024)
-    ap.add_argument("--guide-ref-spp", type=int, default=4096)
-    ap.add_argument("--guide-max-s", type=float, default=40.0)
+    ap.add_argument("--guide-passes", type=int, default=32)
+    ap.add_argument("--guide-max-s", type=float, default=60.0)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
@@ -223,7 +249,7 @@ def main():
         if oxr.device_count() == 0:
             raise SystemExit("bench.py: no HIP device visible")
         print(json.dumps(serving_bench() if a.serving_only else
-                         guide_mse_bench(W=a.guide_width, ref_spp=a.guide_ref_spp, max_s=a.guide_max_s)), flush=True)
+                         guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -317,7 +343,7 @@ def main():
     pt.close()
     # The other two parts of the metric, each reported beside the paths/s
     # value (an error in one is reported, not fatal to the line).
-    guide_leg = lambda: guide_mse_bench(W=a.guide_width, ref_spp=a.guide_ref_spp, max_s=a.guide_max_s)  # noqa: E731
+    guide_leg = lambda: guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s)  # noqa: E731
     for key, skip, fn in (("guided_time_to_mse", a.no_guide, guide_leg),
                           ("serving", a.no_serving, serving_bench)):
         if skip:
