@@ -139,7 +139,7 @@ OXR_D float vmf_pdf(float kappa, float c) {
 
 OXR_D float guide_pdf(const GuideLobes* __restrict__ g, V3 w) {
   float q = 0.0f;
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < G_LOBES; ++k) {
     float4 m = g->mk[k];
     q += g->pi[k] * vmf_pdf(m.w, m.x * w.x + m.y * w.y + m.z * w.z);
@@ -153,7 +153,7 @@ OXR_D V3 guide_sample(const GuideLobes* __restrict__ g, float u0, float u1, floa
   int pick = G_LOBES - 1;
   bool found = false;
   float c = 0.0f;
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < G_LOBES; ++k) {
     c += g->pi[k];
     if (!found && u0 < c) {

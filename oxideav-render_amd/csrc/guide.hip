@@ -181,6 +181,9 @@ __global__ __launch_bounds__(GUIDE_TB) __attribute__((amdgpu_waves_per_eu(4))) v
   for (uint32_t grp = blockIdx.x * (GUIDE_TB / 64) + wave; grp < ngroups; grp += gridDim.x * (GUIDE_TB / 64)) {
     uint32_t i = grp * 16u + q;
     bool valid = i < count;
+    // Queue entries that missed need no lobes (shading never reads them):
+    // a group whose 16 queries all missed is skipped.
+    if (SRC == 1 && !__any(valid && hits[i].y != HIT_MISS)) continue;
     V3 p = valid ? query_pos(SRC, i, pos, qray, hits) : v3(0.0f, 0.0f, 0.0f);
     float feat[8];
     encode4(n, sh.res, p, g, feat);
