@@ -220,6 +220,16 @@ int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* scene);
 int oxr_pt_set_camera(oxr_pt* h, const oxr_camera* cam, int* reset_out);
 int oxr_pt_set_options(oxr_pt* h, const oxr_pt_options* opts, int* reset_out);
 int oxr_pt_reset(oxr_pt* h);
+/* PathTracer::set_environment (src/pathtrace.rs:2449-2454): install an
+ * equirectangular HDR environment (rgba: width*height*4 floats, scene-linear,
+ * row 0 = +Y) scaled by `intensity`, or remove it (rgba = NULL). It replaces
+ * the constant `ambient` sky. Resets the accumulation. */
+int oxr_pt_set_environment(oxr_pt* h, const float* rgba, uint32_t width, uint32_t height, float intensity);
+/* Parity entry points: EnvironmentMap::{pdf, radiance} at unit directions
+ * (3n floats; le 3n) and EnvironmentMap::sample of (u0, u1) pairs (2n floats;
+ * dirs 3n, pdf n, ok n) on the installed map (src/pathtrace.rs:1677-1736). */
+int oxr_pt_env_eval(oxr_pt* h, uint32_t n, const float* dirs, float* pdf, float* le);
+int oxr_pt_env_sample(oxr_pt* h, uint32_t n, const float* u, float* dirs, float* pdf, uint8_t* ok);
 
 /* Add n samples per pixel, continuing the global sample index; a no-op
  * before the first upload. refine(a); refine(b) == refine(a+b) bit for

@@ -56,6 +56,10 @@ def lib():
         L.oxrcpu_sample_spherical_triangle.argtypes = [fp, fp, fp, fp, C.c_float, C.c_float, fp]
         L.oxrcpu_iridescent_fresnel.argtypes = [C.c_float, C.c_float, fp, C.c_float, C.c_float, fp]
         L.oxrcpu_offset_ray_origin.argtypes = [fp, fp, fp]
+        L.oxrcpu_set_environment.argtypes = [C.c_void_p, fp, C.c_uint32, C.c_uint32, C.c_float]
+        L.oxrcpu_env_eval.argtypes = [C.c_void_p, C.c_uint32, fp, fp, fp]
+        L.oxrcpu_env_sample.argtypes = [C.c_void_p, C.c_uint32, fp, fp, fp, bp]
+        L.oxrcpu_env_tables.argtypes = [C.c_void_p, fp, fp, fp, bp]
         gc = C.POINTER(A.GuideConfig)
         L.oxrcpu_guide_resolutions.argtypes = [gc, up, bp]
         L.oxrcpu_bf16_round.restype = C.c_float
@@ -149,6 +153,43 @@ class OracleScene:
         occ = np.zeros(max(n, 1), np.uint8)
         lib().oxrcpu_trace_any(self.h, n, rays, 1 if brute else 0, occ.ctypes.data_as(bp))
         return occ[:n]
+
+    def set_environment(self, rgba, intensity=1.0):
+        """PathTracer::set_environment: rgba (H, W, 4) scene-linear, row 0 =
+        +Y; None removes it."""
+        if rgba is None:
+            lib().oxrcpu_set_environment(self.h, fp(), 0, 0, 0.0)
+            self._env = None
+            return
+        img = np.ascontiguousarray(rgba, dtype=np.float32)
+        self._env = img
+        lib().oxrcpu_set_environment(self.h, img.ctypes.data_as(fp), img.shape[1], img.shape[0], float(intensity))
+
+    def env_eval(self, dirs):
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        pdf = np.zeros(len(d), np.float32)
+        le = np.zeros((len(d), 3), np.float32)
+        lib().oxrcpu_env_eval(self.h, len(d), d.ctypes.data_as(fp), pdf.ctypes.data_as(fp), le.ctypes.data_as(fp))
+        return pdf, le
+
+    def env_sample(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        d = np.zeros((len(u), 3), np.float32)
+        pdf = np.zeros(len(u), np.float32)
+        ok = np.zeros(len(u), np.uint8)
+        lib().oxrcpu_env_sample(self.h, len(u), u.ctypes.data_as(fp), d.ctypes.data_as(fp), pdf.ctypes.data_as(fp),
+                                ok.ctypes.data_as(bp))
+        return d, pdf, ok.astype(bool)
+
+    def env_tables(self):
+        h, w = self._env.shape[:2]
+        mg = np.zeros(h + 1, np.float32)
+        cd = np.zeros(h * (w + 1), np.float32)
+        ms = np.zeros(h * w, np.float32)
+        un = np.zeros(1, np.uint8)
+        lib().oxrcpu_env_tables(self.h, mg.ctypes.data_as(fp), cd.ctypes.data_as(fp), ms.ctypes.data_as(fp),
+                                un.ctypes.data_as(bp))
+        return mg, cd, ms, bool(un[0])
 
     def emissive_table(self):
         L = lib()

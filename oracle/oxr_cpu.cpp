@@ -38,6 +38,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -1215,6 +1216,116 @@ struct BNode {
   uint32_t count;          // 0 = inner (right = left + 1)
 };
 
+// Equirectangular environment with luminance x sin(theta) importance
+// sampling (EnvironmentMap, src/pathtrace.rs:1557-1741).
+struct EnvMap {
+  uint32_t w = 1, h = 1;
+  std::vector<float> img;  // w*h*4, row 0 = +Y
+  float intensity = 1.0f;
+  std::vector<float> marginal, conditional, mass;
+  bool uniform = true;
+
+  // EnvironmentMap::new (:1575-1628).
+  void build(const float* rgba, uint32_t w_, uint32_t h_, float inten) {
+    w = std::max<uint32_t>(w_, 1);
+    h = std::max<uint32_t>(h_, 1);
+    bool valid = rgba && w_ > 0 && h_ > 0;
+    img.assign(rgba ? rgba : nullptr, rgba ? rgba + (size_t)w * h * 4 : nullptr);
+    std::vector<double> weights((size_t)w * h, 0.0);
+    if (valid) {
+      for (uint32_t y = 0; y < h; ++y) {
+        double st = (double)std::sin(PI_F * ((float)y + 0.5f) / (float)h);
+        for (uint32_t x = 0; x < w; ++x) {
+          const float* px = &img[((size_t)y * w + x) * 4];
+          float l = luminance({px[0], px[1], px[2]});
+          weights[(size_t)y * w + x] = (std::isfinite(l) && l > 0.0f) ? (double)l * st : 0.0;
+        }
+      }
+    }
+    double total = 0.0;
+    for (double v : weights) total += v;
+    uniform = !(total > 0.0 && std::isfinite(total));
+    if (uniform)
+      for (size_t i = 0; i < weights.size(); ++i)
+        weights[i] = (double)std::sin(PI_F * ((float)(i / w) + 0.5f) / (float)h);
+    total = 0.0;
+    for (double v : weights) total += v;
+    marginal.assign(1, 0.0f);
+    conditional.clear();
+    double acc = 0.0;
+    for (uint32_t y = 0; y < h; ++y) {
+      double row = 0.0;
+      for (uint32_t x = 0; x < w; ++x) row += weights[(size_t)y * w + x];
+      double racc = 0.0;
+      conditional.push_back(0.0f);
+      for (uint32_t x = 0; x < w; ++x) {
+        racc += weights[(size_t)y * w + x];
+        conditional.push_back(row > 0.0 ? (float)(racc / row) : (float)(x + 1) / (float)w);
+      }
+      acc += row;
+      marginal.push_back((float)(acc / total));
+    }
+    marginal.back() = 1.0f;
+    mass.resize(weights.size());
+    for (size_t i = 0; i < weights.size(); ++i) mass[i] = (float)(weights[i] / total);
+    intensity = std::isfinite(inten) ? fmaxr(inten, 0.0f) : 0.0f;
+  }
+
+  void texel_of(f3 d, uint32_t& x, uint32_t& y, float& st) const {
+    float u = 0.5f + std::atan2(d[0], -d[2]) / (2.0f * PI_F);
+    float th = std::acos(clampr(d[1], -1.0f, 1.0f));
+    float v = th / PI_F;
+    float fx = u * (float)w, fy = v * (float)h;
+    x = std::min<uint32_t>(fx > 0.0f ? (uint32_t)fx : 0u, w - 1);
+    y = std::min<uint32_t>(fy > 0.0f ? (uint32_t)fy : 0u, h - 1);
+    st = std::sin(th);
+  }
+  f3 radiance(f3 d) const {
+    uint32_t x, y;
+    float st;
+    texel_of(d, x, y, st);
+    if (uniform) return {0, 0, 0};
+    const float* p = &img[((size_t)y * w + x) * 4];
+    f3 c = {p[0], p[1], p[2]};
+    if (!finite3(c)) return {0, 0, 0};
+    return scale({fmaxr(c[0], 0.0f), fmaxr(c[1], 0.0f), fmaxr(c[2], 0.0f)}, intensity);
+  }
+  float pdf(f3 d) const {
+    uint32_t x, y;
+    float st;
+    texel_of(d, x, y, st);
+    if (st <= 1.0e-6f) return 0.0f;
+    return mass[(size_t)y * w + x] * (float)(w * h) / (2.0f * PI_F * PI_F * st);
+  }
+  // EnvironmentMap::sample (:1713-1736).
+  bool sample(float u0, float u1, f3& d, f3& le, float& pdf_out) const {
+    auto ppoint = [](const float* c, size_t n, float u) {
+      size_t lo = 0, hi = n;
+      while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (c[mid] <= u) lo = mid + 1;
+        else hi = mid;
+      }
+      return lo;
+    };
+    size_t y = std::min<size_t>(std::max<size_t>(ppoint(marginal.data(), h + 1, u0), 1), h) - 1;
+    const float* row = &conditional[y * (w + 1)];
+    size_t x = std::min<size_t>(std::max<size_t>(ppoint(row, w + 1, u1), 1), w) - 1;
+    float my = (u0 - marginal[y]) / fmaxr(marginal[y + 1] - marginal[y], 1e-12f);
+    float mx = (u1 - row[x]) / fmaxr(row[x + 1] - row[x], 1e-12f);
+    float u = ((float)x + clampr(mx, 0.0f, 0.99999f)) / (float)w;
+    float v = ((float)y + clampr(my, 0.0f, 0.99999f)) / (float)h;
+    float phi = 2.0f * PI_F * (u - 0.5f);
+    float th = PI_F * v;
+    float st = std::sin(th), ct = std::cos(th);
+    d = {st * std::sin(phi), ct, -st * std::cos(phi)};
+    pdf_out = pdf(d);
+    if (!(pdf_out > 0.0f) || !std::isfinite(pdf_out)) return false;
+    le = radiance(d);
+    return true;
+  }
+};
+
 struct Scene {
   uint32_t n = 0;
   std::vector<float> pos;        // 9 per tri
@@ -1231,6 +1342,7 @@ struct Scene {
   std::vector<BNode> nodes;
   std::vector<uint32_t> slots;  // leaf slot -> global id
   float pad = 0.0f;
+  std::shared_ptr<EnvMap> env;  // PathTracer::set_environment (null: the ambient sky)
 
   const float* tri(uint32_t g) const { return &pos[9ull * g]; }
   const PMaterial& mat(uint32_t g) const { return mats[mat_id[g]]; }
@@ -1627,8 +1739,8 @@ struct Integrator {
     if (pt.clamp > 0.0f && m > pt.clamp) return scale(c, pt.clamp / m);
     return c;
   }
-  // No environment map in this restatement: P_env = 0.
-  float p_env() const { return 0.0f; }
+  // Integrator::p_env (src/pathtrace.rs:1915-1921).
+  float p_env() const { return ts.env ? (ts.l_tris.empty() ? 1.0f : 0.5f) : 0.0f; }
 
   bool accept(const Hit& c, const Sampler& s, uint32_t ray, bool primary) const {
     const PMaterial& mat = ts.mat(c.g);
@@ -1716,6 +1828,11 @@ struct Integrator {
         if (k == 0) return false;
         f3 le = {ambient, ambient, ambient};
         float w = 1.0f;
+        if (ts.env) {  // src/pathtrace.rs:2038-2050
+          le = ts.env->radiance(d);
+          w = strategy == OXR_STRATEGY_MIS ? power_heuristic(prev_pdf, p_env() * ts.env->pdf(d))
+              : strategy == OXR_STRATEGY_LIGHT_ONLY ? 0.0f : 1.0f;
+        }
         if (w > 0.0f) radiance = add(radiance, clampc(scale(mul(beta, le), w)));
         break;
       }
@@ -1787,14 +1904,28 @@ struct Integrator {
           radiance = add(radiance, clampc(mul(mul(beta, f), ls.radiance)));
       }
 
-      // NEE: area set (emissive triangles; no environment map here).
+      // NEE: area set (emissive triangles / environment, src/pathtrace.rs:2173-2279).
       auto pl_u = s.pattern(2 + 2 * k);
-      if (strategy != OXR_STRATEGY_BSDF_ONLY && !ts.l_tris.empty()) {
+      if (strategy != OXR_STRATEGY_BSDF_ONLY && (ts.env || !ts.l_tris.empty())) {
         float pe = p_env();
         uint32_t shadow_id = ray_id | 0xffffu;
         if (pl_u[2] < pe) {
-          // environment branch: unreachable without an environment map
-        } else {
+          f3 l, le;
+          float epdf;
+          if (ts.env->sample(pl_u[0], pl_u[1], l, le, epdf)) {
+            float pl = pe * epdf;
+            f3 f;
+            float pb;
+            bsdf.eval(l, f, pb);
+            if (!is_black(f) && !is_black(le) && pl > 0.0f) {
+              float w = strategy == OXR_STRATEGY_MIS ? power_heuristic(pl, pb) : 1.0f;
+              if (visible(spawn(l), l, INF_F, s, shadow_id, 0xffffffffu)) {
+                f3 cc = scale(mul(mul(beta, f), le), w / pl);
+                radiance = add(radiance, clampc(cc));
+              }
+            }
+          }
+        } else if (!ts.l_tris.empty()) {
           float us = pe > 0.0f ? clampr((pl_u[2] - pe) / (1.0f - pe), 0.0f, 0.999999f) : pl_u[2];
           size_t li = light_pick(ts, us);
           uint32_t g = ts.l_tris[li];
@@ -1971,6 +2102,45 @@ oxrcpu_scene* oxrcpu_scene_create(const oxr_scene_desc* d) {
 }
 
 void oxrcpu_scene_destroy(oxrcpu_scene* h) { delete h; }
+
+// PathTracer::set_environment: rgba = w*h*4 floats (row 0 = +Y), or null
+// to remove it.
+void oxrcpu_set_environment(oxrcpu_scene* h, const float* rgba, uint32_t w, uint32_t hgt, float intensity) {
+  if (!rgba) {
+    h->s.env.reset();
+    return;
+  }
+  auto e = std::make_shared<EnvMap>();
+  e->build(rgba, w, hgt, intensity);
+  h->s.env = e;
+}
+
+// EnvironmentMap::{pdf, radiance, sample} of the scene's map.
+void oxrcpu_env_eval(oxrcpu_scene* h, uint32_t n, const float* dirs, float* pdf, float* le) {
+  for (uint32_t i = 0; i < n; ++i) {
+    f3 d = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    pdf[i] = h->s.env->pdf(d);
+    f3 r = h->s.env->radiance(d);
+    for (int c = 0; c < 3; ++c) le[3 * i + c] = r[c];
+  }
+}
+void oxrcpu_env_sample(oxrcpu_scene* h, uint32_t n, const float* u, float* dirs, float* pdf, uint8_t* ok) {
+  for (uint32_t i = 0; i < n; ++i) {
+    f3 d, le;
+    float p = 0.0f;
+    ok[i] = h->s.env->sample(u[2 * i], u[2 * i + 1], d, le, p) ? 1 : 0;
+    for (int c = 0; c < 3; ++c) dirs[3 * i + c] = d[c];
+    pdf[i] = p;
+  }
+}
+// The map's CDF tables (marginal h+1, conditional h*(w+1), mass w*h).
+void oxrcpu_env_tables(oxrcpu_scene* h, float* marginal, float* conditional, float* mass, uint8_t* uniform) {
+  const EnvMap& e = *h->s.env;
+  std::copy(e.marginal.begin(), e.marginal.end(), marginal);
+  std::copy(e.conditional.begin(), e.conditional.end(), conditional);
+  std::copy(e.mass.begin(), e.mass.end(), mass);
+  *uniform = e.uniform;
+}
 
 uint32_t oxrcpu_num_lights(oxrcpu_scene* h) { return (uint32_t)h->s.l_tris.size(); }
 

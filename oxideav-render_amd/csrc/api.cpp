@@ -461,8 +461,10 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   return OXR_OK;
 }
 
-DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const DevScene& sc) {
+DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const DevScene& sc,
+                    const DevEnv* env = nullptr) {
   DevFrame f;
+  std::memset(&f, 0, sizeof(f));
   for (int k = 0; k < 3; ++k) {
     f.eye[k] = cam.eye[k];
     f.forward[k] = cam.forward[k];
@@ -481,8 +483,95 @@ DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const Dev
   f.clamp = std::isfinite(opts.clamp) ? opts.clamp : 0.0f;
   f.ambient = std::isfinite(opts.ambient) ? std::fmax(opts.ambient, 0.0f) : 0.0f;
   f.nslots = sc.num_lights + 1;
+  if (env && env->img) {
+    f.env = *env;
+    f.p_env = sc.num_emissive > 0 ? 0.5f : 1.0f;  // Integrator::p_env (src/pathtrace.rs:1915-1921)
+  }
   return f;
 }
+
+// EnvironmentMap::new (src/pathtrace.rs:1575-1628): luminance x sin(theta)
+// weights in f64, per-row conditional CDFs and the marginal CDF stored f32
+// (last = 1), normalised texel mass; an image with no usable radiance
+// samples uniformly in sin(theta) and radiates nothing.
+struct EnvBuf {
+  float4* d_img = nullptr;
+  float* d_marginal = nullptr;
+  float* d_conditional = nullptr;
+  float* d_mass = nullptr;
+  DevEnv env{};
+
+  ~EnvBuf() { release(); }
+  void release() {
+    dfree(d_img);
+    dfree(d_marginal);
+    dfree(d_conditional);
+    dfree(d_mass);
+    env = DevEnv{};
+  }
+  int upload(const float* rgba, uint32_t w_in, uint32_t h_in, float intensity) {
+    release();
+    const uint32_t w = std::max(w_in, 1u), h = std::max(h_in, 1u);
+    const bool valid = rgba && w_in > 0 && h_in > 0;
+    const float PI = 3.14159265358979323846f;
+    std::vector<double> wt((size_t)w * h, 0.0);
+    std::vector<float4> img((size_t)w * h, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    if (valid) {
+      for (size_t i = 0; i < img.size(); ++i) img[i] = make_float4(rgba[4 * i], rgba[4 * i + 1], rgba[4 * i + 2], 0.0f);
+      for (uint32_t y = 0; y < h; ++y) {
+        double st = (double)std::sin(PI * ((float)y + 0.5f) / (float)h);
+        for (uint32_t x = 0; x < w; ++x) {
+          const float4& c = img[(size_t)y * w + x];
+          float l = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+          wt[(size_t)y * w + x] = (std::isfinite(l) && l > 0.0f) ? (double)l * st : 0.0;
+        }
+      }
+    }
+    double total = 0.0;
+    for (double v : wt) total += v;
+    const bool uniform = !(total > 0.0 && std::isfinite(total));
+    if (uniform) {
+      for (size_t i = 0; i < wt.size(); ++i) wt[i] = (double)std::sin(PI * ((float)(i / w) + 0.5f) / (float)h);
+      total = 0.0;
+      for (double v : wt) total += v;
+    }
+    std::vector<float> marginal(h + 1), conditional((size_t)h * (w + 1)), mass(wt.size());
+    marginal[0] = 0.0f;
+    double acc = 0.0;
+    for (uint32_t y = 0; y < h; ++y) {
+      double row = 0.0;
+      for (uint32_t x = 0; x < w; ++x) row += wt[(size_t)y * w + x];
+      double racc = 0.0;
+      float* c = &conditional[(size_t)y * (w + 1)];
+      c[0] = 0.0f;
+      for (uint32_t x = 0; x < w; ++x) {
+        racc += wt[(size_t)y * w + x];
+        c[x + 1] = row > 0.0 ? (float)(racc / row) : (float)(x + 1) / (float)w;
+      }
+      acc += row;
+      marginal[y + 1] = (float)(acc / total);
+    }
+    marginal[h] = 1.0f;
+    for (size_t i = 0; i < wt.size(); ++i) mass[i] = (float)(wt[i] / total);
+    int rc;
+    if ((rc = dalloc(&d_img, img.size())) || (rc = dalloc(&d_marginal, marginal.size())) ||
+        (rc = dalloc(&d_conditional, conditional.size())) || (rc = dalloc(&d_mass, mass.size())))
+      return rc;
+    HIPC(hipMemcpy(d_img, img.data(), img.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d_marginal, marginal.data(), marginal.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d_conditional, conditional.data(), conditional.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d_mass, mass.data(), mass.size() * sizeof(float), hipMemcpyHostToDevice));
+    env.img = d_img;
+    env.marginal = d_marginal;
+    env.conditional = d_conditional;
+    env.mass = d_mass;
+    env.w = w;
+    env.h = h;
+    env.intensity = std::isfinite(intensity) ? std::fmax(intensity, 0.0f) : 0.0f;
+    env.uniform = uniform ? 1u : 0u;
+    return OXR_OK;
+  }
+};
 
 // What one batch of a guided tracer needs: the network, the shading
 // kernel's view (lobes per queue entry, training records) and the
@@ -1096,6 +1185,7 @@ struct oxr_pt {
   DevJob* d_job = nullptr;
   oxr_guide* guide = nullptr;  // not owned
   bool guide_train = false;
+  EnvBuf env;
 
   ~oxr_pt() {
     (void)hipSetDevice(eng.device);
@@ -1146,7 +1236,7 @@ int oxr_pt::refine(uint32_t n) {
   if (npx > (1ull << 31)) return fail(OXR_ERR_INVALID, "image too large (%llu pixels)", (unsigned long long)npx);
   DevJob job;
   job.sc = scene.sc;
-  job.fr = make_frame(cam, opts, scene.sc);
+  job.fr = make_frame(cam, opts, scene.sc, env.env.img ? &env.env : nullptr);
   job.accum = d_accum;
   job.sumsq = d_sumsq;
   HIPC(hipMemcpyAsync(d_job, &job, sizeof(DevJob), hipMemcpyHostToDevice, eng.stream));
@@ -1529,6 +1619,69 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
   r.worst_paths_per_s = r.trials ? worst : 0.0;
   if (out) *out = r;
   return OXR_OK;
+}
+
+int oxr_pt_set_environment(oxr_pt* h, const float* rgba, uint32_t width, uint32_t height, float intensity) {
+  if (!h) return fail(OXR_ERR_INVALID, "null handle");
+  HIPC(hipSetDevice(h->eng.device));
+  HIPC(hipStreamSynchronize(h->eng.stream));
+  if (!rgba) {
+    h->env.release();
+  } else {
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 28))
+      return fail(OXR_ERR_INVALID, "environment size %ux%u", width, height);
+    int rc = h->env.upload(rgba, width, height, intensity);
+    if (rc) return rc;
+  }
+  return h->reset();
+}
+
+namespace {
+int env_common(oxr_pt* h, uint32_t n, int do_sample, const float* in, float* dirs, float* pdf, float* le,
+               uint8_t* ok) {
+  if (!h || (n && (!in || !pdf))) return fail(OXR_ERR_INVALID, "null argument");
+  if (!h->env.env.img) return fail(OXR_ERR_STATE, "no environment map installed");
+  if (n == 0) return OXR_OK;
+  HIPC(hipSetDevice(h->eng.device));
+  const size_t nin = (size_t)n * (do_sample ? 2 : 3);
+  float *d_in = nullptr, *d_dirs = nullptr, *d_pdf = nullptr, *d_le = nullptr;
+  uint8_t* d_ok = nullptr;
+  int rc;
+  if ((rc = dalloc(&d_in, nin)) || (rc = dalloc(&d_dirs, 3 * (size_t)n)) || (rc = dalloc(&d_pdf, n)) ||
+      (rc = dalloc(&d_le, 3 * (size_t)n)) || (rc = dalloc(&d_ok, n))) {
+    dfree(d_in);
+    dfree(d_dirs);
+    dfree(d_pdf);
+    dfree(d_le);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(d_in, in, nin * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch::env(h->eng.stream, h->env.env, n, do_sample, d_in, d_dirs, d_pdf, d_le, d_ok);
+    e = hipStreamSynchronize(h->eng.stream);
+  }
+  if (e == hipSuccess) e = hipMemcpy(pdf, d_pdf, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && le) e = hipMemcpy(le, d_le, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && dirs && do_sample)
+    e = hipMemcpy(dirs, d_dirs, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ok && do_sample) e = hipMemcpy(ok, d_ok, n, hipMemcpyDeviceToHost);
+  dfree(d_in);
+  dfree(d_dirs);
+  dfree(d_pdf);
+  dfree(d_le);
+  dfree(d_ok);
+  if (e != hipSuccess) return fail(OXR_ERR_DEVICE, "environment query: %s", hipGetErrorString(e));
+  return OXR_OK;
+}
+}  // namespace
+
+int oxr_pt_env_eval(oxr_pt* h, uint32_t n, const float* dirs, float* pdf, float* le) {
+  return env_common(h, n, 0, dirs, nullptr, pdf, le, nullptr);
+}
+
+int oxr_pt_env_sample(oxr_pt* h, uint32_t n, const float* u, float* dirs, float* pdf, uint8_t* ok) {
+  if (!dirs || !ok) return fail(OXR_ERR_INVALID, "null argument");
+  return env_common(h, n, 1, u, dirs, pdf, nullptr, ok);
 }
 
 int oxr_pt_samples(oxr_pt* h, uint32_t* out) {

@@ -67,6 +67,19 @@ struct DevScene {
 constexpr uint32_t TI_HAS_NORMALS = 0x80000000u;
 constexpr uint32_t TI_MAT_MASK = 0x7fffffffu;
 
+// Equirectangular environment map (EnvironmentMap, src/pathtrace.rs:
+// 1557-1741): texels (row 0 = +Y), the luminance x sin(theta) CDFs and the
+// per-texel mass. img == nullptr: no map (the constant ambient sky).
+struct DevEnv {
+  const float4* img;
+  const float* marginal;     // h + 1
+  const float* conditional;  // h * (w + 1)
+  const float* mass;         // w * h
+  uint32_t w, h;
+  float intensity;
+  uint32_t uniform;
+};
+
 // Camera (src/camera.rs:61-91) + the integrator constants.
 struct DevFrame {
   float eye[3], forward[3], side[3], up[3];
@@ -76,6 +89,8 @@ struct DevFrame {
   uint32_t max_bounces, rr_start, seed, strategy;
   float clamp, ambient;
   uint32_t nslots;  // NEE slots per path: punctual lights + 1 area
+  float p_env;      // Integrator::p_env: 0, 1/2 or 1
+  DevEnv env;
 };
 
 // Path flags (PathState meta.z >> 16).

@@ -475,11 +475,11 @@ OXR_D V3 clampc(V3 c, float cl) {
 }
 
 // Light-sampling pdf of emissive triangle g (Integrator::tri_pdf,
-// src/pathtrace.rs:1972-1991); P_env = 0 (no environment map).
-OXR_D float tri_pdf(const DevScene& sc, uint32_t li, uint32_t g, V3 p, V3 q) {
+// src/pathtrace.rs:1972-1991).
+OXR_D float tri_pdf(const DevScene& sc, float p_env, uint32_t li, uint32_t g, V3 p, V3 q) {
   const float* tv = sc.pos + 9ull * g;
   V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
-  float sel = (1.0f - 0.0f) * sc.l_pmf[li];
+  float sel = (1.0f - p_env) * sc.l_pmf[li];
   float omega = triangle_solid_angle(p, a, b, c);
   if (omega >= 1.0e-4f) return sel / omega;
   V3 cr = cross(sub(b, a), sub(c, a));
@@ -490,6 +490,69 @@ OXR_D float tri_pdf(const DevScene& sc, uint32_t li, uint32_t g, V3 p, V3 q) {
   float cos = fabsf(dot(cr, to) / (len * sqrtf(d2)));
   if (cos <= 0.0f) return 0.0f;
   return sel * d2 / (0.5f * len * cos);
+}
+
+// ---------------------------------------------------------------------
+// Environment map (EnvironmentMap::{texel_of, radiance, pdf, sample},
+// src/pathtrace.rs:1677-1736).
+// ---------------------------------------------------------------------
+OXR_D void env_texel(const DevEnv& e, V3 d, uint32_t& x, uint32_t& y, float& st) {
+  float u = 0.5f + atan2f(d.x, -d.z) / (2.0f * PI_F);
+  float th = acosf(clampr(d.y, -1.0f, 1.0f));
+  float v = th / PI_F;
+  float fx = u * (float)e.w, fy = v * (float)e.h;
+  x = fx > 0.0f ? (uint32_t)fx : 0u;
+  y = fy > 0.0f ? (uint32_t)fy : 0u;
+  x = x < e.w - 1u ? x : e.w - 1u;
+  y = y < e.h - 1u ? y : e.h - 1u;
+  st = sinf(th);
+}
+OXR_D V3 env_radiance(const DevEnv& e, V3 d) {
+  uint32_t x, y;
+  float st;
+  env_texel(e, d, x, y, st);
+  if (e.uniform) return v3(0.0f, 0.0f, 0.0f);
+  float4 c = e.img[(size_t)y * e.w + x];
+  if (!(isfinite(c.x) && isfinite(c.y) && isfinite(c.z))) return v3(0.0f, 0.0f, 0.0f);
+  return scale(v3(fmaxr(c.x, 0.0f), fmaxr(c.y, 0.0f), fmaxr(c.z, 0.0f)), e.intensity);
+}
+OXR_D float env_pdf(const DevEnv& e, V3 d) {
+  uint32_t x, y;
+  float st;
+  env_texel(e, d, x, y, st);
+  if (st <= 1.0e-6f) return 0.0f;
+  return e.mass[(size_t)y * e.w + x] * (float)(e.w * e.h) / (2.0f * PI_F * PI_F * st);
+}
+// partition_point(c <= u) over n entries.
+OXR_D uint32_t cdf_point(const float* c, uint32_t n, float u) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (c[mid] <= u)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+OXR_D bool env_sample(const DevEnv& e, float u0, float u1, V3& d, V3& le, float& pdf) {
+  uint32_t y = cdf_point(e.marginal, e.h + 1u, u0);
+  y = (y < 1u ? 1u : (y > e.h ? e.h : y)) - 1u;
+  const float* row = e.conditional + (size_t)y * (e.w + 1u);
+  uint32_t x = cdf_point(row, e.w + 1u, u1);
+  x = (x < 1u ? 1u : (x > e.w ? e.w : x)) - 1u;
+  float my = (u0 - e.marginal[y]) / fmaxr(e.marginal[y + 1] - e.marginal[y], 1e-12f);
+  float mx = (u1 - row[x]) / fmaxr(row[x + 1] - row[x], 1e-12f);
+  float u = ((float)x + clampr(mx, 0.0f, 0.99999f)) / (float)e.w;
+  float v = ((float)y + clampr(my, 0.0f, 0.99999f)) / (float)e.h;
+  float phi = 2.0f * PI_F * (u - 0.5f);
+  float th = PI_F * v;
+  float st = sinf(th), ct = cosf(th);
+  d = v3(st * sinf(phi), ct, -st * cosf(phi));
+  pdf = env_pdf(e, d);
+  if (!(pdf > 0.0f) || !isfinite(pdf)) return false;
+  le = env_radiance(e, d);
+  return true;
 }
 
 OXR_D void load_mat(const DevMaterial& m, V3 sn, MatSample& ms) {
@@ -588,8 +651,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       if (k == 0) {
         flags |= PF_UNCOVERED;
       } else {
+        // The environment map when installed, else the constant sky
+        // (src/pathtrace.rs:2038-2050).
         V3 le = v3(fr.ambient, fr.ambient, fr.ambient);
-        radiance = add(radiance, clampc(scale(mul(beta, le), 1.0f), fr.clamp));
+        float w = 1.0f;
+        if (fr.env.img) {
+          le = env_radiance(fr.env, d);
+          w = fr.strategy == 0u ? power_heuristic(prev_pdf, fr.p_env * env_pdf(fr.env, d))
+                                : (fr.strategy == 1u ? 0.0f : 1.0f);
+        }
+        if (w > 0.0f) radiance = add(radiance, clampc(scale(mul(beta, le), w), fr.clamp));
       }
       alive = false;
     }
@@ -629,7 +700,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
           radiance = add(radiance, mul(beta, em));
         } else {
           uint32_t li = sc.l_lookup[g];
-          float pl = li == 0xffffffffu ? 0.0f : tri_pdf(sc, li, g, prev_pos, p);
+          float pl = li == 0xffffffffu ? 0.0f : tri_pdf(sc, fr.p_env, li, g, prev_pos, p);
           float w;
           if (pl <= 0.0f)
             w = 1.0f;
@@ -738,15 +809,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         any_shadow = true;
       }
     }
-    // NEE: one sample of the emissive triangles (src/pathtrace.rs:2173-2279).
+    // NEE: one sample of the area set — the environment map with
+    // probability P_env, else an emissive triangle (src/pathtrace.rs:2173-2279).
     {
       bool emit = false;
       V3 o2, l, contrib;
       float tmax = 0.0f;
       uint32_t skip = NO_SKIP;
-      if (alive && fr.strategy != 2u && sc.num_emissive > 0) {
-        F4 pl_u = pattern(ps, index, 2 + 2 * k);
-        float us = pl_u.v[2];
+      F4 pl_u;
+      const float pe = fr.p_env;
+      const bool area = alive && fr.strategy != 2u && (fr.env.img != nullptr || sc.num_emissive > 0);
+      if (area) pl_u = pattern(ps, index, 2 + 2 * k);
+      if (area && pl_u.v[2] < pe) {
+        V3 le;
+        float epdf;
+        if (env_sample(fr.env, pl_u.v[0], pl_u.v[1], l, le, epdf)) {
+          float pl = pe * epdf;
+          V3 f;
+          float pb;
+          bsdf.eval(l, f, pb);
+          if (!is_black(f) && !is_black(le) && pl > 0.0f) {
+            if (GUIDE && gmix > 0.0f) pb = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pb;
+            float w = fr.strategy == 0u ? power_heuristic(pl, pb) : 1.0f;
+            o2 = spawn(p, ng, l);
+            tmax = __builtin_huge_valf();
+            contrib = clampc(scale(mul(mul(beta, f), le), w / pl), fr.clamp);
+            emit = true;
+          }
+        }
+      } else if (area && sc.num_emissive > 0) {
+        float us = pe > 0.0f ? clampr((pl_u.v[2] - pe) / (1.0f - pe), 0.0f, 0.999999f) : pl_u.v[2];
         // EmissiveLights::pick: partition_point(c <= u).min(len - 1).
         uint32_t lo = 0, hi = sc.num_emissive;
         while (lo < hi) {
@@ -793,7 +885,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             float dist = sqrtf(dist2);
             l = scale(to, 1.0f / dist);
             float cos_l = -dot(lng, l);
-            float pl = tri_pdf(sc, li, lg, p, q);
+            float pl = tri_pdf(sc, fr.p_env, li, lg, p, q);
             bool lds = (LM.flags & MAT_DS) != 0;
             if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && pl > 0.0f && isfinite(pl)) {
               V3 f;
@@ -1133,6 +1225,31 @@ OXR_D void ms_from_abi(const float* m, MatSample& ms) {
   ms.diffuse_transmission_color = ld3(m + 32);
 }
 
+// Environment map evaluation / sampling on host inputs (parity entry
+// points): in = directions (eval) or (u0, u1) pairs (sample).
+__global__ void k_env(DevEnv e, uint32_t n, int do_sample, const float* __restrict__ in, float* __restrict__ dirs,
+                      float* __restrict__ pdf, float* __restrict__ le, uint8_t* __restrict__ ok) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V3 d, r = v3(0.0f, 0.0f, 0.0f);
+  float p = 0.0f;
+  if (do_sample) {
+    bool got = env_sample(e, in[2 * i], in[2 * i + 1], d, r, p);
+    ok[i] = got ? 1 : 0;
+    dirs[3 * i] = d.x;
+    dirs[3 * i + 1] = d.y;
+    dirs[3 * i + 2] = d.z;
+  } else {
+    d = ld3(in + 3 * i);
+    p = env_pdf(e, d);
+    r = env_radiance(e, d);
+  }
+  pdf[i] = p;
+  le[3 * i] = r.x;
+  le[3 * i + 1] = r.y;
+  le[3 * i + 2] = r.z;
+}
+
 __global__ void k_bsdf(uint32_t n, int do_sample, const float* __restrict__ ms_abi, const float* __restrict__ v,
                        const float* __restrict__ ng, const uint8_t* __restrict__ front, const float* __restrict__ in,
                        float* __restrict__ l_out, float* __restrict__ f_out, float* __restrict__ pdf_out,
@@ -1304,6 +1421,11 @@ void resolve(hipStream_t s, bool rgba8, uint32_t npx, uint32_t samples, const do
 
 void sobol(hipStream_t s, uint32_t n, const uint32_t* idx, const uint32_t* seed, float4* out) {
   hipLaunchKernelGGL(k_sobol, dim3((n + 255) / 256), dim3(256), 0, s, n, idx, seed, out);
+}
+
+void env(hipStream_t s, const DevEnv& e, uint32_t n, int do_sample, const float* in, float* dirs, float* pdf,
+         float* le, uint8_t* ok) {
+  hipLaunchKernelGGL(k_env, dim3((n + 127) / 128), dim3(128), 0, s, e, n, do_sample, in, dirs, pdf, le, ok);
 }
 
 void bsdf(hipStream_t s, uint32_t n, int do_sample, const float* ms, const float* v, const float* ng,

@@ -65,6 +65,8 @@ void zero_ctrl(hipStream_t s, Ctrl* c);
 void resolve(hipStream_t s, bool rgba8, uint32_t npx, uint32_t samples, const double* accum, const float bg[4],
              const uint8_t bgb[4], uint32_t tone_map, float exposure, void* out);
 void sobol(hipStream_t s, uint32_t n, const uint32_t* idx, const uint32_t* seed, float4* out);
+void env(hipStream_t s, const DevEnv& e, uint32_t n, int do_sample, const float* in, float* dirs, float* pdf,
+         float* le, uint8_t* ok);
 void bsdf(hipStream_t s, uint32_t n, int do_sample, const float* ms, const float* v, const float* ng,
           const uint8_t* front, const float* in, float* l, float* f, float* pdf, uint8_t* ok);
 }  // namespace launch

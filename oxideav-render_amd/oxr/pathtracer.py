@@ -162,6 +162,43 @@ class PathTracer:
                                            out.ctypes.data_as(C.POINTER(C.c_uint8))), "resolve_rgba8")
         return out
 
+    # -- environment map ------------------------------------------------------
+    def set_environment(self, rgba, intensity=1.0):
+        """PathTracer::set_environment (src/pathtrace.rs:2449-2454): an
+        equirectangular HDR image (H, W, 4), scene-linear, row 0 = +Y, scaled
+        by `intensity`; None removes it. Replaces the ambient sky; resets."""
+        fpp = C.POINTER(C.c_float)
+        if rgba is None:
+            self._env = None
+            check(self._L.oxr_pt_set_environment(self._h, fpp(), 0, 0, 0.0), "set_environment")
+            return
+        img = np.ascontiguousarray(rgba, dtype=np.float32)
+        if img.ndim != 3 or img.shape[2] != 4:
+            raise InvalidOptions("environment image must be (H, W, 4)")
+        self._env = img
+        check(self._L.oxr_pt_set_environment(self._h, img.ctypes.data_as(fpp), img.shape[1], img.shape[0],
+                                             float(intensity)), "set_environment")
+
+    def env_eval(self, dirs):
+        fpp = C.POINTER(C.c_float)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        pdf = np.zeros(len(d), np.float32)
+        le = np.zeros((len(d), 3), np.float32)
+        check(self._L.oxr_pt_env_eval(self._h, len(d), d.ctypes.data_as(fpp), pdf.ctypes.data_as(fpp),
+                                      le.ctypes.data_as(fpp)), "env_eval")
+        return pdf, le
+
+    def env_sample(self, u):
+        fpp = C.POINTER(C.c_float)
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        d = np.zeros((len(u), 3), np.float32)
+        pdf = np.zeros(len(u), np.float32)
+        ok = np.zeros(len(u), np.uint8)
+        check(self._L.oxr_pt_env_sample(self._h, len(u), u.ctypes.data_as(fpp), d.ctypes.data_as(fpp),
+                                        pdf.ctypes.data_as(fpp), ok.ctypes.data_as(C.POINTER(C.c_uint8))),
+              "env_sample")
+        return d, pdf, ok.astype(bool)
+
     # -- learned path guide ----------------------------------------------------
     def set_guide(self, guide, train=True):
         """Attach an oxr.guide.Guide (None detaches); resets the
