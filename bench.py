@@ -84,7 +84,7 @@ def cpu_baseline(scene, cfg, ambient, target_s=12.0):
                       f"({paths} paths, {dt:.1f} s, band-parallel f64 accumulate)"}
 
 
-def serving_bench(n=512, lam=40.0, seed=0, n_soups=4):
+def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sustained=256):
     """The serving mix: n requests with Poisson arrivals at rate lam, each
     drawing the Cornell box, the material sweep or one of n_soups seeded
     triangle soups (N log-uniform 1k..2M), resolution in {128..1024}^2, spp
@@ -110,6 +110,18 @@ def serving_bench(n=512, lam=40.0, seed=0, n_soups=4):
     out["scene_setup_s"] = setup_s
     out["soup_tris"] = [x.num_tris for x in soups]
     out["sustained"] = out["jobs_per_s"] >= 0.95 * out["offered_rate"]
+    paths = [r.width * r.height * r.spp for r in trace]
+    out["offered_paths_per_s"] = float(sum(paths)) / max(trace[-1].arrival_s, 1e-9)
+    if sustained_lam:
+        # The same request mix at a rate the device sustains: latency with
+        # the queue not growing.
+        tr2 = make_trace(n=n_sustained, lam=sustained_lam, n_scenes=len(sc), seed=seed + 1)
+        for r, p_, w_ in zip(tr2, pick, which):
+            r.scene = int(p_) if p_ < 2 else 2 + int(w_)
+        o2 = run_requests(srv, ids, sc, tr2)
+        o2["sustained"] = o2["jobs_per_s"] >= 0.95 * o2["offered_rate"]
+        o2.pop("server", None)
+        out["at_lower_rate"] = o2
     srv.close()
     return out
 

@@ -106,9 +106,8 @@ def test_full_bounce_with_environment_within_3_sigma():
     pt.refine(spp)
     ag, sg = pt.accum(with_sum_sq=True)
     ac, sc = os_.render(W, W, 0, spp, oo, sum_sq=True)
-    n = ac[..., 3:4]
+    n = np.maximum(ac[..., 3:4], 1.0)
     mg, mc = ag[..., :3] / n, ac[..., :3] / n
-    n = np.maximum(n, 1.0)
     var = (np.maximum(sg / n - mg ** 2, 0) + np.maximum(sc / n - mc ** 2, 0)) / n
     z = np.abs(mg - mc) / np.sqrt(var + 1e-20)
     assert (z > 3.0).mean() < 0.01
