@@ -84,7 +84,7 @@ def cpu_baseline(scene, cfg, ambient, target_s=12.0):
                       f"({paths} paths, {dt:.1f} s, band-parallel f64 accumulate)"}
 
 
-def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sustained=256):
+def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sustained=128):
     """The serving mix: n requests with Poisson arrivals at rate lam, each
     drawing the Cornell box, the material sweep or one of n_soups seeded
     triangle soups (N log-uniform 1k..2M), resolution in {128..1024}^2, spp
@@ -126,7 +126,7 @@ def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sust
     return out
 
 
-def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=32, max_s=60.0):
+def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=32, max_s=30.0):
     """Time-to-MSE on the glossy-caustic scene, plain vs guided (online
     training included in the guided time).
 
@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--guide-only", action="store_true")
     ap.add_argument("--guide-width", type=int, default=1024)
     ap.add_argument("--guide-passes", type=int, default=32)
-    ap.add_argument("--guide-max-s", type=float, default=60.0)
+    ap.add_argument("--guide-max-s", type=float, default=30.0)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
