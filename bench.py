@@ -126,7 +126,10 @@ def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sust
     return out
 
 
-def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=32, max_s=30.0):
+GUIDE_BENCH_CONFIG = {"max_steps": 96}
+
+
+def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=48, max_s=30.0, guide_cfg=None):
     """Time-to-MSE on the glossy-caustic scene, plain vs guided (online
     training included in the guided time).
 
@@ -150,19 +153,21 @@ def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=32, max_s=30.0):
         o.path_trace.seed = seed
         return o
 
+    gcfg = dict(GUIDE_BENCH_CONFIG if guide_cfg is None else guide_cfg)
+
     def run(guide):
         pt = oxr.PathTracer(0)
         pt.sync(sc, opts(1 << 20, 1 if guide else 2))
         g = None
         if guide:
-            g = Guide(0)
+            g = Guide(0, **gcfg)
             pt.set_guide(g, train=True)
         pt.refine(pass_spp)  # warm-up: buffers, graph capture (the guide trains on it too)
         pt.reset()
         if g:  # start from an untrained guide: the training is part of the measured time
             pt.set_guide(None)
             g.close()
-            g = Guide(0)
+            g = Guide(0, **gcfg)
             pt.set_guide(g, train=True)
         prev_a = np.zeros((W, W, 4))
         prev_s = np.zeros((W, W, 3))
@@ -211,7 +216,7 @@ def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=32, max_s=30.0):
     guided, gi, gv = run(True)
     z = (gi - pi) / np.sqrt(pv + gv + 1e-30)
     return {"workload": f"glossy_caustic {W}x{W}, 8 bounces, {pass_spp} spp per pass", "target_mse": target,
-            "plain": plain, "guided": guided,
+            "guide_config": gcfg, "plain": plain, "guided": guided,
             "guided_over_plain_time": guided["time_to_mse_s"] / plain["time_to_mse_s"],
             "unbiased_check": {"mean_abs_z": float(np.abs(z).mean()), "frac_abs_z_gt_4": float((np.abs(z) > 4).mean()),
                                "image_mean_plain": float(pi.mean()), "image_mean_guided": float(gi.mean())}}
@@ -250,7 +255,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--guide-only", action="store_true")
     ap.add_argument("--guide-width", type=int, default=1024)
-    ap.add_argument("--guide-passes", type=int, default=32)
+    ap.add_argument("--guide-passes", type=int, default=48)
+    ap.add_argument("--guide-config", default=None, help="JSON overrides of the guide configuration")
     ap.add_argument("--guide-max-s", type=float, default=30.0)
     a = ap.parse_args()
     if a.copy_calibration:
@@ -261,11 +267,12 @@ def main():
         if oxr.device_count() == 0:
             raise SystemExit("bench.py: no HIP device visible")
         print(json.dumps(serving_bench() if a.serving_only else
-                         guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s)), flush=True)
+                         guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s,
+                                        guide_cfg=json.loads(a.guide_config) if a.guide_config else None)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
-              "the path shards by sample range, see DESIGN.md", file=sys.stderr)
+              "the path shards by sample range across GPUs", file=sys.stderr)
         raise SystemExit(2)
 
     import oxr
@@ -355,7 +362,8 @@ def main():
     pt.close()
     # The other two parts of the metric, each reported beside the paths/s
     # value (an error in one is reported, not fatal to the line).
-    guide_leg = lambda: guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s)  # noqa: E731
+    guide_leg = lambda: guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s,
+                                        guide_cfg=json.loads(a.guide_config) if a.guide_config else None)  # noqa: E731
     for key, skip, fn in (("guided_time_to_mse", a.no_guide, guide_leg),
                           ("serving", a.no_serving, serving_bench)):
         if skip:

@@ -31,6 +31,10 @@
  *   p = mix * q + (1 - mix) * p_bsdf.
  *
  * NEE's MIS weights use the same p, so the estimator remains unbiased.
+ * Russian roulette on a guided path decides on the throughput the BSDF
+ * sampler alone would have produced (beta * p / p_bsdf per guided vertex):
+ * a guided direction into a bright region has a small f / p by design, and
+ * throughput roulette would end most such paths and scale up the rest.
  *
  * Parameters are one f32 array in this order:
  *
@@ -60,6 +64,8 @@ extern "C" {
 #define OXR_GUIDE_HIDDEN 64
 #define OXR_GUIDE_OUTPUTS 48
 #define OXR_GUIDE_LOBES 8
+/* Largest oxr_guide_config.bounces: vertices past it sample the BSDF alone. */
+#define OXR_GUIDE_MAX_BOUNCES 4
 /* f32 count of the MLP part of the parameter array. */
 #define OXR_GUIDE_MLP_PARAMS (64 * 32 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 48 * 64 + 48)
 
@@ -80,6 +86,7 @@ typedef struct oxr_guide_config {
   uint32_t steps;        /* online training steps after each refine pass (4) */
   uint32_t record_shift; /* online samples come from paths with id % 2^shift == 0 (3) */
   uint32_t max_steps;    /* online training stops after this many Adam steps; 0 = never (0) */
+  uint32_t bounces;      /* eligible vertices at bounce k < bounces use the guide, 1..OXR_GUIDE_MAX_BOUNCES (1) */
 } oxr_guide_config;
 
 typedef struct oxr_guide_stats {
@@ -120,6 +127,11 @@ int oxr_guide_gradient(oxr_guide* g, uint32_t n, const float* pos, const float* 
 int oxr_guide_train(oxr_guide* g, uint32_t n, const float* pos, const float* dir, const float* weight,
                     uint32_t steps, double* loss);
 int oxr_guide_stats_get(oxr_guide* g, oxr_guide_stats* out);
+/* The training samples the last online pass collected (inspection and
+ * tests): up to `max` samples of 8 floats each into `out` (may be null),
+ * (position.xyz, weight, direction.xyz, pdf of the proposal that drew the
+ * direction); *n receives the number available. */
+int oxr_guide_collected(oxr_guide* g, uint32_t max, float* out, uint32_t* n);
 
 /* Attach a guide to a tracer (null detaches). With `train` != 0 every
  * refine pass collects training samples and runs cfg.steps Adam steps after

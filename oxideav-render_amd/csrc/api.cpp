@@ -800,7 +800,7 @@ struct Engine {
                     &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
                     xcd_segments);
     span_end(0, e1);
-    if (gb && k < (uint32_t)G_KREC) {  // the guide serves the first G_KREC bounces
+    if (gb && k < gb->use.kmax) {  // the guide serves the first kmax bounces
       hipEvent_t eg{};
       span_begin(eg);
       glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur], P, nullptr, pb.qray[cur], pb.hits,
@@ -973,6 +973,9 @@ struct oxr_guide {
   float4* d_samp = nullptr;
   uint32_t samp_cap = 0;
   uint32_t* d_count = nullptr;
+  uint32_t last_collected = 0;  // samples of the last online pass held in d_samp
+  float4* d_batch = nullptr;     // one strided training batch of them
+  uint32_t batch_cap = 0;
   float* d_scal = nullptr;  // inv weight sum, loss
   uint16_t* d_xs = nullptr;
   uint16_t* d_ds = nullptr;
@@ -996,6 +999,7 @@ struct oxr_guide {
     dfree(d_grad);
     dfree(d_frag);
     dfree(d_samp);
+    dfree(d_batch);
     dfree(d_count);
     dfree(d_scal);
     dfree(d_xs);
@@ -1073,6 +1077,7 @@ struct oxr_guide {
     if (n <= samp_cap) return OXR_OK;
     dfree(d_samp);
     samp_cap = 0;
+    last_collected = 0;
     int rc = dalloc(&d_samp, 2 * (size_t)n);
     if (rc) return rc;
     samp_cap = n;
@@ -1130,12 +1135,23 @@ struct oxr_guide {
     return OXR_OK;
   }
 
+  int ensure_batch(uint32_t n) {
+    if (n <= batch_cap) return OXR_OK;
+    dfree(d_batch);
+    batch_cap = 0;
+    int rc = dalloc(&d_batch, 2 * (size_t)n);
+    if (rc) return rc;
+    batch_cap = n;
+    return OXR_OK;
+  }
+
   // Host batch -> device samples (padded with zero-weight samples to a
   // multiple of 32); returns the padded count.
   int upload_batch(uint32_t n, const float* pos, const float* dir, const float* w, uint32_t* padded) {
     uint32_t np = (n + 31u) & ~31u;
     int rc = ensure_samples(np);
     if (rc) return rc;
+    last_collected = 0;  // the host batch replaces the collected samples
     std::vector<float4> h(2 * (size_t)np, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (uint32_t i = 0; i < n; ++i) {
       h[2 * i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w ? w[i] : 0.0f);
@@ -1268,7 +1284,7 @@ int oxr_pt::refine(uint32_t n) {
     }
     gbuf.net = guide->net;
     gbuf.use = DevGuideUse{guide->d_lobes, train_now ? guide->d_rec : nullptr, guide->cfg.mix,
-                           guide->cfg.record_shift};
+                           guide->cfg.record_shift, std::min<uint32_t>(guide->cfg.bounces, (uint32_t)G_KREC)};
     gbuf.samp = guide->d_samp;
     gbuf.count = guide->d_count;
     gbuf.samp_cap = guide->samp_cap;
@@ -1300,15 +1316,21 @@ int oxr_pt::refine(uint32_t n) {
   return OXR_OK;
 }
 
-// Online training after a refine pass: cfg.steps Adam steps over windows
-// of the samples the pass collected.
+// Online training after a refine pass: cfg.steps Adam steps, each over a
+// strided subset of all the samples the pass collected. The collection
+// order follows the paths (image rows, then bounces), so a contiguous
+// window would train each step on one band of the image.
 int oxr_pt::train_guide() {
   uint32_t count = 0;
   HIPC(hipMemcpy(&count, guide->d_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
   guide->st.collected += count;
+  guide->last_collected = std::min(count, guide->samp_cap);
   uint32_t n = std::min(count, guide->samp_cap) & ~31u;
   if (n < 32) return OXR_OK;
   uint32_t B = std::min<uint32_t>(n, std::max<uint32_t>(32u, guide->cfg.batch & ~31u));
+  int rc = guide->ensure_batch(B);
+  if (rc) return rc;
+  const uint32_t stride = n / B;  // start < stride keeps start + (B - 1) * stride < n
   hipEvent_t a{}, b{};
   if (eng.timing) {
     a = eng.next_event();
@@ -1317,11 +1339,15 @@ int oxr_pt::train_guide() {
   }
   uint32_t steps = guide->cfg.steps;
   if (guide->cfg.max_steps) steps = (uint32_t)std::min<uint64_t>(steps, guide->cfg.max_steps - guide->st.steps);
+  const uint64_t s0 = guide->st.steps;
   for (uint32_t k = 0; k < steps; ++k) {
-    uint64_t start = ((uint64_t)k * B) % n;
-    if (start + B > n) start = 0;
-    int rc = guide->step(eng.stream, guide->d_samp + 2 * start, B);
-    if (rc) return rc;
+    const float4* batch = guide->d_samp;
+    if (stride > 1) {
+      uint32_t start = (uint32_t)((s0 + k) % stride);
+      glaunch::gather(eng.stream, guide->d_samp, start, stride, B, guide->d_batch);
+      batch = guide->d_batch;
+    }
+    if ((rc = guide->step(eng.stream, batch, B))) return rc;
   }
   if (eng.timing) (void)hipEventRecord(b, eng.stream);
   HIPC(hipStreamSynchronize(eng.stream));
@@ -2016,6 +2042,8 @@ int oxr_srv_stats_get(oxr_srv* s, oxr_srv_stats* out) {
 // ---------------------------------------------------------------------
 // Learned path guide (include/oxr/guide.h).
 // ---------------------------------------------------------------------
+static_assert(G_KREC == OXR_GUIDE_MAX_BOUNCES, "one training record per guided vertex");
+
 int oxr_guide_default_config(oxr_guide_config* c) {
   if (!c) return fail(OXR_ERR_INVALID, "null config");
   *c = oxr_guide_config{};
@@ -2030,6 +2058,7 @@ int oxr_guide_default_config(oxr_guide_config* c) {
   c->batch = 65536;
   c->steps = 4;
   c->record_shift = 3;
+  c->bounces = 1;
   return OXR_OK;
 }
 
@@ -2044,6 +2073,8 @@ int oxr_guide_create(int device, const oxr_guide_config* c, oxr_guide** out) {
   if (!(c->beta1 >= 0.0f && c->beta1 < 1.0f && c->beta2 >= 0.0f && c->beta2 < 1.0f) || !(c->eps >= 0.0f))
     return fail(OXR_ERR_INVALID, "Adam betas must be in [0, 1) and eps >= 0");
   if (c->batch < 32 || c->record_shift > 16) return fail(OXR_ERR_INVALID, "batch >= 32 and record_shift <= 16");
+  if (c->bounces < 1 || c->bounces > OXR_GUIDE_MAX_BOUNCES)
+    return fail(OXR_ERR_INVALID, "bounces must be in 1..%d", OXR_GUIDE_MAX_BOUNCES);
   for (int a = 0; a < 3; ++a)
     if (!std::isfinite(c->aabb_min[a]) || !std::isfinite(c->aabb_max[a]))
       return fail(OXR_ERR_INVALID, "aabb must be finite");
@@ -2223,6 +2254,18 @@ int oxr_guide_train(oxr_guide* g, uint32_t n, const float* pos, const float* dir
 int oxr_guide_stats_get(oxr_guide* g, oxr_guide_stats* out) {
   if (!g || !out) return fail(OXR_ERR_INVALID, "null argument");
   *out = g->st;
+  return OXR_OK;
+}
+
+int oxr_guide_collected(oxr_guide* g, uint32_t max, float* out, uint32_t* n) {
+  if (!g || !n) return fail(OXR_ERR_INVALID, "null argument");
+  *n = g->last_collected;
+  uint32_t m = std::min(max, g->last_collected);
+  if (out && m) {
+    HIPC(hipSetDevice(g->device));
+    HIPC(hipStreamSynchronize(g->stream));
+    HIPC(hipMemcpy(out, g->d_samp, (size_t)m * 2 * sizeof(float4), hipMemcpyDeviceToHost));
+  }
   return OXR_OK;
 }
 

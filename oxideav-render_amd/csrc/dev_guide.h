@@ -59,12 +59,12 @@ struct GuideLobes {
 };
 
 // What the shading kernel needs of an attached guide for one bounce. The
-// guide serves bounces k < G_KREC (inference runs only for those).
+// guide serves bounces k < kmax <= G_KREC (inference runs only for those).
 // Training records: for paths with pid % 2^shift == 0, vertex k < G_KREC
 // keeps 4 float4 at rec[((pid >> shift) * G_KREC + k) * 4]:
 //   (position, mixture pdf), (direction, 1 = valid), (throughput after
 //   the vertex, 0), (path radiance once the vertex's own NEE is in, 0).
-constexpr int G_KREC = 4;
+constexpr int G_KREC = 4;  // = OXR_GUIDE_MAX_BOUNCES
 constexpr uint32_t G_PATTERN = 0xA0000000u;  // Sobol' pattern id base of the guide draws
 
 struct DevGuideUse {
@@ -72,6 +72,7 @@ struct DevGuideUse {
   float4* rec;              // null: no training records
   float mix;                // guide probability at eligible vertices
   uint32_t shift;
+  uint32_t kmax;            // vertices k < kmax use the guide
 };
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));

@@ -93,7 +93,7 @@ struct DevFrame {
   DevEnv env;
 };
 
-// Path flags (PathState meta.z >> 16).
+// Path flags (meta.z >> 24).
 constexpr uint32_t PF_UNCOVERED = 1u;
 constexpr uint32_t PF_PENDING = 2u;
 constexpr uint32_t PF_DONE = 4u;

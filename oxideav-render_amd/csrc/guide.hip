@@ -222,7 +222,8 @@ __global__ __launch_bounds__(GUIDE_TB) __attribute__((amdgpu_waves_per_eu(4))) v
 // Training: 1/sum(w), then forward + loss gradient + backward per group
 // of 16 samples; activations/deltas (bf16) to xs/ds for the weight
 // gradient; hash-grid gradients by f32 atomics.
-// Sample i: samp[2i] = (pos, w), samp[2i+1] = (dir, 0). count % 32 == 0.
+// Sample i: samp[2i] = (pos, w), samp[2i+1] = (dir, proposal pdf; unused
+// here). count % 32 == 0.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_guide_wsum(uint32_t count, const float4* __restrict__ samp,
                                                      float* __restrict__ inv_wsum, float* __restrict__ loss) {
@@ -544,6 +545,18 @@ __global__ __launch_bounds__(64) void k_guide_pack(const float* __restrict__ mlp
   frag[f * 64 + lane] = pack8(v);
 }
 
+// Strided batch of the collected samples: dst[i] = src[start + i * stride]
+// for i < count (start < stride, start + (count - 1) * stride < the source
+// count), so that every training step sees samples from the whole pass.
+__global__ void k_guide_gather(const float4* __restrict__ src, uint32_t start, uint32_t stride, uint32_t count,
+                               float4* __restrict__ dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  size_t j = (size_t)start + (size_t)i * stride;
+  dst[2ull * i] = src[2ull * j];
+  dst[2ull * i + 1] = src[2ull * j + 1];
+}
+
 __global__ void k_guide_fill(float* p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0.0f;
@@ -618,6 +631,11 @@ void fill(hipStream_t s, float* p, uint64_t n) {
   int grid = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)cu_count() * 16u);
   if (grid < 1) grid = 1;
   k_guide_fill<<<grid, 256, 0, s>>>(p, n);
+}
+
+void gather(hipStream_t s, const float4* src, uint32_t start, uint32_t stride, uint32_t count, float4* dst) {
+  if (count == 0) return;
+  k_guide_gather<<<(count + 255) / 256, 256, 0, s>>>(src, start, stride, count, dst);
 }
 
 void pdf_sample(hipStream_t s, uint32_t n, const GuideLobes* lobes, const float* dir, const float* u, float* dir_out,

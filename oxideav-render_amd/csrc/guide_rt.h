@@ -22,6 +22,8 @@ void adam(hipStream_t s, uint64_t n_grid, uint64_t n_total, float* p, float* m, 
           float b1, float b2, float eps, float c1, float c2);
 void pack(hipStream_t s, const float* mlp, uint4* frag);
 void fill(hipStream_t s, float* p, uint64_t n);
+// dst[i] = src[start + i * stride], i < count (samples of 2 float4).
+void gather(hipStream_t s, const float4* src, uint32_t start, uint32_t stride, uint32_t count, float4* dst);
 void pdf_sample(hipStream_t s, uint32_t n, const GuideLobes* lobes, const float* dir, const float* u, float* dir_out,
                 float* pdf_out);
 uint32_t wtiles();

@@ -623,7 +623,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     float prev_pdf = b4.w;
     V3 radiance = valid ? f4xyz(rad_s[pid]) : v3(0, 0, 0);
     float4 sg = valid ? sigma_s[pid] : make_float4(0, 0, 0, 0);
-    V3 prev_pos = valid ? f4xyz(prev_s[pid]) : v3(0, 0, 0);
+    float4 pv4 = valid ? prev_s[pid] : make_float4(0, 0, 0, 0);
+    V3 prev_pos = f4xyz(pv4);
+    // Roulette throughput scale of guided paths: the product of
+    // p_mixture / p_bsdf over the guided vertices so far (1 unguided).
+    float rr_scale = (GUIDE && k > 0) ? pv4.w : 1.0f;
     if (valid && (flags & PF_PENDING)) {
       add_pending(pid, nslots, nee, vis, radiance);
       flags &= ~PF_PENDING;
@@ -743,14 +747,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     bsdf.init(ms, neg(d), ngeo, front);
     if (alive && bsdf.black()) alive = false;
     V3 ng = bsdf.ng;
-    // One-sample MIS with the guide at eligible vertices: the first G_KREC
+    // One-sample MIS with the guide at eligible vertices: the first kmax
     // bounces, no transmission, not a near-mirror metal (the BSDF sampler
     // alone serves those).
     float gmix = 0.0f;
     const GuideLobes* GL = nullptr;
     if (GUIDE) {
       GL = gu.lobes + i;
-      if (alive && k < (uint32_t)G_KREC && transmission == 0.0f && M->dt == 0.0f &&
+      if (alive && k < gu.kmax && transmission == 0.0f && M->dt == 0.0f &&
           !(M->metallic >= 0.99f && M->roughness < 0.2f))
         gmix = gu.mix;
     }
@@ -941,6 +945,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         if (ok) {
           pdf = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pbs;
           ok = pdf > 0.0f && isfinite(pdf);
+          // Roulette sees the throughput the BSDF sampler alone would give:
+          // a guided direction into a bright region has a small f / p by
+          // design, and throughput roulette would end most such paths and
+          // scale the survivors up, undoing the guide.
+          rr_scale = pbs > 0.0f ? fminf(rr_scale * (pdf / pbs), 1.0e30f) : 1.0e30f;
         }
       } else {
         ok = bsdf.sample(pb_u.v[0], pb_u.v[1], pb_u.v[2], l, f, pdf);
@@ -966,7 +975,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
           }
           cont = true;
           if (k + 1 >= fr.rr_start) {
-            float q = clampr(max3(beta), 0.05f, 1.0f);
+            float q = clampr(GUIDE ? max3(beta) * rr_scale : max3(beta), 0.05f, 1.0f);
             if (q < 1.0f) {
               if (pb_u.v[3] >= q)
                 cont = false;
@@ -997,7 +1006,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       if (!cont) flags |= PF_DONE;
       beta_s[pid] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
       rad_s[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
-      prev_s[pid] = make_float4(prev_pos.x, prev_pos.y, prev_pos.z, 0.0f);
+      prev_s[pid] = make_float4(prev_pos.x, prev_pos.y, prev_pos.z, GUIDE ? rr_scale : 0.0f);
       sigma_s[pid] = sg;
       meta_s[pid] = make_uint4(meta.x, meta.y, ((k + (cont ? 1u : 0u)) & 0xffffu) | (hero << 16) | (flags << 24), ps);
     }
@@ -1034,7 +1043,7 @@ __global__ void k_guide_collect(uint32_t P, uint32_t nsel, uint32_t shift, const
     uint32_t pos = wave_append(count, emit);
     if (emit && pos < cap) {
       samp[2ull * pos] = make_float4(r0.x, r0.y, r0.z, w);
-      samp[2ull * pos + 1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
+      samp[2ull * pos + 1] = make_float4(r1.x, r1.y, r1.z, r0.w);
     }
   }
 }
@@ -1364,7 +1373,7 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
            uint32_t cur, const DevGuideUse* guide) {
-  DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u};
+  DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
 #define OXR_SHADE(W, G)                                                                                          \
   hipLaunchKernelGGL((k_shade<W, G>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],             \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \

@@ -262,7 +262,8 @@ class GuideConfig(C.Structure):
     _fields_ = [("log2_table", C.c_uint32), ("base_res", C.c_uint32), ("max_res", C.c_uint32), ("seed", C.c_uint32),
                 ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float), ("mix", C.c_float), ("batch", C.c_uint32),
-                ("steps", C.c_uint32), ("record_shift", C.c_uint32), ("max_steps", C.c_uint32)]
+                ("steps", C.c_uint32), ("record_shift", C.c_uint32), ("max_steps", C.c_uint32),
+                ("bounces", C.c_uint32)]
 
 
 class GuideStats(C.Structure):
@@ -273,14 +274,14 @@ class GuideStats(C.Structure):
 
 def guide_config(log2_table=19, base_res=16, max_res=2048, seed=0, aabb_min=(0.0, 0.0, 0.0), aabb_max=(0.0, 0.0, 0.0),
                  lr=5e-3, beta1=0.9, beta2=0.99, eps=1e-15, mix=0.5, batch=65536, steps=4, record_shift=3,
-                 max_steps=0):
+                 max_steps=0, bounces=1):
     """oxr_guide_default_config with overrides."""
     c = GuideConfig()
     c.log2_table, c.base_res, c.max_res, c.seed = log2_table, base_res, max_res, seed
     c.aabb_min = f3(*aabb_min)
     c.aabb_max = f3(*aabb_max)
     c.lr, c.beta1, c.beta2, c.eps, c.mix = lr, beta1, beta2, eps, mix
-    c.batch, c.steps, c.record_shift, c.max_steps = batch, steps, record_shift, max_steps
+    c.batch, c.steps, c.record_shift, c.max_steps, c.bounces = batch, steps, record_shift, max_steps, bounces
     return c
 
 

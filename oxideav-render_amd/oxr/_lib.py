@@ -82,6 +82,7 @@ def lib():
         "oxr_guide_gradient": (C.c_int, [vp, u32, fp, fp, fp, fp, fp, dp]),
         "oxr_guide_train": (C.c_int, [vp, u32, fp, fp, fp, u32, dp]),
         "oxr_guide_stats_get": (C.c_int, [vp, C.POINTER(A.GuideStats)]),
+        "oxr_guide_collected": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
         "oxr_pt_set_guide": (C.c_int, [vp, vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -103,7 +104,7 @@ EXPORTED = [
     "oxr_srv_job_status", "oxr_srv_read_accum", "oxr_srv_release", "oxr_srv_stats_get",
     "oxr_guide_default_config", "oxr_guide_create", "oxr_guide_destroy", "oxr_guide_num_params",
     "oxr_guide_get_params", "oxr_guide_set_params", "oxr_guide_infer", "oxr_guide_pdf", "oxr_guide_sample",
-    "oxr_guide_gradient", "oxr_guide_train", "oxr_guide_stats_get", "oxr_pt_set_guide",
+    "oxr_guide_gradient", "oxr_guide_train", "oxr_guide_stats_get", "oxr_guide_collected", "oxr_pt_set_guide",
 ]
 
 

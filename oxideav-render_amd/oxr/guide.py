@@ -112,6 +112,17 @@ class Guide:
                                       w.ctypes.data_as(fp), steps, C.byref(loss)), "guide_train")
         return loss.value
 
+    def collected(self, max_samples=1 << 20):
+        """Training samples of the last online pass: (pos (n,3), weight (n,),
+        dir (n,3), proposal pdf (n,))."""
+        n = C.c_uint32(0)
+        check(self._L.oxr_guide_collected(self._h, 0, None, C.byref(n)), "guide_collected")
+        m = min(n.value, max_samples)
+        out = np.zeros((max(m, 1), 8), np.float32)
+        check(self._L.oxr_guide_collected(self._h, m, out.ctypes.data_as(fp), C.byref(n)), "guide_collected")
+        out = out[:m]
+        return out[:, 0:3], out[:, 3], out[:, 4:7], out[:, 7]
+
     def stats(self):
         s = A.GuideStats()
         check(self._L.oxr_guide_stats_get(self._h, C.byref(s)), "guide_stats")

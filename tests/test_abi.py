@@ -106,7 +106,7 @@ def test_guide_struct_layouts_match_header():
 #include "oxr/guide.h"
 int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(oxr_guide_config), sizeof(oxr_guide_stats), sizeof(oxr_tune_result),
-         offsetof(oxr_guide_config, max_steps), offsetof(oxr_guide_stats, infer_ms));
+         offsetof(oxr_guide_config, bounces), offsetof(oxr_guide_stats, infer_ms));
   return 0;
 }
 '''
@@ -118,4 +118,4 @@ int main(void) {
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert out == [C.sizeof(A.GuideConfig), C.sizeof(A.GuideStats), C.sizeof(A.TuneResult),
-                   A.GuideConfig.max_steps.offset, A.GuideStats.infer_ms.offset]
+                   A.GuideConfig.bounces.offset, A.GuideStats.infer_ms.offset]
