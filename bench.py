@@ -61,27 +61,45 @@ def bytes_per_closest_ray(pt, scene, cfg, opts_ambient, sample_rows=48, x_step=1
     return 32.0 + 16.0 + 128.0 * n_node + 48.0 * n_tri, n_node, n_tri, len(rays)
 
 
-def cpu_baseline(scene, cfg, ambient, target_s=12.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, cfg, ambient, target_s=12.0, single_s=6.0):
     """The oracle (C++ restatement of the reference's band-parallel
-    refine) on a bounded band of rows of the same frame."""
+    refine) on a bounded band of rows of the same frame: all usable cores
+    (at most 16), then one thread on a smaller band."""
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
     os_ = O.OracleScene(scene)
     W, H = cfg["width"], cfg["height"]
-    rows = min(H, 4 * threads * 2)
-    y0 = max(0, H // 2 - rows // 2)
     opts = O.options(W, H, max_bounces=cfg["bounces"], ambient=ambient)
-    t = time.perf_counter()
-    os_.render(W, H, 0, 1, opts, y0=y0, y1=y0 + rows, threads=threads)
-    dt1 = time.perf_counter() - t
-    spp = int(max(1, min(512, target_s / max(dt1, 1e-3))))
-    t = time.perf_counter()
-    os_.render(W, H, 1, spp, opts, y0=y0, y1=y0 + rows, threads=threads)
-    dt = time.perf_counter() - t
-    paths = rows * W * spp
-    return {"value": paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
+
+    def timed(rows, nthreads, budget_s):
+        y0 = max(0, H // 2 - rows // 2)
+        t = time.perf_counter()
+        os_.render(W, H, 0, 1, opts, y0=y0, y1=y0 + rows, threads=nthreads)
+        dt1 = time.perf_counter() - t
+        spp = int(max(1, min(512, budget_s / max(dt1, 1e-3))))
+        t = time.perf_counter()
+        os_.render(W, H, 1, spp, opts, y0=y0, y1=y0 + rows, threads=nthreads)
+        dt = time.perf_counter() - t
+        return rows * W * spp / dt, spp, y0, dt
+
+    rows = min(H, 4 * threads * 2)
+    v, spp, y0, dt = timed(rows, threads, target_s)
+    v1, spp1, y01, dt1 = timed(4, 1, single_s)
+    return {"value": v, "unit": "paths/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "hardware_threads": os.cpu_count(), "single_thread_paths_per_s": v1,
             "sample": f"oracle refine({spp}) of rows {y0}..{y0 + rows} of the {W}x{H} frame "
-                      f"({paths} paths, {dt:.1f} s, band-parallel f64 accumulate)"}
+                      f"({rows * W * spp} paths, {dt:.1f} s, band-parallel f64 accumulate); one thread: "
+                      f"refine({spp1}) of rows {y01}..{y01 + 4} ({dt1:.1f} s)"}
 
 
 def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sustained=128):
