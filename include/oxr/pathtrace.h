@@ -115,20 +115,108 @@ typedef struct oxr_light {
   float outer_cone_angle;       /* radians                                  */
 } oxr_light;
 
+/* Textures: a decoded image plus the glTF sampler of the texture that
+ * references it (PreparedTexture, src/texture.rs:384-470). The library
+ * builds the linear-light mip chains of src/texture.rs:323-381 at upload
+ * (an sRGB chain for colour slots, a linear chain for data slots). */
+#define OXR_TEXFMT_RGBA8 0u     /* 4 bytes per texel                        */
+#define OXR_TEXFMT_RGBA32F 1u   /* 4 floats per texel, taken as linear      */
+#define OXR_FILTER_NEAREST 0u   /* magnification filters                    */
+#define OXR_FILTER_LINEAR 1u
+#define OXR_MIN_NEAREST 0u      /* minification filters (glTF)              */
+#define OXR_MIN_LINEAR 1u
+#define OXR_MIN_NEAREST_MIP_NEAREST 2u
+#define OXR_MIN_LINEAR_MIP_NEAREST 3u
+#define OXR_MIN_NEAREST_MIP_LINEAR 4u
+#define OXR_MIN_LINEAR_MIP_LINEAR 5u
+#define OXR_WRAP_REPEAT 0u
+#define OXR_WRAP_CLAMP_TO_EDGE 1u
+#define OXR_WRAP_MIRRORED_REPEAT 2u
+
+typedef struct oxr_texture {
+  uint32_t width, height;
+  uint32_t format;              /* OXR_TEXFMT_*                              */
+  const void* pixels;           /* width x height RGBA, row-major, row 0 at
+                                   v = 0                                     */
+  uint32_t mag_filter;          /* OXR_FILTER_* (the effective filter: glTF
+                                   leaves it open; the default is linear)    */
+  uint32_t min_filter;          /* OXR_MIN_* (default linear-mip-linear)     */
+  uint32_t wrap_s, wrap_t;      /* OXR_WRAP_*                                */
+} oxr_texture;
+
+/* A material's reference to a texture (TextureBinding, src/prepare.rs:
+ * 386-431): the texture, its TEXCOORD set and KHR_texture_transform.
+ * The transform maps uv to (T R S) uv with T = offset, R = rotation by
+ * `rotation` radians as in KHR_texture_transform, S = scale. */
+typedef struct oxr_texture_binding {
+  int32_t texture;              /* index into oxr_scene_desc.textures; -1 =
+                                   no texture                                */
+  uint32_t uv_set;              /* 0 or 1                                    */
+  uint32_t has_transform;
+  float offset[2];
+  float rotation;
+  float scale[2];
+} oxr_texture_binding;
+
+/* The texture slots TraceScene::material_oriented samples (src/trace.rs:
+ * 717-895), one entry per material. Colour slots are sampled in sRGB, the
+ * others as linear data; the comment names the channel that is used. The
+ * occlusion texture has no slot: the path tracer ignores it
+ * (src/pathtrace.rs:241-243). */
+typedef struct oxr_material_textures {
+  oxr_texture_binding base_color;         /* sRGB RGBA x base colour     */
+  oxr_texture_binding metallic_roughness; /* G x roughness, B x metallic */
+  oxr_texture_binding normal;             /* tangent-space normal map    */
+  oxr_texture_binding emissive;           /* sRGB RGB x emissive         */
+  oxr_texture_binding specular;           /* A x specular factor         */
+  oxr_texture_binding specular_color;     /* sRGB RGB x specular colour  */
+  oxr_texture_binding transmission;       /* R x transmission            */
+  oxr_texture_binding thickness;          /* G x thickness               */
+  oxr_texture_binding clearcoat;          /* R x clearcoat               */
+  oxr_texture_binding clearcoat_roughness;/* G x clearcoat roughness     */
+  oxr_texture_binding clearcoat_normal;   /* clearcoat normal map        */
+  oxr_texture_binding sheen_color;        /* sRGB RGB x sheen colour     */
+  oxr_texture_binding sheen_roughness;    /* A x sheen roughness         */
+  oxr_texture_binding anisotropy;         /* RG direction, B x strength  */
+  oxr_texture_binding iridescence;        /* R x iridescence             */
+  oxr_texture_binding iridescence_thickness; /* G: thickness min..max    */
+  oxr_texture_binding diffuse_transmission;  /* A x factor               */
+  oxr_texture_binding diffuse_transmission_color; /* sRGB RGB x colour   */
+  float normal_scale;           /* normalTexture.scale (1)                   */
+  float clearcoat_normal_scale; /* clearcoatNormalTexture.scale (1)          */
+  float iridescence_thickness_min; /* iridescenceThicknessMinimum (100); the
+                                      maximum is oxr_material's value        */
+  float pad;
+} oxr_material_textures;
+
 /* Triangle flags. */
 #define OXR_TRI_HAS_NORMALS 1u
+#define OXR_TRI_HAS_UV0 2u      /* TEXCOORD_0 present for the triangle's item */
+#define OXR_TRI_HAS_UV1 4u
+#define OXR_TRI_HAS_TANGENTS 8u
+#define OXR_TRI_HAS_COLORS 16u
 
 typedef struct oxr_scene_desc {
   uint32_t num_tris;
   const float* positions;       /* num_tris x 9: v0,v1,v2 (CCW front)      */
   const float* normals;         /* num_tris x 9 vertex normals, or NULL     */
-  const uint32_t* tri_flags;    /* num_tris OXR_TRI_*, or NULL (all =
-                                   OXR_TRI_HAS_NORMALS when normals given) */
+  const uint32_t* tri_flags;    /* num_tris OXR_TRI_*, or NULL (each
+                                   OXR_TRI_HAS_* set where its array below
+                                   is given)                                 */
   const uint32_t* material_ids; /* num_tris, index into materials          */
   uint32_t num_materials;
   const oxr_material* materials;
   uint32_t num_lights;
   const oxr_light* lights;      /* punctual lights (PreparedScene::lights)  */
+  /* Textured materials (all may be NULL / 0: an untextured scene). */
+  uint32_t num_textures;
+  const oxr_texture* textures;
+  const oxr_material_textures* material_textures; /* num_materials, or NULL */
+  const float* uv0;             /* num_tris x 6: TEXCOORD_0 of v0, v1, v2    */
+  const float* uv1;             /* num_tris x 6: TEXCOORD_1                  */
+  const float* tangents;        /* num_tris x 12: xyz + handedness w per
+                                   vertex                                    */
+  const float* colors;          /* num_tris x 12: COLOR_0 RGBA per vertex    */
 } oxr_scene_desc;
 
 /* Resolved camera (src/camera.rs:61-91): eye + orthonormal basis + the

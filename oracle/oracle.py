@@ -82,6 +82,14 @@ def lib():
         L.oxrcpu_record_rays.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
         L.oxrcpu_bvh4_count.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
+        L.oxrcpu_tex_sample.restype = C.c_int
+        L.oxrcpu_tex_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, fp, fp, fp]
+        L.oxrcpu_tex_grad_lod.restype = C.c_int
+        L.oxrcpu_tex_grad_lod.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, fp, fp]
+        L.oxrcpu_tex_level.restype = C.c_int
+        L.oxrcpu_tex_level.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, up, fp]
+        L.oxrcpu_material_eval.restype = C.c_int
+        L.oxrcpu_material_eval.argtypes = [C.c_void_p, C.c_uint32, up, fp, fp, fp, bp, fp]
         _LIB = L
     return _LIB
 
@@ -211,6 +219,57 @@ class OracleScene:
         m = lib().oxrcpu_record_rays(self.h, C.byref(cam), C.byref(opts), first, n, y0, y1, x_step, max_rays,
                                      out.ctypes.data_as(fp))
         return out[:m]
+
+    def tex_sample(self, tex, uv, lod=None, srgb=False):
+        """PreparedTexture::sample_lod at (n, 2) uv and n levels (0 when
+        lod is None): (n, 4) linear RGBA."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        lod = np.zeros(len(uv), np.float32) if lod is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(lod, np.float32), (len(uv),)))
+        out = np.zeros((len(uv), 4), np.float32)
+        if lib().oxrcpu_tex_sample(self.h, tex, 1 if srgb else 0, len(uv), uv.ctypes.data_as(fp),
+                                   lod.ctypes.data_as(fp), out.ctypes.data_as(fp)) != 0:
+            raise ValueError("no such texture")
+        return out
+
+    def tex_grad_lod(self, tex, duv):
+        """sample_grad's level for (n, 4) (du/dx, dv/dx, du/dy, dv/dy)."""
+        duv = np.ascontiguousarray(duv, np.float32).reshape(-1, 4)
+        out = np.zeros(len(duv), np.float32)
+        if lib().oxrcpu_tex_grad_lod(self.h, tex, len(duv), duv.ctypes.data_as(fp), out.ctypes.data_as(fp)) != 0:
+            raise ValueError("no such texture")
+        return out
+
+    def tex_levels(self, tex, srgb=False):
+        """The mip chain: a list of (h, w, 4) arrays."""
+        wh = np.zeros(2, np.uint32)
+        n = lib().oxrcpu_tex_level(self.h, tex, 1 if srgb else 0, 0, wh.ctypes.data_as(up), fp())
+        if n < 0:
+            raise ValueError("no such texture")
+        out = []
+        for lv in range(n):
+            lib().oxrcpu_tex_level(self.h, tex, 1 if srgb else 0, lv, wh.ctypes.data_as(up), fp())
+            t = np.zeros((int(wh[1]), int(wh[0]), 4), np.float32)
+            lib().oxrcpu_tex_level(self.h, tex, 1 if srgb else 0, lv, wh.ctypes.data_as(up), t.ctypes.data_as(fp))
+            out.append(t)
+        return out
+
+    def material_eval(self, g, bary, width=None, dirs=None, flip=None):
+        """material_oriented at hits (triangle g, barycentrics): (n, 42)
+        floats, see oxrcpu_material_eval."""
+        g = np.ascontiguousarray(g, np.uint32).ravel()
+        n = len(g)
+        bary = np.ascontiguousarray(bary, np.float32).reshape(n, 3)
+        width = np.zeros(n, np.float32) if width is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(width, np.float32), (n,)))
+        dirs = np.zeros((n, 3), np.float32) if dirs is None else np.ascontiguousarray(dirs, np.float32).reshape(n, 3)
+        flip = np.zeros(n, np.uint8) if flip is None else np.ascontiguousarray(flip, np.uint8).ravel()
+        out = np.zeros((n, 42), np.float32)
+        if lib().oxrcpu_material_eval(self.h, n, g.ctypes.data_as(up), bary.ctypes.data_as(fp),
+                                      width.ctypes.data_as(fp), dirs.ctypes.data_as(fp), flip.ctypes.data_as(bp),
+                                      out.ctypes.data_as(fp)) != 0:
+            raise ValueError("bad triangle id")
+        return out
 
     def surface(self, g, bary):
         b, bp_ = _f(bary)

@@ -1023,6 +1023,7 @@ struct PMaterial {  // PreparedMaterial + material_oriented constants
   float diffuse_transmission;
   f3 diffuse_transmission_color;
   float dispersion;
+  float anisotropy, anisotropy_rotation;  // KHR_materials_anisotropy
 };
 
 // PreparedMaterial::from_material (src/prepare.rs:623-668) followed by
@@ -1061,6 +1062,8 @@ static PMaterial prepare_material(const oxr_material& m) {
   for (int k = 0; k < 3; ++k)
     p.diffuse_transmission_color[k] = clampr(fin(m.diffuse_transmission_color[k], 1.0f), 0.0f, 1.0f);
   p.dispersion = fmaxr(fin(m.dispersion, 0.0f), 0.0f);
+  p.anisotropy = clampr(fin(m.anisotropy, 0.0f), 0.0f, 1.0f);
+  p.anisotropy_rotation = fin(m.anisotropy_rotation, 0.0f);
   return p;
 }
 
@@ -1326,6 +1329,214 @@ struct EnvMap {
   }
 };
 
+// ---------------------------------------------------------------------
+// Textures (src/texture.rs:225-515): linear-light mip chains, the glTF
+// sampler's filters and wrap modes, and the texture slots of
+// TraceScene::material_oriented (src/trace.rs:517-605, 640-895).
+// ---------------------------------------------------------------------
+using f4 = std::array<float, 4>;
+
+struct MipLevel {
+  uint32_t w = 0, h = 0;
+  std::vector<f4> t;
+};
+
+// wrap_index (src/texture.rs:280-289); i64 as the reference.
+static int64_t rem_euclid(int64_t a, int64_t n) {
+  int64_t r = a % n;
+  return r < 0 ? r + n : r;
+}
+static int64_t wrap_index(int64_t i, int64_t n, uint32_t mode) {
+  if (mode == OXR_WRAP_CLAMP_TO_EDGE) return std::min(std::max(i, (int64_t)0), n - 1);
+  if (mode == OXR_WRAP_MIRRORED_REPEAT) {
+    int64_t m = rem_euclid(i, 2 * n);
+    return m >= n ? 2 * n - 1 - m : m;
+  }
+  return rem_euclid(i, n);
+}
+static f4 fetch(const MipLevel& l, int64_t x, int64_t y, uint32_t ws, uint32_t wt) {
+  int64_t xi = wrap_index(x, (int64_t)l.w, ws), yi = wrap_index(y, (int64_t)l.h, wt);
+  return l.t[(size_t)yi * l.w + (size_t)xi];
+}
+static f4 lerp4(const f4& a, const f4& b, float t) {
+  return {a[0] + (b[0] - a[0]) * t, a[1] + (b[1] - a[1]) * t, a[2] + (b[2] - a[2]) * t, a[3] + (b[3] - a[3]) * t};
+}
+
+static float srgb_decode(float c) {  // hdr.rs srgb_to_linear
+  if (c <= 0.04045f) return c / 12.92f;
+  return std::pow((c + 0.055f) / 1.055f, 2.4f);
+}
+
+// build_mips (src/texture.rs:323-381): level 0 from the image (sRGB bytes
+// decoded through the 256-entry table, or linear bytes / 255; float
+// texels as given), then 2x2 box filters down to 1x1.
+static std::vector<MipLevel> build_mips(const oxr_texture& tx, bool srgb) {
+  static const std::array<float, 256> lut = [] {
+    std::array<float, 256> t{};
+    for (int i = 0; i < 256; ++i) t[i] = srgb_decode((float)i / 255.0f);
+    return t;
+  }();
+  MipLevel l0;
+  l0.w = tx.width;
+  l0.h = tx.height;
+  size_t n = (size_t)tx.width * tx.height;
+  l0.t.resize(n);
+  if (tx.format == OXR_TEXFMT_RGBA8) {
+    const uint8_t* p = static_cast<const uint8_t*>(tx.pixels);
+    for (size_t i = 0; i < n; ++i) {
+      for (int c = 0; c < 3; ++c) l0.t[i][c] = srgb ? lut[p[4 * i + c]] : (float)p[4 * i + c] / 255.0f;
+      l0.t[i][3] = (float)p[4 * i + 3] / 255.0f;
+    }
+  } else {
+    const float* p = static_cast<const float*>(tx.pixels);
+    for (size_t i = 0; i < n; ++i)
+      for (int c = 0; c < 4; ++c) l0.t[i][c] = p[4 * i + c];
+  }
+  std::vector<MipLevel> levels;
+  levels.push_back(std::move(l0));
+  while (!(levels.back().w == 1 && levels.back().h == 1)) {
+    const MipLevel& pv = levels.back();
+    MipLevel nx;
+    nx.w = std::max(pv.w / 2, 1u);
+    nx.h = std::max(pv.h / 2, 1u);
+    nx.t.resize((size_t)nx.w * nx.h);
+    for (uint32_t y = 0; y < nx.h; ++y)
+      for (uint32_t x = 0; x < nx.w; ++x) {
+        size_t x0 = std::min(2 * x, pv.w - 1), x1 = std::min(2 * x + 1, pv.w - 1);
+        size_t y0 = std::min(2 * y, pv.h - 1), y1 = std::min(2 * y + 1, pv.h - 1);
+        size_t pw = pv.w;
+        f4 acc = {0, 0, 0, 0};
+        for (size_t idx : {y0 * pw + x0, y0 * pw + x1, y1 * pw + x0, y1 * pw + x1})
+          for (int c = 0; c < 4; ++c) acc[c] += pv.t[idx][c];
+        nx.t[(size_t)y * nx.w + x] = {acc[0] * 0.25f, acc[1] * 0.25f, acc[2] * 0.25f, acc[3] * 0.25f};
+      }
+    levels.push_back(std::move(nx));
+  }
+  return levels;
+}
+
+// filter_level (src/texture.rs:481-512).
+static f4 filter_level(const MipLevel& l, float u, float v, bool linear, uint32_t ws, uint32_t wt) {
+  u = std::isfinite(u) ? u : 0.0f;
+  v = std::isfinite(v) ? v : 0.0f;
+  u = clampr(u, -1.0e6f, 1.0e6f);
+  v = clampr(v, -1.0e6f, 1.0e6f);
+  float x = u * (float)l.w, y = v * (float)l.h;
+  if (!linear) return fetch(l, (int64_t)std::floor(x), (int64_t)std::floor(y), ws, wt);
+  float xf = x - 0.5f, yf = y - 0.5f;
+  float x0 = std::floor(xf), y0 = std::floor(yf);
+  float fx = xf - x0, fy = yf - y0;
+  int64_t ix = (int64_t)x0, iy = (int64_t)y0;
+  f4 a = fetch(l, ix, iy, ws, wt), b = fetch(l, ix + 1, iy, ws, wt);
+  f4 c = fetch(l, ix, iy + 1, ws, wt), d = fetch(l, ix + 1, iy + 1, ws, wt);
+  return lerp4(lerp4(a, b, fx), lerp4(c, d, fx), fy);
+}
+
+struct PTexture {  // PreparedTexture: image + sampler, both chains
+  oxr_texture desc{};
+  std::vector<MipLevel> chain[2];  // 0: sRGB, 1: linear
+};
+
+// The texel filter of a minification filter (the base filter of
+// oxideav-mesh3d's MinFilter: X_MIPMAP_Y filters texels with X).
+static bool min_base_linear(uint32_t m) {
+  return m == OXR_MIN_LINEAR || m == OXR_MIN_LINEAR_MIP_NEAREST || m == OXR_MIN_LINEAR_MIP_LINEAR;
+}
+
+// PreparedTexture::sample_lod (src/texture.rs:417-447).
+static f4 sample_lod(const PTexture& t, bool srgb, float u, float v, float lod) {
+  const std::vector<MipLevel>& mips = t.chain[srgb ? 0 : 1];
+  const oxr_texture& s = t.desc;
+  if (std::isnan(lod) || lod <= 0.0f) return filter_level(mips[0], u, v, s.mag_filter == OXR_FILTER_LINEAR, s.wrap_s, s.wrap_t);
+  uint32_t mn = s.min_filter;
+  bool lin = min_base_linear(mn);
+  float max_level = (float)(mips.size() - 1);
+  if (mn == OXR_MIN_NEAREST || mn == OXR_MIN_LINEAR) return filter_level(mips[0], u, v, lin, s.wrap_s, s.wrap_t);
+  if (mn == OXR_MIN_NEAREST_MIP_NEAREST || mn == OXR_MIN_LINEAR_MIP_NEAREST) {
+    size_t l = (size_t)clampr(std::round(lod), 0.0f, max_level);
+    return filter_level(mips[l], u, v, lin, s.wrap_s, s.wrap_t);
+  }
+  float l = clampr(lod, 0.0f, max_level);
+  size_t l0 = (size_t)std::floor(l);
+  size_t l1 = std::min(l0 + 1, mips.size() - 1);
+  float f = l - (float)l0;
+  f4 a = filter_level(mips[l0], u, v, lin, s.wrap_s, s.wrap_t);
+  if (f <= 0.0f || l0 == l1) return a;
+  f4 b = filter_level(mips[l1], u, v, lin, s.wrap_s, s.wrap_t);
+  return lerp4(a, b, f);
+}
+
+// PreparedTexture::sample_grad's level of detail (src/texture.rs:452-470).
+static float grad_lod(const PTexture& t, float dux, float dvx, float duy, float dvy) {
+  float w = (float)t.desc.width, h = (float)t.desc.height;
+  float lx = std::hypot(dux * w, dvx * h), ly = std::hypot(duy * w, dvy * h);
+  float rho = fmaxr(lx, ly);
+  return (std::isfinite(rho) && rho > 0.0f) ? std::log2(rho) : 0.0f;
+}
+
+// A material's texture reference (TextureBinding, src/prepare.rs:386-431)
+// with its KHR_texture_transform as the 2x3 matrix T R S. The rotation
+// follows KHR_texture_transform's definition (oxideav-mesh3d's
+// TextureTransform::apply, not vendored: restated from the extension's
+// specification; parity unpinned).
+struct TBind {
+  int32_t tex = -1;
+  uint32_t uv_set = 0;
+  bool xf = false;
+  float m00 = 1, m01 = 0, m02 = 0, m10 = 0, m11 = 1, m12 = 0;
+};
+static TBind make_bind(const oxr_texture_binding& b, uint32_t ntex) {
+  TBind t;
+  if (b.texture < 0 || (uint32_t)b.texture >= ntex) return t;
+  t.tex = b.texture;
+  t.uv_set = b.uv_set;
+  t.xf = b.has_transform != 0;
+  if (t.xf) {
+    float c = std::cos(b.rotation), s = std::sin(b.rotation);
+    t.m00 = c * b.scale[0];
+    t.m01 = s * b.scale[1];
+    t.m02 = b.offset[0];
+    t.m10 = -s * b.scale[0];
+    t.m11 = c * b.scale[1];
+    t.m12 = b.offset[1];
+  }
+  return t;
+}
+static void transform_uv(const TBind& b, float& u, float& v) {
+  if (!b.xf) return;
+  float nu = b.m00 * u + b.m01 * v + b.m02;
+  float nv = b.m10 * u + b.m11 * v + b.m12;
+  u = nu;
+  v = nv;
+}
+
+// The slots of oxr_material_textures, in its order.
+enum TexSlot {
+  TS_BASE, TS_MR, TS_NORMAL, TS_EMISSIVE, TS_SPEC, TS_SPEC_COLOR, TS_TRANSMISSION, TS_THICKNESS, TS_CC, TS_CC_ROUGH,
+  TS_CC_NORMAL, TS_SHEEN_COLOR, TS_SHEEN_ROUGH, TS_ANISO, TS_IRID, TS_IRID_THICK, TS_DT, TS_DT_COLOR, TS_COUNT
+};
+struct PMatTex {
+  TBind slot[TS_COUNT];
+  float normal_scale = 1.0f, cc_normal_scale = 1.0f, irid_min = 100.0f;
+  bool any = false;
+};
+static PMatTex prepare_mat_tex(const oxr_material_textures& m, uint32_t ntex) {
+  const oxr_texture_binding* b[TS_COUNT] = {
+      &m.base_color, &m.metallic_roughness, &m.normal, &m.emissive, &m.specular, &m.specular_color,
+      &m.transmission, &m.thickness, &m.clearcoat, &m.clearcoat_roughness, &m.clearcoat_normal, &m.sheen_color,
+      &m.sheen_roughness, &m.anisotropy, &m.iridescence, &m.iridescence_thickness, &m.diffuse_transmission,
+      &m.diffuse_transmission_color};
+  PMatTex p;
+  for (int k = 0; k < TS_COUNT; ++k) {
+    p.slot[k] = make_bind(*b[k], ntex);
+    p.any = p.any || p.slot[k].tex >= 0;
+  }
+  p.normal_scale = m.normal_scale;
+  p.cc_normal_scale = m.clearcoat_normal_scale;
+  p.irid_min = m.iridescence_thickness_min;
+  return p;
+}
+
 struct Scene {
   uint32_t n = 0;
   std::vector<float> pos;        // 9 per tri
@@ -1334,6 +1545,13 @@ struct Scene {
   std::vector<uint32_t> mat_id;
   std::vector<PMaterial> mats;
   std::vector<PLight> lights;
+  // Textures and per-triangle attributes (empty: untextured).
+  std::vector<PTexture> texs;
+  std::vector<PMatTex> mtex;     // per material, or empty
+  std::vector<uint32_t> tflags;  // OXR_TRI_* per triangle
+  std::vector<float> uv[2];      // 6 per triangle
+  std::vector<float> tan;        // 12 per triangle
+  std::vector<float> col;        // 12 per triangle
   // EmissiveLights (src/pathtrace.rs:1436-1555).
   std::vector<uint32_t> l_tris;
   std::vector<float> l_pmf, l_cdf;
@@ -1346,6 +1564,12 @@ struct Scene {
 
   const float* tri(uint32_t g) const { return &pos[9ull * g]; }
   const PMaterial& mat(uint32_t g) const { return mats[mat_id[g]]; }
+  const PMatTex* mat_tex(uint32_t g) const {
+    if (mtex.empty()) return nullptr;
+    const PMatTex& m = mtex[mat_id[g]];
+    return m.any ? &m : nullptr;
+  }
+  bool has(uint32_t g, uint32_t bit) const { return !tflags.empty() && (tflags[g] & bit) != 0; }
 };
 
 static void build_emissive(Scene& s) {
@@ -1353,9 +1577,21 @@ static void build_emissive(Scene& s) {
   s.l_lookup.assign(s.n, NO_LIGHT);
   s.l_tris.clear();
   std::vector<float> powf_;
+  // Mean emission per material: factor x the emissive texture's 1x1 mip
+  // (src/pathtrace.rs:1484-1500).
+  std::vector<f3> mean_e(s.mats.size());
+  for (size_t i = 0; i < s.mats.size(); ++i) {
+    const PMaterial& m = s.mats[i];
+    f3 e = m.unlit ? f3{0, 0, 0} : m.emissive;
+    if (!m.unlit && !s.mtex.empty() && s.mtex[i].slot[TS_EMISSIVE].tex >= 0) {
+      const f4& c = s.texs[s.mtex[i].slot[TS_EMISSIVE].tex].chain[0].back().t[0];
+      e = {e[0] * c[0], e[1] * c[1], e[2] * c[2]};
+    }
+    mean_e[i] = e;
+  }
   for (uint32_t g = 0; g < s.n; ++g) {
     const PMaterial& m = s.mat(g);
-    f3 e = m.unlit ? f3{0, 0, 0} : m.emissive;
+    f3 e = mean_e[s.mat_id[g]];
     float lum = luminance(e);
     if (std::isnan(lum) || lum <= 0.0f) continue;
     const float* v = s.tri(g);
@@ -1681,39 +1917,187 @@ static Surface surface(const Scene& s, uint32_t g, f3 b) {
   return sf;
 }
 
-// TraceScene::material_oriented for untextured materials
-// (src/trace.rs:700-923); `flip` reverses the shading normal first.
-static MaterialSample material_at(const PMaterial& m, const Surface& sf, bool flip) {
+// Level of detail of a texture tap (TexLod): the ray cone at the first
+// hit, else level 0 (src/pathtrace.rs:2083-2092).
+struct TexLod {
+  bool cone = false;
+  float width = 0.0f;
+  f3 dir = {0, 0, 0};
+};
+
+static bool tri_uvs(const Scene& s, uint32_t g, uint32_t set, const float*& uv) {
+  if (set > 1 || !s.has(g, set ? OXR_TRI_HAS_UV1 : OXR_TRI_HAS_UV0)) return false;
+  uv = &s.uv[set][6ull * g];
+  return true;
+}
+static f4 interp4v(const float* v12, f3 b) {
+  f4 o;
+  for (int k = 0; k < 4; ++k) o[k] = v12[k] * b[0] + v12[4 + k] * b[1] + v12[8 + k] * b[2];
+  return o;
+}
+
+// TraceScene::lod (src/trace.rs:518-552): the ray-cone level for a binding.
+static float cone_lod(const Scene& s, const TBind& b, uint32_t g, const float* uvs, const TexLod& L) {
+  float uv[3][2];
+  for (int k = 0; k < 3; ++k) {
+    uv[k][0] = uvs[2 * k];
+    uv[k][1] = uvs[2 * k + 1];
+    transform_uv(b, uv[k][0], uv[k][1]);
+  }
+  const oxr_texture& d = s.texs[b.tex].desc;
+  float ta = (float)d.width * (float)d.height *
+             std::fabs((uv[1][0] - uv[0][0]) * (uv[2][1] - uv[0][1]) - (uv[2][0] - uv[0][0]) * (uv[1][1] - uv[0][1]));
+  const float* p = s.tri(g);
+  f3 p0 = {p[0], p[1], p[2]}, p1 = {p[3], p[4], p[5]}, p2 = {p[6], p[7], p[8]};
+  f3 cr = cross(sub(p1, p0), sub(p2, p0));
+  float pa = std::sqrt(dot(cr, cr));
+  if (!(ta > 0.0f && pa > 0.0f && L.width > 0.0f)) return 0.0f;
+  f3 n = {cr[0] / pa, cr[1] / pa, cr[2] / pa};
+  float cos = fmaxr(std::fabs(dot(n, L.dir)), 1.0e-3f);
+  float l = 0.5f * std::log2(ta / pa) + std::log2(L.width / cos);
+  return std::isfinite(l) ? l : 0.0f;
+}
+
+// TraceScene::sample_at (src/trace.rs:554-590): the binding's texel value
+// at the hit, or false when the texture or its UV set is missing.
+static bool tex_tap(const Scene& s, const TBind& b, uint32_t g, f3 bary, const TexLod& L, bool srgb, f4& out) {
+  const float* uvs;
+  if (b.tex < 0 || !tri_uvs(s, g, b.uv_set, uvs)) return false;
+  float lod = L.cone ? cone_lod(s, b, g, uvs, L) : 0.0f;
+  float u = uvs[0] * bary[0] + uvs[2] * bary[1] + uvs[4] * bary[2];
+  float v = uvs[1] * bary[0] + uvs[3] * bary[1] + uvs[5] * bary[2];
+  transform_uv(b, u, v);
+  out = sample_lod(s.texs[b.tex], srgb, u, v, lod);
+  return true;
+}
+
+// TraceScene::tangent_frame (src/trace.rs:640-663).
+static bool tangent_frame(const Scene& s, uint32_t g, f3 b, f3 n, f3& tt, f3& bt) {
+  if (!s.has(g, OXR_TRI_HAS_TANGENTS)) return false;
+  const float* tg12 = &s.tan[12ull * g];
+  f4 tg = interp4v(tg12, b);
+  float w = tg12[3] < 0.0f ? -1.0f : 1.0f;
+  f3 t3 = {tg[0], tg[1], tg[2]};
+  float d = dot(n, t3);
+  tt = normalise({t3[0] - n[0] * d, t3[1] - n[1] * d, t3[2] - n[2] * d});
+  if (dot(tt, tt) < 0.5f) return false;
+  f3 c = cross(n, tt);
+  bt = {c[0] * w, c[1] * w, c[2] * w};
+  return true;
+}
+
+// TraceScene::perturb (src/trace.rs:665-687).
+static f3 perturb(const Scene& s, uint32_t g, f3 b, f3 n, const f4& t, float scale) {
+  f3 tt, bt;
+  if (!tangent_frame(s, g, b, n, tt, bt)) return n;
+  float ts[3] = {(t[0] * 2.0f - 1.0f) * scale, (t[1] * 2.0f - 1.0f) * scale, t[2] * 2.0f - 1.0f};
+  f3 nn = normalise({tt[0] * ts[0] + bt[0] * ts[1] + n[0] * ts[2], tt[1] * ts[0] + bt[1] * ts[1] + n[1] * ts[2],
+                     tt[2] * ts[0] + bt[2] * ts[1] + n[2] * ts[2]});
+  return dot(nn, nn) > 0.5f ? nn : n;
+}
+
+// TraceScene::material_oriented (src/trace.rs:700-923); `flip` reverses
+// the shading normal before the normal maps are applied.
+static MaterialSample material_at(const Scene& s, uint32_t g, f3 bary, const Surface& sf, bool flip,
+                                  const TexLod& L) {
+  const PMaterial& m = s.mat(g);
+  const PMatTex* mt = s.mat_tex(g);
   f3 sn = flip ? neg(sf.shading_normal) : sf.shading_normal;
   MaterialSample ms;
+  f4 t;
+  auto tap = [&](int slot, bool srgb) { return mt && tex_tap(s, mt->slot[slot], g, bary, L, srgb, t); };
   for (int k = 0; k < 4; ++k) ms.base_color[k] = m.base_color[k];
-  ms.metallic = clampr(m.metallic, 0.0f, 1.0f);
-  ms.roughness = clampr(m.roughness, 0.0f, 1.0f);
+  if (tap(TS_BASE, true))
+    for (int k = 0; k < 4; ++k) ms.base_color[k] *= t[k];
+  if (s.has(g, OXR_TRI_HAS_COLORS)) {
+    f4 v = interp4v(&s.col[12ull * g], bary);
+    for (int k = 0; k < 4; ++k) ms.base_color[k] *= v[k];
+  }
+  float metallic = m.metallic, roughness = m.roughness;
+  if (tap(TS_MR, false)) {
+    roughness *= t[1];
+    metallic *= t[2];
+  }
+  ms.metallic = clampr(metallic, 0.0f, 1.0f);
+  ms.roughness = clampr(roughness, 0.0f, 1.0f);
   ms.normal = sn;
+  if (tap(TS_NORMAL, false)) ms.normal = perturb(s, g, bary, sn, t, mt->normal_scale);
   ms.emissive = m.emissive;
+  if (tap(TS_EMISSIVE, true))
+    for (int k = 0; k < 3; ++k) ms.emissive[k] *= t[k];
   ms.unlit = m.unlit;
   ms.double_sided = m.double_sided;
   ms.ior = m.ior;
   ms.specular = m.specular;
+  if (tap(TS_SPEC, false)) ms.specular *= t[3];
   ms.specular_color = m.specular_color;
+  if (tap(TS_SPEC_COLOR, true))
+    for (int k = 0; k < 3; ++k) ms.specular_color[k] *= t[k];
   ms.transmission = m.transmission;
+  if (tap(TS_TRANSMISSION, false)) ms.transmission *= t[0];
   ms.thickness = m.thickness;
+  if (tap(TS_THICKNESS, false)) ms.thickness *= t[1];
   ms.attenuation_color = m.attenuation_color;
   ms.attenuation_distance = m.attenuation_distance;
   ms.clearcoat = m.clearcoat;
+  if (tap(TS_CC, false)) ms.clearcoat *= t[0];
   ms.clearcoat_roughness = m.clearcoat_roughness;
+  if (tap(TS_CC_ROUGH, false)) ms.clearcoat_roughness *= t[1];
   ms.clearcoat_normal = sn;
+  if (tap(TS_CC_NORMAL, false)) ms.clearcoat_normal = perturb(s, g, bary, sn, t, mt->cc_normal_scale);
   ms.sheen_color = m.sheen_color;
+  if (tap(TS_SHEEN_COLOR, true))
+    for (int k = 0; k < 3; ++k) ms.sheen_color[k] *= t[k];
   ms.sheen_roughness = m.sheen_roughness;
-  ms.anisotropy = 0.0f;  // no tangent frame: anisotropy has no effect
+  if (tap(TS_SHEEN_ROUGH, false)) ms.sheen_roughness *= t[3];
+  // KHR_materials_anisotropy (src/trace.rs:835-858): the direction needs
+  // the tangent frame; without one the lobe stays isotropic.
+  ms.anisotropy = 0.0f;
   ms.anisotropy_direction = {0, 0, 0};
+  if (m.anisotropy > 0.0f) {
+    float a = m.anisotropy;
+    float dir[2] = {1.0f, 0.0f};
+    if (tap(TS_ANISO, false)) {
+      dir[0] = t[0] * 2.0f - 1.0f;
+      dir[1] = t[1] * 2.0f - 1.0f;
+      a *= clampr(t[2], 0.0f, 1.0f);
+    }
+    float sr = std::sin(m.anisotropy_rotation), cr = std::cos(m.anisotropy_rotation);
+    float rot[2] = {cr * dir[0] - sr * dir[1], sr * dir[0] + cr * dir[1]};
+    f3 tt, bt;
+    if (tangent_frame(s, g, bary, sn, tt, bt)) {
+      f3 d = normalise({tt[0] * rot[0] + bt[0] * rot[1], tt[1] * rot[0] + bt[1] * rot[1],
+                        tt[2] * rot[0] + bt[2] * rot[1]});
+      if (dot(d, d) > 0.5f) ms.anisotropy_direction = d;
+    }
+    ms.anisotropy = dot(ms.anisotropy_direction, ms.anisotropy_direction) < 0.5f ? 0.0f : a;
+  }
   ms.iridescence = m.iridescence;
+  if (tap(TS_IRID, false)) ms.iridescence *= t[0];
   ms.iridescence_ior = m.iridescence_ior;
   ms.iridescence_thickness = m.iridescence_thickness;
+  if (tap(TS_IRID_THICK, false)) {
+    float lo = mt->irid_min, hi = m.iridescence_thickness;
+    ms.iridescence_thickness = fmaxr(lo + (hi - lo) * clampr(t[1], 0.0f, 1.0f), 0.0f);
+  }
   ms.diffuse_transmission = m.diffuse_transmission;
+  if (tap(TS_DT, false)) ms.diffuse_transmission *= t[3];
   ms.diffuse_transmission_color = m.diffuse_transmission_color;
+  if (tap(TS_DT_COLOR, true))
+    for (int k = 0; k < 3; ++k) ms.diffuse_transmission_color[k] *= t[k];
   ms.dispersion = m.dispersion;
   return ms;
+}
+
+// TraceScene::alpha (src/trace.rs:492-506): base alpha x the base
+// texture's alpha at level 0 x the vertex colour's alpha.
+static float tri_alpha(const Scene& s, uint32_t g, f3 bary) {
+  float a = s.mat(g).base_color[3];
+  const PMatTex* mt = s.mat_tex(g);
+  f4 t;
+  if (mt && tex_tap(s, mt->slot[TS_BASE], g, bary, TexLod{}, true, t)) a *= t[3];
+  if (s.has(g, OXR_TRI_HAS_COLORS)) a *= interp4v(&s.col[12ull * g], bary)[3];
+  return a;
 }
 
 // ---------------------------------------------------------------------
@@ -1739,6 +2123,11 @@ struct Integrator {
     if (pt.clamp > 0.0f && m > pt.clamp) return scale(c, pt.clamp / m);
     return c;
   }
+  // pixel_spread (src/trace.rs:1138-1145): the cone angle of one pixel.
+  float pixel_spread() const {
+    if (cam.projection == OXR_PROJ_ORTHOGRAPHIC) return 0.0f;
+    return std::atan(2.0f * cam.half_h / (float)std::max(height, 1u));
+  }
   // Integrator::p_env (src/pathtrace.rs:1915-1921).
   float p_env() const { return ts.env ? (ts.l_tris.empty() ? 1.0f : 0.5f) : 0.0f; }
 
@@ -1747,9 +2136,9 @@ struct Integrator {
     if (primary && !c.front && !mat.double_sided) return false;
     switch (mat.alpha_mode) {
       case OXR_ALPHA_MASK:
-        return mat.base_color[3] >= mat.alpha_cutoff;
+        return tri_alpha(ts, c.g, c.bary) >= mat.alpha_cutoff;
       case OXR_ALPHA_BLEND:
-        return s.coin(ray, c.g) < mat.base_color[3];
+        return s.coin(ray, c.g) < tri_alpha(ts, c.g, c.bary);
       default:
         return true;
     }
@@ -1757,10 +2146,16 @@ struct Integrator {
   bool visible(f3 o, f3 d, float t_max, const Sampler& s, uint32_t ray, uint32_t skip) const {
     return !occluded(ts, o, d, 0.0f, t_max, [&](const Hit& c) { return c.g != skip && accept(c, s, ray, false); });
   }
-  f3 emission(uint32_t g) const {
+  // Integrator::emission (src/pathtrace.rs:1951-1966): the factor times
+  // the emissive texture at level 0.
+  f3 emission(uint32_t g, f3 bary) const {
     const PMaterial& m = ts.mat(g);
     if (m.unlit || is_black(m.emissive)) return {0, 0, 0};
-    return m.emissive;
+    f3 e = m.emissive;
+    const PMatTex* mt = ts.mat_tex(g);
+    f4 t;
+    if (mt && tex_tap(ts, mt->slot[TS_EMISSIVE], g, bary, TexLod{}, true, t)) e = {e[0] * t[0], e[1] * t[1], e[2] * t[2]};
+    return e;
   }
   float tri_pdf(size_t li, uint32_t g, f3 p, f3 q) const {
     const float* v = ts.tri(g);
@@ -1839,7 +2234,7 @@ struct Integrator {
       Surface surf = surface(ts, hit.g, hit.bary);
       const PMaterial& pmat = ts.mat(hit.g);
       if ((hit.front || pmat.double_sided) && !pmat.unlit && !is_black(pmat.emissive)) {
-        f3 le = emission(hit.g);
+        f3 le = emission(hit.g, hit.bary);
         if (!is_black(le)) {
           if (k == 0) {
             radiance = add(radiance, mul(beta, le));
@@ -1859,7 +2254,13 @@ struct Integrator {
         }
       }
       bool flip = !hit.front && pmat.double_sided;
-      MaterialSample mat = material_at(pmat, surf, flip);
+      TexLod lod;
+      if (k == 0) {  // src/pathtrace.rs:2083-2092
+        lod.cone = true;
+        lod.width = cam.projection == OXR_PROJ_ORTHOGRAPHIC ? pixel_spread() : pixel_spread() * hit.t;
+        lod.dir = d;
+      }
+      MaterialSample mat = material_at(ts, hit.g, hit.bary, surf, flip, lod);
       if (flip) {
         mat.normal = neg(mat.normal);
         mat.clearcoat_normal = neg(mat.clearcoat_normal);
@@ -1969,7 +2370,7 @@ struct Integrator {
                 f3 f;
                 float pb;
                 bsdf.eval(l, f, pb);
-                f3 le = emission(g);
+                f3 le = emission(g, bary);
                 if (!is_black(f) && !is_black(le)) {
                   float w = strategy == OXR_STRATEGY_MIS ? power_heuristic(pl, pb) : 1.0f;
                   f3 o2 = spawn(l);
@@ -2096,6 +2497,38 @@ oxrcpu_scene* oxrcpu_scene_create(const oxr_scene_desc* d) {
   s.mat_id.assign(d->material_ids, d->material_ids + s.n);
   for (uint32_t i = 0; i < d->num_materials; ++i) s.mats.push_back(prepare_material(d->materials[i]));
   for (uint32_t i = 0; i < d->num_lights; ++i) s.lights.push_back(prepare_light(d->lights[i]));
+  // Textures: both chains of every texture (TextureData builds them lazily).
+  for (uint32_t i = 0; i < d->num_textures; ++i) {
+    PTexture t;
+    t.desc = d->textures[i];
+    if (t.desc.width == 0 || t.desc.height == 0 || !t.desc.pixels) {
+      delete h;
+      return nullptr;
+    }
+    t.chain[0] = build_mips(t.desc, true);
+    t.chain[1] = build_mips(t.desc, false);
+    t.desc.pixels = nullptr;
+    s.texs.push_back(std::move(t));
+  }
+  if (d->material_textures)
+    for (uint32_t i = 0; i < d->num_materials; ++i)
+      s.mtex.push_back(prepare_mat_tex(d->material_textures[i], d->num_textures));
+  const float* attr[4] = {d->uv0, d->uv1, d->tangents, d->colors};
+  const uint32_t bits[4] = {OXR_TRI_HAS_UV0, OXR_TRI_HAS_UV1, OXR_TRI_HAS_TANGENTS, OXR_TRI_HAS_COLORS};
+  const size_t width[4] = {6, 6, 12, 12};
+  std::vector<float>* dst[4] = {&s.uv[0], &s.uv[1], &s.tan, &s.col};
+  bool any_attr = false;
+  for (int k = 0; k < 4; ++k)
+    if (attr[k]) {
+      dst[k]->assign(attr[k], attr[k] + width[k] * s.n);
+      any_attr = true;
+    }
+  if (any_attr) {
+    s.tflags.assign(s.n, 0u);
+    for (uint32_t g = 0; g < s.n; ++g)
+      for (int k = 0; k < 4; ++k)
+        if (attr[k] && (!d->tri_flags || (d->tri_flags[g] & bits[k]))) s.tflags[g] |= bits[k];
+  }
   build_emissive(s);
   build_bvh(s);
   return h;
@@ -2390,6 +2823,85 @@ void oxrcpu_surface(oxrcpu_scene* h, uint32_t g, const float* bary, float* out9)
     out9[3 + k] = sf.geometric_normal[k];
     out9[6 + k] = sf.shading_normal[k];
   }
+}
+
+// PreparedTexture::sample_lod of texture `tex` in sRGB (srgb != 0) or
+// linear space at n (u, v) pairs and levels of detail: 4 floats each.
+int oxrcpu_tex_sample(oxrcpu_scene* h, uint32_t tex, int srgb, uint32_t n, const float* uv, const float* lod,
+                      float* out) {
+  if (tex >= h->s.texs.size()) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    f4 r = sample_lod(h->s.texs[tex], srgb != 0, uv[2 * i], uv[2 * i + 1], lod[i]);
+    for (int c = 0; c < 4; ++c) out[4 * i + c] = r[c];
+  }
+  return 0;
+}
+
+// PreparedTexture::sample_grad's level of detail for (du/dx, dv/dx,
+// du/dy, dv/dy) quadruples.
+int oxrcpu_tex_grad_lod(oxrcpu_scene* h, uint32_t tex, uint32_t n, const float* duv, float* lod) {
+  if (tex >= h->s.texs.size()) return -1;
+  for (uint32_t i = 0; i < n; ++i)
+    lod[i] = grad_lod(h->s.texs[tex], duv[4 * i], duv[4 * i + 1], duv[4 * i + 2], duv[4 * i + 3]);
+  return 0;
+}
+
+// Mip level `level` of a texture's chain: returns the level count; with
+// `texels` non-null copies w*h*4 floats (w, h into wh[0..1]).
+int oxrcpu_tex_level(oxrcpu_scene* h, uint32_t tex, int srgb, uint32_t level, uint32_t* wh, float* texels) {
+  if (tex >= h->s.texs.size()) return -1;
+  const std::vector<MipLevel>& c = h->s.texs[tex].chain[srgb ? 0 : 1];
+  if (level < c.size()) {
+    wh[0] = c[level].w;
+    wh[1] = c[level].h;
+    if (texels)
+      for (size_t i = 0; i < c[level].t.size(); ++i)
+        for (int k = 0; k < 4; ++k) texels[4 * i + k] = c[level].t[i][k];
+  }
+  return (int)c.size();
+}
+
+// TraceScene::material_oriented at n hits (triangle g, barycentrics b),
+// with the ray cone (width, dir) when width > 0 and level 0 otherwise,
+// and the double-sided flip; then the material's emission at level 0
+// and the alpha of the any-hit filter. 42 floats per hit: the 35 of
+// oxr_material_sample, base alpha, emissive (3), emission (3), alpha.
+int oxrcpu_material_eval(oxrcpu_scene* h, uint32_t n, const uint32_t* g, const float* bary, const float* width,
+                         const float* dir, const uint8_t* flip, float* out) {
+  const Scene& s = h->s;
+  oxr_camera cam{};
+  oxr_pt_options opt{};
+  Integrator it{s, cam, 1, 1, 0.0f, opt};
+  for (uint32_t i = 0; i < n; ++i) {
+    if (g[i] >= s.n) return -1;
+    f3 b = {bary[3 * i], bary[3 * i + 1], bary[3 * i + 2]};
+    TexLod L;
+    if (width[i] > 0.0f) {
+      L.cone = true;
+      L.width = width[i];
+      L.dir = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    }
+    Surface sf = surface(s, g[i], b);
+    MaterialSample m = material_at(s, g[i], b, sf, flip[i] != 0, L);
+    float* o = out + 42ull * i;
+    float v[42] = {m.base_color[0], m.base_color[1], m.base_color[2], m.metallic, m.roughness,
+                   m.normal[0], m.normal[1], m.normal[2], m.ior, m.specular,
+                   m.specular_color[0], m.specular_color[1], m.specular_color[2], m.transmission, m.thickness,
+                   m.clearcoat, m.clearcoat_roughness, m.clearcoat_normal[0], m.clearcoat_normal[1],
+                   m.clearcoat_normal[2], m.sheen_color[0], m.sheen_color[1], m.sheen_color[2], m.sheen_roughness,
+                   m.anisotropy, m.anisotropy_direction[0], m.anisotropy_direction[1], m.anisotropy_direction[2],
+                   m.iridescence, m.iridescence_ior, m.iridescence_thickness, m.diffuse_transmission,
+                   m.diffuse_transmission_color[0], m.diffuse_transmission_color[1],
+                   m.diffuse_transmission_color[2], m.base_color[3], m.emissive[0], m.emissive[1], m.emissive[2],
+                   0.0f, 0.0f, 0.0f};
+    f3 e = it.emission(g[i], b);
+    v[38] = e[0];
+    v[39] = e[1];
+    v[40] = e[2];
+    v[41] = tri_alpha(s, g[i], b);
+    for (int k = 0; k < 42; ++k) o[k] = v[k];
+  }
+  return 0;
 }
 
 // Resolve (PathTracer::hdr / image): accum W*H*4 doubles.

@@ -143,6 +143,59 @@ def light(kind, color=(1.0, 1.0, 1.0), intensity=1.0, position=(0.0, 0.0, 0.0),
     return l
 
 
+TEXFMT_RGBA8, TEXFMT_RGBA32F = 0, 1
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
+(MIN_NEAREST, MIN_LINEAR, MIN_NEAREST_MIP_NEAREST, MIN_LINEAR_MIP_NEAREST, MIN_NEAREST_MIP_LINEAR,
+ MIN_LINEAR_MIP_LINEAR) = range(6)
+WRAP_REPEAT, WRAP_CLAMP_TO_EDGE, WRAP_MIRRORED_REPEAT = 0, 1, 2
+TRI_HAS_NORMALS, TRI_HAS_UV0, TRI_HAS_UV1, TRI_HAS_TANGENTS, TRI_HAS_COLORS = 1, 2, 4, 8, 16
+
+
+class Texture(C.Structure):
+    """oxr_texture."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("pixels", C.c_void_p),
+                ("mag_filter", C.c_uint32), ("min_filter", C.c_uint32), ("wrap_s", C.c_uint32),
+                ("wrap_t", C.c_uint32)]
+
+
+class TextureBinding(C.Structure):
+    """oxr_texture_binding."""
+    _fields_ = [("texture", C.c_int32), ("uv_set", C.c_uint32), ("has_transform", C.c_uint32),
+                ("offset", C.c_float * 2), ("rotation", C.c_float), ("scale", C.c_float * 2)]
+
+
+TEXTURE_SLOTS = ("base_color", "metallic_roughness", "normal", "emissive", "specular", "specular_color",
+                 "transmission", "thickness", "clearcoat", "clearcoat_roughness", "clearcoat_normal", "sheen_color",
+                 "sheen_roughness", "anisotropy", "iridescence", "iridescence_thickness", "diffuse_transmission",
+                 "diffuse_transmission_color")
+
+
+class MaterialTextures(C.Structure):
+    """oxr_material_textures."""
+    _fields_ = [(n, TextureBinding) for n in TEXTURE_SLOTS] + [
+        ("normal_scale", C.c_float), ("clearcoat_normal_scale", C.c_float),
+        ("iridescence_thickness_min", C.c_float), ("pad", C.c_float)]
+
+
+def no_binding():
+    b = TextureBinding()
+    b.texture = -1
+    b.scale[0] = b.scale[1] = 1.0
+    return b
+
+
+def material_textures():
+    """An oxr_material_textures with every slot empty and the glTF
+    default scales."""
+    m = MaterialTextures()
+    for n in TEXTURE_SLOTS:
+        setattr(m, n, no_binding())
+    m.normal_scale = 1.0
+    m.clearcoat_normal_scale = 1.0
+    m.iridescence_thickness_min = 100.0
+    return m
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("num_tris", C.c_uint32),
@@ -154,6 +207,13 @@ class SceneDesc(C.Structure):
         ("materials", C.POINTER(Material)),
         ("num_lights", C.c_uint32),
         ("lights", C.POINTER(Light)),
+        ("num_textures", C.c_uint32),
+        ("textures", C.POINTER(Texture)),
+        ("material_textures", C.POINTER(MaterialTextures)),
+        ("uv0", C.POINTER(C.c_float)),
+        ("uv1", C.POINTER(C.c_float)),
+        ("tangents", C.POINTER(C.c_float)),
+        ("colors", C.POINTER(C.c_float)),
     ]
 
 

@@ -87,8 +87,11 @@ class Scene:
         self._nrm = []
         self._has_n = []
         self._mid = []
+        self._attr = {"uv0": [], "uv1": [], "tangents": [], "colors": []}
         self.materials = []
         self.lights = []
+        self.textures = []         # (A.Texture, pixel array) pairs
+        self.material_tex = {}     # material index -> A.MaterialTextures
         self.camera = None
         self._packed = None
 
@@ -121,20 +124,80 @@ class Scene:
         t = self.material_table()
         return A.Material.from_buffer_copy(t[i:i + 1].tobytes())
 
-    def add_tris(self, pos, mat, normals=None):
-        """pos: (n, 9) corners; normals: (n, 9) or None (geometric)."""
+    _ATTR = {"uv0": (6, A.TRI_HAS_UV0), "uv1": (6, A.TRI_HAS_UV1), "tangents": (12, A.TRI_HAS_TANGENTS),
+             "colors": (12, A.TRI_HAS_COLORS)}
+
+    def add_tris(self, pos, mat, normals=None, uv0=None, uv1=None, tangents=None, colors=None):
+        """pos: (n, 9) corners; normals: (n, 9) or None (geometric);
+        uv0 / uv1: (n, 6) TEXCOORD_0/1 per corner; tangents: (n, 12) xyzw
+        per corner; colors: (n, 12) COLOR_0 RGBA per corner."""
         pos = np.ascontiguousarray(pos, dtype=F32).reshape(-1, 9)
         self._pos.append(pos)
+        flags = np.zeros(len(pos), dtype=np.uint32)
         if normals is None:
             self._nrm.append(np.zeros_like(pos))
-            self._has_n.append(np.zeros(len(pos), dtype=np.uint32))
         else:
             self._nrm.append(np.ascontiguousarray(normals, dtype=F32).reshape(-1, 9))
-            self._has_n.append(np.full(len(pos), A.OXR_TRI_HAS_NORMALS, dtype=np.uint32))
+            flags |= A.OXR_TRI_HAS_NORMALS
+        for name, arr in (("uv0", uv0), ("uv1", uv1), ("tangents", tangents), ("colors", colors)):
+            width, bit = self._ATTR[name]
+            if arr is None:
+                self._attr[name].append(np.zeros((len(pos), width), F32))
+            else:
+                self._attr[name].append(np.ascontiguousarray(arr, dtype=F32).reshape(len(pos), width))
+                flags |= bit
+        self._has_n.append(flags)
         if np.isscalar(mat):
             self._mid.append(np.full(len(pos), mat, dtype=np.uint32))
         else:
             self._mid.append(np.asarray(mat, dtype=np.uint32))
+        self._packed = None
+
+    def add_texture(self, pixels, mag=A.FILTER_LINEAR, min_filter=A.MIN_LINEAR_MIP_LINEAR, wrap_s=A.WRAP_REPEAT,
+                    wrap_t=A.WRAP_REPEAT):
+        """pixels: (H, W, 4) uint8 (RGBA8) or float32 (linear RGBA); row 0
+        at v = 0. Returns the texture index."""
+        px = np.ascontiguousarray(pixels)
+        if px.ndim != 3 or px.shape[2] != 4:
+            raise ValueError("texture pixels must be (H, W, 4)")
+        if px.dtype == np.uint8:
+            fmt = A.TEXFMT_RGBA8
+        else:
+            px = np.ascontiguousarray(px, dtype=F32)
+            fmt = A.TEXFMT_RGBA32F
+        t = A.Texture()
+        t.width, t.height, t.format = px.shape[1], px.shape[0], fmt
+        t.pixels = px.ctypes.data
+        t.mag_filter, t.min_filter, t.wrap_s, t.wrap_t = mag, min_filter, wrap_s, wrap_t
+        self.textures.append((t, px))
+        return len(self.textures) - 1
+
+    def set_material_textures(self, mat, normal_scale=1.0, clearcoat_normal_scale=1.0,
+                              iridescence_thickness_min=100.0, **slots):
+        """Bind textures to material `mat`. Each slot (base_color,
+        metallic_roughness, normal, emissive, or a KHR extension slot) takes
+        a texture index or a dict(texture=, uv_set=0, offset=(0, 0),
+        rotation=0, scale=(1, 1)); giving offset, rotation or scale turns
+        the KHR_texture_transform on."""
+        m = A.material_textures()
+        m.normal_scale, m.clearcoat_normal_scale = normal_scale, clearcoat_normal_scale
+        m.iridescence_thickness_min = iridescence_thickness_min
+        for name, v in slots.items():
+            if name not in A.TEXTURE_SLOTS:
+                raise ValueError(f"unknown texture slot {name!r}")
+            b = A.no_binding()
+            if isinstance(v, dict):
+                b.texture = int(v["texture"])
+                b.uv_set = int(v.get("uv_set", 0))
+                if any(k in v for k in ("offset", "rotation", "scale")):
+                    b.has_transform = 1
+                    b.offset[0], b.offset[1] = v.get("offset", (0.0, 0.0))
+                    b.rotation = float(v.get("rotation", 0.0))
+                    b.scale[0], b.scale[1] = v.get("scale", (1.0, 1.0))
+            else:
+                b.texture = int(v)
+            setattr(m, name, b)
+        self.material_tex[int(mat)] = m
         self._packed = None
 
     def pack(self):
@@ -144,13 +207,22 @@ class Scene:
                 nrm = np.ascontiguousarray(np.concatenate(self._nrm), dtype=F32)
                 hn = np.ascontiguousarray(np.concatenate(self._has_n), dtype=np.uint32)
                 mid = np.ascontiguousarray(np.concatenate(self._mid), dtype=np.uint32)
+                attr = {k: np.ascontiguousarray(np.concatenate(v), dtype=F32) for k, v in self._attr.items()}
             else:
                 pos = np.zeros((0, 9), F32)
                 nrm = np.zeros((0, 9), F32)
                 hn = np.zeros(0, np.uint32)
                 mid = np.zeros(0, np.uint32)
+                attr = {k: np.zeros((0, self._ATTR[k][0]), F32) for k in self._attr}
             self._packed = (pos, nrm, hn, mid)
+            self._packed_attr = attr
         return self._packed
+
+    def attributes(self):
+        """Per-triangle uv0, uv1, tangents and colors arrays (zeros where a
+        triangle has none; see the tri flags)."""
+        self.pack()
+        return self._packed_attr
 
     @property
     def num_tris(self):
@@ -168,14 +240,17 @@ class Scene:
         """oxr_scene_desc pointing at this scene's arrays (kept alive by
         the returned object)."""
         pos, nrm, hn, mid = self.pack()
+        attr = self.attributes()
         d = A.SceneDesc()
         d.num_tris = len(pos)
         fp = C.POINTER(C.c_float)
         up = C.POINTER(C.c_uint32)
         d.positions = pos.ctypes.data_as(fp)
-        any_n = bool(hn.any())
+        any_n = bool((hn & A.OXR_TRI_HAS_NORMALS).any())
         d.normals = nrm.ctypes.data_as(fp) if any_n else fp()
-        d.tri_flags = hn.ctypes.data_as(up) if any_n else up()
+        d.tri_flags = hn.ctypes.data_as(up) if bool(hn.any()) else up()
+        for name, (_, bit) in self._ATTR.items():
+            setattr(d, name, attr[name].ctypes.data_as(fp) if bool((hn & bit).any()) else fp())
         d.material_ids = mid.ctypes.data_as(up)
         mats = self.material_table()
         if len(mats) == 0:
@@ -187,7 +262,16 @@ class Scene:
         d.materials = mats.ctypes.data_as(C.POINTER(A.Material))
         d.num_lights = len(self.lights)
         d.lights = C.cast(lights, C.POINTER(A.Light))
-        d._keep = (pos, nrm, hn, mid, mats, lights)
+        texs = (A.Texture * max(len(self.textures), 1))(*[t for t, _ in self.textures])
+        d.num_textures = len(self.textures)
+        d.textures = C.cast(texs, C.POINTER(A.Texture))
+        mtex = None
+        if self.material_tex:
+            mtex = (A.MaterialTextures * max(d.num_materials, 1))()
+            for i in range(d.num_materials):
+                mtex[i] = self.material_tex.get(i, A.material_textures())
+            d.material_textures = C.cast(mtex, C.POINTER(A.MaterialTextures))
+        d._keep = (pos, nrm, hn, mid, mats, lights, attr, texs, mtex, [px for _, px in self.textures])
         return d
 
     def bounds(self):
