@@ -319,6 +319,23 @@ int oxr_pt_set_environment(oxr_pt* h, const float* rgba, uint32_t width, uint32_
 int oxr_pt_env_eval(oxr_pt* h, uint32_t n, const float* dirs, float* pdf, float* le);
 int oxr_pt_env_sample(oxr_pt* h, uint32_t n, const float* u, float* dirs, float* pdf, uint8_t* ok);
 
+/* Texture parity entry points (the tests of the texture layer).
+ * oxr_pt_tex_sample: PreparedTexture::sample_lod (src/texture.rs:417-447)
+ * of texture `tex` in sRGB (srgb != 0) or linear space at n (u, v) pairs
+ * (2n floats) and levels of detail (n): 4 floats each. OXR_ERR_INVALID
+ * when no material slot uses the texture in that space.
+ * oxr_pt_material_eval: TraceScene::material_oriented (src/trace.rs:
+ * 700-923) at n hits (triangle g[i], barycentrics 3n) with the ray cone
+ * (width[i], dir 3n; src/trace.rs:518-552) where width > 0 and level 0
+ * otherwise, and the double-sided flip (flip[i]; normals reversed back as
+ * the integrator uses them); then Integrator::emission (src/pathtrace.rs:
+ * 1951-1966) and the any-hit alpha (src/trace.rs:492-506). 43 floats per
+ * hit: the 35 of oxr_material_sample, the base alpha, the emissive colour,
+ * the emission, the alpha. */
+int oxr_pt_tex_sample(oxr_pt* h, uint32_t tex, int srgb, uint32_t n, const float* uv, const float* lod, float* out);
+int oxr_pt_material_eval(oxr_pt* h, uint32_t n, const uint32_t* g, const float* bary, const float* width,
+                         const float* dir, const uint8_t* flip, float* out);
+
 /* Add n samples per pixel, continuing the global sample index; a no-op
  * before the first upload. refine(a); refine(b) == refine(a+b) bit for
  * bit. Returns after the device finished. */

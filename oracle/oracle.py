@@ -255,7 +255,7 @@ class OracleScene:
         return out
 
     def material_eval(self, g, bary, width=None, dirs=None, flip=None):
-        """material_oriented at hits (triangle g, barycentrics): (n, 42)
+        """material_oriented at hits (triangle g, barycentrics): (n, 43)
         floats, see oxrcpu_material_eval."""
         g = np.ascontiguousarray(g, np.uint32).ravel()
         n = len(g)
@@ -264,7 +264,7 @@ class OracleScene:
             np.broadcast_to(np.asarray(width, np.float32), (n,)))
         dirs = np.zeros((n, 3), np.float32) if dirs is None else np.ascontiguousarray(dirs, np.float32).reshape(n, 3)
         flip = np.zeros(n, np.uint8) if flip is None else np.ascontiguousarray(flip, np.uint8).ravel()
-        out = np.zeros((n, 42), np.float32)
+        out = np.zeros((n, 43), np.float32)
         if lib().oxrcpu_material_eval(self.h, n, g.ctypes.data_as(up), bary.ctypes.data_as(fp),
                                       width.ctypes.data_as(fp), dirs.ctypes.data_as(fp), flip.ctypes.data_as(bp),
                                       out.ctypes.data_as(fp)) != 0:

@@ -2863,8 +2863,9 @@ int oxrcpu_tex_level(oxrcpu_scene* h, uint32_t tex, int srgb, uint32_t level, ui
 
 // TraceScene::material_oriented at n hits (triangle g, barycentrics b),
 // with the ray cone (width, dir) when width > 0 and level 0 otherwise,
-// and the double-sided flip; then the material's emission at level 0
-// and the alpha of the any-hit filter. 42 floats per hit: the 35 of
+// and the double-sided flip (normals reversed back, as the integrator
+// uses them); then the material's emission at level 0
+// and the alpha of the any-hit filter. 43 floats per hit: the 35 of
 // oxr_material_sample, base alpha, emissive (3), emission (3), alpha.
 int oxrcpu_material_eval(oxrcpu_scene* h, uint32_t n, const uint32_t* g, const float* bary, const float* width,
                          const float* dir, const uint8_t* flip, float* out) {
@@ -2883,8 +2884,12 @@ int oxrcpu_material_eval(oxrcpu_scene* h, uint32_t n, const uint32_t* g, const f
     }
     Surface sf = surface(s, g[i], b);
     MaterialSample m = material_at(s, g[i], b, sf, flip[i] != 0, L);
-    float* o = out + 42ull * i;
-    float v[42] = {m.base_color[0], m.base_color[1], m.base_color[2], m.metallic, m.roughness,
+    if (flip[i]) {  // the integrator's flip back (src/pathtrace.rs:2097-2101)
+      m.normal = neg(m.normal);
+      m.clearcoat_normal = neg(m.clearcoat_normal);
+    }
+    float* o = out + 43ull * i;
+    float v[43] = {m.base_color[0], m.base_color[1], m.base_color[2], m.metallic, m.roughness,
                    m.normal[0], m.normal[1], m.normal[2], m.ior, m.specular,
                    m.specular_color[0], m.specular_color[1], m.specular_color[2], m.transmission, m.thickness,
                    m.clearcoat, m.clearcoat_roughness, m.clearcoat_normal[0], m.clearcoat_normal[1],
@@ -2893,13 +2898,13 @@ int oxrcpu_material_eval(oxrcpu_scene* h, uint32_t n, const uint32_t* g, const f
                    m.iridescence, m.iridescence_ior, m.iridescence_thickness, m.diffuse_transmission,
                    m.diffuse_transmission_color[0], m.diffuse_transmission_color[1],
                    m.diffuse_transmission_color[2], m.base_color[3], m.emissive[0], m.emissive[1], m.emissive[2],
-                   0.0f, 0.0f, 0.0f};
+                   0.0f, 0.0f, 0.0f, 0.0f};
     f3 e = it.emission(g[i], b);
-    v[38] = e[0];
-    v[39] = e[1];
-    v[40] = e[2];
-    v[41] = tri_alpha(s, g[i], b);
-    for (int k = 0; k < 42; ++k) o[k] = v[k];
+    v[39] = e[0];
+    v[40] = e[1];
+    v[41] = e[2];
+    v[42] = tri_alpha(s, g[i], b);
+    for (int k = 0; k < 43; ++k) o[k] = v[k];
   }
   return 0;
 }

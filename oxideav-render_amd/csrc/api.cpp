@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -210,7 +211,105 @@ DevMaterial prepare_material(const oxr_material& m) {
   d.irid_thick = std::fmax(fin(m.iridescence_thickness, 400.0f), 0.0f);
   d.dt = cl(fin(m.diffuse_transmission, 0.0f), 0.0f, 1.0f);
   d.dispersion = std::fmax(fin(m.dispersion, 0.0f), 0.0f);
+  d.aniso = cl(fin(m.anisotropy, 0.0f), 0.0f, 1.0f);
+  d.aniso_rot = fin(m.anisotropy_rotation, 0.0f);
   return d;
+}
+
+// ---------------------------------------------------------------------
+// Textures: the mip chains of src/texture.rs:323-381 (level 0 decoded
+// through the sRGB byte table or divided by 255; float texels as given;
+// then 2x2 box filters with edge clamping down to 1x1), built once per
+// (texture, colour space) a material uses.
+// ---------------------------------------------------------------------
+struct TexBuilder {
+  const oxr_scene_desc* d;
+  std::vector<float4> pool;
+  std::vector<DevTexLevel> levels;
+  std::vector<DevTexChain> chains;
+  std::vector<int32_t> chain_of;  // [texture * 2 + linear] -> chain, -1 = not built
+
+  int validate() {
+    if (d->num_textures > 0 && !d->textures) return fail(OXR_ERR_INVALID, "null textures");
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+      const oxr_texture& t = d->textures[i];
+      if (t.width == 0 || t.height == 0 || !t.pixels)
+        return fail(OXR_ERR_INVALID, "texture %u: empty image", i);
+      if (t.format > OXR_TEXFMT_RGBA32F || t.mag_filter > OXR_FILTER_LINEAR || t.min_filter > OXR_MIN_LINEAR_MIP_LINEAR ||
+          t.wrap_s > OXR_WRAP_MIRRORED_REPEAT || t.wrap_t > OXR_WRAP_MIRRORED_REPEAT)
+        return fail(OXR_ERR_INVALID, "texture %u: unknown format, filter or wrap mode", i);
+    }
+    chain_of.assign(2ull * d->num_textures, -1);
+    return OXR_OK;
+  }
+
+  uint32_t chain(uint32_t tex, bool srgb) {
+    int32_t& c = chain_of[2ull * tex + (srgb ? 0 : 1)];
+    if (c >= 0) return (uint32_t)c;
+    static const std::array<float, 256> lut = [] {
+      std::array<float, 256> t{};
+      for (int i = 0; i < 256; ++i) t[i] = srgb_to_linear((float)i / 255.0f);
+      return t;
+    }();
+    const oxr_texture& t = d->textures[tex];
+    DevTexChain ch{};
+    ch.level0 = (uint32_t)levels.size();
+    ch.w0 = t.width;
+    ch.h0 = t.height;
+    ch.mag_linear = t.mag_filter == OXR_FILTER_LINEAR ? 1u : 0u;
+    ch.min_filter = t.min_filter;
+    ch.wrap_s = t.wrap_s;
+    ch.wrap_t = t.wrap_t;
+    size_t n0 = (size_t)t.width * t.height;
+    size_t base = pool.size();
+    pool.resize(base + n0);
+    if (t.format == OXR_TEXFMT_RGBA8) {
+      const uint8_t* p = static_cast<const uint8_t*>(t.pixels);
+      for (size_t i = 0; i < n0; ++i) {
+        const uint8_t* q = p + 4 * i;
+        pool[base + i] = srgb ? make_float4(lut[q[0]], lut[q[1]], lut[q[2]], (float)q[3] / 255.0f)
+                              : make_float4((float)q[0] / 255.0f, (float)q[1] / 255.0f, (float)q[2] / 255.0f,
+                                            (float)q[3] / 255.0f);
+      }
+    } else {
+      const float* p = static_cast<const float*>(t.pixels);
+      for (size_t i = 0; i < n0; ++i) pool[base + i] = make_float4(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);
+    }
+    levels.push_back(DevTexLevel{(uint64_t)base, t.width, t.height});
+    while (!(levels.back().w == 1 && levels.back().h == 1)) {
+      const DevTexLevel pv = levels.back();
+      uint32_t w = std::max(pv.w / 2, 1u), h = std::max(pv.h / 2, 1u);
+      size_t nb = pool.size();
+      pool.resize(nb + (size_t)w * h);
+      for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+          size_t x0 = std::min(2 * x, pv.w - 1), x1 = std::min(2 * x + 1, pv.w - 1);
+          size_t y0 = std::min(2 * y, pv.h - 1), y1 = std::min(2 * y + 1, pv.h - 1);
+          const float4* src = pool.data() + pv.off;
+          float a[4] = {0, 0, 0, 0};
+          for (size_t idx : {y0 * pv.w + x0, y0 * pv.w + x1, y1 * pv.w + x0, y1 * pv.w + x1}) {
+            a[0] += src[idx].x;
+            a[1] += src[idx].y;
+            a[2] += src[idx].z;
+            a[3] += src[idx].w;
+          }
+          pool[nb + (size_t)y * w + x] = make_float4(a[0] * 0.25f, a[1] * 0.25f, a[2] * 0.25f, a[3] * 0.25f);
+        }
+      levels.push_back(DevTexLevel{(uint64_t)nb, w, h});
+    }
+    ch.nlevels = (uint32_t)levels.size() - ch.level0;
+    c = (int32_t)chains.size();
+    chains.push_back(ch);
+    return (uint32_t)c;
+  }
+  // The 1x1 level of a chain.
+  float4 top(uint32_t c) const { return pool[levels[chains[c].level0 + chains[c].nlevels - 1].off]; }
+};
+
+// Colour space of each oxr_material_textures slot (src/trace.rs:717-895).
+static bool slot_srgb(uint32_t slot) {
+  return slot == TS_BASE || slot == TS_EMISSIVE || slot == TS_SPEC_COLOR || slot == TS_SHEEN_COLOR ||
+         slot == TS_DT_COLOR;
 }
 
 // The fallback material of primitives without one
@@ -284,6 +383,18 @@ struct SceneBuf {
   float* d_lpmf = nullptr;
   float* d_lcdf = nullptr;
   uint32_t* d_llookup = nullptr;
+  // Textures and per-triangle attributes.
+  DevTexChain* d_chains = nullptr;
+  DevTexLevel* d_levels = nullptr;
+  float4* d_texels = nullptr;
+  DevTexBind* d_binds = nullptr;
+  uint2* d_mattex = nullptr;
+  float* d_uv[2] = {nullptr, nullptr};
+  float* d_tan = nullptr;
+  float* d_col = nullptr;
+  uint32_t* d_attr = nullptr;
+  DevScene* d_self = nullptr;  // a device copy of `sc` (the textured-alpha traversal reads it)
+  std::vector<int32_t> chain_of;  // [texture * 2 + linear] -> chain (parity entry points)
   oxr_bvh_info bvh{};
   uint32_t spill_cap = 4;
   size_t bytes = 0;
@@ -303,6 +414,18 @@ struct SceneBuf {
     dfree(d_lpmf);
     dfree(d_lcdf);
     dfree(d_llookup);
+    dfree(d_chains);
+    dfree(d_levels);
+    dfree(d_texels);
+    dfree(d_binds);
+    dfree(d_mattex);
+    dfree(d_uv[0]);
+    dfree(d_uv[1]);
+    dfree(d_tan);
+    dfree(d_col);
+    dfree(d_attr);
+    dfree(d_self);
+    chain_of.clear();
     valid = false;
     bytes = 0;
   }
@@ -339,9 +462,65 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   const oxr_material fb = fallback_material();
   mats.push_back(prepare_material(fb));
   auto src_of = [&](uint32_t mid) -> const oxr_material& { return mid < d->num_materials ? d->materials[mid] : fb; };
+  // Texture bindings per material (the fallback material has none).
+  TexBuilder tb{d};
+  int rc;
+  if ((rc = tb.validate())) return rc;
+  std::vector<DevTexBind> binds;
+  std::vector<uint2> mattex(mats.size(), make_uint2(0u, 0u));
+  if (d->material_textures) {
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+      const oxr_material_textures& mt = d->material_textures[i];
+      const oxr_texture_binding* slot[TS_COUNT] = {
+          &mt.base_color, &mt.metallic_roughness, &mt.normal, &mt.emissive, &mt.specular, &mt.specular_color,
+          &mt.transmission, &mt.thickness, &mt.clearcoat, &mt.clearcoat_roughness, &mt.clearcoat_normal,
+          &mt.sheen_color, &mt.sheen_roughness, &mt.anisotropy, &mt.iridescence, &mt.iridescence_thickness,
+          &mt.diffuse_transmission, &mt.diffuse_transmission_color};
+      uint32_t first = (uint32_t)binds.size();
+      for (uint32_t k = 0; k < TS_COUNT; ++k) {
+        const oxr_texture_binding& b = *slot[k];
+        if (b.texture < 0 || (uint32_t)b.texture >= d->num_textures) continue;
+        DevTexBind db{};
+        db.slot = k;
+        db.chain = tb.chain((uint32_t)b.texture, slot_srgb(k));
+        db.uv_set = b.uv_set;
+        db.xf = b.has_transform ? 1u : 0u;
+        db.m00 = 1.0f;
+        db.m11 = 1.0f;
+        if (db.xf) {  // KHR_texture_transform: T R S
+          float c = std::cos(b.rotation), s = std::sin(b.rotation);
+          db.m00 = c * b.scale[0];
+          db.m01 = s * b.scale[1];
+          db.m02 = b.offset[0];
+          db.m10 = -s * b.scale[0];
+          db.m11 = c * b.scale[1];
+          db.m12 = b.offset[1];
+        }
+        db.aux = k == TS_NORMAL ? mt.normal_scale
+                 : k == TS_CC_NORMAL ? mt.clearcoat_normal_scale
+                 : k == TS_IRID_THICK ? mt.iridescence_thickness_min
+                                      : 0.0f;
+        binds.push_back(db);
+      }
+      mattex[i] = make_uint2(first, (uint32_t)binds.size() - first);
+    }
+  }
+  // Per-triangle attribute flags: an array's bit where it is given (and
+  // tri_flags, when given, sets it).
+  const float* attr_src[4] = {d->uv0, d->uv1, d->tangents, d->colors};
+  const uint32_t attr_bit[4] = {TA_UV0, TA_UV1, TA_TANGENTS, TA_COLORS};
+  bool any_attr = false;
+  for (int k = 0; k < 4; ++k) any_attr = any_attr || attr_src[k] != nullptr;
+  std::vector<uint32_t> attr;
+  if (any_attr) {
+    attr.assign(std::max(n, 1u), 0u);
+    for (uint32_t g = 0; g < n; ++g)
+      for (int k = 0; k < 4; ++k)
+        if (attr_src[k] && (!d->tri_flags || (d->tri_flags[g] & attr_bit[k]))) attr[g] |= attr_bit[k];
+  }
   std::vector<uint32_t> info(std::max(n, 1u), 0), tflags(std::max(n, 1u), 0);
   std::vector<float> alpha(std::max(n, 1u), 1.0f);
-  bool has_blend = false;
+  bool has_blend = false, has_alpha_tex = false;
   for (uint32_t g = 0; g < n; ++g) {
     uint32_t mid = d->material_ids[g];
     if (mid >= d->num_materials) mid = fallback;
@@ -350,13 +529,45 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
     const DevMaterial& m = mats[mid];
     const oxr_material& sm = src_of(mid);
     uint32_t f = (m.flags & MAT_DS) ? TRI_DOUBLE_SIDED : 0u;
-    if (sm.alpha_mode == OXR_ALPHA_MASK && !(m.base[3] >= sm.alpha_cutoff)) f |= TRI_MASK_REJECT;
+    // Alpha that varies over the triangle: a base-colour texture it can
+    // sample or vertex colours (TraceScene::alpha, src/trace.rs:492-506).
+    bool varies = false;
+    if (sm.alpha_mode == OXR_ALPHA_MASK || sm.alpha_mode == OXR_ALPHA_BLEND) {
+      uint32_t a = any_attr ? attr[g] : 0u;
+      varies = (a & TA_COLORS) != 0;
+      for (uint32_t j = 0; j < mattex[mid].y && !varies; ++j) {
+        const DevTexBind& b = binds[mattex[mid].x + j];
+        varies = b.slot == TS_BASE && b.uv_set <= 1u && (a & (b.uv_set ? TA_UV1 : TA_UV0));
+      }
+    }
+    alpha[g] = m.base[3];
+    if (sm.alpha_mode == OXR_ALPHA_MASK) {
+      if (varies) {
+        f |= TRI_MASK_TEX | TRI_ALPHA_TEX;
+        alpha[g] = sm.alpha_cutoff;  // the record holds the cutoff
+      } else if (!(m.base[3] >= sm.alpha_cutoff)) {
+        f |= TRI_MASK_REJECT;
+      }
+    }
     if (sm.alpha_mode == OXR_ALPHA_BLEND) {
-      f |= TRI_BLEND;
+      f |= TRI_BLEND | (varies ? TRI_ALPHA_TEX : 0u);
       has_blend = true;
     }
+    has_alpha_tex = has_alpha_tex || varies;
     tflags[g] = f;
-    alpha[g] = m.base[3];
+  }
+  // Mean emission per material: the factor times the emissive texture's
+  // 1x1 level (src/pathtrace.rs:1484-1500).
+  std::vector<std::array<float, 3>> mean_e(mats.size());
+  for (size_t i = 0; i < mats.size(); ++i) {
+    const DevMaterial& m = mats[i];
+    mean_e[i] = {m.emissive[0], m.emissive[1], m.emissive[2]};
+    for (uint32_t j = 0; j < mattex[i].y; ++j) {
+      const DevTexBind& b = binds[mattex[i].x + j];
+      if (b.slot != TS_EMISSIVE) continue;
+      float4 c = tb.top(b.chain);
+      mean_e[i] = {m.emissive[0] * c.x, m.emissive[1] * c.y, m.emissive[2] * c.z};
+    }
   }
   // Punctual lights (zero-power lights contribute exactly nothing and are
   // dropped).
@@ -371,9 +582,10 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   std::vector<float> lpow;
   for (uint32_t g = 0; g < n; ++g) {
     const DevMaterial& m = mats[info[g] & TI_MAT_MASK];
-    float e0 = (m.flags & MAT_UNLIT) ? 0.0f : m.emissive[0];
-    float e1 = (m.flags & MAT_UNLIT) ? 0.0f : m.emissive[1];
-    float e2 = (m.flags & MAT_UNLIT) ? 0.0f : m.emissive[2];
+    const std::array<float, 3>& me = mean_e[info[g] & TI_MAT_MASK];
+    float e0 = (m.flags & MAT_UNLIT) ? 0.0f : me[0];
+    float e1 = (m.flags & MAT_UNLIT) ? 0.0f : me[1];
+    float e2 = (m.flags & MAT_UNLIT) ? 0.0f : me[2];
     float lum = 0.2126f * e0 + 0.7152f * e1 + 0.0722f * e2;
     if (std::isnan(lum) || lum <= 0.0f) continue;
     const float* p = d->positions + 9ull * g;
@@ -409,7 +621,6 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
     }
     bvh4.nodes.push_back(root);
   }
-  int rc;
   auto up = [&](auto** dst, const auto* srcp, size_t count) -> int {
     int r = dalloc(dst, count);
     if (r) return r;
@@ -433,6 +644,22 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   if ((rc = up(&d_lpmf, pmf.data(), pmf.size()))) return rc;
   if ((rc = up(&d_lcdf, cdf.data(), cdf.size()))) return rc;
   if ((rc = up(&d_llookup, lookup.data(), lookup.size()))) return rc;
+  const bool has_tex = !binds.empty() || any_attr;
+  if (!tb.chains.empty()) {
+    if ((rc = up(&d_chains, tb.chains.data(), tb.chains.size()))) return rc;
+    if ((rc = up(&d_levels, tb.levels.data(), tb.levels.size()))) return rc;
+    if ((rc = up(&d_texels, tb.pool.data(), tb.pool.size()))) return rc;
+  }
+  if (!binds.empty()) {
+    if ((rc = up(&d_binds, binds.data(), binds.size()))) return rc;
+    if ((rc = up(&d_mattex, mattex.data(), mattex.size()))) return rc;
+  }
+  const size_t attr_w[4] = {6, 6, 12, 12};
+  float** attr_dst[4] = {&d_uv[0], &d_uv[1], &d_tan, &d_col};
+  for (int k = 0; k < 4; ++k)
+    if (attr_src[k] && n > 0 && (rc = up(attr_dst[k], attr_src[k], attr_w[k] * n))) return rc;
+  if (any_attr && (rc = up(&d_attr, attr.data(), attr.size()))) return rc;
+  chain_of = tb.chain_of;
   sc.nodes = d_nodes;
   sc.tris = d_tris;
   sc.pos = d_pos;
@@ -449,6 +676,19 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   sc.num_lights = (uint32_t)lights.size();
   sc.num_emissive = (uint32_t)ltris.size();
   sc.has_blend = has_blend ? 1u : 0u;
+  sc.has_tex = has_tex ? 1u : 0u;
+  sc.has_alpha_tex = has_alpha_tex ? 1u : 0u;
+  sc.tex.chains = d_chains;
+  sc.tex.levels = d_levels;
+  sc.tex.texels = d_texels;
+  sc.tex.binds = d_binds;
+  sc.tex.mat = d_mattex;
+  sc.tex.uv[0] = d_uv[0];
+  sc.tex.uv[1] = d_uv[1];
+  sc.tex.tan = d_tan;
+  sc.tex.col = d_col;
+  sc.tex.attr = d_attr;
+  if ((rc = up(&d_self, &sc, 1))) return rc;
   bvh.num_nodes = (uint32_t)bvh4.nodes.size();
   bvh.num_leaves = bvh4.num_leaves;
   bvh.num_tris = (uint32_t)bvh4.tris.size();
@@ -483,6 +723,8 @@ DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const Dev
   f.clamp = std::isfinite(opts.clamp) ? opts.clamp : 0.0f;
   f.ambient = std::isfinite(opts.ambient) ? std::fmax(opts.ambient, 0.0f) : 0.0f;
   f.nslots = sc.num_lights + 1;
+  // pixel_spread (src/trace.rs:1138-1145): the cone angle of one pixel row.
+  f.spread = cam.projection == OXR_PROJ_ORTHOGRAPHIC ? 0.0f : std::atan(2.0f * cam.half_h / (float)f.height);
   if (env && env->img) {
     f.env = *env;
     f.p_env = sc.num_emissive > 0 ? 0.5f : 1.0f;  // Integrator::p_env (src/pathtrace.rs:1915-1921)
@@ -784,6 +1026,8 @@ struct Engine {
     DevScene sc;
     uint32_t guided;
     GuideBatch gb;
+    const DevScene* alpha_sc;
+    uint32_t tex;
   };
   bool graphs = true;
   uint32_t gen = 0;  // bumped when path or spill buffers move
@@ -798,7 +1042,7 @@ struct Engine {
     span_begin(e1);
     launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, multi, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
                     &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
-                    xcd_segments);
+                    xcd_segments, alpha_sc);
     span_end(0, e1);
     if (gb && k < gb->use.kmax) {  // the guide serves the first kmax bounces
       hipEvent_t eg{};
@@ -809,13 +1053,14 @@ struct Engine {
     }
     hipEvent_t e2{};
     span_begin(e2);
-    launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr);
+    launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr,
+                  tex_variant);
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);
     launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray, pb.sidx, pb.meta, nslots, 0,
                    &d_ctrl->scount, &d_ctrl->work[8], pb.vis, d_spill, spill_cap, &d_ctrl->error,
-                   xcd_segments);
+                   xcd_segments, alpha_sc);
     span_end(1, e3);
   }
 
@@ -836,6 +1081,8 @@ struct Engine {
     key.sc = sc_any;
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
+    key.alpha_sc = alpha_sc;
+    key.tex = tex_variant ? 1u : 0u;
     if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
@@ -872,9 +1119,17 @@ struct Engine {
   // loop, then per-segment reduction into the job accumulators.
   // multi: the batch mixes scenes (each path's scene from its job); else
   // every path traces sc_any.
+  // The kernel variants of the current batch: the textured shading and
+  // the textured-alpha traversal (whose single-scene launches read the
+  // scene's device copy through alpha_sc).
+  bool tex_variant = false;
+  const DevScene* alpha_sc = nullptr;
+
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
                  const std::vector<Reduce>& red, const DevScene& sc_any, bool multi, int& rc,
-                 const GuideBatch* gb = nullptr) {
+                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr) {
+    tex_variant = tex;
+    alpha_sc = asc;
     const int grid_shade = (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
     hipEvent_t ea{};
     span_begin(ea);
@@ -1303,7 +1558,8 @@ int oxr_pt::refine(uint32_t n) {
       std::vector<Engine::Reduce> red{{0u, npx_c, S, d_accum + 4 * px0, d_sumsq + 3 * px0}};
       rc = OXR_OK;
       if (gb && gb->use.rec) glaunch::fill(eng.stream, reinterpret_cast<float*>(guide->d_rec), rec_floats);
-      eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, false, rc, gb);
+      eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, false, rc, gb, scene.sc.has_tex != 0,
+                    scene.sc.has_alpha_tex ? scene.d_self : nullptr);
       if (rc) return rc;
     }
   }
@@ -1474,9 +1730,16 @@ struct oxr_srv {
       if ((rc = eng.ensure_spill(spill))) return rc;
       eng.begin_stats();
       rc = OXR_OK;
-      bool multi = false;
-      for (auto& [id, S_] : took) multi = multi || jobs[id].scene != jobs[took[0].first].scene;
-      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, multi, rc);
+      bool multi = false, tex = false;
+      const DevScene* asc = nullptr;  // a mixed batch reads each path's scene through its job
+      for (auto& [id, S_] : took) {
+        multi = multi || jobs[id].scene != jobs[took[0].first].scene;
+        const SceneBuf& sb = *scenes[jobs[id].scene];
+        tex = tex || sb.sc.has_tex;
+        if (sb.sc.has_alpha_tex && !asc) asc = sb.d_self;
+      }
+      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, multi, rc, nullptr,
+                    tex, asc);
       if (rc) return rc;
       if ((rc = eng.end_stats())) return rc;
       st.rounds++;
@@ -1708,6 +1971,71 @@ int oxr_pt_env_eval(oxr_pt* h, uint32_t n, const float* dirs, float* pdf, float*
 int oxr_pt_env_sample(oxr_pt* h, uint32_t n, const float* u, float* dirs, float* pdf, uint8_t* ok) {
   if (!dirs || !ok) return fail(OXR_ERR_INVALID, "null argument");
   return env_common(h, n, 1, u, dirs, pdf, nullptr, ok);
+}
+
+extern "C++" {
+namespace {
+// Host arrays in, one kernel, host array out, through temporary device
+// buffers (the parity entry points).
+struct DevTmp {
+  std::vector<void*> ptrs;
+  ~DevTmp() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  int in(T** d, const T* src, size_t n) {
+    int rc = dalloc(d, std::max<size_t>(n, 1));
+    if (rc) return rc;
+    ptrs.push_back(*d);
+    if (n && src && hipMemcpy(*d, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(OXR_ERR_DEVICE, "upload failed");
+    return OXR_OK;
+  }
+};
+}  // namespace
+}
+
+int oxr_pt_tex_sample(oxr_pt* h, uint32_t tex, int srgb, uint32_t n, const float* uv, const float* lod, float* out) {
+  if (!h || (n && (!uv || !lod || !out))) return fail(OXR_ERR_INVALID, "null argument");
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
+  size_t key = 2ull * tex + (srgb ? 0 : 1);
+  if (key >= h->scene.chain_of.size() || h->scene.chain_of[key] < 0)
+    return fail(OXR_ERR_INVALID, "texture %u is not used in that colour space by any material", tex);
+  if (n == 0) return OXR_OK;
+  HIPC(hipSetDevice(h->eng.device));
+  DevTmp t;
+  float *d_uv, *d_lod;
+  float4* d_out;
+  int rc;
+  if ((rc = t.in(&d_uv, uv, 2ull * n)) || (rc = t.in(&d_lod, lod, n)) || (rc = t.in(&d_out, (const float4*)nullptr, n)))
+    return rc;
+  launch::tex_sample(h->eng.stream, h->scene.sc.tex, (uint32_t)h->scene.chain_of[key], n, d_uv, d_lod, d_out);
+  HIPC(hipStreamSynchronize(h->eng.stream));
+  HIPC(hipMemcpy(out, d_out, 16ull * n, hipMemcpyDeviceToHost));
+  return OXR_OK;
+}
+
+int oxr_pt_material_eval(oxr_pt* h, uint32_t n, const uint32_t* g, const float* bary, const float* width,
+                         const float* dir, const uint8_t* flip, float* out) {
+  if (!h || (n && (!g || !bary || !width || !dir || !flip || !out))) return fail(OXR_ERR_INVALID, "null argument");
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
+  for (uint32_t i = 0; i < n; ++i)
+    if (g[i] >= h->scene.sc.num_tris) return fail(OXR_ERR_INVALID, "triangle %u out of range", g[i]);
+  if (n == 0) return OXR_OK;
+  HIPC(hipSetDevice(h->eng.device));
+  DevTmp t;
+  uint32_t* d_g;
+  float *d_b, *d_w, *d_d, *d_out;
+  uint8_t* d_f;
+  int rc;
+  if ((rc = t.in(&d_g, g, n)) || (rc = t.in(&d_b, bary, 3ull * n)) || (rc = t.in(&d_w, width, n)) ||
+      (rc = t.in(&d_d, dir, 3ull * n)) || (rc = t.in(&d_f, flip, n)) ||
+      (rc = t.in(&d_out, (const float*)nullptr, 43ull * n)))
+    return rc;
+  launch::material_eval(h->eng.stream, h->scene.sc, n, d_g, d_b, d_w, d_d, d_f, d_out);
+  HIPC(hipStreamSynchronize(h->eng.stream));
+  HIPC(hipMemcpy(out, d_out, 43ull * n * sizeof(float), hipMemcpyDeviceToHost));
+  return OXR_OK;
 }
 
 int oxr_pt_samples(oxr_pt* h, uint32_t* out) {

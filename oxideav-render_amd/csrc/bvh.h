@@ -41,6 +41,8 @@ static_assert(sizeof(TriRec) == 48, "TriRec is 48 bytes");
 constexpr uint32_t TRI_DOUBLE_SIDED = 1u;
 constexpr uint32_t TRI_MASK_REJECT = 2u;  // MASK with alpha < cutoff
 constexpr uint32_t TRI_BLEND = 4u;
+constexpr uint32_t TRI_ALPHA_TEX = 8u;  // alpha varies over the triangle (texture / vertex colours)
+constexpr uint32_t TRI_MASK_TEX = 16u;  // MASK with varying alpha: the slot holds the cutoff
 
 constexpr uint32_t NODE_EMPTY = 0xffffffffu;
 

@@ -4,11 +4,13 @@
 
 #include <stdint.h>
 
+#include "dev_tex.h"
+
 namespace oxr {
 
 // Per-material constants (PreparedMaterial + the untextured reads of
 // TraceScene::material_oriented, oxideav-render src/prepare.rs:623-668,
-// src/trace.rs:724-922), resolved and clamped on the host. 144 bytes.
+// src/trace.rs:724-922), resolved and clamped on the host. 160 bytes.
 struct alignas(16) DevMaterial {
   float base[4];
   float metallic, roughness, ior, specular;
@@ -24,9 +26,11 @@ struct alignas(16) DevMaterial {
   float iridescence;
   float irid_ior, irid_thick, dt, dispersion;
   float dt_color[3];
-  float pad;
+  float aniso;      // KHR_materials_anisotropy strength (needs tangents)
+  float aniso_rot;  // its rotation
+  float pad[3];
 };
-static_assert(sizeof(DevMaterial) == 144, "DevMaterial is 144 bytes");
+static_assert(sizeof(DevMaterial) == 160, "DevMaterial is 160 bytes");
 
 constexpr uint32_t MAT_DS = 1u;
 constexpr uint32_t MAT_UNLIT = 2u;
@@ -61,7 +65,10 @@ struct DevScene {
   const uint32_t* l_lookup;
   uint32_t num_tris, num_mats, num_lights, num_emissive;
   uint32_t has_blend;
-  uint32_t pad[3];
+  uint32_t has_tex;        // textures, vertex colours or tangents: the TEX shading variant
+  uint32_t has_alpha_tex;  // MASK / BLEND alpha varies over triangles: the ALPHA traversal variant
+  uint32_t pad;
+  DevTextures tex;
 };
 
 constexpr uint32_t TI_HAS_NORMALS = 0x80000000u;
@@ -91,6 +98,8 @@ struct DevFrame {
   uint32_t nslots;  // NEE slots per path: punctual lights + 1 area
   float p_env;      // Integrator::p_env: 0, 1/2 or 1
   DevEnv env;
+  float spread;     // pixel_spread (src/trace.rs:1138-1145): the ray-cone angle
+  float pad_f[3];
 };
 
 // Path flags (meta.z >> 24).

@@ -101,8 +101,9 @@ OXR_D bool accept_tri(uint32_t flags, float alpha, bool front, bool cull, uint32
 
 // One ray's traversal state. step() visits one node (its leaves are
 // tested right away) and returns true once the ray is finished.
-template <bool ANY>
+template <bool ANY, bool ALPHA = false>
 struct Trav {
+  const DevScene* asc;  // ALPHA: the scene whose textured alpha the filter evaluates
   V3 inv, oi;
   WRay wr;
   float tmin, tmax;
@@ -135,6 +136,18 @@ struct Trav {
     st.bottom = 0;
     st.spn = 0;
     st.overflow = false;
+  }
+
+  // Textured alpha of a candidate (TraceScene::alpha, src/trace.rs:
+  // 492-515): MASK compares it with the cutoff held in the triangle
+  // record; BLEND's coin then uses it in place of the constant alpha.
+  OXR_D bool tex_alpha_filter(uint32_t fl, uint32_t gid, float, float b1, float b2, float& aw) const {
+    if (!(fl & TRI_ALPHA_TEX)) return true;
+    uint32_t mid = asc->tri_info[gid] & TI_MAT_MASK;
+    float al = tex_alpha(asc->tex, asc->mats[mid].base[3], mid, gid, v3(1.0f - b1 - b2, b1, b2));
+    if ((fl & TRI_MASK_TEX) && !(al >= aw)) return false;
+    aw = al;
+    return true;
   }
 
   // nodes / tris: the scene's BVH (uniform across the wave when every
@@ -174,14 +187,17 @@ struct Trav {
           if (!intersect_tri(wr, f4xyz(a), f4xyz(b), f4xyz(e), tmin, tmax, t, b1, b2, front)) continue;
           uint32_t gid = __float_as_uint(a.w);
           uint32_t fl = __float_as_uint(b.w);
+          float aw = e.w;
           if (ANY) {
             if (gid == skip) continue;
-            if (!accept_tri(fl, e.w, front, false, path_seed, ray_id, gid)) continue;
+            if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
+            if (!accept_tri(fl, aw, front, false, path_seed, ray_id, gid)) continue;
             hit_any = true;
             return true;
           } else {
             if (best != HIT_MISS && !(t < tmax || (t == tmax && gid < (best & 0x7fffffffu)))) continue;
-            if (!accept_tri(fl, e.w, front, cull, path_seed, ray_id, gid)) continue;
+            if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
+            if (!accept_tri(fl, aw, front, cull, path_seed, ray_id, gid)) continue;
             tmax = t;
             best = gid | (front ? 0u : HIT_BACK);
             bu = b1;
@@ -233,7 +249,7 @@ constexpr uint32_t REFILL = 64;
 // so with nseg = 8 each XCD traces its own run of neighbouring rays and
 // keeps their BVH nodes in its own L2. `work` holds one counter per segment;
 // gridDim.x is a multiple of nseg.
-template <bool ANY, class Setup, class Step, class Finish>
+template <bool ANY, bool ALPHA, class Setup, class Step, class Finish>
 OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, Setup setup, Step step,
                       Finish finish) {
   const uint32_t lane = lane_id();
@@ -243,7 +259,8 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, 
   const uint32_t s0 = (uint32_t)((uint64_t)count * seg / nseg);
   const uint32_t s1 = (uint32_t)((uint64_t)count * (seg + 1) / nseg);
   uint32_t* wk = work + seg;
-  Trav<ANY> tr;
+  Trav<ANY, ALPHA> tr;
+  tr.asc = nullptr;
   uint32_t ri = s0 + wave * 64u + lane;
   bool active = ri < s1;
   bool exhausted = (uint64_t)s0 + (uint64_t)nwaves * 64u >= s1;
@@ -283,14 +300,14 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, 
 // bounce index k in the t_min slot (t_min = 0); else t_min as given.
 // MULTI: the launch mixes scenes (server rounds); each ray finds its scene
 // through its path's job. Otherwise every ray traces `sc`.
-template <bool PATH, bool MULTI>
+template <bool PATH, bool MULTI, bool ALPHA>
 __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob* __restrict__ jobs,
                                                      const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                      const uint32_t* __restrict__ qpid,
                                                      const uint4* __restrict__ meta, uint32_t fixed_count,
                                                      const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
                                                      uint4* __restrict__ hits, uint32_t* spill, uint32_t spill_cap,
-                                                     uint32_t* err, uint32_t nseg) {
+                                                     uint32_t* err, uint32_t nseg, const DevScene* asc) {
   __shared__ uint32_t s_stack[STACK * TRACE_TB];
   Stack st;
   st.lds = s_stack + threadIdx.x;
@@ -299,7 +316,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
   const Node4* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
-  auto setup = [&](Trav<false>& tr, uint32_t i) {
+  auto setup = [&](Trav<false, ALPHA>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     uint32_t path_seed = 0, ray_id = 0;
     float tmin;
@@ -313,6 +330,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
         ray_nodes = js.nodes;
         ray_tris = js.tris;
         blend = js.has_blend;
+        if (ALPHA) tr.asc = &js;
       }
       tmin = 0.0f;
       cull = k == 0;
@@ -325,12 +343,13 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
       tmin = a.w;
       cull = cull_all != 0;
     }
+    if (ALPHA && !MULTI) tr.asc = asc;
     tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
   };
-  auto step = [&](Trav<false>& tr) {
+  auto step = [&](Trav<false, ALPHA>& tr) {
     return MULTI ? tr.step(st, ray_nodes, ray_tris) : tr.step(st, sc.nodes, sc.tris);
   };
-  auto finish = [&](Trav<false>& tr, uint32_t i) {
+  auto finish = [&](Trav<false, ALPHA>& tr, uint32_t i) {
     uint32_t prim = tr.best;
     if (st.overflow) {
       atomicOr(err, ERR_STACK);
@@ -339,19 +358,20 @@ __global__ __launch_bounds__(TRACE_TB) void k_closest(DevScene sc, const DevJob*
     hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(tr.tmax), prim, __float_as_uint(tr.bu),
                          __float_as_uint(tr.bv));
   };
-  trace_loop<false>(count, work, nseg, st, setup, step, finish);
+  trace_loop<false, ALPHA>(count, work, nseg, st, setup, step, finish);
 }
 
 // Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
 // sidx[i] = path * nslots + slot; writes vis[slot] = 2 when unoccluded.
 // Else: record = oxr_ray, writes occ[i] (1 = occluded). MULTI as k_closest.
-template <bool PATH, bool MULTI>
+template <bool PATH, bool MULTI, bool ALPHA>
 __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                     const uint32_t* __restrict__ sidx, const uint4* __restrict__ meta,
                                                     uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr,
                                                     uint32_t* work, uint8_t* __restrict__ out, uint32_t* spill,
-                                                    uint32_t spill_cap, uint32_t* err, uint32_t nseg) {
+                                                    uint32_t spill_cap, uint32_t* err, uint32_t nseg,
+                                                    const DevScene* asc) {
   __shared__ uint32_t s_stack[STACK * TRACE_TB];
   Stack st;
   st.lds = s_stack + threadIdx.x;
@@ -360,7 +380,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
   const Node4* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
-  auto setup = [&](Trav<true>& tr, uint32_t i) {
+  auto setup = [&](Trav<true, ALPHA>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     float tmin = 0.0f;
     uint32_t skip = NO_SKIP, path_seed = 0, ray_id = 0;
@@ -374,6 +394,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
         ray_nodes = js.nodes;
         ray_tris = js.tris;
         blend = js.has_blend;
+        if (ALPHA) tr.asc = &js;
       }
       if (blend) {
         uint4 m = meta[pid];
@@ -384,12 +405,13 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
     } else {
       tmin = a.w;
     }
+    if (ALPHA && !MULTI) tr.asc = asc;
     tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
   };
-  auto step = [&](Trav<true>& tr) {
+  auto step = [&](Trav<true, ALPHA>& tr) {
     return MULTI ? tr.step(st, ray_nodes, ray_tris) : tr.step(st, sc.nodes, sc.tris);
   };
-  auto finish = [&](Trav<true>& tr, uint32_t i) {
+  auto finish = [&](Trav<true, ALPHA>& tr, uint32_t i) {
     bool occ = tr.hit_any;
     if (st.overflow) {
       atomicOr(err, ERR_STACK);
@@ -401,7 +423,7 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
       out[i] = occ ? 1 : 0;
     }
   };
-  trace_loop<true>(count, work, nseg, st, setup, step, finish);
+  trace_loop<true, ALPHA>(count, work, nseg, st, setup, step, finish);
 }
 
 // ---------------------------------------------------------------------
@@ -580,13 +602,86 @@ OXR_D void load_mat(const DevMaterial& m, V3 sn, MatSample& ms) {
   ms.diffuse_transmission_color = ld3(m.dt_color);
 }
 
+// The textured reads of TraceScene::material_oriented (src/trace.rs:
+// 717-895) on top of the factors load_mat put in `ms`. snf: the shading
+// normal, already reversed on a double-sided back face (`flip`); the
+// mapped normals are reversed back afterwards (src/pathtrace.rs:2093-2101).
+OXR_D void tex_apply(const DevTextures& T, uint2 run, uint32_t g, uint32_t attr, V3 bary, const float* pos9,
+                     const TexCone& cone, V3 snf, bool flip, const DevMaterial& M, MatSample& ms) {
+  V3 normal = snf, ccn = snf;
+  float irid_thick = ms.iridescence_thickness;
+  for (uint32_t i = 0; i < run.y; ++i) {
+    const DevTexBind b = T.binds[run.x + i];
+    F4 t;
+    if (!tex_tap(T, b, g, attr, bary, pos9, cone, t)) continue;
+    switch (b.slot) {
+      case TS_BASE: ms.base = v3(ms.base.x * t.v[0], ms.base.y * t.v[1], ms.base.z * t.v[2]); break;
+      case TS_MR:
+        ms.roughness *= t.v[1];
+        ms.metallic *= t.v[2];
+        break;
+      case TS_NORMAL: normal = tex_perturb(T, g, attr, bary, snf, t, b.aux); break;
+      case TS_SPEC: ms.specular *= t.v[3]; break;
+      case TS_SPEC_COLOR: ms.specular_color = mul(ms.specular_color, v3(t.v[0], t.v[1], t.v[2])); break;
+      case TS_TRANSMISSION: ms.transmission *= t.v[0]; break;
+      case TS_THICKNESS: ms.thickness *= t.v[1]; break;
+      case TS_CC: ms.clearcoat *= t.v[0]; break;
+      case TS_CC_ROUGH: ms.clearcoat_roughness *= t.v[1]; break;
+      case TS_CC_NORMAL: ccn = tex_perturb(T, g, attr, bary, snf, t, b.aux); break;
+      case TS_SHEEN_COLOR: ms.sheen_color = mul(ms.sheen_color, v3(t.v[0], t.v[1], t.v[2])); break;
+      case TS_SHEEN_ROUGH: ms.sheen_roughness *= t.v[3]; break;
+      case TS_IRID: ms.iridescence *= t.v[0]; break;
+      case TS_IRID_THICK: {
+        float lo = b.aux, hi = M.irid_thick;
+        irid_thick = fmaxr(lo + (hi - lo) * clampr(t.v[1], 0.0f, 1.0f), 0.0f);
+        break;
+      }
+      case TS_DT: ms.diffuse_transmission *= t.v[3]; break;
+      case TS_DT_COLOR:
+        ms.diffuse_transmission_color = mul(ms.diffuse_transmission_color, v3(t.v[0], t.v[1], t.v[2]));
+        break;
+      default: break;  // emissive: Integrator::emission; anisotropy: below
+    }
+  }
+  if (attr & TA_COLORS) {
+    F4 c = tex_interp4(T.col + 12ull * g, bary);
+    ms.base = v3(ms.base.x * c.v[0], ms.base.y * c.v[1], ms.base.z * c.v[2]);
+  }
+  ms.metallic = clampr(ms.metallic, 0.0f, 1.0f);
+  ms.roughness = clampr(ms.roughness, 0.0f, 1.0f);
+  ms.iridescence_thickness = irid_thick;
+  // KHR_materials_anisotropy (src/trace.rs:835-858).
+  if (M.aniso > 0.0f) {
+    float a = M.aniso;
+    float d0 = 1.0f, d1 = 0.0f;
+    const DevTexBind* ab = tex_find(T, run, TS_ANISO);
+    F4 t;
+    if (ab && tex_tap(T, *ab, g, attr, bary, pos9, cone, t)) {
+      d0 = t.v[0] * 2.0f - 1.0f;
+      d1 = t.v[1] * 2.0f - 1.0f;
+      a *= clampr(t.v[2], 0.0f, 1.0f);
+    }
+    float sr = sinf(M.aniso_rot), cr = cosf(M.aniso_rot);
+    float r0 = cr * d0 - sr * d1, r1 = sr * d0 + cr * d1;
+    V3 dir = v3(0.0f, 0.0f, 0.0f), tt, bt;
+    if (tex_tangent_frame(T, g, attr, bary, snf, tt, bt)) {
+      V3 d = normalise(v3(tt.x * r0 + bt.x * r1, tt.y * r0 + bt.y * r1, tt.z * r0 + bt.z * r1));
+      if (dot(d, d) > 0.5f) dir = d;
+    }
+    ms.anisotropy = dot(dir, dir) < 0.5f ? 0.0f : a;
+    ms.anisotropy_direction = dir;
+  }
+  ms.normal = flip ? neg(normal) : normal;
+  ms.clearcoat_normal = flip ? neg(ccn) : ccn;
+}
+
 OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_origin(p, ng) : offset_ray_origin(p, neg(ng)); }
 
 // ---------------------------------------------------------------------
 // Shading: one path vertex of Integrator::sample (src/pathtrace.rs:
 // 2027-2322), with shadow rays deferred to k_anyhit.
 // ---------------------------------------------------------------------
-template <int WAVES, bool GUIDE>
+template <int WAVES, bool GUIDE, bool TEX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
                                              const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
@@ -673,10 +768,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     uint32_t g = h.y & 0x7fffffffu;
     bool front = !(h.y & HIT_BACK);
     V3 p = v3(0, 0, 0), ngeo = v3(0, 0, 1), sn = v3(0, 0, 1);
+    V3 bary = v3(1.0f, 0.0f, 0.0f);
     uint32_t mid = 0;
     if (alive) {
       float u = __uint_as_float(h.z), v = __uint_as_float(h.w);
-      V3 bary = v3(1.0f - u - v, u, v);
+      bary = v3(1.0f - u - v, u, v);
       const float* tv = sc.pos + 9ull * g;
       V3 p0 = ld3(tv), p1 = ld3(tv + 3), p2 = ld3(tv + 6);
       p = v3(p0.x * bary.x + p1.x * bary.y + p2.x * bary.z, p0.y * bary.x + p1.y * bary.y + p2.y * bary.z,
@@ -698,6 +794,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     const DevMaterial* M = sc.mats + mid;
     if (alive) {
       V3 em = ld3(M->emissive);
+      if (TEX && !is_black(em)) em = mul(em, tex_emission_factor(sc.tex, mid, g, bary));
       bool ds = (M->flags & MAT_DS) != 0;
       bool unlit = (M->flags & MAT_UNLIT) != 0;
       if ((front || ds) && !unlit && !is_black(em)) {
@@ -728,6 +825,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     if (alive && k >= fr.max_bounces) alive = false;
     MatSample ms;
     load_mat(*M, sn, ms);
+    if (TEX && alive) {
+      uint2 run = sc.tex.mat ? sc.tex.mat[mid] : make_uint2(0u, 0u);
+      uint32_t attr = sc.tex.attr ? sc.tex.attr[g] : 0u;
+      if (run.y || (attr & TA_COLORS) || (M->aniso > 0.0f && (attr & TA_TANGENTS))) {
+        bool flip = !front && (M->flags & MAT_DS);
+        TexCone cone{0.0f, d};
+        if (k == 0) cone.width = fr.projection == 1u ? fr.spread : fr.spread * t;  // src/pathtrace.rs:2083-2092
+        tex_apply(sc.tex, run, g, attr, bary, sc.pos + 9ull * g, cone, flip ? neg(sn) : sn, flip, *M, ms);
+      }
+    }
     float thickness = M->thickness, transmission = M->transmission;
     if (alive && M->dispersion > 0.0f && thickness > 0.0f && transmission > 0.0f) {
       uint32_t c;
@@ -897,6 +1004,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
               float pb;
               bsdf.eval(l, f, pb);
               V3 le = (LM.flags & MAT_UNLIT) ? v3(0, 0, 0) : ld3(LM.emissive);
+              if (TEX && !is_black(le))
+                le = mul(le, tex_emission_factor(sc.tex, sc.tri_info[lg] & TI_MAT_MASK, lg, bary));
               if (!is_black(f) && !is_black(le)) {
                 if (GUIDE && gmix > 0.0f) pb = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pb;
                 float w = fr.strategy == 0u ? power_heuristic(pl, pb) : 1.0f;
@@ -1046,6 +1155,77 @@ __global__ void k_guide_collect(uint32_t P, uint32_t nsel, uint32_t shift, const
       samp[2ull * pos + 1] = make_float4(r1.x, r1.y, r1.z, r0.w);
     }
   }
+}
+
+// ---------------------------------------------------------------------
+// Parity entry points of the texture layer: PreparedTexture::sample_lod
+// on host (u, v, lod) triples, and material_oriented + emission + the
+// any-hit alpha at host (triangle, barycentrics) hits, 43 floats each in
+// the layout of the oracle's material evaluation.
+// ---------------------------------------------------------------------
+__global__ void k_tex_sample(DevTextures T, uint32_t chain, uint32_t n, const float* __restrict__ uv,
+                             const float* __restrict__ lod, float4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  F4 r = tex_sample_lod(T, chain, uv[2 * i], uv[2 * i + 1], lod[i]);
+  out[i] = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+}
+
+__global__ void k_material_eval(DevScene sc, uint32_t n, const uint32_t* __restrict__ gids,
+                                const float* __restrict__ bary3, const float* __restrict__ width,
+                                const float* __restrict__ dir3, const uint8_t* __restrict__ flips,
+                                float* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t g = gids[i];
+  V3 bary = v3(bary3[3 * i], bary3[3 * i + 1], bary3[3 * i + 2]);
+  const float* tv = sc.pos + 9ull * g;
+  V3 p0 = ld3(tv), p1 = ld3(tv + 3), p2 = ld3(tv + 6);
+  V3 cr = cross(sub(p1, p0), sub(p2, p0));
+  float len = sqrtf(dot(cr, cr));
+  V3 sn = len > 0.0f ? v3(cr.x / len, cr.y / len, cr.z / len) : v3(0.0f, 0.0f, 1.0f);
+  uint32_t info = sc.tri_info[g];
+  uint32_t mid = info & TI_MAT_MASK;
+  if (info & TI_HAS_NORMALS) {
+    const float* nv = sc.nrm + 9ull * g;
+    V3 n0 = ld3(nv), n1 = ld3(nv + 3), n2 = ld3(nv + 6);
+    V3 nn = normalise(v3(n0.x * bary.x + n1.x * bary.y + n2.x * bary.z, n0.y * bary.x + n1.y * bary.y + n2.y * bary.z,
+                         n0.z * bary.x + n1.z * bary.y + n2.z * bary.z));
+    sn = dot(nn, nn) > 0.5f ? nn : sn;
+  }
+  const DevMaterial& M = sc.mats[mid];
+  bool flip = flips[i] != 0;
+  MatSample ms;
+  load_mat(M, sn, ms);
+  TexCone cone{width[i], v3(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2])};
+  uint2 run = sc.tex.mat ? sc.tex.mat[mid] : make_uint2(0u, 0u);
+  uint32_t attr = sc.tex.attr ? sc.tex.attr[g] : 0u;
+  float base_a = M.base[3];
+  V3 emissive = ld3(M.emissive);
+  if (run.y || (attr & TA_COLORS) || (M.aniso > 0.0f && (attr & TA_TANGENTS))) {
+    tex_apply(sc.tex, run, g, attr, bary, tv, cone, flip ? neg(sn) : sn, flip, M, ms);
+    F4 t;
+    const DevTexBind* b = tex_find(sc.tex, run, TS_BASE);
+    if (b && tex_tap(sc.tex, *b, g, attr, bary, tv, cone, t)) base_a *= t.v[3];
+    if (attr & TA_COLORS) base_a *= tex_interp4(sc.tex.col + 12ull * g, bary).v[3];
+    b = tex_find(sc.tex, run, TS_EMISSIVE);
+    if (b && tex_tap(sc.tex, *b, g, attr, bary, tv, cone, t)) emissive = mul(emissive, v3(t.v[0], t.v[1], t.v[2]));
+  } else if (flip) {
+    ms.normal = sn;  // flipped and flipped back
+    ms.clearcoat_normal = sn;
+  }
+  V3 le = ((M.flags & MAT_UNLIT) || is_black(ld3(M.emissive))) ? v3(0, 0, 0)
+                                                               : mul(ld3(M.emissive), tex_emission_factor(sc.tex, mid, g, bary));
+  float alpha = tex_alpha(sc.tex, M.base[3], mid, g, bary);
+  float v[43] = {ms.base.x, ms.base.y, ms.base.z, ms.metallic, ms.roughness, ms.normal.x, ms.normal.y, ms.normal.z,
+                 ms.ior, ms.specular, ms.specular_color.x, ms.specular_color.y, ms.specular_color.z, ms.transmission,
+                 ms.thickness, ms.clearcoat, ms.clearcoat_roughness, ms.clearcoat_normal.x, ms.clearcoat_normal.y,
+                 ms.clearcoat_normal.z, ms.sheen_color.x, ms.sheen_color.y, ms.sheen_color.z, ms.sheen_roughness,
+                 ms.anisotropy, ms.anisotropy_direction.x, ms.anisotropy_direction.y, ms.anisotropy_direction.z,
+                 ms.iridescence, ms.iridescence_ior, ms.iridescence_thickness, ms.diffuse_transmission,
+                 ms.diffuse_transmission_color.x, ms.diffuse_transmission_color.y, ms.diffuse_transmission_color.z,
+                 base_a, emissive.x, emissive.y, emissive.z, le.x, le.y, le.z, alpha};
+  for (int k = 0; k < 43; ++k) out[43ull * i + k] = v[k];
 }
 
 // Per-sample radiance: the pending NEE of the last vertex, then the
@@ -1315,14 +1495,26 @@ hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, con
 }
 
 int trace_blocks_per_cu(bool any) {
-  int nb = 0, nb2 = 0;
-  hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_anyhit<true, false>, TRACE_TB, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_closest<true, false>, TRACE_TB, 0);
-  hipError_t e2 = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_anyhit<true, true>, TRACE_TB, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_closest<true, true>, TRACE_TB, 0);
-  // The persistent grid must fit on the device for either variant.
-  if (e != hipSuccess || e2 != hipSuccess || nb <= 0 || nb2 <= 0) return 4;
-  return std::min(nb, nb2);
+  // The persistent grid must fit on the device for every variant.
+  int nb[4] = {0, 0, 0, 0};
+  hipError_t e[4];
+  if (any) {
+    e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], k_anyhit<true, false, false>, TRACE_TB, 0);
+    e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], k_anyhit<true, true, false>, TRACE_TB, 0);
+    e[2] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[2], k_anyhit<true, false, true>, TRACE_TB, 0);
+    e[3] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[3], k_anyhit<true, true, true>, TRACE_TB, 0);
+  } else {
+    e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], k_closest<true, false, false>, TRACE_TB, 0);
+    e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], k_closest<true, true, false>, TRACE_TB, 0);
+    e[2] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[2], k_closest<true, false, true>, TRACE_TB, 0);
+    e[3] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[3], k_closest<true, true, true>, TRACE_TB, 0);
+  }
+  int m = 1 << 30;
+  for (int k = 0; k < 4; ++k) {
+    if (e[k] != hipSuccess || nb[k] <= 0) return 4;
+    m = std::min(m, nb[k]);
+  }
+  return m;
 }
 
 int trace_block_threads() { return TRACE_TB; }
@@ -1331,34 +1523,50 @@ int stack_lds_entries() { return STACK; }
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
-             uint32_t spill_cap, uint32_t* err, uint32_t nseg) {
+             uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc) {
   if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
-#define OXR_CLOSEST(P, M)                                                                                        \
-  hipLaunchKernelGGL((k_closest<P, M>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta,         \
-                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err, nseg)
-  if (!path)
-    OXR_CLOSEST(false, false);
+  // asc (the scene in device memory) selects the textured-alpha variant;
+  // a mixed batch finds each path's scene through its job instead.
+  const bool alpha = asc != nullptr;
+#define OXR_CLOSEST(P, M, A)                                                                                     \
+  hipLaunchKernelGGL((k_closest<P, M, A>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta,      \
+                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err, nseg, asc)
+  if (!path && alpha)
+    OXR_CLOSEST(false, false, true);
+  else if (!path)
+    OXR_CLOSEST(false, false, false);
+  else if (multi && alpha)
+    OXR_CLOSEST(true, true, true);
   else if (multi)
-    OXR_CLOSEST(true, true);
+    OXR_CLOSEST(true, true, false);
+  else if (alpha)
+    OXR_CLOSEST(true, false, true);
   else
-    OXR_CLOSEST(true, false);
+    OXR_CLOSEST(true, false, false);
 #undef OXR_CLOSEST
 }
 
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots,
             uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill,
-            uint32_t spill_cap, uint32_t* err, uint32_t nseg) {
+            uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc) {
   if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
-#define OXR_ANYHIT(P, M)                                                                                         \
-  hipLaunchKernelGGL((k_anyhit<P, M>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta, nslots, \
-                     fixed_count, count_ptr, work, out, spill, spill_cap, err, nseg)
-  if (!path)
-    OXR_ANYHIT(false, false);
+  const bool alpha = asc != nullptr;
+#define OXR_ANYHIT(P, M, A)                                                                                      \
+  hipLaunchKernelGGL((k_anyhit<P, M, A>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta,      \
+                     nslots, fixed_count, count_ptr, work, out, spill, spill_cap, err, nseg, asc)
+  if (!path && alpha)
+    OXR_ANYHIT(false, false, true);
+  else if (!path)
+    OXR_ANYHIT(false, false, false);
+  else if (multi && alpha)
+    OXR_ANYHIT(true, true, true);
   else if (multi)
-    OXR_ANYHIT(true, true);
+    OXR_ANYHIT(true, true, false);
+  else if (alpha)
+    OXR_ANYHIT(true, false, true);
   else
-    OXR_ANYHIT(true, false);
+    OXR_ANYHIT(true, false, false);
 #undef OXR_ANYHIT
 }
 
@@ -1372,20 +1580,26 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 }
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide) {
+           uint32_t cur, const DevGuideUse* guide, bool tex) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
-#define OXR_SHADE(W, G)                                                                                          \
-  hipLaunchKernelGGL((k_shade<W, G>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],             \
+#define OXR_SHADE(W, G, T)                                                                                       \
+  hipLaunchKernelGGL((k_shade<W, G, T>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],          \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
                      pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
-  if (guide)
-    OXR_SHADE(1, true);
+  // Textured scenes take the TEX variant at one wave per SIMD; the
+  // occupancy variants are for untextured shading.
+  if (guide && tex)
+    OXR_SHADE(1, true, true);
+  else if (guide)
+    OXR_SHADE(1, true, false);
+  else if (tex)
+    OXR_SHADE(1, false, true);
   else if (waves >= 3)
-    OXR_SHADE(3, false);
+    OXR_SHADE(3, false, false);
   else if (waves == 2)
-    OXR_SHADE(2, false);
+    OXR_SHADE(2, false, false);
   else
-    OXR_SHADE(1, false);
+    OXR_SHADE(1, false, false);
 #undef OXR_SHADE
 }
 
@@ -1394,6 +1608,18 @@ void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathB
   uint32_t nsel = (uint32_t)(((uint64_t)P + (1ull << gu.shift) - 1ull) >> gu.shift);
   hipLaunchKernelGGL(k_guide_collect, dim3((nsel + 255) / 256), dim3(256), 0, s, P, nsel, gu.shift, gu.rec, pb.out,
                      samp, count, cap);
+}
+
+void tex_sample(hipStream_t s, const DevTextures& T, uint32_t chain, uint32_t n, const float* uv, const float* lod,
+                float4* out) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_tex_sample, dim3((n + 255) / 256), dim3(256), 0, s, T, chain, n, uv, lod, out);
+}
+
+void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t* g, const float* bary,
+                   const float* width, const float* dir, const uint8_t* flip, float* out) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_material_eval, dim3((n + 127) / 128), dim3(128), 0, s, sc, n, g, bary, width, dir, flip, out);
 }
 
 void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb) {

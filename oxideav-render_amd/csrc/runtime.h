@@ -40,20 +40,30 @@ int stack_lds_entries();
 // a multiple of it).
 // path: rays of the wavefront (else host rays of the parity entry points);
 // multi: the launch mixes scenes (each ray's scene from its job), else all
-// rays trace `sc`.
+// rays trace `sc`. asc non-null: the textured-alpha variant (MASK / BLEND
+// alpha varying over triangles); single-scene launches read that scene's
+// device copy through it.
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
-             uint32_t spill_cap, uint32_t* err, uint32_t nseg = 1);
+             uint32_t spill_cap, uint32_t* err, uint32_t nseg = 1, const DevScene* asc = nullptr);
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
             const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,
-            uint32_t* err, uint32_t nseg = 1);
+            uint32_t* err, uint32_t nseg = 1, const DevScene* asc = nullptr);
 void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl);
 void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev);
 // guide: null for the plain estimator.
+// Texture parity entry points: sample_lod of a chain at (u, v, lod)
+// triples; material_oriented + emission + alpha at hits (43 floats each).
+void tex_sample(hipStream_t s, const DevTextures& T, uint32_t chain, uint32_t n, const float* uv, const float* lod,
+                float4* out);
+void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t* g, const float* bary,
+                   const float* width, const float* dir, const uint8_t* flip, float* out);
+// tex: the TEX variant (textures, vertex colours or tangents in a scene
+// of the batch).
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide);
+           uint32_t cur, const DevGuideUse* guide, bool tex = false);
 // Training samples (2 float4 each) of the guide-recorded paths of a batch,
 // appended at *count (entries past cap are dropped).
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,

@@ -53,6 +53,8 @@ def lib():
         "oxr_pt_set_environment": (C.c_int, [vp, fp, u32, u32, C.c_float]),
         "oxr_pt_env_eval": (C.c_int, [vp, u32, fp, fp, fp]),
         "oxr_pt_env_sample": (C.c_int, [vp, u32, fp, fp, fp, bp]),
+        "oxr_pt_tex_sample": (C.c_int, [vp, u32, C.c_int, u32, fp, fp, fp]),
+        "oxr_pt_material_eval": (C.c_int, [vp, u32, up, fp, fp, fp, bp, fp]),
         "oxr_sobol_owen_4d": (C.c_int, [C.c_int, u32, up, up, fp]),
         "oxr_trace_closest": (C.c_int, [vp, u32, C.POINTER(A.Ray), u32, C.POINTER(A.Hit)]),
         "oxr_trace_any": (C.c_int, [vp, u32, C.POINTER(A.Ray), bp]),
@@ -97,7 +99,7 @@ EXPORTED = [
     "oxr_last_error", "oxr_device_count", "oxr_pt_create", "oxr_pt_destroy", "oxr_pt_upload_scene",
     "oxr_pt_set_camera", "oxr_pt_set_options", "oxr_pt_reset", "oxr_pt_refine", "oxr_pt_samples",
     "oxr_pt_read_accum", "oxr_pt_resolve_hdr", "oxr_pt_resolve_rgba8", "oxr_pt_set_timing", "oxr_pt_stats_get",
-    "oxr_pt_autotune", "oxr_pt_set_environment", "oxr_pt_env_eval", "oxr_pt_env_sample",
+    "oxr_pt_autotune", "oxr_pt_set_environment", "oxr_pt_env_eval", "oxr_pt_tex_sample", "oxr_pt_material_eval", "oxr_pt_env_sample",
     "oxr_sobol_owen_4d", "oxr_trace_closest", "oxr_trace_any", "oxr_trace_closest_device", "oxr_pt_bvh_info",
     "oxr_pt_bvh_export", "oxr_bsdf_eval", "oxr_bsdf_sample",
     "oxr_srv_create", "oxr_srv_destroy", "oxr_srv_add_scene", "oxr_srv_submit", "oxr_srv_step",

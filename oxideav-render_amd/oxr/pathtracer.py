@@ -179,6 +179,35 @@ class PathTracer:
         check(self._L.oxr_pt_set_environment(self._h, img.ctypes.data_as(fpp), img.shape[1], img.shape[0],
                                              float(intensity)), "set_environment")
 
+    def tex_sample(self, tex, uv, lod=None, srgb=False):
+        """sample_lod of texture `tex` (in the colour space a material
+        uses it in) at (n, 2) uv and n levels: (n, 4)."""
+        fpp = C.POINTER(C.c_float)
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        lod = np.zeros(len(uv), np.float32) if lod is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(lod, np.float32), (len(uv),)))
+        out = np.zeros((len(uv), 4), np.float32)
+        check(self._L.oxr_pt_tex_sample(self._h, tex, 1 if srgb else 0, len(uv), uv.ctypes.data_as(fpp),
+                                        lod.ctypes.data_as(fpp), out.ctypes.data_as(fpp)), "tex_sample")
+        return out
+
+    def material_eval(self, g, bary, width=None, dirs=None, flip=None):
+        """material_oriented + emission + alpha at hits: (n, 43)."""
+        fpp = C.POINTER(C.c_float)
+        g = np.ascontiguousarray(g, np.uint32).ravel()
+        n = len(g)
+        bary = np.ascontiguousarray(bary, np.float32).reshape(n, 3)
+        width = np.zeros(n, np.float32) if width is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(width, np.float32), (n,)))
+        dirs = np.zeros((n, 3), np.float32) if dirs is None else np.ascontiguousarray(dirs, np.float32).reshape(n, 3)
+        flip = np.zeros(n, np.uint8) if flip is None else np.ascontiguousarray(flip, np.uint8).ravel()
+        out = np.zeros((n, 43), np.float32)
+        check(self._L.oxr_pt_material_eval(self._h, n, g.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           bary.ctypes.data_as(fpp), width.ctypes.data_as(fpp),
+                                           dirs.ctypes.data_as(fpp), flip.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                           out.ctypes.data_as(fpp)), "material_eval")
+        return out
+
     def env_eval(self, dirs):
         fpp = C.POINTER(C.c_float)
         d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)

@@ -332,6 +332,39 @@ def uv_sphere(radius, segments, rings):
     return pos[idx].reshape(-1, 9), n[idx].reshape(-1, 9)
 
 
+def uv_sphere_textured(radius, segments, rings, center=(0.0, 0.0, 0.0)):
+    """uv_sphere's grid with per-corner TEXCOORD_0 (u along phi, v along
+    theta) and tangents (d/dphi, w = 1): (pos, nrm, uv, tangents)."""
+    r_ = np.arange(rings + 1, dtype=F32)
+    s_ = np.arange(segments + 1, dtype=F32)
+    v = r_ / F32(rings)
+    u = s_ / F32(segments)
+    theta = (v * F32(math.pi)).astype(F32)
+    phi = (u * F32(2.0) * F32(math.pi)).astype(F32)
+    st, ct = np.sin(theta).astype(F32), np.cos(theta).astype(F32)
+    sp, cp = np.sin(phi).astype(F32), np.cos(phi).astype(F32)
+    n = np.stack([st[:, None] * cp[None, :], np.broadcast_to(ct[:, None], (rings + 1, segments + 1)),
+                  -st[:, None] * sp[None, :]], axis=-1).astype(F32).reshape(-1, 3)
+    pos = (n * F32(radius) + np.asarray(center, F32)).astype(F32)
+    uv = np.stack(np.broadcast_arrays(u[None, :], v[:, None]), axis=-1).astype(F32).reshape(-1, 2)
+    tg = np.stack([np.broadcast_to(-sp[None, :], (rings + 1, segments + 1)), np.zeros((rings + 1, segments + 1), F32),
+                   np.broadcast_to(-cp[None, :], (rings + 1, segments + 1)),
+                   np.ones((rings + 1, segments + 1), F32)], axis=-1).astype(F32).reshape(-1, 4)
+    row = segments + 1
+    a = (np.arange(rings)[:, None] * row + np.arange(segments)[None, :]).reshape(-1)
+    b = a + row
+    idx = np.stack([a, b, a + 1, a + 1, b, b + 1], axis=1).reshape(-1, 3)
+    return pos[idx].reshape(-1, 9), n[idx].reshape(-1, 9), uv[idx].reshape(-1, 6), tg[idx].reshape(-1, 12)
+
+
+def quad_uv(p, uv=((0, 0), (1, 0), (1, 1), (0, 1))):
+    """quad() with per-corner TEXCOORD_0 of its corners p0..p3."""
+    t, nrm = quad(p)
+    uv = np.asarray(uv, F32)
+    uvs = np.stack([np.concatenate([uv[0], uv[1], uv[2]]), np.concatenate([uv[0], uv[2], uv[3]])]).astype(F32)
+    return t, nrm, uvs
+
+
 def rotate_y(points9, deg, center):
     """Rotate (n, 9) corners about the Y axis through `center`."""
     c, s = math.cos(math.radians(deg)), math.sin(math.radians(deg))
@@ -569,6 +602,118 @@ def glass_cube(volume):
     s.add_tris(t, e, normals=n)
     s.camera = CameraSpec((0.0, 0.0, 4.0), (0.0, 0.0, 0.0), yfov_deg=math.degrees(0.15))
     s.ambient = 0.0
+    return s
+
+
+def _srgb_byte(x):
+    x = np.clip(np.asarray(x, np.float64), 0.0, 1.0)
+    s = np.where(x <= 0.0031308, x * 12.92, 1.055 * np.power(x, 1.0 / 2.4) - 0.055)
+    return np.round(s * 255.0).astype(np.uint8)
+
+
+def textured_showcase(segments=48, rings=24):
+    """Every texture path of the tracer in one small scene:
+    - a 4 m floor with an RGBA8 checker base-colour texture (0.8 / 0.2 in
+      1 m cells through a KHR_texture_transform scale of 4) and a
+      metallic-roughness texture;
+    - a normal-mapped anisotropic metal sphere (tangents, bump map, an
+      anisotropy direction texture) and a clearcoat / sheen sphere with
+      extension textures;
+    - a double-sided quad emitter with an sRGB emissive gradient;
+    - an alpha-MASK quad (a disc texture, cutoff 0.5) and a BLEND quad
+      whose alpha comes from vertex colours;
+    - a vertex-coloured box, a point light."""
+    s = Scene("textured_showcase")
+    # Textures.
+    n = 64
+    yy, xx = np.mgrid[0:n, 0:n]
+    cell = ((xx * 4 // n) + (yy * 4 // n)) % 2
+    check = np.zeros((n, n, 4), np.uint8)
+    check[..., :3] = np.where(cell[..., None] == 0, _srgb_byte(0.8), _srgb_byte(0.2))
+    check[..., 3] = 255
+    t_check = s.add_texture(check)
+    mr = np.zeros((16, 16, 4), np.uint8)
+    mr[..., 1] = np.linspace(60, 255, 16, dtype=np.float32)[None, :].astype(np.uint8)  # roughness ramp
+    mr[..., 2] = 0
+    mr[..., 3] = 255
+    t_mr = s.add_texture(mr, min_filter=A.MIN_LINEAR_MIP_NEAREST)
+    bump = np.zeros((32, 64, 4), np.float32)
+    by, bx = np.mgrid[0:32, 0:64]
+    bump[..., 0] = 0.5 + 0.25 * np.sin(2 * np.pi * bx / 8.0)
+    bump[..., 1] = 0.5 + 0.25 * np.cos(2 * np.pi * by / 8.0)
+    bump[..., 2] = 0.9
+    bump[..., 3] = 1.0
+    t_bump = s.add_texture(bump, wrap_t=A.WRAP_CLAMP_TO_EDGE)
+    aniso = np.zeros((8, 8, 4), np.uint8)
+    aniso[..., 0] = 255
+    aniso[..., 1] = 128
+    aniso[..., 2] = np.linspace(64, 255, 8, dtype=np.float32)[:, None].astype(np.uint8)
+    aniso[..., 3] = 255
+    t_aniso = s.add_texture(aniso, mag=A.FILTER_NEAREST, min_filter=A.MIN_NEAREST_MIP_LINEAR)
+    grad = np.zeros((1, 16, 4), np.uint8)
+    grad[0, :, 0] = 255
+    grad[0, :, 1] = np.linspace(40, 255, 16).astype(np.uint8)
+    grad[0, :, 2] = np.linspace(255, 40, 16).astype(np.uint8)
+    grad[0, :, 3] = 255
+    t_grad = s.add_texture(grad, wrap_s=A.WRAP_MIRRORED_REPEAT)
+    disc = np.zeros((32, 32, 4), np.uint8)
+    dy, dx = np.mgrid[0:32, 0:32]
+    disc[..., :3] = (200, 120, 60)
+    disc[..., 3] = np.where((dx - 15.5) ** 2 + (dy - 15.5) ** 2 < 14.0 ** 2, 255, 0).astype(np.uint8)
+    t_disc = s.add_texture(disc)
+    cc = np.zeros((8, 8, 4), np.uint8)
+    cc[..., 0] = np.linspace(80, 255, 8).astype(np.uint8)[None, :]
+    cc[..., 1] = np.linspace(255, 40, 8).astype(np.uint8)[:, None]
+    cc[..., 2] = 200
+    cc[..., 3] = 180
+    t_cc = s.add_texture(cc)
+    # Floor.
+    floor = s.add_material(A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.8))
+    t, nr, uv = quad_uv([[-2, 0, 2], [2, 0, 2], [2, 0, -2], [-2, 0, -2]])
+    s.add_tris(t, floor, normals=nr, uv0=uv)
+    s.set_material_textures(floor, base_color=dict(texture=t_check, scale=(4.0, 4.0)), metallic_roughness=t_mr)
+    # Normal-mapped anisotropic sphere.
+    metal = s.add_material(A.material((0.9, 0.7, 0.4, 1.0), metallic=1.0, roughness=0.35, anisotropy=0.6,
+                                      anisotropy_rotation=0.3))
+    sp, sn, suv, stg = uv_sphere_textured(0.5, segments, rings, center=(-0.7, 0.5, 0.0))
+    s.add_tris(sp, metal, normals=sn, uv0=suv, tangents=stg)
+    s.set_material_textures(metal, normal=t_bump, anisotropy=t_aniso, normal_scale=0.8)
+    # Clearcoat / sheen sphere with extension textures (uv on TEXCOORD_1).
+    coat = s.add_material(A.material((0.3, 0.5, 0.8, 1.0), metallic=0.0, roughness=0.6, clearcoat=1.0,
+                                     clearcoat_roughness=0.2, sheen_color=(0.6, 0.6, 0.6), sheen_roughness=0.5))
+    cp, cn, cuv, ctg = uv_sphere_textured(0.4, segments, rings, center=(0.6, 0.4, 0.3))
+    s.add_tris(cp, coat, normals=cn, uv1=cuv, tangents=ctg)
+    s.set_material_textures(coat, clearcoat=dict(texture=t_cc, uv_set=1), clearcoat_normal=dict(texture=t_bump, uv_set=1),
+                            sheen_color=dict(texture=t_cc, uv_set=1), sheen_roughness=dict(texture=t_cc, uv_set=1),
+                            clearcoat_normal_scale=0.5)
+    # Textured emitter (double-sided).
+    em = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(6.0, 6.0, 6.0), double_sided=True))
+    t, nr, uv = quad_uv([[-0.5, 2.2, -0.5], [0.5, 2.2, -0.5], [0.5, 2.2, 0.5], [-0.5, 2.2, 0.5]],
+                        uv=((0, 0), (2, 0), (2, 1), (0, 1)))
+    s.add_tris(t, em, normals=nr, uv0=uv)
+    s.set_material_textures(em, emissive=t_grad)
+    # Alpha-masked disc quad and a vertex-colour BLEND quad.
+    mask = A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.7, double_sided=True)
+    mask.alpha_mode = A.OXR_ALPHA_MASK
+    mask.alpha_cutoff = 0.5
+    mk = s.add_material(mask)
+    t, nr, uv = quad_uv([[1.2, 0.05, -1.0], [1.2, 0.05, 0.2], [1.2, 1.25, 0.2], [1.2, 1.25, -1.0]])
+    s.add_tris(t, mk, normals=nr, uv0=uv)
+    s.set_material_textures(mk, base_color=t_disc)
+    blend = A.material((0.9, 0.9, 0.9, 1.0), metallic=0.0, roughness=0.5, double_sided=True)
+    blend.alpha_mode = A.OXR_ALPHA_BLEND
+    bl = s.add_material(blend)
+    t, nr = quad([[-1.6, 0.05, 0.6], [-0.4, 0.05, 0.6], [-0.4, 1.0, 0.6], [-1.6, 1.0, 0.6]])
+    cols = np.tile(np.array([1.0, 0.6, 0.6, 0.9, 0.6, 1.0, 0.6, 0.2, 0.6, 0.6, 1.0, 0.5], np.float32), (2, 1))
+    s.add_tris(t, bl, normals=nr, colors=cols)
+    # Vertex-coloured box.
+    box = s.add_material(lambert((1.0, 1.0, 1.0)))
+    bt, bn = cuboid((0.2, 0.0, -1.4), (0.8, 0.5, -0.8))
+    bc = np.tile(np.array([0.9, 0.2, 0.2, 1.0, 0.2, 0.9, 0.2, 1.0, 0.2, 0.2, 0.9, 1.0], np.float32), (len(bt), 1))
+    s.add_tris(bt, box, colors=bc)
+    s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=6.0, position=(1.5, 2.5, 1.5), direction=(0, -1, 0)))
+    s.camera = CameraSpec((0.0, 1.6, 3.4), (0.0, 0.5, -0.2), yfov_deg=45.0)
+    s.ambient = 0.05
     return s
 
 

@@ -202,7 +202,7 @@ def test_textured_emitter_power_uses_the_mean_texel():
     np.testing.assert_allclose(pmf, [1 / 6, 1 / 6, 1 / 3, 1 / 3], rtol=1e-6)
     # Emission itself samples the texture at level 0: the left half lit.
     e = o.material_eval([2, 2], [[0.9, 0.05, 0.05], [0.05, 0.9, 0.05]])
-    assert e[0, 38] == pytest.approx(1.0) and e[1, 38] == pytest.approx(0.0, abs=1e-7)
+    assert e[0, 39] == pytest.approx(1.0) and e[1, 39] == pytest.approx(0.0, abs=1e-7)
 
 
 def masked_scene():
@@ -238,7 +238,7 @@ def test_alpha_mask_texture():
     sc = masked_scene()
     o = O.OracleScene(sc)
     r = o.material_eval([0, 0], [[0.9, 0.05, 0.05], [0.05, 0.05, 0.9]])
-    assert r[0, 41] == 0.0 and r[1, 41] == 1.0   # uv (0.1, 0.05) left, (0.9, ...) right
+    assert r[0, 42] == 0.0 and r[1, 42] == 1.0   # uv (0.1, 0.05) left, (0.9, ...) right
     W = 16
     img = o.render(W, W, 0, 4, O.options(W, W, max_bounces=1))
     m = img[..., 0] / img[..., 3]
