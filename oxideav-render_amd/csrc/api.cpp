@@ -840,6 +840,7 @@ struct Engine {
   size_t spill_alloc = 0;
   uint32_t spill_cap = 4;
   int grid_closest = 0, grid_any = 0;
+  int grid_closest_alpha = 0, grid_any_alpha = 0;  // the textured-alpha variants
   uint32_t path_budget = 1u << 24;
   uint32_t max_budget = 1u << 24;
   int shade_waves = 1;
@@ -885,6 +886,8 @@ struct Engine {
     num_cu = prop.multiProcessorCount;
     grid_closest = num_cu * launch::trace_blocks_per_cu(false);
     grid_any = num_cu * launch::trace_blocks_per_cu(true);
+    grid_closest_alpha = num_cu * launch::trace_blocks_per_cu(false, true);
+    grid_any_alpha = num_cu * launch::trace_blocks_per_cu(true, true);
     if (const char* b = std::getenv("OXR_PATH_BUDGET")) {
       long v = std::atol(b);
       if (v >= 4096 && v <= (1l << 26)) path_budget = (uint32_t)v;
@@ -1040,7 +1043,8 @@ struct Engine {
     launch::prep(stream, d_ctrl, cur, k > 0 ? 1u : 0u);
     hipEvent_t e1{};
     span_begin(e1);
-    launch::closest(stream, grid_closest, sc_any, d_jobs, pb.pjob, true, multi, pb.qray[cur], pb.qpid[cur], pb.meta, 0,
+    launch::closest(stream, alpha_sc ? grid_closest_alpha : grid_closest, sc_any, d_jobs, pb.pjob, true, multi,
+                    pb.qray[cur], pb.qpid[cur], pb.meta, 0,
                     &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
                     xcd_segments, alpha_sc);
     span_end(0, e1);
@@ -1058,7 +1062,8 @@ struct Engine {
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);
-    launch::anyhit(stream, grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray, pb.sidx, pb.meta, nslots, 0,
+    launch::anyhit(stream, alpha_sc ? grid_any_alpha : grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray,
+                   pb.sidx, pb.meta, nslots, 0,
                    &d_ctrl->scount, &d_ctrl->work[8], pb.vis, d_spill, spill_cap, &d_ctrl->error,
                    xcd_segments, alpha_sc);
     span_end(1, e3);

@@ -1494,27 +1494,27 @@ hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, con
   return hipMemcpyToSymbol(HIP_SYMBOL(c_srgb_th), srgb255, sizeof(float) * 255);
 }
 
-int trace_blocks_per_cu(bool any) {
-  // The persistent grid must fit on the device for every variant.
-  int nb[4] = {0, 0, 0, 0};
-  hipError_t e[4];
-  if (any) {
+int trace_blocks_per_cu(bool any, bool alpha) {
+  // The persistent grid must fit on the device for the single- and the
+  // mixed-scene variant of the kernel (the textured-alpha variants hold
+  // more registers and get their own grid).
+  int nb[2] = {0, 0};
+  hipError_t e[2];
+  if (any && alpha) {
+    e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], k_anyhit<true, false, true>, TRACE_TB, 0);
+    e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], k_anyhit<true, true, true>, TRACE_TB, 0);
+  } else if (any) {
     e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], k_anyhit<true, false, false>, TRACE_TB, 0);
     e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], k_anyhit<true, true, false>, TRACE_TB, 0);
-    e[2] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[2], k_anyhit<true, false, true>, TRACE_TB, 0);
-    e[3] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[3], k_anyhit<true, true, true>, TRACE_TB, 0);
+  } else if (alpha) {
+    e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], k_closest<true, false, true>, TRACE_TB, 0);
+    e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], k_closest<true, true, true>, TRACE_TB, 0);
   } else {
     e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], k_closest<true, false, false>, TRACE_TB, 0);
     e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], k_closest<true, true, false>, TRACE_TB, 0);
-    e[2] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[2], k_closest<true, false, true>, TRACE_TB, 0);
-    e[3] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[3], k_closest<true, true, true>, TRACE_TB, 0);
   }
-  int m = 1 << 30;
-  for (int k = 0; k < 4; ++k) {
-    if (e[k] != hipSuccess || nb[k] <= 0) return 4;
-    m = std::min(m, nb[k]);
-  }
-  return m;
+  if (e[0] != hipSuccess || e[1] != hipSuccess || nb[0] <= 0 || nb[1] <= 0) return 4;
+  return std::min(nb[0], nb[1]);
 }
 
 int trace_block_threads() { return TRACE_TB; }

@@ -32,7 +32,7 @@ struct PathBuffers {
 
 namespace launch {
 hipError_t upload_constants(const uint32_t* sobol128, const float* sheen512, const float* srgb255);
-int trace_blocks_per_cu(bool any);
+int trace_blocks_per_cu(bool any, bool alpha = false);
 int trace_block_threads();
 int stack_lds_entries();
 // work: nseg counters (closest: Ctrl::work, any-hit: Ctrl::work + 8);
