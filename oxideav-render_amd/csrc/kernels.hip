@@ -398,7 +398,11 @@ __global__ __launch_bounds__(TRACE_TB) void k_anyhit(DevScene sc, const DevJob* 
       }
       if (blend) {
         uint4 m = meta[pid];
+        // The shading kernel has already advanced the bounce of a path that
+        // continues (it sets PF_DONE on the others): the shadow rays belong
+        // to the vertex before.
         uint32_t k = (m.z & 0xffffu);
+        if (!((m.z >> 24) & PF_DONE)) k -= 1u;
         path_seed = hash_mix(m.w, m.y);
         ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (j & 0x7fffu)));
       }

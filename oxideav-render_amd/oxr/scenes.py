@@ -611,7 +611,7 @@ def _srgb_byte(x):
     return np.round(s * 255.0).astype(np.uint8)
 
 
-def textured_showcase(segments=48, rings=24):
+def textured_showcase(segments=48, rings=24, parts=None):
     """Every texture path of the tracer in one small scene:
     - a 4 m floor with an RGBA8 checker base-colour texture (0.8 / 0.2 in
       1 m cells through a KHR_texture_transform scale of 4) and a
@@ -622,8 +622,11 @@ def textured_showcase(segments=48, rings=24):
     - a double-sided quad emitter with an sRGB emissive gradient;
     - an alpha-MASK quad (a disc texture, cutoff 0.5) and a BLEND quad
       whose alpha comes from vertex colours;
-    - a vertex-coloured box, a point light."""
+    - a vertex-coloured box, a point light.
+    parts: the subset of {"floor", "metal", "coat", "emitter", "mask",
+    "blend", "box"} to build (None: all)."""
     s = Scene("textured_showcase")
+    want = (lambda k: True) if parts is None else (lambda k: k in parts)
     # Textures.
     n = 64
     yy, xx = np.mgrid[0:n, 0:n]
@@ -670,19 +673,22 @@ def textured_showcase(segments=48, rings=24):
     # Floor.
     floor = s.add_material(A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.8))
     t, nr, uv = quad_uv([[-2, 0, 2], [2, 0, 2], [2, 0, -2], [-2, 0, -2]])
-    s.add_tris(t, floor, normals=nr, uv0=uv)
+    if want("floor"):
+        s.add_tris(t, floor, normals=nr, uv0=uv)
     s.set_material_textures(floor, base_color=dict(texture=t_check, scale=(4.0, 4.0)), metallic_roughness=t_mr)
     # Normal-mapped anisotropic sphere.
     metal = s.add_material(A.material((0.9, 0.7, 0.4, 1.0), metallic=1.0, roughness=0.35, anisotropy=0.6,
                                       anisotropy_rotation=0.3))
     sp, sn, suv, stg = uv_sphere_textured(0.5, segments, rings, center=(-0.7, 0.5, 0.0))
-    s.add_tris(sp, metal, normals=sn, uv0=suv, tangents=stg)
+    if want("metal"):
+        s.add_tris(sp, metal, normals=sn, uv0=suv, tangents=stg)
     s.set_material_textures(metal, normal=t_bump, anisotropy=t_aniso, normal_scale=0.8)
     # Clearcoat / sheen sphere with extension textures (uv on TEXCOORD_1).
     coat = s.add_material(A.material((0.3, 0.5, 0.8, 1.0), metallic=0.0, roughness=0.6, clearcoat=1.0,
                                      clearcoat_roughness=0.2, sheen_color=(0.6, 0.6, 0.6), sheen_roughness=0.5))
     cp, cn, cuv, ctg = uv_sphere_textured(0.4, segments, rings, center=(0.6, 0.4, 0.3))
-    s.add_tris(cp, coat, normals=cn, uv1=cuv, tangents=ctg)
+    if want("coat"):
+        s.add_tris(cp, coat, normals=cn, uv1=cuv, tangents=ctg)
     s.set_material_textures(coat, clearcoat=dict(texture=t_cc, uv_set=1), clearcoat_normal=dict(texture=t_bump, uv_set=1),
                             sheen_color=dict(texture=t_cc, uv_set=1), sheen_roughness=dict(texture=t_cc, uv_set=1),
                             clearcoat_normal_scale=0.5)
@@ -690,7 +696,8 @@ def textured_showcase(segments=48, rings=24):
     em = s.add_material(lambert((0.0, 0.0, 0.0), emissive=(6.0, 6.0, 6.0), double_sided=True))
     t, nr, uv = quad_uv([[-0.5, 2.2, -0.5], [0.5, 2.2, -0.5], [0.5, 2.2, 0.5], [-0.5, 2.2, 0.5]],
                         uv=((0, 0), (2, 0), (2, 1), (0, 1)))
-    s.add_tris(t, em, normals=nr, uv0=uv)
+    if want("emitter"):
+        s.add_tris(t, em, normals=nr, uv0=uv)
     s.set_material_textures(em, emissive=t_grad)
     # Alpha-masked disc quad and a vertex-colour BLEND quad.
     mask = A.material((1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.7, double_sided=True)
@@ -698,19 +705,22 @@ def textured_showcase(segments=48, rings=24):
     mask.alpha_cutoff = 0.5
     mk = s.add_material(mask)
     t, nr, uv = quad_uv([[1.2, 0.05, -1.0], [1.2, 0.05, 0.2], [1.2, 1.25, 0.2], [1.2, 1.25, -1.0]])
-    s.add_tris(t, mk, normals=nr, uv0=uv)
+    if want("mask"):
+        s.add_tris(t, mk, normals=nr, uv0=uv)
     s.set_material_textures(mk, base_color=t_disc)
     blend = A.material((0.9, 0.9, 0.9, 1.0), metallic=0.0, roughness=0.5, double_sided=True)
     blend.alpha_mode = A.OXR_ALPHA_BLEND
     bl = s.add_material(blend)
     t, nr = quad([[-1.6, 0.05, 0.6], [-0.4, 0.05, 0.6], [-0.4, 1.0, 0.6], [-1.6, 1.0, 0.6]])
     cols = np.tile(np.array([1.0, 0.6, 0.6, 0.9, 0.6, 1.0, 0.6, 0.2, 0.6, 0.6, 1.0, 0.5], np.float32), (2, 1))
-    s.add_tris(t, bl, normals=nr, colors=cols)
+    if want("blend"):
+        s.add_tris(t, bl, normals=nr, colors=cols)
     # Vertex-coloured box.
     box = s.add_material(lambert((1.0, 1.0, 1.0)))
     bt, bn = cuboid((0.2, 0.0, -1.4), (0.8, 0.5, -0.8))
     bc = np.tile(np.array([0.9, 0.2, 0.2, 1.0, 0.2, 0.9, 0.2, 1.0, 0.2, 0.2, 0.9, 1.0], np.float32), (len(bt), 1))
-    s.add_tris(bt, box, colors=bc)
+    if want("box"):
+        s.add_tris(bt, box, colors=bc)
     s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=6.0, position=(1.5, 2.5, 1.5), direction=(0, -1, 0)))
     s.camera = CameraSpec((0.0, 1.6, 3.4), (0.0, 0.5, -0.2), yfov_deg=45.0)
     s.ambient = 0.05

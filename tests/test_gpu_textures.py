@@ -100,3 +100,44 @@ def test_textured_full_bounce_within_3_sigma():
     var = (np.maximum(sg / n - mg ** 2, 0) + np.maximum(sc / n - mc ** 2, 0)) / n
     z = np.abs(mg - mc) / np.sqrt(var + 1e-20)
     assert (z > 3.0).mean() < 0.01
+
+
+def test_blend_shadow_coins_match_oracle():
+    """Stochastic BLEND (Integrator::accept, src/pathtrace.rs:1924-1934) on
+    shadow rays: the coin's ray id is the vertex's bounce (k << 16 with
+    0x8000 | j for punctual light j, 0xffff for the area set), so the
+    device's per-pixel direct lighting equals the oracle's. A constant
+    alpha 0.5 quad between a point light, a small emitter and the floor."""
+    import oxr
+    from oxr import _abi as A
+    s = scenes.Scene("blend_shadow")
+    fl = s.add_material(scenes.lambert((0.7, 0.7, 0.7)))
+    t, nr = scenes.quad([[-2, 0, 2], [2, 0, 2], [2, 0, -2], [-2, 0, -2]])
+    s.add_tris(t, fl, normals=nr)
+    bm = A.material((0.9, 0.5, 0.3, 0.5), metallic=0.0, roughness=0.6, double_sided=True)
+    bm.alpha_mode = A.OXR_ALPHA_BLEND
+    bl = s.add_material(bm)
+    t, nr = scenes.quad([[-0.8, 0.7, -0.8], [0.8, 0.7, -0.8], [0.8, 0.7, 0.8], [-0.8, 0.7, 0.8]])
+    s.add_tris(t, bl, normals=nr)
+    em = s.add_material(scenes.lambert((0.0, 0.0, 0.0), emissive=(8.0, 8.0, 8.0)))
+    t, nr = scenes.quad([[-0.3, 1.6, 0.3], [0.3, 1.6, 0.3], [0.3, 1.6, -0.3], [-0.3, 1.6, -0.3]])
+    s.add_tris(t[:, [0, 1, 2, 6, 7, 8, 3, 4, 5]], em)  # facing down
+    s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=3.0, position=(0.4, 2.0, 0.2), direction=(0, -1, 0)))
+    s.camera = scenes.CameraSpec((0.0, 2.5, 2.5), (0.0, 0.2, 0.0), yfov_deg=50.0)
+    s.ambient = 0.0
+    W, spp = 48, 16
+    pt = oxr.PathTracer(0)
+    o = oxr.RenderOptions(width=W, height=W, ambient=0.0)
+    o.path_trace.samples_per_pixel = spp
+    o.path_trace.max_bounces = 1
+    pt.sync(s, o)
+    pt.refine(spp)
+    ag = pt.accum()
+    pt.close()
+    ac = O.OracleScene(s).render(W, W, 0, spp, O.options(W, W, max_bounces=1))
+    assert np.array_equal(ag[..., 3], ac[..., 3])
+    cov = ac[..., 3] > 0
+    mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
+    mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
+    rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
+    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
