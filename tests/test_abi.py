@@ -119,3 +119,28 @@ int main(void) {
         out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert out == [C.sizeof(A.GuideConfig), C.sizeof(A.GuideStats), C.sizeof(A.TuneResult),
                    A.GuideConfig.bounces.offset, A.GuideStats.infer_ms.offset]
+
+
+def test_texture_struct_layouts_match_header():
+    from oxr import _abi as A
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "oxr/pathtrace.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(oxr_scene_desc), sizeof(oxr_texture), sizeof(oxr_texture_binding),
+         sizeof(oxr_material_textures), offsetof(oxr_scene_desc, textures), offsetof(oxr_scene_desc, colors),
+         offsetof(oxr_material_textures, normal_scale));
+  return 0;
+}
+'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "t.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(td, "t")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out == [C.sizeof(A.SceneDesc), C.sizeof(A.Texture), C.sizeof(A.TextureBinding),
+                   C.sizeof(A.MaterialTextures), A.SceneDesc.textures.offset, A.SceneDesc.colors.offset,
+                   A.MaterialTextures.normal_scale.offset]
