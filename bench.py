@@ -144,10 +144,10 @@ def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sust
     return out
 
 
-GUIDE_BENCH_CONFIG = {"max_steps": 96}
+GUIDE_BENCH_CONFIG = {"max_steps": 160}
 
 
-def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=48, max_s=30.0, guide_cfg=None):
+def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=80, max_s=30.0, guide_cfg=None):
     """Time-to-MSE on the glossy-caustic scene, plain vs guided (online
     training included in the guided time).
 
@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--guide-only", action="store_true")
     ap.add_argument("--guide-width", type=int, default=1024)
-    ap.add_argument("--guide-passes", type=int, default=48)
+    ap.add_argument("--guide-passes", type=int, default=80)
     ap.add_argument("--guide-config", default=None, help="JSON overrides of the guide configuration")
     ap.add_argument("--guide-max-s", type=float, default=30.0)
     a = ap.parse_args()
