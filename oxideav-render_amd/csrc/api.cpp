@@ -1058,7 +1058,7 @@ struct Engine {
     hipEvent_t e2{};
     span_begin(e2);
     launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr,
-                  tex_variant);
+                  tex_variant, multi_jobs);
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);
@@ -1087,7 +1087,7 @@ struct Engine {
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
     key.alpha_sc = alpha_sc;
-    key.tex = tex_variant ? 1u : 0u;
+    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u);
     if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
@@ -1129,12 +1129,15 @@ struct Engine {
   // scene's device copy through alpha_sc).
   bool tex_variant = false;
   const DevScene* alpha_sc = nullptr;
+  bool multi_jobs = true;  // the batch holds paths of more than one job
 
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
                  const std::vector<Reduce>& red, const DevScene& sc_any, bool multi, int& rc,
                  const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr) {
     tex_variant = tex;
     alpha_sc = asc;
+    multi_jobs = false;  // every segment of job slot 0: the single-job shading variant
+    for (uint32_t q = 0; q < segs.n; ++q) multi_jobs = multi_jobs || segs.s[q].job != 0u;
     const int grid_shade = (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
     hipEvent_t ea{};
     span_begin(ea);

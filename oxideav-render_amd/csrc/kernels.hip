@@ -695,7 +695,10 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // Shading: one path vertex of Integrator::sample (src/pathtrace.rs:
 // 2027-2322), with shadow rays deferred to k_anyhit.
 // ---------------------------------------------------------------------
-template <int WAVES, bool GUIDE, bool TEX>
+// MULTI: the batch mixes jobs (each path's job through pjob); else every
+// path belongs to job 0, whose scene and frame the wave reads with uniform
+// (scalar) loads instead of a per-lane dependent lookup.
+template <int WAVES, bool GUIDE, bool TEX, bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
                                              const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
@@ -719,7 +722,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     bool valid = i < count;
     uint32_t pid = valid ? qpid[i] : 0u;
     bool alive = valid;
-    const DevJob& job = jobs[valid ? pjob[pid] : 0u];
+    const DevJob& job = jobs[MULTI ? (valid ? pjob[pid] : 0u) : 0u];
     const DevScene& sc = job.sc;
     const DevFrame& fr = job.fr;
     uint4 meta = valid ? meta_s[pid] : make_uint4(0, 0, 0, 0);
@@ -1594,26 +1597,31 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 }
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex) {
+           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
-#define OXR_SHADE(W, G, T)                                                                                       \
-  hipLaunchKernelGGL((k_shade<W, G, T>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],          \
+#define OXR_SHADE(W, G, T, M)                                                                                    \
+  hipLaunchKernelGGL((k_shade<W, G, T, M>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],       \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
                      pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
-  // Textured scenes take the TEX variant at one wave per SIMD; the
-  // occupancy variants are for untextured shading.
-  if (guide && tex)
-    OXR_SHADE(1, true, true);
+  // Mixed batches (the server) and textured or guided scenes run at one
+  // wave per SIMD; the occupancy variants are for plain single-scene
+  // shading.
+  if (multi && tex)
+    OXR_SHADE(1, false, true, true);
+  else if (multi)
+    OXR_SHADE(1, false, false, true);
+  else if (guide && tex)
+    OXR_SHADE(1, true, true, false);
   else if (guide)
-    OXR_SHADE(1, true, false);
+    OXR_SHADE(1, true, false, false);
   else if (tex)
-    OXR_SHADE(1, false, true);
+    OXR_SHADE(1, false, true, false);
   else if (waves >= 3)
-    OXR_SHADE(3, false, false);
+    OXR_SHADE(3, false, false, false);
   else if (waves == 2)
-    OXR_SHADE(2, false, false);
+    OXR_SHADE(2, false, false, false);
   else
-    OXR_SHADE(1, false, false);
+    OXR_SHADE(1, false, false, false);
 #undef OXR_SHADE
 }
 

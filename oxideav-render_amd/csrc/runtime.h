@@ -62,8 +62,9 @@ void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t
                    const float* width, const float* dir, const uint8_t* flip, float* out);
 // tex: the TEX variant (textures, vertex colours or tangents in a scene
 // of the batch).
+// multi: the batch mixes jobs (the server's rounds).
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex = false);
+           uint32_t cur, const DevGuideUse* guide, bool tex = false, bool multi = true);
 // Training samples (2 float4 each) of the guide-recorded paths of a batch,
 // appended at *count (entries past cap are dropped).
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
