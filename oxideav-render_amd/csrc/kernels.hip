@@ -717,10 +717,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   // Uniform trip count so every lane reaches the wave-level appends.
   const uint32_t rounds = (count + stride - 1) / stride;
+  // Software pipelining: the next round's queue entry (path id, hit, ray
+  // direction) is in flight while this round shades, so its dependent
+  // loads (path state, triangle, material) start one memory latency
+  // earlier.
+  uint32_t pid_n = i0 < count ? qpid[i0] : 0u;
+  uint4 h_n = i0 < count ? hits[i0] : make_uint4(0, HIT_MISS, 0, 0);
+  float4 rd_n = i0 < count ? qray[2 * i0 + 1] : make_float4(0, 0, 1, 0);
   for (uint32_t r = 0; r < rounds; ++r) {
     uint32_t i = i0 + r * stride;
     bool valid = i < count;
-    uint32_t pid = valid ? qpid[i] : 0u;
+    uint32_t pid = valid ? pid_n : 0u;
+    const uint4 h_cur = h_n;
+    const float4 rd_cur = rd_n;
+    {
+      uint32_t in = i + stride;
+      if (r + 1 < rounds && in < count) {
+        pid_n = qpid[in];
+        h_n = hits[in];
+        rd_n = qray[2 * in + 1];
+      }
+    }
     bool alive = valid;
     const DevJob& job = jobs[MULTI ? (valid ? pjob[pid] : 0u) : 0u];
     const DevScene& sc = job.sc;
@@ -749,8 +766,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     if (GUIDE && gu.rec && valid && k >= 1 && k - 1 < (uint32_t)G_KREC && (pid & rec_mask) == 0u)
       gu.rec[((size_t)(pid >> gu.shift) * G_KREC + (k - 1)) * 4 + 3] =
           make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
-    uint4 h = valid ? hits[i] : make_uint4(0, HIT_MISS, 0, 0);
-    float4 rd = valid ? qray[2 * i + 1] : make_float4(0, 0, 1, 0);
+    uint4 h = valid ? h_cur : make_uint4(0, HIT_MISS, 0, 0);
+    float4 rd = valid ? rd_cur : make_float4(0, 0, 1, 0);
     V3 d = f4xyz(rd);
     bool miss = h.y == HIT_MISS;
     float t = __uint_as_float(h.x);
