@@ -1,0 +1,613 @@
+"""The prepare layer for the HIP tracer's Python side (oxideav-render
+src/prepare.rs): a glTF-shaped scene graph (Scene3D) flattened into the
+world-space triangle soup, material table, textures and lights that
+`oxr_pt_upload_scene` takes, plus the scene's camera instances.
+
+It restates PreparedScene::build for static scenes (src/prepare.rs:
+1098-1237): preorder node traversal from the roots (`preorder_nodes`,
+:1282-1296), world transforms, `flatten_primitive` (:1352-1517: index
+streams, strips and fans, garbage indices dropped, winding flipped under a
+mirroring world transform, per-corner normals or flat ones, UV sets,
+tangents, COLOR_0), `face_tangents` (:1522-1575) for normal-mapped
+primitives without tangents, `PreparedMaterial::from_material` /
+`fallback` (:599-672), `PreparedLight::from_scene` / `from_light_spec`
+(:848-919) and the fallback-light rule. Line and point primitives are
+flattened (they count as items) but trace nothing, as in TraceScene.
+
+The scene-graph types mirror oxideav-mesh3d's Scene3D, which is a
+crates.io dependency not vendored with the reference: node transforms
+compose as glTF's T * R * S with (x, y, z, w) quaternions, primitives
+transform with the world matrix (positions), its inverse transpose
+(normals) and its upper 3x3 (tangents), and strips and fans triangulate
+as the glTF specification defines. Animation, skinning, morph targets and
+material variants (which that crate evaluates) are not restated.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _abi as A
+from . import scenes as S
+
+F32 = np.float32
+
+TOPOLOGIES = ("triangles", "triangle_strip", "triangle_fan", "lines", "line_strip", "line_loop", "points")
+
+
+@dataclass
+class TextureRef:
+    """A material's texture reference: texture index, TEXCOORD set and an
+    optional KHR_texture_transform (offset, rotation, scale, and a
+    texCoord override)."""
+    texture: int
+    tex_coord: int = 0
+    offset: Optional[tuple] = None
+    rotation: float = 0.0
+    scale: Optional[tuple] = None
+    transform_tex_coord: Optional[int] = None
+
+    def effective_uv_set(self):
+        return self.transform_tex_coord if self.transform_tex_coord is not None else self.tex_coord
+
+    def has_transform(self):
+        off = self.offset or (0.0, 0.0)
+        sc = self.scale or (1.0, 1.0)
+        ident = off == (0.0, 0.0) and self.rotation == 0.0 and tuple(sc) == (1.0, 1.0)
+        finite = all(math.isfinite(v) for v in (*off, self.rotation, *sc))
+        return finite and not ident
+
+
+@dataclass
+class Texture:
+    """A decoded image (H, W, 4) uint8 or float32, row 0 at v = 0, with its
+    glTF sampler."""
+    pixels: np.ndarray
+    mag_filter: int = A.FILTER_LINEAR
+    min_filter: int = A.MIN_LINEAR_MIP_LINEAR
+    wrap_s: int = A.WRAP_REPEAT
+    wrap_t: int = A.WRAP_REPEAT
+
+
+@dataclass
+class Material:
+    """A glTF metallic-roughness material. `params` holds the remaining
+    oxr_material fields (KHR extension scalars: specular, transmission,
+    clearcoat, ...); `textures` maps oxr_material_textures slot names to
+    TextureRefs."""
+    base_color: tuple = (1.0, 1.0, 1.0, 1.0)
+    metallic: float = 1.0
+    roughness: float = 1.0
+    emissive_factor: tuple = (0.0, 0.0, 0.0)
+    emissive_strength: Optional[float] = None
+    alpha_mode: int = A.OXR_ALPHA_OPAQUE
+    alpha_cutoff: float = 0.5
+    double_sided: bool = False
+    unlit: bool = False
+    ior: Optional[float] = None
+    normal_scale: float = 1.0
+    params: dict = field(default_factory=dict)
+    textures: dict = field(default_factory=dict)
+
+
+@dataclass
+class Primitive:
+    positions: np.ndarray
+    topology: str = "triangles"
+    normals: Optional[np.ndarray] = None
+    tangents: Optional[np.ndarray] = None
+    uvs: List[np.ndarray] = field(default_factory=list)
+    colors: List[np.ndarray] = field(default_factory=list)
+    indices: Optional[np.ndarray] = None
+    material: Optional[int] = None
+
+
+@dataclass
+class Mesh:
+    primitives: List[Primitive]
+
+
+@dataclass
+class Light:
+    """KHR_lights_punctual: kind 'directional', 'point' or 'spot'."""
+    kind: str
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+    range: Optional[float] = None
+    inner_cone_angle: float = 0.0
+    outer_cone_angle: float = math.pi / 4.0
+
+
+@dataclass
+class Camera:
+    """A glTF camera: perspective (yfov, aspect) or orthographic (xmag,
+    ymag)."""
+    perspective: bool = True
+    yfov: float = 0.8
+    aspect: Optional[float] = None
+    znear: float = 0.1
+    xmag: float = 1.0
+    ymag: float = 1.0
+
+
+@dataclass
+class Node:
+    mesh: Optional[int] = None
+    light: Optional[int] = None
+    camera: Optional[int] = None
+    children: List[int] = field(default_factory=list)
+    matrix: Optional[np.ndarray] = None        # 4x4 row-major (column vectors)
+    translation: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)     # quaternion x, y, z, w
+    scale: tuple = (1.0, 1.0, 1.0)
+
+
+@dataclass
+class Scene3D:
+    nodes: List[Node] = field(default_factory=list)
+    roots: List[int] = field(default_factory=list)
+    meshes: List[Mesh] = field(default_factory=list)
+    materials: List[Material] = field(default_factory=list)
+    textures: List[Texture] = field(default_factory=list)
+    lights: List[Light] = field(default_factory=list)
+    cameras: List[Camera] = field(default_factory=list)
+
+
+@dataclass
+class LightSpec:
+    """RenderOptions' light (src/options.rs:219-262): the fallback
+    directional light."""
+    azimuth_deg: float = 45.0
+    elevation_deg: float = 45.0
+    intensity: float = 1.0
+
+
+@dataclass
+class DrawItem:
+    node: int
+    mesh: int
+    primitive: int
+    topology: str               # 'triangles', 'lines' or 'points'
+    world: np.ndarray
+    positions: np.ndarray       # corners, (n, 3)
+    normals: np.ndarray
+    has_vertex_normals: bool
+    tangents: np.ndarray
+    uvs: List[np.ndarray]
+    colors: np.ndarray
+    material: int
+    bounds_min: np.ndarray
+    bounds_max: np.ndarray
+
+    def primitive_count(self):
+        per = {"triangles": 3, "lines": 2, "points": 1}[self.topology]
+        return len(self.positions) // per
+
+
+@dataclass
+class PreparedScene:
+    items: List[DrawItem]
+    materials: list             # A.Material, the fallback last
+    material_textures: dict     # material index -> slot -> TextureRef
+    textures: List[Texture]
+    lights: list                # A.Light
+    cameras: list               # (node, camera index, Camera, world)
+    scene_lights: bool
+    bounds: Optional[tuple]
+
+    def default_material(self):
+        return len(self.materials) - 1
+
+
+# ---------------------------------------------------------------------------
+# Transforms.
+# ---------------------------------------------------------------------------
+def _quat_matrix(q):
+    x, y, z, w = (float(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]], np.float64)
+
+
+def local_matrix(node):
+    if node.matrix is not None:
+        return np.asarray(node.matrix, np.float64).reshape(4, 4)
+    m = np.eye(4)
+    m[:3, :3] = _quat_matrix(node.rotation) * np.asarray(node.scale, np.float64)[None, :]
+    m[:3, 3] = node.translation
+    return m
+
+
+def preorder_nodes(scene):
+    """src/prepare.rs:1282-1296."""
+    visited = [False] * len(scene.nodes)
+    out, stack = [], list(reversed(scene.roots))
+    while stack:
+        i = stack.pop()
+        if i < 0 or i >= len(visited) or visited[i]:
+            continue
+        visited[i] = True
+        out.append(i)
+        stack.extend(reversed(scene.nodes[i].children))
+    return out
+
+
+def world_transforms(scene):
+    """World matrix of every node reachable from the roots (first visit)."""
+    worlds = [None] * len(scene.nodes)
+    stack = [(r, np.eye(4)) for r in reversed(scene.roots)]
+    while stack:
+        i, parent = stack.pop()
+        if i < 0 or i >= len(worlds) or worlds[i] is not None:
+            continue
+        worlds[i] = parent @ local_matrix(scene.nodes[i])
+        for c in reversed(scene.nodes[i].children):
+            stack.append((c, worlds[i]))
+    return [None if w is None else w.astype(F32) for w in worlds]
+
+
+def det3(m):
+    """src/prepare.rs:1314-1318."""
+    m = np.asarray(m, F32)
+    return F32(m[0, 0] * (m[1, 1] * m[2, 2] - m[1, 2] * m[2, 1]) - m[0, 1] * (m[1, 0] * m[2, 2] - m[1, 2] * m[2, 0])
+               + m[0, 2] * (m[1, 0] * m[2, 1] - m[1, 1] * m[2, 0]))
+
+
+def normal_matrix(m):
+    """src/prepare.rs:1320-1346: the inverse transpose (cofactors / det)."""
+    m = np.asarray(m, F32)
+    d = det3(m)
+    if not math.isfinite(d) or abs(d) <= 1.0e-20:
+        return np.eye(3, dtype=F32)
+    inv = F32(1.0) / d
+
+    def c(r0, c0, r1, c1):
+        return m[r0, c0] * m[r1, c1] - m[r0, c1] * m[r1, c0]
+    return np.array([[c(1, 1, 2, 2) * inv, -c(1, 0, 2, 2) * inv, c(1, 0, 2, 1) * inv],
+                     [-c(0, 1, 2, 2) * inv, c(0, 0, 2, 2) * inv, -c(0, 0, 2, 1) * inv],
+                     [c(0, 1, 1, 2) * inv, -c(0, 0, 1, 2) * inv, c(0, 0, 1, 1) * inv]], F32)
+
+
+def _normalise(v):
+    v = np.asarray(v, F32)
+    n = np.sqrt((v * v).sum(-1, keepdims=True))
+    out = np.where(n < F32(1.1920929e-7), F32(0.0), v / np.where(n == 0, F32(1.0), n))
+    return out.astype(F32)
+
+
+def transformed(prim, world):
+    """The primitive in world space: positions by the world matrix, normals
+    by its inverse transpose, tangent directions by its upper 3x3."""
+    w = np.asarray(world, F32)
+    p = np.asarray(prim.positions, F32).reshape(-1, 3)
+    pos = (p @ w[:3, :3].T + w[:3, 3]).astype(F32)
+    nrm = None
+    if prim.normals is not None:
+        nrm = (np.asarray(prim.normals, F32).reshape(-1, 3) @ normal_matrix(w).T).astype(F32)
+    tan = None
+    if prim.tangents is not None:
+        t = np.asarray(prim.tangents, F32).reshape(-1, 4)
+        tan = np.concatenate([t[:, :3] @ w[:3, :3].T, t[:, 3:4]], 1).astype(F32)
+    return Primitive(positions=pos, topology=prim.topology, normals=nrm, tangents=tan, uvs=prim.uvs,
+                     colors=prim.colors, indices=prim.indices, material=prim.material)
+
+
+def triangle_indices(prim, seq):
+    """Triangles of a triangle, strip or fan index stream (glTF 2.0
+    section 3.7.2.1)."""
+    n = len(seq)
+    if prim.topology == "triangles":
+        return [tuple(seq[3 * i:3 * i + 3]) for i in range(n // 3)]
+    if prim.topology == "triangle_strip":
+        return [(seq[i], seq[i + 1 + i % 2], seq[i + 2 - i % 2]) for i in range(max(n - 2, 0))]
+    if prim.topology == "triangle_fan":
+        return [(seq[i + 1], seq[i + 2], seq[0]) for i in range(max(n - 2, 0))]
+    return []
+
+
+def face_tangents(pos, nrm, uv):
+    """src/prepare.rs:1522-1575 (Lengyel 2001, Gram-Schmidt per corner)."""
+    out = np.zeros((len(pos), 4), F32)
+    for t in range(len(pos) // 3):
+        p0, p1, p2 = pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]
+        w0, w1, w2 = uv[3 * t], uv[3 * t + 1], uv[3 * t + 2]
+        e1, e2 = p1 - p0, p2 - p0
+        du1, dv1 = w1[0] - w0[0], w1[1] - w0[1]
+        du2, dv2 = w2[0] - w0[0], w2[1] - w0[1]
+        det = F32(du1 * dv2 - du2 * dv1)
+        if abs(det) > 1.0e-12:
+            r = F32(1.0) / det
+            tan = ((e1 * dv2 - e2 * dv1) * r).astype(F32)
+            bit = ((e2 * du1 - e1 * du2) * r).astype(F32)
+        else:
+            tan, bit = np.array([1, 0, 0], F32), np.array([0, 1, 0], F32)
+        for k in range(3):
+            n = nrm[3 * t + k] if 3 * t + k < len(nrm) else np.array([0, 0, 1], F32)
+            d = F32(np.dot(n, tan))
+            tt = _normalise(tan - n * d)
+            if np.dot(tt, tt) < 0.5:
+                a = np.array([1, 0, 0], F32) if abs(n[0]) < 0.9 else np.array([0, 1, 0], F32)
+                tt = _normalise(np.cross(a, n))
+            w = -1.0 if np.dot(np.cross(n, tt), bit) < 0.0 else 1.0
+            out[3 * t + k] = (*tt, w)
+    return out
+
+
+def flatten_primitive(prim, node, mesh, pi, world, material, needs_tangents, tangent_uv_set):
+    """src/prepare.rs:1352-1517. One departure: for a normal-mapped
+    primitive that has vertex normals but no tangents the reference first
+    asks oxideav-mesh3d's Primitive::compute_tangents (:1371-1377, not
+    vendored); here such a primitive takes the per-face tangents that the
+    reference uses when that call yields none (parity unpinned there)."""
+    pos_all = np.asarray(prim.positions, F32).reshape(-1, 3)
+    nv = len(pos_all)
+    if nv == 0:
+        return None
+    normals_src = None if prim.normals is None or len(prim.normals) != nv else np.asarray(prim.normals, F32)
+    tangents_src = None if prim.tangents is None or len(prim.tangents) != nv else np.asarray(prim.tangents, F32)
+    topo = ("triangles" if prim.topology in TOPOLOGIES[:3] else "lines" if prim.topology in TOPOLOGIES[3:6]
+            else "points")
+    seq = list(range(nv)) if prim.indices is None else [int(i) for i in np.asarray(prim.indices).ravel()]
+    flip = det3(world) < 0.0
+    finite = np.all(np.isfinite(pos_all), axis=1)
+
+    def ok(i):
+        return 0 <= i < nv and bool(finite[i])
+    corners = []
+    if topo == "triangles":
+        for a, b, c in triangle_indices(prim, seq):
+            if ok(a) and ok(b) and ok(c):
+                corners.extend((a, c, b) if flip else (a, b, c))
+    elif topo == "lines":
+        if prim.topology == "lines":
+            pairs = [(seq[2 * i], seq[2 * i + 1]) for i in range(len(seq) // 2)]
+        else:
+            pairs = [(seq[i], seq[i + 1]) for i in range(max(len(seq) - 1, 0))]
+            if prim.topology == "line_loop" and len(seq) >= 2:
+                pairs.append((seq[-1], seq[0]))
+        for a, b in pairs:
+            if ok(a) and ok(b):
+                corners.extend((a, b))
+    else:
+        corners = [i for i in seq if ok(i)]
+    if not corners:
+        return None
+    idx = np.asarray(corners, np.int64)
+    positions = pos_all[idx]
+    if normals_src is not None:
+        nrm = _normalise(normals_src[idx])
+        nrm = np.where(np.all(np.isfinite(nrm), axis=1, keepdims=True), nrm, np.array([0, 0, 1], F32)).astype(F32)
+        has_vn = True
+    elif topo == "triangles":
+        t = positions.reshape(-1, 3, 3)
+        fn = _normalise(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]))
+        nrm = np.repeat(fn, 3, axis=0).astype(F32)
+        has_vn = False
+    else:
+        nrm, has_vn = np.zeros((0, 3), F32), False
+    uvs = [np.asarray(u, F32)[idx] if len(u) == nv else np.zeros((0, 2), F32) for u in prim.uvs]
+    if tangents_src is not None:
+        d = _normalise(tangents_src[idx, :3])
+        tangents = np.concatenate([d, np.where(tangents_src[idx, 3:4] < 0.0, -1.0, 1.0)], 1).astype(F32)
+    else:
+        tangents = np.zeros((0, 4), F32)
+    if (len(tangents) == 0 and needs_tangents and topo == "triangles" and tangent_uv_set < len(uvs)
+            and len(uvs[tangent_uv_set])):
+        tangents = face_tangents(positions, nrm, uvs[tangent_uv_set])
+    colors = (np.asarray(prim.colors[0], F32)[idx] if prim.colors and len(prim.colors[0]) == nv
+              else np.zeros((0, 4), F32))
+    return DrawItem(node=node, mesh=mesh, primitive=pi, topology=topo, world=np.asarray(world, F32),
+                    positions=positions, normals=nrm, has_vertex_normals=has_vn, tangents=tangents, uvs=uvs,
+                    colors=colors, material=material, bounds_min=positions.min(0), bounds_max=positions.max(0))
+
+
+# ---------------------------------------------------------------------------
+# Materials and lights.
+# ---------------------------------------------------------------------------
+def _fin(v, d):
+    return v if math.isfinite(v) else d
+
+
+def prepared_material(m):
+    """PreparedMaterial::from_material's factor rules (src/prepare.rs:
+    623-668) as an oxr_material (the library applies material_oriented's
+    finite/clamp rules to the rest at upload)."""
+    strength = max(_fin(m.emissive_strength, 1.0), 0.0) if m.emissive_strength is not None else 1.0
+    base = tuple(_fin(v, 1.0) for v in m.base_color[:3]) + (min(max(_fin(m.base_color[3], 1.0), 0.0), 1.0),)
+    out = A.material(base, metallic=min(max(_fin(m.metallic, 1.0), 0.0), 1.0),
+                     roughness=min(max(_fin(m.roughness, 1.0), 0.0), 1.0), **m.params)
+    out.emissive = A.f3(*(max(_fin(v, 0.0), 0.0) * strength for v in m.emissive_factor))
+    out.flags = (A.OXR_MAT_DOUBLE_SIDED if m.double_sided else 0) | (A.OXR_MAT_UNLIT if m.unlit else 0)
+    out.alpha_mode = m.alpha_mode
+    out.alpha_cutoff = m.alpha_cutoff
+    out.ior = max(_fin(m.ior, 1.5), 1.0) if m.ior is not None else 1.5
+    return out
+
+
+def fallback_material():
+    """PreparedMaterial::fallback (src/prepare.rs:603-621)."""
+    return A.material((0.7, 0.7, 0.75, 1.0), metallic=0.0, roughness=1.0)
+
+
+def light_from_spec(spec):
+    """PreparedLight::from_light_spec (src/prepare.rs:848-863)."""
+    az, el = math.radians(spec.azimuth_deg), math.radians(spec.elevation_deg)
+    toward = _normalise(np.array([math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)], F32))
+    return A.light(A.OXR_LIGHT_DIRECTIONAL, color=(1.0, 1.0, 1.0), intensity=math.pi * max(spec.intensity, 0.0),
+                   position=(0.0, 0.0, 0.0), direction=tuple(float(-v) for v in toward), range_=0.0, inner=0.0,
+                   outer=math.pi / 4.0)
+
+
+def light_from_scene(light, world):
+    """PreparedLight::from_scene (src/prepare.rs:865-919)."""
+    w = np.asarray(world, F32)
+    position = (float(w[0, 3]), float(w[1, 3]), float(w[2, 3]))
+    direction = _normalise(np.array([-w[0, 2], -w[1, 2], -w[2, 2]], F32))
+    if np.dot(direction, direction) < 0.5:
+        direction = np.array([0.0, 0.0, -1.0], F32)
+    kind = {"directional": A.OXR_LIGHT_DIRECTIONAL, "point": A.OXR_LIGHT_POINT, "spot": A.OXR_LIGHT_SPOT}[light.kind]
+    rng = light.range if (light.range is not None and math.isfinite(light.range) and light.range > 0.0) else 0.0
+    inner = outer = 0.0
+    if light.kind == "directional":
+        rng = 0.0
+    elif light.kind == "spot":
+        oc, ic = light.outer_cone_angle, light.inner_cone_angle
+        outer = min(max(oc, 1.0e-4), math.pi / 2.0) if math.isfinite(oc) else math.pi / 4.0
+        inner = min(max(ic, 0.0), outer) if math.isfinite(ic) else 0.0
+    return A.light(kind, color=tuple(max(_fin(c, 0.0), 0.0) for c in light.color),
+                   intensity=max(_fin(light.intensity, 0.0), 0.0), position=position,
+                   direction=tuple(float(v) for v in direction), range_=rng, inner=inner, outer=outer)
+
+
+# ---------------------------------------------------------------------------
+# PreparedScene::build.
+# ---------------------------------------------------------------------------
+def build(scene, use_scene_lights=True, fallback_light=LightSpec(), generate_tangents=True):
+    """PreparedScene::build (src/prepare.rs:1098-1237) for the rest pose."""
+    ntex = len(scene.textures)
+    materials, mat_tex = [], {}
+    for i, m in enumerate(scene.materials):
+        materials.append(prepared_material(m))
+        binds = {slot: r for slot, r in m.textures.items() if 0 <= r.texture < ntex}
+        if binds:
+            mat_tex[i] = (binds, m)
+    default_material = len(materials)
+    materials.append(fallback_material())
+    worlds = world_transforms(scene)
+    items, lights, cameras = [], [], []
+    for nid in preorder_nodes(scene):
+        node, world = scene.nodes[nid], worlds[nid]
+        if world is None:
+            continue
+        if node.light is not None and 0 <= node.light < len(scene.lights):
+            lights.append(light_from_scene(scene.lights[node.light], world))
+        if node.camera is not None and 0 <= node.camera < len(scene.cameras):
+            cameras.append((nid, node.camera, scene.cameras[node.camera], world))
+        if node.mesh is None or not (0 <= node.mesh < len(scene.meshes)):
+            continue
+        for pi, prim in enumerate(scene.meshes[node.mesh].primitives):
+            mi = prim.material if (prim.material is not None and 0 <= prim.material < default_material) \
+                else default_material
+            normal_ref = mat_tex.get(mi, ({}, None))[0].get("normal")
+            needs = generate_tangents and normal_ref is not None
+            it = flatten_primitive(transformed(prim, world), nid, node.mesh, pi, world, mi, needs,
+                                   normal_ref.effective_uv_set() if normal_ref else 0)
+            if it is not None:
+                items.append(it)
+    scene_lights = use_scene_lights and len(lights) > 0
+    if not scene_lights:
+        lights = [light_from_spec(fallback_light)] if fallback_light is not None else []
+    bounds = None
+    for it in items:
+        bounds = (it.bounds_min, it.bounds_max) if bounds is None else (
+            np.minimum(bounds[0], it.bounds_min), np.maximum(bounds[1], it.bounds_max))
+    return PreparedScene(items=items, materials=materials, material_textures=mat_tex, textures=scene.textures,
+                         lights=lights, cameras=cameras, scene_lights=scene_lights, bounds=bounds)
+
+
+def to_scene(prep, name="prepared"):
+    """The tracer's scene (oxr_scene_desc) of a PreparedScene: the triangle
+    items' corners in order, as TraceScene::new takes them
+    (src/trace.rs:329-343)."""
+    s = S.Scene(name)
+    for m in prep.materials:
+        s.add_material(m)
+    for t in prep.textures:
+        s.add_texture(t.pixels, mag=t.mag_filter, min_filter=t.min_filter, wrap_s=t.wrap_s, wrap_t=t.wrap_t)
+    for mi, (binds, m) in prep.material_textures.items():
+        slots = {}
+        for slot, r in binds.items():
+            if slot not in A.TEXTURE_SLOTS:
+                continue  # occlusion: the path tracer ignores it (src/pathtrace.rs:241-246)
+            b = dict(texture=r.texture, uv_set=r.effective_uv_set())
+            if r.has_transform():
+                b.update(offset=tuple(r.offset or (0.0, 0.0)), rotation=r.rotation, scale=tuple(r.scale or (1.0, 1.0)))
+            slots[slot] = b
+        extra = {"normal_scale": _fin(m.normal_scale, 1.0)}
+        extra.update({k: m.params[k] for k in ("clearcoat_normal_scale", "iridescence_thickness_min")
+                      if k in m.params})
+        s.set_material_textures(mi, **extra, **slots)
+    for it in prep.items:
+        if it.topology != "triangles":
+            continue
+        n = len(it.positions) // 3
+        kw = {}
+        for k in range(min(2, len(it.uvs))):
+            if len(it.uvs[k]):
+                kw["uv%d" % k] = it.uvs[k].reshape(n, 6)
+        if len(it.tangents):
+            kw["tangents"] = it.tangents.reshape(n, 12)
+        if len(it.colors):
+            kw["colors"] = it.colors.reshape(n, 12)
+        s.add_tris(it.positions.reshape(n, 9), it.material,
+                   normals=it.normals.reshape(n, 9) if it.has_vertex_normals else None, **kw)
+    s.lights = list(prep.lights)
+    return s
+
+
+# ---------------------------------------------------------------------------
+# An example scene graph.
+# ---------------------------------------------------------------------------
+def showcase_graph(res=32):
+    """A small glTF-shaped scene graph that exercises the prepare rules: a
+    strip floor scaled by its node, an indexed box with COLOR_0 placed twice
+    (once under a mirroring node), a normal-mapped textured panel without
+    tangents (generated per face), an emissive fan, a line loop and a
+    material-less primitive, lit by a spot and a point light on
+    transformed nodes."""
+    u = (np.arange(res) + 0.5) / res
+    uu, vv = np.meshgrid(u, u)
+    checker = ((np.floor(uu * 4) + np.floor(vv * 4)) % 2)[..., None]
+    base = np.concatenate([40 + 180 * checker, 90 + 60 * checker, 200 - 150 * checker,
+                           np.full_like(checker, 255)], 2).astype(np.uint8)
+    nx, ny = 0.5 * np.sin(uu * 6 * math.pi), 0.5 * np.cos(vv * 6 * math.pi)
+    nz = np.ones_like(nx)
+    ln = np.sqrt(nx * nx + ny * ny + nz * nz)
+    nmap = np.stack([nx / ln, ny / ln, nz / ln], 2) * 127.5 + 127.5
+    nmap = np.concatenate([nmap, np.full(nmap.shape[:2] + (1,), 255)], 2).round().astype(np.uint8)
+    textures = [Texture(base), Texture(nmap)]
+    materials = [
+        Material(base_color=(0.8, 0.8, 0.8, 1.0), metallic=0.0, roughness=0.7),
+        Material(base_color=(1.0, 1.0, 1.0, 1.0), metallic=0.0, roughness=0.4),
+        Material(base_color=(1.0, 1.0, 1.0, 1.0), metallic=0.2, roughness=0.35, double_sided=True,
+                 textures={"base_color": TextureRef(0), "normal": TextureRef(1)}, normal_scale=0.8),
+        Material(base_color=(0.0, 0.0, 0.0, 1.0), emissive_factor=(1.0, 0.8, 0.6), emissive_strength=6.0),
+    ]
+    quad = np.array([[-1, 0, 1], [1, 0, 1], [-1, 0, -1], [1, 0, -1]], np.float32)
+    floor = Primitive(positions=quad, topology="triangle_strip", normals=np.array([[0, 1, 0]] * 4, np.float32),
+                      material=0)
+    c = np.array([[x, y, z] for x in (-0.3, 0.3) for y in (0.0, 0.6) for z in (-0.3, 0.3)], np.float32)
+    box_idx = np.array([0, 6, 4, 0, 2, 6, 1, 7, 3, 1, 5, 7, 0, 5, 1, 0, 4, 5,
+                        2, 7, 6, 2, 3, 7, 0, 3, 2, 0, 1, 3, 4, 7, 5, 4, 6, 7], np.uint16)
+    cols = np.concatenate([(c[:, :1] + 0.3) / 0.6, (c[:, 1:2]) / 0.6, (c[:, 2:3] + 0.3) / 0.6,
+                           np.ones((8, 1), np.float32)], 1).astype(np.float32)
+    box = Primitive(positions=c, indices=box_idx, colors=[cols], material=1)
+    panel = Primitive(positions=np.array([[-0.5, 0, 0], [0.5, 0, 0], [0.5, 1, 0], [-0.5, 1, 0]], np.float32),
+                      indices=np.array([0, 1, 2, 0, 2, 3], np.uint32),
+                      normals=np.array([[0, 0, 1]] * 4, np.float32),
+                      uvs=[np.array([[0, 1], [1, 1], [1, 0], [0, 0]], np.float32)], material=2)
+    ring = [[0.0, 0.0, 0.0]] + [[0.25 * math.cos(a), 0.0, -0.25 * math.sin(a)]
+                                for a in np.linspace(0.0, 2.0 * math.pi, 9)]
+    lamp = Primitive(positions=np.array(ring, np.float32), topology="triangle_fan", material=3)
+    wire = Primitive(positions=quad, topology="line_loop", material=0)
+    plain = Primitive(positions=np.array([[1.2, 0.01, 0.8], [1.6, 0.01, 0.8], [1.4, 0.5, 0.6]], np.float32))
+    h = math.sqrt(0.5)
+    nodes = [
+        Node(children=[1, 2, 3, 4, 5, 6, 7, 8]),
+        Node(mesh=0, scale=(3.0, 1.0, 3.0)),                                   # floor
+        Node(mesh=1, translation=(-0.9, 0.0, 0.2), rotation=(0.0, 0.2588190, 0.0, 0.9659258)),
+        Node(mesh=1, translation=(0.9, 0.0, -0.4), scale=(-1.0, 1.3, 1.0)),   # mirrored box
+        Node(mesh=2, translation=(0.0, 0.05, -1.2), rotation=(0.0, 0.0, 0.0, 1.0), scale=(1.6, 1.6, 1.6)),
+        Node(mesh=3, translation=(0.3, 2.2, 0.3), rotation=(1.0, 0.0, 0.0, 0.0)),  # fan faces down
+        Node(light=0, translation=(0.0, 3.0, 1.5), rotation=(-0.4, 0.0, 0.0, 0.9165151)),
+        Node(light=1, translation=(-1.5, 1.5, 1.0)),
+        Node(mesh=4),
+    ]
+    meshes = [Mesh([floor, wire]), Mesh([box]), Mesh([panel]), Mesh([lamp]), Mesh([plain])]
+    lights = [Light("spot", color=(1.0, 0.9, 0.8), intensity=30.0, range=12.0, inner_cone_angle=0.3,
+                    outer_cone_angle=0.6),
+              Light("point", color=(0.6, 0.7, 1.0), intensity=4.0)]
+    del h
+    return Scene3D(nodes=nodes, roots=[0], meshes=meshes, materials=materials, textures=textures, lights=lights,
+                   cameras=[Camera(yfov=0.9)])

@@ -1,0 +1,58 @@
+"""A prepared scene graph on the GPU against the oracle: the example graph
+(strip floor, mirrored indexed box with COLOR_0, a normal-mapped panel with
+generated tangents, an emissive fan, spot and point lights on transformed
+nodes) goes through the prepare layer (src/prepare.rs:1098-1237) into
+`sync`, and its direct-lighting image matches the oracle's per pixel, its
+full-bounce image within 3 sigma."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oxr import prepare as P
+from oxr import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene():
+    s = P.to_scene(P.build(P.showcase_graph()), "graph")
+    s.camera = scenes.CameraSpec((0.0, 1.7, 3.4), (0.0, 0.5, -0.4), yfov_deg=50.0)
+    s.ambient = 0.1
+    return s
+
+
+def _pair(W, spp, bounces):
+    import oxr
+    s = _scene()
+    pt = oxr.PathTracer(0)
+    o = oxr.RenderOptions(width=W, height=W, ambient=s.ambient)
+    o.path_trace.samples_per_pixel = spp
+    o.path_trace.max_bounces = bounces
+    pt.sync(s, o)
+    pt.refine(spp)
+    ag, sg = pt.accum(with_sum_sq=True)
+    pt.close()
+    ac, sc = O.OracleScene(s).render(W, W, 0, spp, O.options(W, W, max_bounces=bounces, ambient=s.ambient),
+                                     sum_sq=True)
+    return ag, sg, ac, sc
+
+
+def test_prepared_graph_direct_lighting_matches_oracle():
+    ag, _, ac, _ = _pair(64, 8, 1)
+    assert np.array_equal(ag[..., 3], ac[..., 3])
+    cov = ac[..., 3] > 0
+    mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
+    mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
+    assert mc.max() > 0.1
+    rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
+    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+
+
+def test_prepared_graph_full_bounce_within_3_sigma():
+    W, spp = 40, 64
+    ag, sg, ac, sc = _pair(W, spp, 6)
+    n = np.maximum(ac[..., 3:4], 1.0)
+    mg, mc = ag[..., :3] / n, ac[..., :3] / n
+    var = (np.maximum(sg / n - mg ** 2, 0) + np.maximum(sc / n - mc ** 2, 0)) / n
+    z = np.abs(mg - mc) / np.sqrt(var + 1e-20)
+    assert (z > 3.0).mean() < 0.01

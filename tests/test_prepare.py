@@ -1,0 +1,360 @@
+"""The prepare layer (src/prepare.rs:1098-1575) on the CPU: the
+reference's own prepare tests (src/prepare.rs:1599-1776, animation
+aside), the glTF rules that fix strips, fans and T * R * S, closed forms
+for tangents and the normal matrix, the material and texture-binding
+rules, and prepared lights checked through the oracle's integrator
+(irradiance of a Lambert plane under the prepared spot and fallback
+lights)."""
+import math
+
+import numpy as np
+import pytest
+
+from oxr import _abi as A
+from oxr import prepare as P
+from oxr import scenes
+
+
+def tri_scene(**node_kw):
+    """src/prepare.rs:1585-1593: one triangle in z = 0 under a node."""
+    prim = P.Primitive(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32))
+    s = P.Scene3D(meshes=[P.Mesh([prim])], nodes=[P.Node(mesh=0, **node_kw)], roots=[0])
+    return s
+
+
+def face_z(t):
+    return np.cross(t[1] - t[0], t[2] - t[0])[2]
+
+
+def test_flat_normals_and_world_space_positions():
+    p = P.build(tri_scene(translation=(0.0, 0.0, 2.0)))
+    assert len(p.items) == 1
+    it = p.items[0]
+    assert list(it.positions[1]) == [1.0, 0.0, 2.0]
+    assert not it.has_vertex_normals
+    assert list(it.normals[0]) == [0.0, 0.0, 1.0]
+    assert it.material == p.default_material()
+    assert len(p.lights) == 1 and not p.scene_lights
+    assert list(p.bounds[1]) == [1.0, 1.0, 2.0]
+    fb = p.materials[p.default_material()]
+    assert list(fb.base_color) == pytest.approx([0.7, 0.7, 0.75, 1.0])
+    assert fb.metallic == 0.0 and fb.roughness == 1.0
+
+
+def test_mirrored_world_keeps_ccw_front_facing():
+    p = P.build(tri_scene(scale=(-1.0, 1.0, 1.0)))
+    assert face_z(p.items[0].positions) > 0.0
+    # The unflipped winding would face -Z.
+    q = P.build(tri_scene())
+    assert face_z(q.items[0].positions) > 0.0
+    assert list(p.items[0].positions[1]) == [0.0, 1.0, 0.0]  # (a, c, b)
+
+
+def test_strips_lines_and_garbage_indices():
+    quad = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    strip = P.Primitive(positions=quad, topology="triangle_strip")
+    loop = P.Primitive(positions=quad, topology="line_loop")
+    bad = P.Primitive(positions=quad[:3], indices=np.array([0, 1, 2, 0, 1, 99], np.uint32))
+    s = P.Scene3D(meshes=[P.Mesh([strip, loop, bad])], nodes=[P.Node(mesh=0)], roots=[0])
+    p = P.build(s)
+    assert len(p.items) == 3
+    assert p.items[0].primitive_count() == 2
+    assert p.items[1].topology == "lines" and p.items[1].primitive_count() == 4
+    assert p.items[2].primitive_count() == 1
+    for t in p.items[0].positions.reshape(-1, 3, 3):
+        assert face_z(t) > 0.0
+    # Line loop closes back to the first vertex.
+    assert list(p.items[1].positions[-1]) == list(quad[0])
+    # Lines trace nothing: the tracer's soup holds the triangles only.
+    assert P.to_scene(p).num_tris == 3
+
+
+def test_strip_and_fan_follow_gltf_triangulation():
+    # glTF 2.0 3.7.2.1: strip i -> (i, i+1+i%2, i+2-i%2); fan i -> (i+1, i+2, 0).
+    ring = np.array([[0, 0, 0]] + [[math.cos(a), math.sin(a), 0.0] for a in np.linspace(0, 1.5 * math.pi, 5)],
+                    np.float32)
+    fan = P.Primitive(positions=ring, topology="triangle_fan")
+    strip = P.Primitive(positions=ring[[1, 2, 0, 3, 4]], topology="triangle_strip", indices=None)
+    tris = P.triangle_indices(strip, list(range(5)))
+    assert tris == [(0, 1, 2), (1, 3, 2), (2, 3, 4)]
+    assert P.triangle_indices(fan, list(range(6))) == [(1, 2, 0), (2, 3, 0), (3, 4, 0), (4, 5, 0)]
+    p = P.build(P.Scene3D(meshes=[P.Mesh([fan])], nodes=[P.Node(mesh=0)], roots=[0]))
+    assert p.items[0].primitive_count() == 4
+    for t in p.items[0].positions.reshape(-1, 3, 3):
+        assert face_z(t) > 0.0
+    # Fewer than three vertices make no triangle and no item.
+    short = P.Primitive(positions=ring[:2], topology="triangle_strip")
+    assert P.build(P.Scene3D(meshes=[P.Mesh([short])], nodes=[P.Node(mesh=0)], roots=[0])).items == []
+
+
+def test_non_finite_positions_drop_their_triangles():
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [np.nan, 0, 0]], np.float32)
+    prim = P.Primitive(positions=pos, indices=np.array([0, 1, 2, 0, 3, 2], np.uint16))
+    p = P.build(P.Scene3D(meshes=[P.Mesh([prim])], nodes=[P.Node(mesh=0)], roots=[0]))
+    assert p.items[0].primitive_count() == 1
+
+
+def test_scene_lights_resolve_with_world_transforms(oracle):
+    s = tri_scene()
+    s.lights.append(P.Light("spot", color=(1.0, 0.5, 0.25), intensity=10.0, range=5.0, inner_cone_angle=0.2,
+                            outer_cone_angle=0.4))
+    h = math.sqrt(0.5)
+    s.nodes.append(P.Node(light=0, translation=(0.0, 3.0, 0.0), rotation=(-h, 0.0, 0.0, h)))
+    s.roots.append(1)
+    p = P.build(s)
+    assert p.scene_lights
+    l = p.lights[0]
+    assert l.kind == A.OXR_LIGHT_SPOT
+    assert abs(l.direction[1] + 1.0) < 1e-5, list(l.direction)
+    assert list(l.position) == pytest.approx([0.0, 3.0, 0.0])
+    assert l.range == 5.0
+    assert l.inner_cone_angle == pytest.approx(0.2) and l.outer_cone_angle == pytest.approx(0.4)
+    # Irradiance through the integrator: a white Lambert plane reflects
+    # E / pi. Straight below at y = 1: 10 / 2^2 * (1 - (2/5)^4).
+    window = 1.0 - (2.0 / 5.0) ** 4
+    below = lambert_radiance(oracle, l, (0.0, 1.0, 0.0))
+    assert below[0] == pytest.approx(10.0 / 4.0 * window / math.pi, rel=1e-4)
+    assert below[1] == pytest.approx(0.5 * below[0], rel=1e-5)
+    assert np.all(lambert_radiance(oracle, l, (3.0, 2.0, 0.0)) == 0.0)   # outside the outer cone
+    assert np.all(lambert_radiance(oracle, l, (0.0, -3.0, 0.0)) == 0.0)  # beyond range
+    p2 = P.build(s, use_scene_lights=False)
+    assert not p2.scene_lights
+    assert p2.lights[0].kind == A.OXR_LIGHT_DIRECTIONAL
+
+
+def lambert_radiance(oracle, light, at):
+    """Mean direct radiance of a small white Lambert patch at `at` facing +Y
+    lit by `light` alone (a one-bounce render of the patch)."""
+    x, y, z = at
+    s = scenes.Scene("patch")
+    m = s.add_material(scenes.lambert((1.0, 1.0, 1.0)))
+    e = 1.0e-3
+    t, n = scenes.quad([[x - e, y, z + e], [x + e, y, z + e], [x + e, y, z - e], [x - e, y, z - e]])
+    s.add_tris(t, m, normals=n)
+    s.lights.append(light)
+    s.camera = scenes.CameraSpec((x, y + 0.5, z + 1e-4), (x, y, z), yfov_deg=0.01)
+    s.ambient = 0.0
+    os_ = oracle.OracleScene(s)
+    acc = os_.render(2, 2, 0, 1, oracle.options(2, 2, spp=1, max_bounces=1))
+    assert np.all(acc[..., 3] == 1.0)
+    return acc[..., :3].reshape(-1, 3).mean(0)
+
+
+def test_fallback_light_matches_legacy_lambert_scale(oracle):
+    l = P.light_from_spec(P.LightSpec(azimuth_deg=0.0, elevation_deg=90.0, intensity=1.0))
+    assert l.kind == A.OXR_LIGHT_DIRECTIONAL
+    assert -l.direction[1] == pytest.approx(1.0, abs=1e-6)
+    assert l.intensity == pytest.approx(math.pi)
+    # E / pi = 1: a white Lambert plane facing the light reflects 1.
+    assert lambert_radiance(oracle, l, (0.0, 0.0, 0.0)) == pytest.approx([1.0, 1.0, 1.0], rel=1e-5)
+    d = P.light_from_spec(P.LightSpec())  # az 45, el 45
+    assert list(d.direction) == pytest.approx([-0.5, -math.sqrt(0.5), -0.5], abs=1e-6)
+    assert P.light_from_spec(P.LightSpec(intensity=-3.0)).intensity == 0.0
+    assert P.build(tri_scene(), fallback_light=None).lights == []
+
+
+def test_light_sanitising():
+    s = tri_scene()
+    s.lights += [P.Light("spot", intensity=float("nan"), color=(-1.0, float("inf"), 2.0),
+                         outer_cone_angle=3.0, inner_cone_angle=5.0, range=-1.0),
+                 P.Light("spot", outer_cone_angle=float("nan"), inner_cone_angle=float("nan")),
+                 P.Light("point", range=float("inf")), P.Light("directional", range=4.0)]
+    for i in range(4):
+        s.nodes.append(P.Node(light=i, scale=(0.0, 0.0, 0.0)))  # degenerate: default direction
+        s.roots.append(i + 1)
+    L = P.build(s).lights
+    assert L[0].intensity == 0.0 and list(L[0].color) == [0.0, 0.0, 2.0]
+    assert L[0].outer_cone_angle == pytest.approx(math.pi / 2) and L[0].inner_cone_angle == pytest.approx(math.pi / 2)
+    assert L[0].range == 0.0
+    assert L[1].outer_cone_angle == pytest.approx(math.pi / 4) and L[1].inner_cone_angle == 0.0
+    assert L[2].range == 0.0 and L[2].outer_cone_angle == 0.0
+    assert L[3].range == 0.0
+    assert all(list(l.direction) == [0.0, 0.0, -1.0] for l in L)
+
+
+def test_camera_instances_collected():
+    s = tri_scene()
+    s.cameras.append(P.Camera(yfov=0.8, znear=0.1))
+    s.nodes.append(P.Node(camera=0, translation=(0.0, 1.0, 5.0)))
+    s.roots.append(1)
+    p = P.build(s)
+    assert len(p.cameras) == 1
+    node, ci, cam, world = p.cameras[0]
+    assert node == 1 and ci == 0 and cam.yfov == 0.8
+    assert list(world[:3, 3]) == [0.0, 1.0, 5.0]
+
+
+def test_normal_matrix_of_scale_is_inverse():
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0] = 2.0
+    n = P.normal_matrix(m)
+    assert abs(n[0, 0] - 0.5) < 1e-6 and abs(n[1, 1] - 1.0) < 1e-6
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        m = np.eye(4, dtype=np.float32)
+        m[:3, :3] = rng.normal(size=(3, 3))
+        np.testing.assert_allclose(P.normal_matrix(m), np.linalg.inv(m[:3, :3].astype(np.float64)).T, rtol=2e-4,
+                                   atol=2e-4)
+    assert np.array_equal(P.normal_matrix(np.zeros((4, 4), np.float32)), np.eye(3))
+
+
+def test_node_hierarchy_composes_trs_in_preorder():
+    # Parent: translate (1, 0, 0); child: rotate 90 deg about +Z, scale 2.
+    h = math.sqrt(0.5)
+    prim = P.Primitive(positions=np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32))
+    s = P.Scene3D(meshes=[P.Mesh([prim])], roots=[0, 0, 3],
+                  nodes=[P.Node(children=[1, 2], translation=(1.0, 0.0, 0.0)),
+                         P.Node(mesh=0, rotation=(0.0, 0.0, h, h), scale=(2.0, 2.0, 2.0)),
+                         P.Node(mesh=0, children=[1]),   # shared child: visited once
+                         P.Node(mesh=7)])                 # bad mesh index: skipped
+    assert P.preorder_nodes(s) == [0, 1, 2, 3]
+    p = P.build(s)
+    assert [it.node for it in p.items] == [1, 2]
+    np.testing.assert_allclose(p.items[0].positions, [[1, 2, 0], [-1, 0, 0], [1, 0, 2]], atol=1e-6)
+    # A matrix node takes the matrix as given.
+    m = np.eye(4)
+    m[:3, 3] = (0.0, 5.0, 0.0)
+    s2 = P.Scene3D(meshes=[P.Mesh([prim])], roots=[0], nodes=[P.Node(mesh=0, matrix=m, translation=(9, 9, 9))])
+    np.testing.assert_allclose(P.build(s2).items[0].positions[0], [1, 5, 0])
+
+
+def test_vertex_normals_follow_the_inverse_transpose():
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    nrm = np.array([[1, 1, 0]] * 3, np.float32)
+    prim = P.Primitive(positions=pos, normals=nrm)
+    p = P.build(P.Scene3D(meshes=[P.Mesh([prim])], roots=[0], nodes=[P.Node(mesh=0, scale=(2.0, 1.0, 1.0))]))
+    it = p.items[0]
+    assert it.has_vertex_normals
+    want = np.array([0.5, 1.0, 0.0]) / math.hypot(0.5, 1.0)
+    np.testing.assert_allclose(it.normals, [want] * 3, atol=1e-6)
+    # A wrong-length normal stream falls back to flat normals.
+    bad = P.Primitive(positions=pos, normals=nrm[:2])
+    q = P.build(P.Scene3D(meshes=[P.Mesh([bad])], roots=[0], nodes=[P.Node(mesh=0)]))
+    assert not q.items[0].has_vertex_normals
+
+
+def test_face_tangents_closed_form():
+    # uv = (x, y) on a +Z triangle: tangent +X, bitangent +Y, w = +1;
+    # mirroring u flips the handedness.
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    nrm = np.array([[0, 0, 1]] * 3, np.float32)
+    t = P.face_tangents(pos, nrm, np.array([[0, 0], [1, 0], [0, 1]], np.float32))
+    np.testing.assert_allclose(t, [[1, 0, 0, 1]] * 3, atol=1e-6)
+    t = P.face_tangents(pos, nrm, np.array([[0, 0], [-1, 0], [0, 1]], np.float32))
+    np.testing.assert_allclose(t, [[-1, 0, 0, -1]] * 3, atol=1e-6)
+    # Degenerate UVs: default frame, Gram-Schmidt against the normal.
+    t = P.face_tangents(pos, np.array([[1, 0, 0]] * 3, np.float32), np.zeros((3, 2), np.float32))
+    assert np.all(np.abs(np.einsum("ij,j->i", t[:, :3], [1, 0, 0])) < 1e-6)
+    np.testing.assert_allclose(np.linalg.norm(t[:, :3], axis=1), 1.0, atol=1e-6)
+
+
+def test_tangents_generated_only_for_normal_mapped_materials():
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    uv = np.array([[0, 0], [1, 0], [0, 1]], np.float32)
+    tex = P.Texture(np.full((2, 2, 4), 128, np.uint8))
+    mats = [P.Material(textures={"normal": P.TextureRef(0, tex_coord=1)}), P.Material(),
+            P.Material(textures={"normal": P.TextureRef(5)})]  # missing texture: not bound
+    prims = [P.Primitive(positions=pos, uvs=[np.zeros((3, 2), np.float32), uv], material=0),
+             P.Primitive(positions=pos, uvs=[uv], material=1),
+             P.Primitive(positions=pos, uvs=[uv], material=2),
+             P.Primitive(positions=pos, uvs=[uv], material=9)]
+    s = P.Scene3D(meshes=[P.Mesh(prims)], roots=[0], nodes=[P.Node(mesh=0)], materials=mats, textures=[tex])
+    p = P.build(s)
+    np.testing.assert_allclose(p.items[0].tangents, [[1, 0, 0, 1]] * 3, atol=1e-6)  # from TEXCOORD_1
+    assert [len(it.tangents) for it in p.items[1:]] == [0, 0, 0]
+    assert [it.material for it in p.items] == [0, 1, 2, 3]
+    assert len(P.build(s, generate_tangents=False).items[0].tangents) == 0
+    # Given tangents are normalised, with w reduced to its sign.
+    given = P.Primitive(positions=pos, uvs=[uv], material=0,
+                        tangents=np.array([[0, 3, 0, -0.2]] * 3, np.float32))
+    s.meshes[0].primitives = [given]
+    np.testing.assert_allclose(P.build(s).items[0].tangents, [[0, 1, 0, -1]] * 3)
+
+
+def test_material_rules():
+    m = P.Material(base_color=(float("nan"), 0.5, 2.0, 1.5), metallic=float("nan"), roughness=-1.0,
+                   emissive_factor=(1.0, -2.0, float("nan")), emissive_strength=4.0, ior=0.5,
+                   double_sided=True, alpha_mode=A.OXR_ALPHA_MASK, alpha_cutoff=0.3,
+                   params=dict(transmission=0.25))
+    q = P.prepared_material(m)
+    assert list(q.base_color) == [1.0, 0.5, 2.0, 1.0]
+    assert q.metallic == 1.0 and q.roughness == 0.0
+    assert list(q.emissive) == [4.0, 0.0, 0.0]
+    assert q.ior == 1.0
+    assert q.flags & A.OXR_MAT_DOUBLE_SIDED
+    assert q.alpha_mode == A.OXR_ALPHA_MASK and q.alpha_cutoff == pytest.approx(0.3)
+    assert q.transmission == 0.25
+    assert P.prepared_material(P.Material()).ior == 1.5
+    assert P.prepared_material(P.Material(ior=float("nan"), emissive_strength=-1.0,
+                                          emissive_factor=(1.0, 1.0, 1.0))).ior == 1.5
+    assert list(P.prepared_material(P.Material(emissive_strength=-1.0, emissive_factor=(1, 1, 1))).emissive) \
+        == [0.0, 0.0, 0.0]
+
+
+def test_texture_binding_transform_filter():
+    tex = P.Texture(np.zeros((1, 1, 4), np.uint8))
+    refs = {"base_color": P.TextureRef(0, offset=(0.0, 0.0), rotation=0.0, scale=(1.0, 1.0)),
+            "emissive": P.TextureRef(0, tex_coord=0, transform_tex_coord=1, offset=(0.5, 0.0)),
+            "metallic_roughness": P.TextureRef(0, scale=(float("inf"), 1.0)),
+            "occlusion": P.TextureRef(0)}
+    s = P.Scene3D(meshes=[P.Mesh([P.Primitive(positions=np.eye(3, dtype=np.float32), material=0)])],
+                  roots=[0], nodes=[P.Node(mesh=0)], materials=[P.Material(textures=refs)], textures=[tex])
+    sc = P.to_scene(P.build(s))
+    mt = sc.material_tex[0]
+    assert mt.base_color.texture == 0 and mt.base_color.has_transform == 0
+    assert mt.emissive.uv_set == 1 and mt.emissive.has_transform == 1 and mt.emissive.offset[0] == 0.5
+    assert mt.metallic_roughness.has_transform == 0
+    assert mt.normal.texture == -1
+
+
+def test_to_scene_packs_items_in_order():
+    quad = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    cols = np.array([[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1], [1, 1, 1, 1]], np.float32)
+    uv = quad[:, :2].copy()
+    a = P.Primitive(positions=quad, topology="triangle_strip", colors=[cols], uvs=[uv, uv * 2])
+    b = P.Primitive(positions=quad[:3], normals=np.array([[0, 0, 2]] * 3, np.float32), material=0)
+    s = P.Scene3D(meshes=[P.Mesh([a, b])], roots=[0, 1],
+                  nodes=[P.Node(mesh=0), P.Node(mesh=0, translation=(0.0, 0.0, 1.0))],
+                  materials=[P.Material(base_color=(0.2, 0.4, 0.6, 1.0))])
+    p = P.build(s)
+    sc = P.to_scene(p)
+    assert sc.num_tris == 6
+    assert sc.num_materials == 2
+    np.testing.assert_array_equal(sc.material_ids, [1, 1, 0, 1, 1, 0])
+    pos, nrm, flags, _ = sc.pack()
+    np.testing.assert_array_equal(pos[1], [1, 0, 0, 1, 1, 0, 0, 1, 0])  # strip triangle 1: (1, 3, 2)
+    np.testing.assert_array_equal(pos[3, 2::3], [1, 1, 1])
+    assert flags[0] == A.TRI_HAS_UV0 | A.TRI_HAS_UV1 | A.TRI_HAS_COLORS
+    assert flags[2] == A.OXR_TRI_HAS_NORMALS
+    np.testing.assert_array_equal(nrm[2], [0, 0, 1] * 3)
+    at = sc.attributes()
+    np.testing.assert_array_equal(at["colors"][1], np.concatenate([cols[1], cols[3], cols[2]]))
+    np.testing.assert_array_equal(at["uv1"][0], (uv[[0, 1, 2]] * 2).ravel())
+
+
+def test_showcase_graph_prepares_outward_boxes_and_generated_tangents():
+    g = P.showcase_graph()
+    p = P.build(g)
+    assert p.scene_lights and len(p.lights) == 2
+    assert [it.topology for it in p.items] == ["triangles", "lines", "triangles", "triangles", "triangles",
+                                               "triangles", "triangles"]
+    assert [it.primitive_count() for it in p.items] == [2, 4, 12, 12, 2, 8, 1]
+    for k in (2, 3):  # both box instances, one under a mirroring node
+        it = p.items[k]
+        centre = (it.bounds_min + it.bounds_max) / 2
+        t = it.positions.reshape(-1, 3, 3)
+        n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+        assert np.all(np.einsum("ij,ij->i", n, t.mean(1) - centre) > 0.0), k
+        assert len(it.colors) == 36
+    assert p.materials[0].base_color[0] == pytest.approx(0.8)
+    panel = p.items[4]
+    assert len(panel.tangents) == 6
+    # u runs along +x and v down the panel (-y), so B = -y = w * (N x T)
+    # with N x T = +y: w = -1.
+    np.testing.assert_allclose(panel.tangents, [[1, 0, 0, -1]] * 6, atol=1e-6)
+    lamp = p.items[5]
+    assert np.all(lamp.normals[:, 1] < -0.99)  # the fan faces down after the node's half turn
+    assert p.items[6].material == p.default_material()
+    sc = P.to_scene(p)
+    assert sc.num_tris == 37
+    assert list(sc.material(3).emissive) == pytest.approx([6.0, 4.8, 3.6])
