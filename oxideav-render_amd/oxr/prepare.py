@@ -1,7 +1,8 @@
 """The prepare layer for the HIP tracer's Python side (oxideav-render
 src/prepare.rs): a glTF-shaped scene graph (Scene3D) flattened into the
 world-space triangle soup, material table, textures and lights that
-`oxr_pt_upload_scene` takes, plus the scene's camera instances.
+`oxr_pt_upload_scene` takes, plus the scene's camera instances and the
+camera that Camera::resolve picks among them (src/camera.rs:177-475).
 
 It restates PreparedScene::build for static scenes (src/prepare.rs:
 1098-1237): preorder node traversal from the roots (`preorder_nodes`,
@@ -506,10 +507,108 @@ def build(scene, use_scene_lights=True, fallback_light=LightSpec(), generate_tan
                          lights=lights, cameras=cameras, scene_lights=scene_lights, bounds=bounds)
 
 
-def to_scene(prep, name="prepared"):
+# ---------------------------------------------------------------------------
+# Cameras (src/camera.rs:177-475).
+# ---------------------------------------------------------------------------
+def _n3(v):
+    v = np.asarray(v, F32)
+    n = F32(math.sqrt(float(np.dot(v, v))))
+    return (v / n).astype(F32) if n >= F32(1.1920929e-7) else np.zeros(3, F32)
+
+
+def _camera(eye, fwd, side, up, half_w, half_h, projection):
+    c = A.Camera()
+    c.eye, c.forward, c.side, c.up = (A.f3(*(float(x) for x in v)) for v in (eye, fwd, side, up))
+    c.half_w, c.half_h, c.projection = float(half_w), float(half_h), projection
+    return c
+
+
+def camera_from_instance(cam, world, width, height):
+    """Camera::from_scene_camera (src/camera.rs:208-332): a glTF camera at
+    its node's world matrix (scale ignored), filling the viewport aspect."""
+    w = np.asarray(world, F32)
+    eye = w[:3, 3].copy()
+    fwd = _n3(-w[:3, 2])
+    if np.dot(fwd, fwd) < 0.5:
+        fwd = np.array([0.0, 0.0, -1.0], F32)
+    side = _n3(np.cross(fwd, _n3(w[:3, 1])))
+    if np.dot(side, side) < 0.5:
+        side = _n3(np.cross(fwd, np.array([0.0, 1.0, 0.0], F32)))
+        if np.dot(side, side) < 0.5:
+            side = np.array([1.0, 0.0, 0.0], F32)
+    up = np.cross(side, fwd).astype(F32)
+    aspect = F32(max(width, 1)) / F32(max(height, 1))
+    if cam.perspective:
+        yfov = cam.yfov if (math.isfinite(cam.yfov) and 1.0e-4 < cam.yfov < math.pi - 1.0e-3) else math.pi / 3.0
+        th = F32(math.tan(float(F32(yfov) * F32(0.5))))
+        return _camera(eye, fwd, side, up, th * aspect, th, A.OXR_PROJ_PERSPECTIVE)
+    hh = F32(abs(cam.ymag)) if (math.isfinite(cam.ymag) and abs(cam.ymag) > 1.0e-6) else F32(1.0)
+    return _camera(eye, fwd, side, up, hh * aspect, hh, A.OXR_PROJ_ORTHOGRAPHIC)
+
+
+def frame_bounds(width, height, bounds, projection=A.OXR_PROJ_PERSPECTIVE, fov_deg=60.0, orbit=None,
+                 target_offset=(0.0, 0.0, 0.0)):
+    """Camera::build (src/camera.rs:357-475): auto-frame the AABB, looking
+    down -Z, or orbit it when `orbit` = (elevation_deg, azimuth_deg,
+    distance multiplier) is given (RenderOptions::camera)."""
+    mn, mx = (np.asarray(b, F32) for b in bounds)
+    off = np.asarray(target_offset, F32)
+    if not np.all(np.isfinite(off)):
+        off = np.zeros(3, F32)
+    c = ((mn + mx) * F32(0.5) + off).astype(F32)
+    extent = F32(max(float(mx[0] - mn[0]), float(mx[1] - mn[1]), float(mx[2] - mn[2]), 1.0e-3))
+    aspect = F32(max(width, 1)) / F32(max(height, 1))
+    radius = extent * F32(0.5) * F32(1.2)
+    fov_y = F32(math.radians(fov_deg))
+    persp = projection == A.OXR_PROJ_PERSPECTIVE
+    auto = radius / F32(math.tan(float(fov_y * F32(0.5)))) if persp else extent * F32(1.5)
+    if orbit is None:
+        eye = c + np.array([0.0, 0.0, auto], F32)
+    else:
+        el, az = math.radians(orbit[0]), math.radians(orbit[1])
+        d = np.array([math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)], F32)
+        eye = (c + d * F32(auto * F32(orbit[2]))).astype(F32)
+    world_up = np.array([0.0, 1.0, 0.0], F32)
+    fwd = _n3(c - eye)
+    side = _n3(np.cross(fwd, world_up))
+    up = np.cross(side, fwd).astype(F32)
+    if persp:
+        th = F32(math.tan(float(fov_y * F32(0.5))))
+        return _camera(eye, fwd, side, up, th * aspect, th, projection)
+    zoom = orbit[2] if (orbit is not None and math.isfinite(orbit[2]) and orbit[2] > 0.0) else 1.0
+    half = radius * F32(zoom)
+    hw, hh = (half * aspect, half) if aspect >= 1.0 else (half, half / aspect)
+    return _camera(eye, fwd, side, up, hw, hh, projection)
+
+
+class PreparedCamera:
+    """Camera::resolve (src/camera.rs:177-190) for a PreparedScene: the
+    scene camera instance `scene_camera` when it exists, otherwise the
+    auto-framed (or orbiting) camera over the prepared bounds (a unit box
+    around the origin when there are none, src/prepare.rs:1259-1261).
+    `resolve(width, height)` gives the oxr_camera."""
+
+    def __init__(self, prep, scene_camera=None, projection=A.OXR_PROJ_PERSPECTIVE, fov_deg=60.0, orbit=None,
+                 target_offset=(0.0, 0.0, 0.0)):
+        self.bounds = prep.bounds if prep.bounds is not None else ((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+        self.instance = (prep.cameras[scene_camera] if scene_camera is not None
+                         and 0 <= scene_camera < len(prep.cameras) else None)
+        self.projection, self.fov_deg, self.orbit, self.target_offset = projection, fov_deg, orbit, target_offset
+
+    def resolve(self, width, height):
+        if self.instance is not None:
+            _, _, cam, world = self.instance
+            return camera_from_instance(cam, world, width, height)
+        return frame_bounds(width, height, self.bounds, self.projection, self.fov_deg, self.orbit,
+                            self.target_offset)
+
+
+def to_scene(prep, name="prepared", scene_camera=None):
     """The tracer's scene (oxr_scene_desc) of a PreparedScene: the triangle
     items' corners in order, as TraceScene::new takes them
-    (src/trace.rs:329-343)."""
+    (src/trace.rs:329-343), and the camera Camera::resolve picks
+    (`scene_camera` = RenderOptions::scene_camera; replace `.camera` for
+    an orbit or another field of view)."""
     s = S.Scene(name)
     for m in prep.materials:
         s.add_material(m)
@@ -543,6 +642,7 @@ def to_scene(prep, name="prepared"):
         s.add_tris(it.positions.reshape(n, 9), it.material,
                    normals=it.normals.reshape(n, 9) if it.has_vertex_normals else None, **kw)
     s.lights = list(prep.lights)
+    s.camera = PreparedCamera(prep, scene_camera)
     return s
 
 
@@ -555,7 +655,7 @@ def showcase_graph(res=32):
     (once under a mirroring node), a normal-mapped textured panel without
     tangents (generated per face), an emissive fan, a line loop and a
     material-less primitive, lit by a spot and a point light on
-    transformed nodes."""
+    transformed nodes and seen through a camera node."""
     u = (np.arange(res) + 0.5) / res
     uu, vv = np.meshgrid(u, u)
     checker = ((np.floor(uu * 4) + np.floor(vv * 4)) % 2)[..., None]
@@ -592,9 +692,8 @@ def showcase_graph(res=32):
     lamp = Primitive(positions=np.array(ring, np.float32), topology="triangle_fan", material=3)
     wire = Primitive(positions=quad, topology="line_loop", material=0)
     plain = Primitive(positions=np.array([[1.2, 0.01, 0.8], [1.6, 0.01, 0.8], [1.4, 0.5, 0.6]], np.float32))
-    h = math.sqrt(0.5)
     nodes = [
-        Node(children=[1, 2, 3, 4, 5, 6, 7, 8]),
+        Node(children=[1, 2, 3, 4, 5, 6, 7, 8, 9]),
         Node(mesh=0, scale=(3.0, 1.0, 3.0)),                                   # floor
         Node(mesh=1, translation=(-0.9, 0.0, 0.2), rotation=(0.0, 0.2588190, 0.0, 0.9659258)),
         Node(mesh=1, translation=(0.9, 0.0, -0.4), scale=(-1.0, 1.3, 1.0)),   # mirrored box
@@ -603,11 +702,11 @@ def showcase_graph(res=32):
         Node(light=0, translation=(0.0, 3.0, 1.5), rotation=(-0.4, 0.0, 0.0, 0.9165151)),
         Node(light=1, translation=(-1.5, 1.5, 1.0)),
         Node(mesh=4),
+        Node(camera=0, translation=(0.0, 1.7, 3.4), rotation=(-0.1522881, 0.0, 0.0, 0.9883358)),  # 17.5 deg down
     ]
     meshes = [Mesh([floor, wire]), Mesh([box]), Mesh([panel]), Mesh([lamp]), Mesh([plain])]
     lights = [Light("spot", color=(1.0, 0.9, 0.8), intensity=30.0, range=12.0, inner_cone_angle=0.3,
                     outer_cone_angle=0.6),
               Light("point", color=(0.6, 0.7, 1.0), intensity=4.0)]
-    del h
     return Scene3D(nodes=nodes, roots=[0], meshes=meshes, materials=materials, textures=textures, lights=lights,
                    cameras=[Camera(yfov=0.9)])

@@ -1,22 +1,21 @@
 """A prepared scene graph on the GPU against the oracle: the example graph
 (strip floor, mirrored indexed box with COLOR_0, a normal-mapped panel with
 generated tangents, an emissive fan, spot and point lights on transformed
-nodes) goes through the prepare layer (src/prepare.rs:1098-1237) into
-`sync`, and its direct-lighting image matches the oracle's per pixel, its
-full-bounce image within 3 sigma."""
+nodes, seen through its camera node) goes through the prepare layer
+(src/prepare.rs:1098-1237, src/camera.rs:177-332) into `sync`, and its
+direct-lighting image matches the oracle's per pixel, its full-bounce
+image within 3 sigma."""
 import numpy as np
 import pytest
 
 import oracle as O
 from oxr import prepare as P
-from oxr import scenes
 
 pytestmark = pytest.mark.gpu
 
 
 def _scene():
-    s = P.to_scene(P.build(P.showcase_graph()), "graph")
-    s.camera = scenes.CameraSpec((0.0, 1.7, 3.4), (0.0, 0.5, -0.4), yfov_deg=50.0)
+    s = P.to_scene(P.build(P.showcase_graph()), "graph", scene_camera=0)
     s.ambient = 0.1
     return s
 

@@ -358,3 +358,71 @@ def test_showcase_graph_prepares_outward_boxes_and_generated_tangents():
     sc = P.to_scene(p)
     assert sc.num_tris == 37
     assert list(sc.material(3).emissive) == pytest.approx([6.0, 4.8, 3.6])
+
+
+def _vec(c, name):
+    return np.array(list(getattr(c, name)), np.float64)
+
+
+def test_scene_camera_from_its_node():
+    """Camera::from_scene_camera (src/camera.rs:208-332): eye and axes from
+    the node's world matrix with its scale ignored, the viewport aspect,
+    sanitised yfov and ymag."""
+    world = P.local_matrix(P.Node(translation=(0.0, 1.0, 5.0), scale=(2.0, 2.0, 2.0))).astype(np.float32)
+    c = P.camera_from_instance(P.Camera(yfov=0.8, aspect=1.0), world, 200, 100)
+    np.testing.assert_allclose(_vec(c, "eye"), [0, 1, 5])
+    np.testing.assert_allclose(_vec(c, "forward"), [0, 0, -1])
+    np.testing.assert_allclose(_vec(c, "side"), [1, 0, 0])
+    np.testing.assert_allclose(_vec(c, "up"), [0, 1, 0])
+    assert c.half_h == pytest.approx(math.tan(0.4), rel=1e-6)
+    assert c.half_w == pytest.approx(2.0 * math.tan(0.4), rel=1e-6)
+    assert c.projection == A.OXR_PROJ_PERSPECTIVE
+    for bad in (0.0, float("nan"), 3.2):
+        assert P.camera_from_instance(P.Camera(yfov=bad), world, 1, 1).half_h == pytest.approx(math.tan(math.pi / 6))
+    o = P.camera_from_instance(P.Camera(perspective=False, ymag=-2.0), world, 100, 100)
+    assert o.projection == A.OXR_PROJ_ORTHOGRAPHIC and o.half_h == 2.0 and o.half_w == 2.0
+    assert P.camera_from_instance(P.Camera(perspective=False, ymag=0.0), world, 100, 50).half_w == 2.0
+    # A quarter turn about +Y: the camera's -Z looks down -X.
+    h = math.sqrt(0.5)
+    r = P.local_matrix(P.Node(rotation=(0.0, h, 0.0, h))).astype(np.float32)
+    c = P.camera_from_instance(P.Camera(), r, 64, 64)
+    np.testing.assert_allclose(_vec(c, "forward"), [-1, 0, 0], atol=1e-6)
+    np.testing.assert_allclose(_vec(c, "side"), [0, 0, -1], atol=1e-6)
+    # Looking straight down: the node's +Y up hint keeps the frame valid.
+    d = P.local_matrix(P.Node(rotation=(-h, 0.0, 0.0, h))).astype(np.float32)
+    c = P.camera_from_instance(P.Camera(), d, 64, 64)
+    np.testing.assert_allclose(_vec(c, "forward"), [0, -1, 0], atol=1e-6)
+    np.testing.assert_allclose(_vec(c, "up"), [0, 0, -1], atol=1e-6)
+
+
+def test_auto_frame_and_orbit():
+    """Camera::build (src/camera.rs:357-475): distance radius / tan(fov/2)
+    with radius 0.6 * extent, looking down -Z; an orbit multiplies that
+    distance; orthographic frames 1.2x the half extent on the smaller axis."""
+    b = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    c = P.frame_bounds(100, 100, b)
+    dist = 1.2 / math.tan(math.radians(30.0))
+    np.testing.assert_allclose(_vec(c, "eye"), [0, 0, dist], rtol=1e-6)
+    np.testing.assert_allclose(_vec(c, "forward"), [0, 0, -1])
+    assert c.half_h == pytest.approx(math.tan(math.radians(30.0)), rel=1e-6)
+    o = P.frame_bounds(100, 100, b, orbit=(30.0, 90.0, 2.0), target_offset=(0.0, 1.0, 0.0))
+    want = np.array([0, 1, 0]) + 2 * dist * np.array([math.cos(math.radians(30)), 0.5, 0.0])
+    np.testing.assert_allclose(_vec(o, "eye"), want, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(_vec(o, "forward"), -(want - [0, 1, 0]) / np.linalg.norm(want - [0, 1, 0]), atol=1e-6)
+    w = P.frame_bounds(200, 100, b, projection=A.OXR_PROJ_ORTHOGRAPHIC)
+    assert (w.half_w, w.half_h) == pytest.approx((2.4, 1.2))
+    t = P.frame_bounds(100, 200, b, projection=A.OXR_PROJ_ORTHOGRAPHIC, orbit=(0.0, 0.0, 0.5))
+    assert (t.half_w, t.half_h) == pytest.approx((0.6, 1.2))
+    # Camera::resolve: the named scene camera, else the bounds (a unit box
+    # when there is no geometry).
+    g = P.showcase_graph()
+    prep = P.build(g)
+    assert len(prep.cameras) == 1
+    pc = P.PreparedCamera(prep, scene_camera=0).resolve(64, 64)
+    np.testing.assert_allclose(_vec(pc, "eye"), [0, 1.7, 3.4], atol=1e-6)
+    assert _vec(pc, "forward")[1] < -0.29
+    fb = P.PreparedCamera(prep, scene_camera=5).resolve(64, 64)  # no such camera: auto-frame
+    centre = (prep.bounds[0] + prep.bounds[1]) / 2
+    np.testing.assert_allclose(_vec(fb, "eye")[:2], centre[:2], atol=1e-6)
+    empty = P.PreparedCamera(P.build(P.Scene3D())).resolve(10, 10)
+    np.testing.assert_allclose(_vec(empty, "eye"), [0, 0, 0.6 / math.tan(math.radians(30))], rtol=1e-6)
