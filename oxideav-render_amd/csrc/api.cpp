@@ -678,6 +678,14 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   sc.has_blend = has_blend ? 1u : 0u;
   sc.has_tex = has_tex ? 1u : 0u;
   sc.has_alpha_tex = has_alpha_tex ? 1u : 0u;
+  // Any factor that is not exactly zero (NaN included) selects the
+  // extension-lobe shading; a texture only scales a factor.
+  bool has_ext = false;
+  for (const DevMaterial& m : mats)
+    has_ext = has_ext || !(m.transmission == 0.0f) || !(m.clearcoat == 0.0f) || !(m.sheen[0] == 0.0f) ||
+              !(m.sheen[1] == 0.0f) || !(m.sheen[2] == 0.0f) || !(m.iridescence == 0.0f) || !(m.dt == 0.0f) ||
+              !(m.aniso == 0.0f);
+  sc.has_ext = has_ext ? 1u : 0u;
   sc.tex.chains = d_chains;
   sc.tex.levels = d_levels;
   sc.tex.texels = d_texels;
@@ -1045,26 +1053,26 @@ struct Engine {
     span_begin(e1);
     launch::closest(stream, alpha_sc ? grid_closest_alpha : grid_closest, sc_any, d_jobs, pb.pjob, true, multi,
                     pb.qray[cur], pb.qpid[cur], pb.meta, 0,
-                    &d_ctrl->qcount[cur], &d_ctrl->work[0], 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
+                    &d_ctrl->qcount[cur].v, &d_ctrl->work[0].v, 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
                     xcd_segments, alpha_sc);
     span_end(0, e1);
     if (gb && k < gb->use.kmax) {  // the guide serves the first kmax bounces
       hipEvent_t eg{};
       span_begin(eg);
-      glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur], P, nullptr, pb.qray[cur], pb.hits,
+      glaunch::infer(stream, gb->net, 0u, &d_ctrl->qcount[cur].v, P, nullptr, pb.qray[cur], pb.hits,
                      const_cast<GuideLobes*>(gb->use.lobes), nullptr);
       span_end(4, eg);
     }
     hipEvent_t e2{};
     span_begin(e2);
     launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr,
-                  tex_variant, multi_jobs);
+                  tex_variant, multi_jobs, ext_variant);
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);
     launch::anyhit(stream, alpha_sc ? grid_any_alpha : grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray,
                    pb.sidx, pb.meta, nslots, 0,
-                   &d_ctrl->scount, &d_ctrl->work[8], pb.vis, d_spill, spill_cap, &d_ctrl->error,
+                   &d_ctrl->scount.v, &d_ctrl->work[8].v, pb.vis, d_spill, spill_cap, &d_ctrl->error,
                    xcd_segments, alpha_sc);
     span_end(1, e3);
   }
@@ -1087,7 +1095,7 @@ struct Engine {
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
     key.alpha_sc = alpha_sc;
-    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u);
+    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u) | (ext_variant ? 4u : 0u);
     if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
@@ -1128,13 +1136,15 @@ struct Engine {
   // the textured-alpha traversal (whose single-scene launches read the
   // scene's device copy through alpha_sc).
   bool tex_variant = false;
+  bool ext_variant = true;
   const DevScene* alpha_sc = nullptr;
   bool multi_jobs = true;  // the batch holds paths of more than one job
 
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
                  const std::vector<Reduce>& red, const DevScene& sc_any, bool multi, int& rc,
-                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr) {
+                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr, bool ext = true) {
     tex_variant = tex;
+    ext_variant = ext;
     alpha_sc = asc;
     multi_jobs = false;  // every segment of job slot 0: the single-job shading variant
     for (uint32_t q = 0; q < segs.n; ++q) multi_jobs = multi_jobs || segs.s[q].job != 0u;
@@ -1153,7 +1163,7 @@ struct Engine {
         // Long paths: stop launching once every path has ended.
         if (k >= 8 && (k % 4) == 0) {
           uint32_t left = 0;
-          if (hipMemcpyAsync(&left, &d_ctrl->qcount[cur], sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
+          if (hipMemcpyAsync(&left, &d_ctrl->qcount[cur].v, sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
                   hipSuccess ||
               hipStreamSynchronize(stream) != hipSuccess) {
             rc = fail(OXR_ERR_DEVICE, "queue readback failed");
@@ -1567,7 +1577,7 @@ int oxr_pt::refine(uint32_t n) {
       rc = OXR_OK;
       if (gb && gb->use.rec) glaunch::fill(eng.stream, reinterpret_cast<float*>(guide->d_rec), rec_floats);
       eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, false, rc, gb, scene.sc.has_tex != 0,
-                    scene.sc.has_alpha_tex ? scene.d_self : nullptr);
+                    scene.sc.has_alpha_tex ? scene.d_self : nullptr, scene.sc.has_ext != 0);
       if (rc) return rc;
     }
   }
@@ -1738,16 +1748,17 @@ struct oxr_srv {
       if ((rc = eng.ensure_spill(spill))) return rc;
       eng.begin_stats();
       rc = OXR_OK;
-      bool multi = false, tex = false;
+      bool multi = false, tex = false, ext = false;
       const DevScene* asc = nullptr;  // a mixed batch reads each path's scene through its job
       for (auto& [id, S_] : took) {
         multi = multi || jobs[id].scene != jobs[took[0].first].scene;
         const SceneBuf& sb = *scenes[jobs[id].scene];
         tex = tex || sb.sc.has_tex;
+        ext = ext || sb.sc.has_ext;
         if (sb.sc.has_alpha_tex && !asc) asc = sb.d_self;
       }
       eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, multi, rc, nullptr,
-                    tex, asc);
+                    tex, asc, ext);
       if (rc) return rc;
       if ((rc = eng.end_stats())) return rc;
       st.rounds++;
@@ -2167,10 +2178,10 @@ static int trace_host(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint32_t flags
   launch::zero_ctrl(e.stream, e.d_ctrl);
   if (occ)
     launch::anyhit(e.stream, e.grid_any, h->scene.sc, nullptr, nullptr, false, false, d_rays, nullptr, nullptr, 1, n, nullptr,
-                   &e.d_ctrl->work[8], (uint8_t*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
+                   &e.d_ctrl->work[8].v, (uint8_t*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   else
     launch::closest(e.stream, e.grid_closest, h->scene.sc, nullptr, nullptr, false, false, d_rays, nullptr, nullptr, n,
-                    nullptr, &e.d_ctrl->work[0], flags & 1u, (uint4*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
+                    nullptr, &e.d_ctrl->work[0].v, flags & 1u, (uint4*)d_out, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(occ ? (void*)occ : (void*)hits, d_out, obytes, hipMemcpyDeviceToHost, e.stream));
   Ctrl c{};
@@ -2201,7 +2212,7 @@ int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays, void* d_
   hipStream_t s = stream ? (hipStream_t)stream : e.stream;
   launch::zero_ctrl(s, e.d_ctrl);
   launch::closest(s, e.grid_closest, h->scene.sc, nullptr, nullptr, false, false, (const float4*)d_rays, nullptr, nullptr, n,
-                  nullptr, &e.d_ctrl->work[0], 0, (uint4*)d_hits, e.d_spill, e.spill_cap, &e.d_ctrl->error);
+                  nullptr, &e.d_ctrl->work[0].v, 0, (uint4*)d_hits, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
   return OXR_OK;
 }

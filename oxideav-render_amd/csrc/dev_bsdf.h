@@ -264,7 +264,14 @@ struct Microfacet {
   }
 };
 
-struct Bsdf {
+// EXT = false: the scene has no material with clearcoat, sheen,
+// iridescence, transmission, diffuse transmission or anisotropy. Those
+// factors are then compile-time zeros: every branch they guard takes the
+// way a zero factor takes at run time, and their lobe probabilities, which
+// a zero factor makes exactly 0, are constants, so the code and the state
+// they need drop out and the results stay bit for bit those of EXT = true.
+template <bool EXT>
+struct BsdfT {
   V3 v, n, ng, ncc, base;
   float metallic;
   Microfacet mf;
@@ -303,13 +310,13 @@ struct Bsdf {
     float r0 = powi2((ior - 1.0f) / (ior + 1.0f));
     base = m.base;
     float alpha = fmaxr(m.roughness * m.roughness, MIN_ALPHA);
-    float aniso = clampr(m.anisotropy, 0.0f, 1.0f);
+    float aniso = EXT ? clampr(m.anisotropy, 0.0f, 1.0f) : 0.0f;
     basis(n, mf.t, mf.b);
     mf.n = n;
     mf.at = alpha;
     mf.ab = alpha;
     mf.aniso = false;
-    if (aniso > 0.0f) {
+    if (EXT && aniso > 0.0f) {
       V3 d0 = m.anisotropy_direction;
       float dn = dot(d0, n);
       V3 t = normalise(v3(d0.x - n.x * dn, d0.y - n.y * dn, d0.z - n.z * dn));
@@ -328,22 +335,28 @@ struct Bsdf {
     specular = clampr(m.specular, 0.0f, 1.0f);
     f0d = v3(fminr(r0 * m.specular_color.x, 1.0f), fminr(r0 * m.specular_color.y, 1.0f),
              fminr(r0 * m.specular_color.z, 1.0f));
-    transmission = m.transmission;
-    clearcoat = inside ? 0.0f : m.clearcoat;
-    cc_alpha = fmaxr(m.clearcoat_roughness * m.clearcoat_roughness, MIN_ALPHA);
-    sheen = inside ? v3(0.0f, 0.0f, 0.0f) : m.sheen_color;
-    sheen_rough = m.sheen_roughness;
-    sheen_alpha = fmaxr(m.sheen_roughness * m.sheen_roughness, 1.0e-3f);
-    irid = inside ? 0.0f : clampr(m.iridescence, 0.0f, 1.0f);
-    irid_ior = isfinite(m.iridescence_ior) ? fmaxr(m.iridescence_ior, 1.0f) : 1.3f;
-    irid_thick = isfinite(m.iridescence_thickness) ? fmaxr(m.iridescence_thickness, 0.0f) : 400.0f;
-    dt = clampr(m.diffuse_transmission, 0.0f, 1.0f);
-    dt_color = v3(clampr(m.diffuse_transmission_color.x, 0.0f, 1.0f), clampr(m.diffuse_transmission_color.y, 0.0f, 1.0f),
-                  clampr(m.diffuse_transmission_color.z, 0.0f, 1.0f));
+    if (EXT) {
+      transmission = m.transmission;
+      clearcoat = inside ? 0.0f : m.clearcoat;
+      cc_alpha = fmaxr(m.clearcoat_roughness * m.clearcoat_roughness, MIN_ALPHA);
+      sheen = inside ? v3(0.0f, 0.0f, 0.0f) : m.sheen_color;
+      sheen_rough = m.sheen_roughness;
+      sheen_alpha = fmaxr(m.sheen_roughness * m.sheen_roughness, 1.0e-3f);
+      irid = inside ? 0.0f : clampr(m.iridescence, 0.0f, 1.0f);
+      irid_ior = isfinite(m.iridescence_ior) ? fmaxr(m.iridescence_ior, 1.0f) : 1.3f;
+      irid_thick = isfinite(m.iridescence_thickness) ? fmaxr(m.iridescence_thickness, 0.0f) : 400.0f;
+      dt = clampr(m.diffuse_transmission, 0.0f, 1.0f);
+      dt_color = v3(clampr(m.diffuse_transmission_color.x, 0.0f, 1.0f),
+                    clampr(m.diffuse_transmission_color.y, 0.0f, 1.0f),
+                    clampr(m.diffuse_transmission_color.z, 0.0f, 1.0f));
+    } else {
+      transmission = clearcoat = cc_alpha = sheen_rough = sheen_alpha = irid = irid_ior = irid_thick = dt = 0.0f;
+      sheen = dt_color = v3(0.0f, 0.0f, 0.0f);
+    }
     float cv = fmaxr(dot(n, v), 1.0e-4f);
-    w_cc = clearcoat > 0.0f ? clearcoat * schlick1(0.04f, dot(ncc, v)) : 0.0f;
+    w_cc = EXT && clearcoat > 0.0f ? clearcoat * schlick1(0.04f, dot(ncc, v)) : 0.0f;
     float sheen_max = max3(sheen);
-    float e_v = sheen_max > 0.0f ? sheen_albedo(cv, sheen_rough) : 0.0f;
+    float e_v = EXT && sheen_max > 0.0f ? sheen_albedo(cv, sheen_rough) : 0.0f;
     float s_scale = (1.0f - w_cc) * (1.0f - sheen_max * e_v);
     V3 fd_v, fm_v;
     fresnel_pair(cv, fd_v, fm_v);
@@ -362,10 +375,12 @@ struct Bsdf {
     if (sum > 0.0f && isfinite(sum)) {
       probs[0] = p0 / sum;
       probs[1] = p1 / sum;
-      probs[2] = p2 / sum;
-      probs[3] = p3 / sum;
-      probs[4] = p4 / sum;
-      probs[5] = p5 / sum;
+      // p2..p5 carry a zero factor (transmission, w_cc, sheen, dt) when
+      // !EXT: exactly 0 over a finite positive sum.
+      probs[2] = EXT ? p2 / sum : 0.0f;
+      probs[3] = EXT ? p3 / sum : 0.0f;
+      probs[4] = EXT ? p4 / sum : 0.0f;
+      probs[5] = EXT ? p5 / sum : 0.0f;
     } else {
 #pragma unroll
       for (int i = 0; i < LOBES; ++i) probs[i] = 0.0f;
@@ -390,7 +405,7 @@ struct Bsdf {
   OXR_D void fresnel_pair(float c, V3& fd_out, V3& fm_out) const {
     V3 fd = scale(fresnel_d(c), specular);
     V3 fm = schlick(base, c);
-    if (irid <= 0.0f) {
+    if (!EXT || irid <= 0.0f) {
       fd_out = fd;
       fm_out = fm;
       return;
@@ -421,7 +436,7 @@ struct Bsdf {
                       spec_c * ((1.0f - m) * fd.y + m * fm.y) + diff_k * base.y,
                       spec_c * ((1.0f - m) * fd.z + m * fm.z) + diff_k * base.z);
         float smax = max3(sheen);
-        if (smax > 0.0f) {
+        if (EXT && smax > 0.0f) {
           float sc = fminr(1.0f - smax * sheen_albedo(nv, sheen_rough), 1.0f - smax * sheen_albedo(nl, sheen_rough));
           float nh = dot(n, h);
           float sh = d_charlie(nh, sheen_alpha) * v_neubelt(nl, nv);
@@ -431,7 +446,7 @@ struct Bsdf {
         pdf += (probs[LOBE_DIFFUSE] + probs[LOBE_SHEEN]) * nl / PI_F;
         pdf += probs[LOBE_SPECULAR] * mf.pdf_reflect(v, h);
       }
-      if (clearcoat > 0.0f) {
+      if (EXT && clearcoat > 0.0f) {
         float cnv = fmaxr(dot(ncc, v), 1.0e-4f);
         float cnl = dot(ncc, l);
         if (cnl > 0.0f) {
@@ -447,7 +462,7 @@ struct Bsdf {
     }
     f_out = v3(0.0f, 0.0f, 0.0f);
     pdf_out = 0.0f;
-    if (metallic >= 1.0f || (transmission <= 0.0f && dt <= 0.0f)) return;
+    if (!EXT || metallic >= 1.0f || (transmission <= 0.0f && dt <= 0.0f)) return;
     float nl = -dot(n, l);
     if (nl <= 0.0f) return;
     float smax = max3(sheen);
@@ -519,13 +534,13 @@ struct Bsdf {
     float acc = 0.0f;
     int lobe = -1;
 #pragma unroll
-    for (int i = 0; i < LOBES; ++i) {
+    for (int i = 0; i < (EXT ? LOBES : 2); ++i) {
       acc += probs[i];
       if (lobe < 0 && probs[i] > 0.0f && u2 < acc) lobe = i;
     }
     if (lobe < 0) {
 #pragma unroll
-      for (int i = 0; i < LOBES; ++i)
+      for (int i = 0; i < (EXT ? LOBES : 2); ++i)
         if (probs[i] > 0.0f) lobe = i;
       if (lobe < 0) return false;
     }
@@ -536,7 +551,7 @@ struct Bsdf {
       V3 d = cosine_hemisphere(u0, u1);
       if (lobe == LOBE_DIFFUSE_T) d.z = -d.z;
       l = to_world(t, b, n, d);
-    } else if (lobe == LOBE_CLEARCOAT) {
+    } else if (EXT && lobe == LOBE_CLEARCOAT) {
       V3 t, b;
       basis(ncc, t, b);
       V3 vl = v3(dot(v, t), dot(v, b), dot(v, ncc));
@@ -546,7 +561,7 @@ struct Bsdf {
     } else {
       V3 h;
       if (!mf.sample(v, u0, u1, h)) return false;
-      if (lobe == LOBE_SPECULAR) {
+      if (!EXT || lobe == LOBE_SPECULAR) {
         l = reflect(v, h);
       } else if (thin) {
         V3 r = reflect(v, h);
@@ -570,6 +585,8 @@ struct Bsdf {
     return true;
   }
 };
+
+using Bsdf = BsdfT<true>;
 
 // ---- spherical triangles (pathtrace.rs:1749-1871), f64 inside ----------
 struct D3 {

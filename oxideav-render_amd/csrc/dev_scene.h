@@ -67,7 +67,7 @@ struct DevScene {
   uint32_t has_blend;
   uint32_t has_tex;        // textures, vertex colours or tangents: the TEX shading variant
   uint32_t has_alpha_tex;  // MASK / BLEND alpha varies over triangles: the ALPHA traversal variant
-  uint32_t pad;
+  uint32_t has_ext;        // a material has an extension lobe: the EXT shading variant
   DevTextures tex;
 };
 
@@ -135,11 +135,21 @@ struct SegTable {
 };
 
 // Device control block: queue counters, work counters, error, stats.
+// A counter alone in its 256-byte block. Device-scope atomics on one
+// address (and on one line) serialise device-wide, about 11 ns apiece on
+// MI355X; counters that one kernel bumps concurrently (the two append
+// queues, the traversal segments) therefore live on lines of their own.
+struct alignas(256) PadCounter {
+  uint32_t v;
+  uint32_t pad_[63];
+};
+constexpr uint32_t PAD_COUNTER_WORDS = sizeof(PadCounter) / sizeof(uint32_t);
+
 struct Ctrl {
-  uint32_t qcount[2];
-  uint32_t scount;
+  PadCounter qcount[2];
+  PadCounter scount;
+  PadCounter work[16];  // traversal work counters: closest [0, 8), any-hit [8, 16), one per queue segment
   uint32_t error;
-  uint32_t work[16];  // traversal work counters: closest [0, 8), any-hit [8, 16), one per XCD segment
   unsigned long long rays_closest;
   unsigned long long rays_shadow;
 };

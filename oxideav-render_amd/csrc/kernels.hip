@@ -52,6 +52,43 @@ OXR_D uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + (uint32_t)__popcll(below);
 }
 
+// Shadow-ray staging of a wave (all 64 lanes present, wave-uniform
+// `staged`): an emitting lane stores its ray at the wave's next free
+// entries in LDS; flush_shadow appends them with one atomic and copies them
+// out, contiguous and coalesced.
+OXR_D void stage_shadow(float4* sa, float4* sb, uint32_t* si, uint32_t wbase, uint32_t& staged, bool emit,
+                        float4 a, float4 b, uint32_t slot) {
+  const uint64_t m = __ballot(emit);
+  const uint32_t lane = lane_id();
+  if (emit) {
+    const uint64_t below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    const uint32_t e = wbase + staged + (uint32_t)__popcll(below);
+    sa[e] = a;
+    sb[e] = b;
+    si[e] = slot;
+  }
+  staged += (uint32_t)__popcll(m);
+}
+
+OXR_D void flush_shadow(const float4* sa, const float4* sb, const uint32_t* si, uint32_t wbase, uint32_t& staged,
+                        uint32_t* counter, float4* __restrict__ sray, uint32_t* __restrict__ sidx) {
+  if (staged == 0u) return;
+  const uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0u) base = atomicAdd(counter, staged);
+  base = __shfl(base, 0);
+  // The staging stores of other lanes come first: LDS operations of a wave
+  // complete in order; the fence keeps the compiler from reordering them.
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  for (uint32_t e = lane; e < staged; e += 64u) {
+    const uint32_t q = base + e;
+    sray[2 * (size_t)q] = sa[wbase + e];
+    sray[2 * (size_t)q + 1] = sb[wbase + e];
+    sidx[q] = si[wbase + e];
+  }
+  staged = 0;
+}
+
 // ---------------------------------------------------------------------
 // BVH traversal (TraceScene::closest_hit_filtered / occluded_filtered,
 // src/trace.rs:405-450), 4-wide nodes, per-thread stack: a 32-entry ring
@@ -262,7 +299,7 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, 
   const uint32_t wave = (blockIdx.x / nseg) * (TRACE_TB / 64) + threadIdx.x / 64;
   const uint32_t s0 = (uint32_t)((uint64_t)count * seg / nseg);
   const uint32_t s1 = (uint32_t)((uint64_t)count * (seg + 1) / nseg);
-  uint32_t* wk = work + seg;
+  uint32_t* wk = work + (size_t)seg * PAD_COUNTER_WORDS;
   Trav<ANY, ALPHA> tr;
   tr.asc = nullptr;
   uint32_t ri = s0 + wave * 64u + lane;
@@ -448,7 +485,7 @@ __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_
                          uint32_t* __restrict__ qpid, float4* __restrict__ beta, float4* __restrict__ rad,
                          float4* __restrict__ sigma, uint4* __restrict__ meta, uint32_t* __restrict__ pjob, Ctrl* ctrl) {
   uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid == 0) ctrl->qcount[0] = P;
+  if (pid == 0) ctrl->qcount[0].v = P;
   if (pid >= P) return;
   uint32_t si = 0;
   while (si + 1 < segs.n && segs.s[si + 1].path0 <= pid) ++si;
@@ -484,12 +521,12 @@ __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_
 
 __global__ void k_prep(Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
   if (threadIdx.x == 0) {
-    ctrl->rays_closest += ctrl->qcount[cur];
-    if (count_shadow_prev) ctrl->rays_shadow += ctrl->scount;
-    ctrl->qcount[cur ^ 1u] = 0;
-    ctrl->scount = 0;
+    ctrl->rays_closest += ctrl->qcount[cur].v;
+    if (count_shadow_prev) ctrl->rays_shadow += ctrl->scount.v;
+    ctrl->qcount[cur ^ 1u].v = 0;
+    ctrl->scount.v = 0;
   }
-  if (threadIdx.x < 16) ctrl->work[threadIdx.x] = 0;
+  if (threadIdx.x < 16) ctrl->work[threadIdx.x].v = 0;
 }
 
 // Adds the visible NEE contributions of the previous vertex in light
@@ -698,7 +735,18 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // MULTI: the batch mixes jobs (each path's job through pjob); else every
 // path belongs to job 0, whose scene and frame the wave reads with uniform
 // (scalar) loads instead of a per-lane dependent lookup.
-template <int WAVES, bool GUIDE, bool TEX, bool MULTI>
+// EXT = false: no material of the batch has an extension lobe (clearcoat,
+// sheen, iridescence, transmission, diffuse transmission, anisotropy), so
+// no path ever enters a medium: the medium state stays zero and is neither
+// read nor written, and the BSDF drops those lobes (BsdfT<false>).
+// Shadow rays of a single-job batch with at most STAGE_SLOTS slots per path
+// (punctual lights + the area sample) are staged per wave in LDS and
+// appended with one atomic per wave and round: appends to the one shadow
+// queue counter serialise device-wide, and one per light made them the
+// shading kernel's bottleneck.
+constexpr uint32_t STAGE_SLOTS = 4;
+
+template <int WAVES, bool GUIDE, bool TEX, bool MULTI, bool EXT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
                                              const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
@@ -709,10 +757,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                                              uint4* __restrict__ meta_s, float4* __restrict__ nee,
                                              uint8_t* __restrict__ vis, Ctrl* ctrl, uint32_t cur,
                                              DevGuideUse gu) {
-  const uint32_t count = ctrl->qcount[cur];
+  __shared__ float4 s_sa[MULTI ? 1 : STAGE_SLOTS * 256], s_sb[MULTI ? 1 : STAGE_SLOTS * 256];
+  __shared__ uint32_t s_si[MULTI ? 1 : STAGE_SLOTS * 256];
+  const bool stage = !MULTI && nslots <= STAGE_SLOTS;
+  const uint32_t wbase = (threadIdx.x / 64u) * STAGE_SLOTS * 64u;  // this wave's staging region
+  const uint32_t count = ctrl->qcount[cur].v;
   const uint32_t rec_mask = GUIDE ? (1u << gu.shift) - 1u : 0u;
-  uint32_t* ncount = &ctrl->qcount[cur ^ 1u];
-  uint32_t* scount = &ctrl->scount;
+  uint32_t* ncount = &ctrl->qcount[cur ^ 1u].v;
+  uint32_t* scount = &ctrl->scount.v;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   // Uniform trip count so every lane reaches the wave-level appends.
@@ -739,6 +791,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       }
     }
     bool alive = valid;
+    uint32_t staged = 0;  // shadow rays staged by this wave this round (wave-uniform)
     const DevJob& job = jobs[MULTI ? (valid ? pjob[pid] : 0u) : 0u];
     const DevScene& sc = job.sc;
     const DevFrame& fr = job.fr;
@@ -751,7 +804,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     V3 beta = f4xyz(b4);
     float prev_pdf = b4.w;
     V3 radiance = valid ? f4xyz(rad_s[pid]) : v3(0, 0, 0);
-    float4 sg = valid ? sigma_s[pid] : make_float4(0, 0, 0, 0);
+    float4 sg = (EXT && valid) ? sigma_s[pid] : make_float4(0, 0, 0, 0);
     float4 pv4 = valid ? prev_s[pid] : make_float4(0, 0, 0, 0);
     V3 prev_pos = f4xyz(pv4);
     // Roulette throughput scale of guided paths: the product of
@@ -772,7 +825,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     bool miss = h.y == HIT_MISS;
     float t = __uint_as_float(h.x);
     // Beer-Lambert inside a medium; an escaping ray ends the path.
-    if (alive && sg.w != 0.0f) {
+    if (EXT && alive && sg.w != 0.0f) {
       if (miss) {
         alive = false;
       } else {
@@ -869,8 +922,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         tex_apply(sc.tex, run, g, attr, bary, sc.pos + 9ull * g, cone, flip ? neg(sn) : sn, flip, *M, ms);
       }
     }
-    float thickness = M->thickness, transmission = M->transmission;
-    if (alive && M->dispersion > 0.0f && thickness > 0.0f && transmission > 0.0f) {
+    const float thickness = M->thickness, transmission = EXT ? M->transmission : 0.0f;
+    if (EXT && alive && M->dispersion > 0.0f && thickness > 0.0f && transmission > 0.0f) {
       uint32_t c;
       if (hero != 0xffu) {
         c = hero;
@@ -884,7 +937,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       }
       ms.ior = dispersion_ior(ms.ior, M->dispersion, (int)c);
     }
-    Bsdf bsdf;
+    BsdfT<EXT> bsdf;
     bsdf.init(ms, neg(d), ngeo, front);
     if (alive && bsdf.black()) alive = false;
     V3 ng = bsdf.ng;
@@ -895,7 +948,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     const GuideLobes* GL = nullptr;
     if (GUIDE) {
       GL = gu.lobes + i;
-      if (alive && k < gu.kmax && transmission == 0.0f && M->dt == 0.0f &&
+      if (alive && k < gu.kmax && transmission == 0.0f && (!EXT || M->dt == 0.0f) &&
           !(M->metallic >= 0.99f && M->roughness < 0.2f))
         gmix = gu.mix;
     }
@@ -944,12 +997,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
           }
         }
       }
-      uint32_t pos = wave_append(scount, emit);
+      const float4 ra = make_float4(o2.x, o2.y, o2.z, __uint_as_float(NO_SKIP));
+      const float4 rb = make_float4(l.x, l.y, l.z, tmax);
+      const size_t slot = (size_t)pid * nslots + j;
+      if (stage) {
+        stage_shadow(s_sa, s_sb, s_si, wbase, staged, emit, ra, rb, (uint32_t)slot);
+      } else {
+        uint32_t pos = wave_append(scount, emit);
+        if (emit) {
+          sray[2 * pos] = ra;
+          sray[2 * pos + 1] = rb;
+          sidx[pos] = (uint32_t)slot;
+        }
+      }
       if (emit) {
-        size_t slot = (size_t)pid * nslots + j;
-        sray[2 * pos] = make_float4(o2.x, o2.y, o2.z, __uint_as_float(NO_SKIP));
-        sray[2 * pos + 1] = make_float4(l.x, l.y, l.z, tmax);
-        sidx[pos] = (uint32_t)slot;
         nee[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
         vis[slot] = 1;
         any_shadow = true;
@@ -1053,16 +1114,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
           }
         }
       }
-      uint32_t pos = wave_append(scount, emit);
+      const float4 ra = make_float4(o2.x, o2.y, o2.z, __uint_as_float(skip));
+      const float4 rb = make_float4(l.x, l.y, l.z, tmax);
+      const size_t slot = (size_t)pid * nslots + (nslots - 1);
+      if (stage) {
+        stage_shadow(s_sa, s_sb, s_si, wbase, staged, emit, ra, rb, (uint32_t)slot);
+      } else {
+        uint32_t pos = wave_append(scount, emit);
+        if (emit) {
+          sray[2 * pos] = ra;
+          sray[2 * pos + 1] = rb;
+          sidx[pos] = (uint32_t)slot;
+        }
+      }
       if (emit) {
-        size_t slot = (size_t)pid * nslots + (nslots - 1);
-        sray[2 * pos] = make_float4(o2.x, o2.y, o2.z, __uint_as_float(skip));
-        sray[2 * pos + 1] = make_float4(l.x, l.y, l.z, tmax);
-        sidx[pos] = (uint32_t)slot;
         nee[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
         vis[slot] = 1;
         any_shadow = true;
       }
+      // The round's staged shadow rays go out in one append.
+      if (stage) flush_shadow(s_sa, s_sb, s_si, wbase, staged, scount, sray, sidx);
     }
     if (any_shadow) flags |= PF_PENDING;
     // BSDF sampling, medium transitions, roulette (src/pathtrace.rs:2281-2322).
@@ -1150,7 +1221,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       beta_s[pid] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
       rad_s[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
       prev_s[pid] = make_float4(prev_pos.x, prev_pos.y, prev_pos.z, GUIDE ? rr_scale : 0.0f);
-      sigma_s[pid] = sg;
+      if (EXT) sigma_s[pid] = sg;
       meta_s[pid] = make_uint4(meta.x, meta.y, ((k + (cont ? 1u : 0u)) & 0xffffu) | (hero << 16) | (flags << 24), ps);
     }
   }
@@ -1309,13 +1380,13 @@ __global__ void k_reduce(uint32_t npx, uint32_t S, const float4* __restrict__ ou
 
 __global__ void k_zero_ctrl(Ctrl* c) {
   if (threadIdx.x == 0) {
-    c->qcount[0] = c->qcount[1] = 0;
-    c->scount = 0;
+    c->qcount[0].v = c->qcount[1].v = 0;
+    c->scount.v = 0;
     c->error = 0;
     c->rays_closest = 0;
     c->rays_shadow = 0;
   }
-  if (threadIdx.x < 16) c->work[threadIdx.x] = 0;
+  if (threadIdx.x < 16) c->work[threadIdx.x].v = 0;
 }
 
 __global__ void k_fill_f64(double* p, size_t n) {
@@ -1614,31 +1685,46 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 }
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi) {
+           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, bool ext) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
-#define OXR_SHADE(W, G, T, M)                                                                                    \
-  hipLaunchKernelGGL((k_shade<W, G, T, M>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],       \
+#define OXR_SHADE(W, G, T, M, X)                                                                                 \
+  hipLaunchKernelGGL((k_shade<W, G, T, M, X>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],    \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
                      pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
   // Mixed batches (the server) and textured or guided scenes run at one
   // wave per SIMD; the occupancy variants are for plain single-scene
-  // shading.
+  // shading. Each has a variant without the extension lobes.
   if (multi && tex)
-    OXR_SHADE(1, false, true, true);
+    OXR_SHADE(1, false, true, true, true);
+  else if (multi && ext)
+    OXR_SHADE(1, false, false, true, true);
   else if (multi)
-    OXR_SHADE(1, false, false, true);
+    OXR_SHADE(1, false, false, true, false);
   else if (guide && tex)
-    OXR_SHADE(1, true, true, false);
+    OXR_SHADE(1, true, true, false, true);
+  else if (guide && ext)
+    OXR_SHADE(1, true, false, false, true);
   else if (guide)
-    OXR_SHADE(1, true, false, false);
+    OXR_SHADE(1, true, false, false, false);
+  else if (tex && ext)
+    OXR_SHADE(1, false, true, false, true);
   else if (tex)
-    OXR_SHADE(1, false, true, false);
-  else if (waves >= 3)
-    OXR_SHADE(3, false, false, false);
-  else if (waves == 2)
-    OXR_SHADE(2, false, false, false);
-  else
-    OXR_SHADE(1, false, false, false);
+    OXR_SHADE(1, false, true, false, false);
+  else if (ext) {
+    if (waves >= 3)
+      OXR_SHADE(3, false, false, false, true);
+    else if (waves == 2)
+      OXR_SHADE(2, false, false, false, true);
+    else
+      OXR_SHADE(1, false, false, false, true);
+  } else {
+    if (waves >= 3)
+      OXR_SHADE(3, false, false, false, false);
+    else if (waves == 2)
+      OXR_SHADE(2, false, false, false, false);
+    else
+      OXR_SHADE(1, false, false, false, false);
+  }
 #undef OXR_SHADE
 }
 

@@ -241,6 +241,25 @@ def test_full_bounce_images_agree_within_3_sigma(oxr_mod, oracle, make, W, spp, 
     assert (z > 3).mean() < 0.01
 
 
+@pytest.mark.parametrize("make", [scenes.cornell, scenes.reference_box,
+                                  lambda: scenes.material_sweep(segments=32, rings=16, floor_half=6)],
+                         ids=["cornell", "reference_box", "sweep_small"])
+def test_shading_without_extension_lobes_is_bit_identical(oxr_mod, make):
+    """Scenes whose materials have no clearcoat, sheen, iridescence,
+    transmission, diffuse transmission or anisotropy shade with the variant
+    that leaves those lobes out; one unused extension material in the table
+    selects the full variant. Both accumulate the same bits."""
+    s = make()
+    _, a, sa = _render_gpu(oxr_mod, s, 40, 8, 8)
+    full = make()
+    full.add_material(A.material((1.0, 1.0, 1.0, 1.0), clearcoat=1.0, sheen_color=(0.5, 0.5, 0.5), transmission=0.5,
+                                 diffuse_transmission=0.3, iridescence=1.0))
+    assert full.num_tris == s.num_tris
+    _, b, sb = _render_gpu(oxr_mod, full, 40, 8, 8)
+    assert np.array_equal(a, b)
+    assert np.array_equal(sa, sb)
+
+
 def test_refinement_is_additive_and_deterministic(oxr_mod):
     s = scenes.area_light_floor(0.5, 0.0)
     pt, a, _ = _render_gpu(oxr_mod, s, 9, 8, 4)
