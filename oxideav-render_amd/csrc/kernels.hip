@@ -52,41 +52,66 @@ OXR_D uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + (uint32_t)__popcll(below);
 }
 
-// Shadow-ray staging of a wave (all 64 lanes present, wave-uniform
-// `staged`): an emitting lane stores its ray at the wave's next free
-// entries in LDS; flush_shadow appends them with one atomic and copies them
-// out, contiguous and coalesced.
-OXR_D void stage_shadow(float4* sa, float4* sb, uint32_t* si, uint32_t wbase, uint32_t& staged, bool emit,
-                        float4 a, float4 b, uint32_t slot) {
-  const uint64_t m = __ballot(emit);
+// Queue staging of the shading kernel. Appends to one queue counter are
+// device-scope atomics that serialise device-wide, so a block gathers its
+// round's queue entries in LDS and appends them with one atomic per queue.
+// stage_entry: an entering lane stores its entry at the wave's next free
+// staging entries (all 64 lanes present, `staged` wave-uniform).
+OXR_D void stage_entry(float4* sa, float4* sb, uint32_t* si, uint32_t wbase, uint32_t& staged, bool enter,
+                       float4 a, float4 b, uint32_t idx) {
+  const uint64_t m = __ballot(enter);
   const uint32_t lane = lane_id();
-  if (emit) {
+  if (enter) {
     const uint64_t below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
     const uint32_t e = wbase + staged + (uint32_t)__popcll(below);
     sa[e] = a;
     sb[e] = b;
-    si[e] = slot;
+    si[e] = idx;
   }
   staged += (uint32_t)__popcll(m);
 }
 
-OXR_D void flush_shadow(const float4* sa, const float4* sb, const uint32_t* si, uint32_t wbase, uint32_t& staged,
-                        uint32_t* counter, float4* __restrict__ sray, uint32_t* __restrict__ sidx) {
-  if (staged == 0u) return;
-  const uint32_t lane = lane_id();
-  uint32_t base = 0;
-  if (lane == 0u) base = atomicAdd(counter, staged);
-  base = __shfl(base, 0);
-  // The staging stores of other lanes come first: LDS operations of a wave
-  // complete in order; the fence keeps the compiler from reordering them.
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  for (uint32_t e = lane; e < staged; e += 64u) {
-    const uint32_t q = base + e;
-    sray[2 * (size_t)q] = sa[wbase + e];
-    sray[2 * (size_t)q + 1] = sb[wbase + e];
-    sidx[q] = si[wbase + e];
+// The block's staged entries of two queues (shadow rays and next rays) go
+// out with one atomic per queue: every thread of the block calls this once
+// per round. s_cnt / s_base: 8 words each.
+OXR_D void block_flush(uint32_t* s_cnt, uint32_t* s_base, const float4* sa, const float4* sb, const uint32_t* si,
+                       uint32_t sw, uint32_t& sstaged, uint32_t* scount, float4* __restrict__ sray,
+                       uint32_t* __restrict__ sidx, const float4* na, const float4* nb, const uint32_t* np,
+                       uint32_t nw, uint32_t& nstaged, uint32_t* ncount, float4* __restrict__ nray,
+                       uint32_t* __restrict__ npid) {
+  const uint32_t w = threadIdx.x / 64u, lane = lane_id();
+  if (lane == 0u) {
+    s_cnt[w] = sstaged;
+    s_cnt[4 + w] = nstaged;
   }
-  staged = 0;
+  __syncthreads();
+  if (lane == 0u && w < 2u) {  // wave 0: shadow queue, wave 1: next-ray queue
+    const uint32_t* c = s_cnt + 4u * w;
+    const uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    const uint32_t b = tot ? atomicAdd(w == 0u ? scount : ncount, tot) : 0u;
+    uint32_t* o = s_base + 4u * w;
+    o[0] = b;
+    o[1] = b + c0;
+    o[2] = b + c0 + c1;
+    o[3] = b + c0 + c1 + c2;
+  }
+  __syncthreads();
+  const uint32_t bs = s_base[w], bn = s_base[4 + w];
+  for (uint32_t e = lane; e < sstaged; e += 64u) {
+    const uint32_t q = bs + e;
+    sray[2 * (size_t)q] = sa[sw + e];
+    sray[2 * (size_t)q + 1] = sb[sw + e];
+    sidx[q] = si[sw + e];
+  }
+  for (uint32_t e = lane; e < nstaged; e += 64u) {
+    const uint32_t q = bn + e;
+    nray[2 * (size_t)q] = na[nw + e];
+    nray[2 * (size_t)q + 1] = nb[nw + e];
+    npid[q] = np[nw + e];
+  }
+  sstaged = 0;
+  nstaged = 0;
 }
 
 // ---------------------------------------------------------------------
@@ -739,11 +764,10 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // sheen, iridescence, transmission, diffuse transmission, anisotropy), so
 // no path ever enters a medium: the medium state stays zero and is neither
 // read nor written, and the BSDF drops those lobes (BsdfT<false>).
-// Shadow rays of a single-job batch with at most STAGE_SLOTS slots per path
-// (punctual lights + the area sample) are staged per wave in LDS and
-// appended with one atomic per wave and round: appends to the one shadow
-// queue counter serialise device-wide, and one per light made them the
-// shading kernel's bottleneck.
+// With at most STAGE_SLOTS shadow slots per path (punctual lights + the
+// area sample) a block stages its round's shadow rays and next rays in LDS
+// and appends each queue with one atomic per round (block_flush); one
+// atomic per wave and light made the counter the kernel's bottleneck.
 constexpr uint32_t STAGE_SLOTS = 4;
 
 template <int WAVES, bool GUIDE, bool TEX, bool MULTI, bool EXT>
@@ -757,10 +781,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                                              uint4* __restrict__ meta_s, float4* __restrict__ nee,
                                              uint8_t* __restrict__ vis, Ctrl* ctrl, uint32_t cur,
                                              DevGuideUse gu) {
-  __shared__ float4 s_sa[MULTI ? 1 : STAGE_SLOTS * 256], s_sb[MULTI ? 1 : STAGE_SLOTS * 256];
-  __shared__ uint32_t s_si[MULTI ? 1 : STAGE_SLOTS * 256];
-  const bool stage = !MULTI && nslots <= STAGE_SLOTS;
-  const uint32_t wbase = (threadIdx.x / 64u) * STAGE_SLOTS * 64u;  // this wave's staging region
+  __shared__ float4 s_sa[STAGE_SLOTS * 256], s_sb[STAGE_SLOTS * 256], s_na[256], s_nb[256];
+  __shared__ uint32_t s_si[STAGE_SLOTS * 256], s_np[256], s_cnt[8], s_base[8];
+  // Uniform over the grid: every thread reaches each block_flush.
+  const bool stage = nslots <= STAGE_SLOTS;
+  const uint32_t wbase = (threadIdx.x / 64u) * STAGE_SLOTS * 64u;  // this wave's shadow staging
+  const uint32_t nbase = (threadIdx.x / 64u) * 64u;                // this wave's next-ray staging
   const uint32_t count = ctrl->qcount[cur].v;
   const uint32_t rec_mask = GUIDE ? (1u << gu.shift) - 1u : 0u;
   uint32_t* ncount = &ctrl->qcount[cur ^ 1u].v;
@@ -791,7 +817,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       }
     }
     bool alive = valid;
-    uint32_t staged = 0;  // shadow rays staged by this wave this round (wave-uniform)
+    uint32_t staged = 0, nstaged = 0;  // entries this wave staged this round (wave-uniform)
     const DevJob& job = jobs[MULTI ? (valid ? pjob[pid] : 0u) : 0u];
     const DevScene& sc = job.sc;
     const DevFrame& fr = job.fr;
@@ -954,11 +980,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     }
     // NEE: punctual lights, one shadow-ray slot each.
     bool any_shadow = false;
-    for (uint32_t j = 0; j < sc.num_lights; ++j) {
+    // Uniform trip count (nslots - 1 covers every scene of the batch) so
+    // every lane reaches the appends.
+    for (uint32_t j = 0; j + 1u < nslots; ++j) {
       bool emit = false;
       V3 o2, l, contrib;
       float tmax = 0.0f;
-      if (alive) {
+      if (alive && j < sc.num_lights) {
         const DevLight& L = sc.lights[j];
         V3 lrad;
         bool ok = true;
@@ -1001,7 +1029,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       const float4 rb = make_float4(l.x, l.y, l.z, tmax);
       const size_t slot = (size_t)pid * nslots + j;
       if (stage) {
-        stage_shadow(s_sa, s_sb, s_si, wbase, staged, emit, ra, rb, (uint32_t)slot);
+        stage_entry(s_sa, s_sb, s_si, wbase, staged, emit, ra, rb, (uint32_t)slot);
       } else {
         uint32_t pos = wave_append(scount, emit);
         if (emit) {
@@ -1118,7 +1146,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       const float4 rb = make_float4(l.x, l.y, l.z, tmax);
       const size_t slot = (size_t)pid * nslots + (nslots - 1);
       if (stage) {
-        stage_shadow(s_sa, s_sb, s_si, wbase, staged, emit, ra, rb, (uint32_t)slot);
+        stage_entry(s_sa, s_sb, s_si, wbase, staged, emit, ra, rb, (uint32_t)slot);
       } else {
         uint32_t pos = wave_append(scount, emit);
         if (emit) {
@@ -1132,8 +1160,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         vis[slot] = 1;
         any_shadow = true;
       }
-      // The round's staged shadow rays go out in one append.
-      if (stage) flush_shadow(s_sa, s_sb, s_si, wbase, staged, scount, sray, sidx);
     }
     if (any_shadow) flags |= PF_PENDING;
     // BSDF sampling, medium transitions, roulette (src/pathtrace.rs:2281-2322).
@@ -1210,11 +1236,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       r[1] = make_float4(nl.x, nl.y, nl.z, 1.0f);
       r[2] = make_float4(beta.x, beta.y, beta.z, 0.0f);
     }
-    uint32_t pos = wave_append(ncount, cont);
-    if (cont) {
-      nray[2 * pos] = make_float4(no.x, no.y, no.z, __uint_as_float(k + 1));
-      nray[2 * pos + 1] = make_float4(nl.x, nl.y, nl.z, __builtin_huge_valf());
-      npid[pos] = pid;
+    {
+      const float4 na = make_float4(no.x, no.y, no.z, __uint_as_float(k + 1));
+      const float4 nb = make_float4(nl.x, nl.y, nl.z, __builtin_huge_valf());
+      if (stage) {
+        stage_entry(s_na, s_nb, s_np, nbase, nstaged, cont, na, nb, pid);
+      } else {
+        uint32_t pos = wave_append(ncount, cont);
+        if (cont) {
+          nray[2 * pos] = na;
+          nray[2 * pos + 1] = nb;
+          npid[pos] = pid;
+        }
+      }
     }
     if (valid) {
       if (!cont) flags |= PF_DONE;
@@ -1224,6 +1258,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       if (EXT) sigma_s[pid] = sg;
       meta_s[pid] = make_uint4(meta.x, meta.y, ((k + (cont ? 1u : 0u)) & 0xffffu) | (hero << 16) | (flags << 24), ps);
     }
+    if (stage)
+      block_flush(s_cnt, s_base, s_sa, s_sb, s_si, wbase, staged, scount, sray, sidx, s_na, s_nb, s_np, nbase, nstaged,
+                  ncount, nray, npid);
   }
 }
 
