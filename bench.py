@@ -44,7 +44,8 @@ def workload(name):
 
 def bytes_per_closest_ray(pt, scene, cfg, opts_ambient, sample_rows=48, x_step=16, spp=2):
     """Algorithmic HBM bytes per closest-hit ray: 32 (ray) + 16 (hit) +
-    128 per 4-wide node + 48 per triangle, with node/triangle counts from
+    64 per 4-wide node (quantised child boxes) + 48 per triangle, with
+    node/triangle counts from
     the oracle's traversal of the exported device BVH over the oracle's own
     path rays (same samples as the device)."""
     import oracle as O
@@ -58,7 +59,7 @@ def bytes_per_closest_ray(pt, scene, cfg, opts_ambient, sample_rows=48, x_step=1
     rays = np.concatenate(rays)
     c = O.bvh4_count(nodes, tris, rays).astype(np.float64)
     n_node, n_tri = c[:, 0].mean(), c[:, 1].mean()
-    return 32.0 + 16.0 + 128.0 * n_node + 48.0 * n_tri, n_node, n_tri, len(rays)
+    return 32.0 + 16.0 + 64.0 * n_node + 48.0 * n_tri, n_node, n_tri, len(rays)
 
 
 def cpu_model():
@@ -306,7 +307,7 @@ def main():
     t = time.perf_counter()
     pt.sync(scene, opts)
     sync_s = time.perf_counter() - t
-    tune = pt.autotune(spp=min(a.spp_per_step, 8)) if not a.no_autotune else None
+    tune = pt.autotune(spp=min(a.spp_per_step, 16)) if not a.no_autotune else None
     for _ in range(a.warmup):
         pt.refine(a.spp_per_step)
     pt.reset()

@@ -374,7 +374,8 @@ int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
  * paths per wavefront batch in {2M, 4M, 8M, 16M, 32M} (within the device
  * memory cap) x shading-kernel occupancy in {1, 2} waves per SIMD x the
  * traversal queue whole or split into 8 per-XCD segments — keeps the
- * fastest, and resets the accumulation. */
+ * fastest (each candidate is warmed once and timed twice, the faster run
+ * counting), and resets the accumulation. */
 typedef struct oxr_tune_result {
   uint32_t path_budget;     /* chosen paths per batch                  */
   uint32_t shade_waves;     /* chosen shading occupancy                */
@@ -468,7 +469,8 @@ typedef struct oxr_bvh_info {
 int oxr_pt_bvh_info(oxr_pt* h, oxr_bvh_info* out);
 
 /* Copy of the flattened device BVH (for the oracle's traversal count):
- * nodes = num_nodes * 128 bytes, tris = num_tris * 48 bytes. */
+ * nodes = num_nodes * 64 bytes (4-wide nodes with 8-bit quantised child
+ * boxes), tris = num_tris * 48 bytes. */
 int oxr_pt_bvh_export(oxr_pt* h, void* nodes, void* tris);
 
 /* Per-point material inputs of Bsdf::new (the MaterialSample fields the

@@ -3001,10 +3001,11 @@ done:
   return m;
 }
 
-// Traversal cost of rays on the device's flattened 4-wide BVH (128-byte
-// nodes: six float4 of child bounds, child refs, leaf counts; 48-byte
-// triangles: v0, gid, v1, flags, v2, alpha), restating the device's
-// closest-hit order: leaves of a node first, inner children near to far.
+// Traversal cost of rays on the device's flattened 4-wide BVH (64-byte
+// nodes with 8-bit quantised child boxes, decoded as the device decodes
+// them; 48-byte triangles: v0, gid, v1, flags, v2, alpha), restating the
+// device's closest-hit order: leaves of a node first, inner children near
+// to far.
 // counts[2*i] = nodes fetched, counts[2*i+1] = triangles fetched.
 void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts) {
   for (uint32_t r = 0; r < n_rays; ++r) {
@@ -3019,14 +3020,22 @@ void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, c
     uint32_t best = 0xffffffffu;
     for (;;) {
       ++nn;
-      const float* N = nodes + 32ull * node;
+      // 64-byte node: org.xyz, exponents; byte planes lo.x hi.x lo.y hi.y
+      // lo.z hi.z and counts (child c in byte c); child refs at words 12-15.
+      const float* N = nodes + 16ull * node;
       const uint32_t* NU = reinterpret_cast<const uint32_t*>(N);
       float tn[4];
+      float sk[3], ok[3];
+      for (int k = 0; k < 3; ++k) {
+        sk[k] = std::ldexp(inv[k], (int)(int8_t)((NU[3] >> (8 * k)) & 0xffu));
+        ok[k] = std::fma(N[k], inv[k], -oi[k]);
+      }
       for (int c = 0; c < 4; ++c) {
         float t0[3], t1[3];
         for (int k = 0; k < 3; ++k) {
-          t0[k] = std::fma(N[8 * k + c], inv[k], -oi[k]);
-          t1[k] = std::fma(N[8 * k + 4 + c], inv[k], -oi[k]);
+          const uint32_t wl = k < 2 ? NU[4 + 2 * k] : NU[8], wh = k < 2 ? NU[5 + 2 * k] : NU[9];
+          t0[k] = std::fma((float)((wl >> (8 * c)) & 0xffu), sk[k], ok[k]);
+          t1[k] = std::fma((float)((wh >> (8 * c)) & 0xffu), sk[k], ok[k]);
         }
         float tnear = tmin, tfar = tmax;
         for (int k = 0; k < 3; ++k) {
@@ -3034,14 +3043,15 @@ void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, c
           tnear = std::fmax(tnear, neg ? t1[k] : t0[k]);
           tfar = std::fmin(tfar, neg ? t0[k] : t1[k]);
         }
-        bool hit = NU[24 + c] != 0xffffffffu && tnear <= tfar * (1.0f + 1.0f / 65536.0f);
+        bool hit = NU[12 + c] != 0xffffffffu && tnear <= tfar * (1.0f + 1.0f / 65536.0f);
         tn[c] = hit ? tnear : INF_F;
       }
       for (int c = 0; c < 4; ++c) {
-        if (tn[c] != INF_F && NU[28 + c] > 0) {
-          for (uint32_t k = 0; k < NU[28 + c]; ++k) {
+        const uint32_t cnt = (NU[10] >> (8 * c)) & 0xffu;
+        if (tn[c] != INF_F && cnt > 0) {
+          for (uint32_t k = 0; k < cnt; ++k) {
             ++nt;
-            const float* T = tris + 12ull * (NU[24 + c] + k);
+            const float* T = tris + 12ull * (NU[12 + c] + k);
             float v9[9] = {T[0], T[1], T[2], T[4], T[5], T[6], T[8], T[9], T[10]};
             float t, b1, b2;
             bool front;
@@ -3054,7 +3064,7 @@ void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, c
           tn[c] = INF_F;
         }
       }
-      uint32_t ci[4] = {NU[24], NU[25], NU[26], NU[27]};
+      uint32_t ci[4] = {NU[12], NU[13], NU[14], NU[15]};
       for (int a = 0; a < 4; ++a)
         for (int b = a + 1; b < 4; ++b)
           if (tn[a] < tn[b]) {

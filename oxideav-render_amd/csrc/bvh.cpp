@@ -523,4 +523,93 @@ Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float
   return out;
 }
 
+namespace {
+
+// Largest f32 not above x (x finite).
+float f32_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+  return f;
+}
+
+// One axis of a node: org, the exponent and its children's byte planes.
+struct AxisQ {
+  float org = 0.0f;
+  int e = 0;
+  uint32_t lo = 0, hi = 0;
+};
+
+AxisQ quantize_axis(const float* lo, const float* hi, const uint32_t* child) {
+  double L = std::numeric_limits<double>::infinity(), H = -L;
+  bool unbounded = false;
+  for (int c = 0; c < 4; ++c)
+    if (child[c] != NODE_EMPTY) {
+      unbounded = unbounded || !std::isfinite(lo[c]) || !std::isfinite(hi[c]);
+      L = std::min(L, (double)lo[c]);
+      H = std::max(H, (double)hi[c]);
+    }
+  AxisQ a;
+  if (unbounded) {
+    // A child box with a non-finite plane (a triangle with a non-finite
+    // corner): a NaN corner makes every plane of this axis NaN, which the
+    // slab test's min/max ignore, so no child is culled on this axis.
+    a.org = std::numeric_limits<float>::quiet_NaN();
+    for (int c = 0; c < 4; ++c)
+      if (child[c] != NODE_EMPTY) a.hi |= 255u << (8 * c);
+    return a;
+  }
+  if (!(L <= H)) return a;  // no child: never decoded
+  // A step of at least a few ulps of the coordinates, and fine enough that
+  // the node's extent spans at most ~250 of them.
+  const double mag = std::max(std::fabs(L), std::fabs(H));
+  int e = std::ilogb(std::max(mag, 1e-30)) - 21;
+  while (std::ldexp(250.0, e) < H - L) ++e;
+  e = std::max(e, -126);
+  for (;; ++e) {
+    const double step = std::ldexp(1.0, e);
+    const float org = f32_down(L - step);
+    // Widest plane: ceil((H - org) / step) + 1 must fit in a byte.
+    if (std::ceil(((double)H - (double)org) / step) + 1.0 > 255.0) continue;
+    a.org = org;
+    a.e = e;
+    for (int c = 0; c < 4; ++c) {
+      if (child[c] == NODE_EMPTY) continue;
+      const double ql = std::floor(((double)lo[c] - (double)org) / step) - 1.0;
+      const double qh = std::ceil(((double)hi[c] - (double)org) / step) + 1.0;
+      a.lo |= (uint32_t)std::max(ql, 0.0) << (8 * c);
+      a.hi |= (uint32_t)std::min(qh, 255.0) << (8 * c);
+    }
+    return a;
+  }
+}
+
+}  // namespace
+
+std::vector<Node4Q> quantize_nodes(const std::vector<Node4>& nodes) {
+  std::vector<Node4Q> out(nodes.size());
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const Node4& n = nodes[i];
+    Node4Q& q = out[i];
+    const AxisQ x = quantize_axis(n.lo_x, n.hi_x, n.child);
+    const AxisQ y = quantize_axis(n.lo_y, n.hi_y, n.child);
+    const AxisQ z = quantize_axis(n.lo_z, n.hi_z, n.child);
+    q.org[0] = x.org;
+    q.org[1] = y.org;
+    q.org[2] = z.org;
+    q.exps = ((uint32_t)(uint8_t)(int8_t)x.e) | ((uint32_t)(uint8_t)(int8_t)y.e << 8) |
+             ((uint32_t)(uint8_t)(int8_t)z.e << 16);
+    q.qlo_x = x.lo;
+    q.qhi_x = x.hi;
+    q.qlo_y = y.lo;
+    q.qhi_y = y.hi;
+    q.qlo_z = z.lo;
+    q.qhi_z = z.hi;
+    q.counts = 0;
+    for (int c = 0; c < 4; ++c) q.counts |= n.count[c] << (8 * c);  // leaves hold at most 8 triangles
+    q.pad = 0;
+    for (int c = 0; c < 4; ++c) q.child[c] = n.child[c];
+  }
+  return out;
+}
+
 }  // namespace oxr

@@ -60,4 +60,27 @@ struct Bvh4 {
 // max_leaf: largest leaf (1..8). threads: 0 = hardware concurrency.
 Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float* alpha, int max_leaf, int threads);
 
+// 64-byte wide node, the device's traversal format: the child boxes of a
+// Node4 quantised to 8 bits per plane (after Ylitie, Karras and Laine,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
+// BVHs", HPG 2017). Planes are steps of 2^e[k] on axis k from the corner
+// org, rounded outwards and widened by one more step, so each decoded box
+// holds the exact box with a margin of a step; the slab test then reads 64
+// bytes a node instead of 128. Four 16-byte quads:
+//   q0: org.xyz (f32), then e.x, e.y, e.z (int8) and a zero byte;
+//   q1: lo.x, hi.x, lo.y, hi.y: one byte per child (child c in byte c);
+//   q2: lo.z, hi.z, count (one byte per child), zero;
+//   q3: child[4] as in Node4 (NODE_EMPTY: unused slot).
+struct alignas(64) Node4Q {
+  float org[3];
+  uint32_t exps;
+  uint32_t qlo_x, qhi_x, qlo_y, qhi_y;
+  uint32_t qlo_z, qhi_z, counts, pad;
+  uint32_t child[4];
+};
+static_assert(sizeof(Node4Q) == 64, "Node4Q is 64 bytes");
+
+// The quantised copy of a Node4 array (same indices, same children).
+std::vector<Node4Q> quantize_nodes(const std::vector<Node4>& nodes);
+
 }  // namespace oxr

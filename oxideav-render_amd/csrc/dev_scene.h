@@ -48,11 +48,11 @@ struct alignas(16) DevLight {
 };
 static_assert(sizeof(DevLight) == 64, "DevLight is 64 bytes");
 
-struct Node4;
+struct Node4Q;
 struct TriRec;
 
 struct DevScene {
-  const Node4* nodes;
+  const Node4Q* nodes;
   const TriRec* tris;
   const float* pos;         // 9 per global triangle
   const float* nrm;         // 9 per global triangle (or null)

@@ -218,22 +218,26 @@ struct Trav {
 
   // nodes / tris: the scene's BVH (uniform across the wave when every
   // ray of the launch belongs to one scene, so they stay in SGPRs).
-  OXR_D bool step(Stack& st, const Node4* __restrict__ nodes, const TriRec* __restrict__ tris) {
-    const float4* np = reinterpret_cast<const float4*>(nodes + node);
-    float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-    uint4 cn = reinterpret_cast<const uint4*>(np)[7];
-    float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-    float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-    float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+  // Node4Q: a plane q (byte) of axis k sits at org[k] + q * 2^e[k], so its
+  // slab distance is q * (2^e[k] * inv[k]) + (org[k] * inv[k] - oi[k]).
+  OXR_D bool step(Stack& st, const Node4Q* __restrict__ nodes, const TriRec* __restrict__ tris) {
+    const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+    const uint4 q0 = np[0], q1 = np[1], q2 = np[2], ch = np[3];
+    const float sx = ldexpf(inv.x, (int)(int8_t)(q0.w & 0xffu));
+    const float sy = ldexpf(inv.y, (int)(int8_t)((q0.w >> 8) & 0xffu));
+    const float sz = ldexpf(inv.z, (int)(int8_t)((q0.w >> 16) & 0xffu));
+    const float ox = fmaf(__uint_as_float(q0.x), inv.x, -oi.x);
+    const float oy = fmaf(__uint_as_float(q0.y), inv.y, -oi.y);
+    const float oz = fmaf(__uint_as_float(q0.z), inv.z, -oi.z);
     uint32_t chl[4] = {ch.x, ch.y, ch.z, ch.w};
-    uint32_t cnt[4] = {cn.x, cn.y, cn.z, cn.w};
+    uint32_t cnt[4] = {q2.z & 0xffu, (q2.z >> 8) & 0xffu, (q2.z >> 16) & 0xffu, q2.z >> 24};
     float tn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      float ax = fmaf(lxa[c], inv.x, -oi.x), bx = fmaf(hxa[c], inv.x, -oi.x);
-      float ay = fmaf(lya[c], inv.y, -oi.y), by = fmaf(hya[c], inv.y, -oi.y);
-      float az = fmaf(lza[c], inv.z, -oi.z), bz = fmaf(hza[c], inv.z, -oi.z);
+      const uint32_t sh = 8u * c;
+      float ax = fmaf((float)((q1.x >> sh) & 0xffu), sx, ox), bx = fmaf((float)((q1.y >> sh) & 0xffu), sx, ox);
+      float ay = fmaf((float)((q1.z >> sh) & 0xffu), sy, oy), by = fmaf((float)((q1.w >> sh) & 0xffu), sy, oy);
+      float az = fmaf((float)((q2.x >> sh) & 0xffu), sz, oz), bz = fmaf((float)((q2.y >> sh) & 0xffu), sz, oz);
       float nx = negx ? bx : ax, fx = negx ? ax : bx;
       float ny = negy ? by : ay, fy = negy ? ay : by;
       float nz = negz ? bz : az, fz = negz ? az : bz;
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
-  const Node4* ray_nodes = sc.nodes;
+  const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<false, ALPHA>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
-  const Node4* ray_nodes = sc.nodes;
+  const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<true, ALPHA>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];

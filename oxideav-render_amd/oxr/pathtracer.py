@@ -259,10 +259,10 @@ class PathTracer:
 
     def bvh_export(self):
         info = self.bvh_info()
-        nodes = np.zeros(info["num_nodes"] * 32, np.uint32)
+        nodes = np.zeros(info["num_nodes"] * 16, np.uint32)
         tris = np.zeros(max(info["num_tris"], 1) * 12, np.uint32)
         check(self._L.oxr_pt_bvh_export(self._h, nodes.ctypes.data, tris.ctypes.data), "bvh_export")
-        return nodes.reshape(-1, 32), tris.reshape(-1, 12)[:info["num_tris"]]
+        return nodes.reshape(-1, 16), tris.reshape(-1, 12)[:info["num_tris"]]
 
     # -- parity entry points -------------------------------------------------
     def upload(self, scene):

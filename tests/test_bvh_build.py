@@ -1,6 +1,7 @@
 """Host-side BVH builder (oxideav-render_amd/csrc/bvh.cpp), compiled with a
 small harness: every triangle lands in exactly one leaf, every leaf box
-holds its triangles, and the tree is the same with 1 and 8 threads —
+holds its triangles, the quantised device nodes hold their boxes, and the
+tree is the same with 1 and 8 threads —
 including sizes where large subtrees are handed to the work queue
 (more than 65536 triangles per side) below the multi-thread cut-off, and
 nodes big enough (2^20 items) to split their passes over threads."""
@@ -26,6 +27,12 @@ int main(int argc, char** argv) {
   if (fread(pos.data(), 1, sz, f) != (size_t)sz) return 2;
   fclose(f);
   uint32_t n = pos.size() / 9;
+  float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
+  for (uint32_t i = 0; i < 3 * n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      bmin[k] = std::fmin(bmin[k], pos[3 * i + k]);
+      bmax[k] = std::fmax(bmax[k], pos[3 * i + k]);
+    }
   std::vector<uint32_t> flags(n, 0); std::vector<float> alpha(n, 1.0f);
   oxr::Bvh4 b = oxr::build_bvh4(pos.data(), n, flags.data(), alpha.data(), 4, atoi(argv[2]));
   // Leaf containment: every triangle of a leaf lies inside the leaf's box.
@@ -42,7 +49,68 @@ int main(int argc, char** argv) {
         ++inleaf;
       }
     }
-  printf("%zu %zu %u %ld %ld\n", b.nodes.size(), b.tris.size(), b.max_depth, bad, inleaf);
+  // Quantised nodes: same children and counts; every decoded box holds the
+  // exact box with a margin of one step (exact arithmetic); and the
+  // device's float slab test on them never culls a child that the exact
+  // box's test keeps, over random rays.
+  std::vector<oxr::Node4Q> q = oxr::quantize_nodes(b.nodes);
+  long qbad = 0, culled = 0;
+  for (size_t i = 0; i < b.nodes.size(); ++i) {
+    const oxr::Node4& nd = b.nodes[i];
+    const oxr::Node4Q& nq = q[i];
+    const float* lo[3] = {nd.lo_x, nd.lo_y, nd.lo_z};
+    const float* hi[3] = {nd.hi_x, nd.hi_y, nd.hi_z};
+    const uint32_t wl[3] = {nq.qlo_x, nq.qlo_y, nq.qlo_z}, wh[3] = {nq.qhi_x, nq.qhi_y, nq.qhi_z};
+    for (int c = 0; c < 4; ++c) {
+      if (nq.child[c] != nd.child[c] || ((nq.counts >> (8 * c)) & 0xffu) != nd.count[c]) ++qbad;
+      if (nd.child[c] == oxr::NODE_EMPTY) continue;
+      for (int k = 0; k < 3; ++k) {
+        const double step = std::ldexp(1.0, (int)(int8_t)((nq.exps >> (8 * k)) & 0xffu));
+        const double dl = (double)nq.org[k] + ((wl[k] >> (8 * c)) & 0xffu) * step;
+        const double dh = (double)nq.org[k] + ((wh[k] >> (8 * c)) & 0xffu) * step;
+        if (!(dl + step <= (double)lo[k][c]) || !(dh - step >= (double)hi[k][c])) ++qbad;
+      }
+    }
+  }
+  srand(7);
+  const size_t nn = b.nodes.size() < 4000 ? b.nodes.size() : 4000;
+  for (int r = 0; r < 300; ++r) {
+    float o[3], d[3], inv[3], oi[3];
+    for (int k = 0; k < 3; ++k) {
+      o[k] = bmin[k] + (bmax[k] - bmin[k]) * (1.4f * rand() / (float)RAND_MAX - 0.2f);
+      d[k] = 2.0f * rand() / (float)RAND_MAX - 1.0f;
+      if (r % 10 == 0 && k == r % 3) d[k] = 0.0f;  // axis-parallel rays too
+      inv[k] = 1.0f / d[k];
+      oi[k] = o[k] * inv[k];
+    }
+    for (size_t i = 0; i < nn; ++i) {
+      const oxr::Node4& nd = b.nodes[i];
+      const oxr::Node4Q& nq = q[i];
+      const float* lo[3] = {nd.lo_x, nd.lo_y, nd.lo_z};
+      const float* hi[3] = {nd.hi_x, nd.hi_y, nd.hi_z};
+      const uint32_t wl[3] = {nq.qlo_x, nq.qlo_y, nq.qlo_z}, wh[3] = {nq.qhi_x, nq.qhi_y, nq.qhi_z};
+      float sk[3], ok[3];
+      for (int k = 0; k < 3; ++k) {
+        sk[k] = std::ldexp(inv[k], (int)(int8_t)((nq.exps >> (8 * k)) & 0xffu));
+        ok[k] = std::fma(nq.org[k], inv[k], -oi[k]);
+      }
+      for (int c = 0; c < 4; ++c) {
+        if (nd.child[c] == oxr::NODE_EMPTY) continue;
+        float n0 = 0.0f, f0 = 1e30f, n1 = 0.0f, f1 = 1e30f;
+        for (int k = 0; k < 3; ++k) {
+          float a = std::fma(lo[k][c], inv[k], -oi[k]), bb = std::fma(hi[k][c], inv[k], -oi[k]);
+          float qa = std::fma((float)((wl[k] >> (8 * c)) & 0xffu), sk[k], ok[k]);
+          float qb = std::fma((float)((wh[k] >> (8 * c)) & 0xffu), sk[k], ok[k]);
+          bool neg = std::signbit(inv[k]);
+          n0 = std::fmax(n0, neg ? bb : a); f0 = std::fmin(f0, neg ? a : bb);
+          n1 = std::fmax(n1, neg ? qb : qa); f1 = std::fmin(f1, neg ? qa : qb);
+        }
+        const float eps = 1.0f + 1.0f / 65536.0f;
+        if (n0 <= f0 * eps && !(n1 <= f1 * eps)) ++culled;
+      }
+    }
+  }
+  printf("%zu %zu %u %ld %ld %ld %ld\n", b.nodes.size(), b.tris.size(), b.max_depth, bad, inleaf, qbad, culled);
   FILE* g = fopen(argv[3], "wb");
   for (const auto& t : b.tris) fwrite(&t.gid, 4, 1, g);
   fclose(g);
@@ -80,11 +148,44 @@ def test_every_triangle_in_one_leaf_any_thread_count(harness, n):
         gf = d / f"g{n}_{threads}.u32"
         r = subprocess.run([str(exe), str(pf), str(threads), str(gf)], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
-        nodes, ntris, depth, bad, inleaf = map(int, r.stdout.split())
+        nodes, ntris, depth, bad, inleaf, qbad, culled = map(int, r.stdout.split())
         gids = np.fromfile(gf, dtype=np.uint32)
         assert ntris == n and inleaf == n and bad == 0
+        assert qbad == 0 and culled == 0
         assert np.array_equal(np.sort(gids), np.arange(n, dtype=np.uint32))
         outs.append((nodes, depth, gids))
     assert outs[0][1] < 40
     # Same tree for any thread count (leaf order included).
     assert outs[0][0] == outs[1][0] and np.array_equal(outs[0][2], outs[1][2])
+
+
+@pytest.mark.parametrize("case", ["far_from_origin", "flat", "mixed_scale"])
+def test_quantised_nodes_contain_their_boxes(harness, case):
+    """The 64-byte device nodes: boxes decoded from the byte planes hold the
+    exact boxes with a margin of one step, and the float slab test on them
+    keeps every child the exact test keeps, also for small triangles far
+    from the origin (the step floor of a few ulps), flat scenes (a zero
+    extent axis) and nodes spanning tiny and large triangles."""
+    d, exe = harness
+    rng = np.random.default_rng(11)
+    n = 20000
+    if case == "far_from_origin":
+        pos = _soup(n, 3) * 1e-3 + np.float32(1000.0)
+    elif case == "flat":
+        pos = _soup(n, 4)
+        pos[:, 2::3] = 0.25
+    else:
+        pos = _soup(n, 5)
+        big = rng.random(n) < 0.01
+        pos[big, 3:] = pos[big, :3].repeat(2, 0).reshape(-1, 6) + rng.normal(size=(big.sum(), 6)).astype(np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    pf = d / f"q_{case}.f32"
+    pos.tofile(pf)
+    gf = d / f"qg_{case}.u32"
+    r = subprocess.run([str(exe), str(pf), "8", str(gf)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    nodes, ntris, depth, bad, inleaf, qbad, culled = map(int, r.stdout.split())
+    assert ntris == n and bad == 0
+    assert qbad == 0
+    assert culled == 0
+
