@@ -42,24 +42,20 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def bytes_per_closest_ray(pt, scene, cfg, opts_ambient, sample_rows=48, x_step=16, spp=2):
+def bytes_per_closest_ray(pt, spp):
     """Algorithmic HBM bytes per closest-hit ray: 32 (ray) + 16 (hit) +
-    64 per 4-wide node (quantised child boxes) + 48 per triangle, with
-    node/triangle counts from
-    the oracle's traversal of the exported device BVH over the oracle's own
-    path rays (same samples as the device)."""
-    import oracle as O
-    nodes, tris = pt.bvh_export()
-    os_ = O.OracleScene(scene)
-    W, H = cfg["width"], cfg["height"]
-    rays = []
-    for y in np.linspace(0, H - 1, sample_rows).astype(int):
-        opts = O.options(W, H, max_bounces=cfg["bounces"], ambient=opts_ambient)
-        rays.append(os_.record_rays(W, H, 0, spp, int(y), int(y) + 1, x_step=x_step, max_rays=1 << 16, opts=opts))
-    rays = np.concatenate(rays)
-    c = O.bvh4_count(nodes, tris, rays).astype(np.float64)
-    n_node, n_tri = c[:, 0].mean(), c[:, 1].mean()
-    return 32.0 + 16.0 + 64.0 * n_node + 48.0 * n_tri, n_node, n_tri, len(rays)
+    64 per 4-wide node (quantised child boxes) + 48 per triangle, with the
+    node and triangle fetches counted by the device itself (the counting
+    closest-hit variant) over every closest ray of one refine(spp) of the
+    workload; the accumulation is reset afterwards."""
+    pt.set_timing(False, count=True)
+    pt.refine(spp)
+    s = pt.stats()
+    pt.set_timing(False)
+    pt.reset()
+    n = max(s["closest_rays"], 1)
+    n_node, n_tri = s["closest_nodes"] / n, s["closest_tris"] / n
+    return 32.0 + 16.0 + 64.0 * n_node + 48.0 * n_tri, n_node, n_tri, s["closest_rays"]
 
 
 def cpu_model():
@@ -356,7 +352,7 @@ def main():
         "kernel_ms_per_step_source": "one extra step with per-kernel HIP events, x steps",
     }
     if not a.no_roofline and tot["launches_closest"]:
-        bpr, n_node, n_tri, nr = bytes_per_closest_ray(pt, scene, cfg, ambient)
+        bpr, n_node, n_tri, nr = bytes_per_closest_ray(pt, a.spp_per_step)
         avg_ms = tot["ms_closest"] / tot["launches_closest"]
         bytes_per_launch = bpr * tot["closest_rays"] / tot["launches_closest"]
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -374,7 +370,7 @@ def main():
                            "algorithmic_bytes_per_launch": bytes_per_launch,
                            "kernel": "k_closest<true>", "avg_launch_ms": avg_ms,
                            "bytes_per_ray": bpr, "nodes_per_ray": n_node, "tris_per_ray": n_tri,
-                           "rays_sampled": nr}
+                           "rays_counted": nr}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cfg, ambient)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -352,7 +352,9 @@ int oxr_pt_resolve_hdr(oxr_pt* h, const oxr_display* disp, float* out);
 int oxr_pt_resolve_rgba8(oxr_pt* h, const oxr_display* disp, uint8_t* out);
 
 /* Counters of the last refine: paths traced, closest-hit rays, shadow
- * rays, and per-kernel device time (ms) when timing was enabled. */
+ * rays, per-kernel device time (ms) when timing was enabled, and the BVH
+ * nodes and triangles the closest-hit rays fetched when traversal counting
+ * was enabled. */
 typedef struct oxr_pt_stats {
   uint64_t paths;
   uint64_t closest_rays;
@@ -365,7 +367,15 @@ typedef struct oxr_pt_stats {
   uint32_t launches_anyhit;
   uint32_t graph_replays;   /* batches whose bounce loop ran as a hipGraph */
   uint32_t graph_captures;  /* of which (re)captured the graph first      */
+  uint64_t closest_nodes;   /* node fetches of the closest-hit rays        */
+  uint64_t closest_tris;    /* triangle fetches of the closest-hit rays    */
 } oxr_pt_stats;
+/* enable: bit 0 (OXR_TIMING_EVENTS) times every kernel launch with HIP
+ * events; bit 1 (OXR_TIMING_COUNTS) counts the closest-hit traversal work
+ * of single-scene refines (a counting variant of the kernel). Either runs
+ * the bounce loop eagerly instead of as a graph. */
+#define OXR_TIMING_EVENTS 1
+#define OXR_TIMING_COUNTS 2
 int oxr_pt_set_timing(oxr_pt* h, int enable);
 int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
 

@@ -852,11 +852,12 @@ struct Engine {
   int grid_closest_alpha = 0, grid_any_alpha = 0;  // the textured-alpha variants
   uint32_t path_budget = 1u << 24;
   uint32_t max_budget = 1u << 24;
-  int shade_waves = 1;
+  int shade_waves = 2;  // plain single-scene shading: two waves per SIMD measured fastest
   uint32_t xcd_segments = 1;  // persistent traversal queue split per XCD (1 or 8)
   void* d_tmp = nullptr;
   size_t tmp_alloc = 0;
   bool timing = false;
+  bool count_trav = false;  // closest-hit launches of single-scene batches count their fetches
   oxr_pt_stats stats{};
   double guide_infer_ms = 0.0;
   std::vector<hipEvent_t> ev_pool;
@@ -1055,7 +1056,7 @@ struct Engine {
     launch::closest(stream, alpha_sc ? grid_closest_alpha : grid_closest, sc_any, d_jobs, pb.pjob, true, multi,
                     pb.qray[cur], pb.qpid[cur], pb.meta, 0,
                     &d_ctrl->qcount[cur].v, &d_ctrl->work[0].v, 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
-                    xcd_segments, alpha_sc);
+                    xcd_segments, alpha_sc, count_trav ? &d_ctrl->trav[0].v : nullptr);
     span_end(0, e1);
     if (gb && k < gb->use.kmax) {  // the guide serves the first kmax bounces
       hipEvent_t eg{};
@@ -1081,7 +1082,7 @@ struct Engine {
   // Returns true when the loop ran as a graph.
   bool bounce_graph(const DevJob* d_jobs, uint32_t P, uint32_t nslots, uint32_t max_bounces, int grid_shade,
                     const DevScene& sc_any, bool multi, const GuideBatch* gb) {
-    if (!graphs || timing || max_bounces > 32) return false;
+    if (!graphs || timing || count_trav || max_bounces > 32) return false;
     GraphKey key;
     std::memset(&key, 0, sizeof(key));
     key.max_bounces = max_bounces;
@@ -1193,6 +1194,8 @@ struct Engine {
     if (c.error) return fail(OXR_ERR_DEVICE, "traversal stack overflow");
     stats.closest_rays = c.rays_closest;
     stats.shadow_rays = c.rays_shadow;
+    stats.closest_nodes = c.trav[0].v;
+    stats.closest_tris = c.trav[1].v;
     for (auto& s : spans) {
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, s.a, s.b);
@@ -1878,8 +1881,9 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
   spp = std::max(spp, 1u);
   HIPC(hipSetDevice(h->eng.device));
   Engine& e = h->eng;
-  const bool timing = e.timing;
+  const bool timing = e.timing, counting = e.count_trav;
   e.timing = false;
+  e.count_trav = false;
   const uint64_t npx = (uint64_t)std::max(h->opts.width, 1u) * std::max(h->opts.height, 1u);
   const uint32_t budgets[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25};
   const int waves[] = {1, 2};
@@ -1922,6 +1926,7 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
   e.xcd_segments = best_x;
   if (e.pb.cap > 2ull * best_b) e.free_paths();  // drop oversized trial buffers
   e.timing = timing;
+  e.count_trav = counting;
   int rc = h->reset();
   if (rc) return rc;
   r.path_budget = best_b;
@@ -2109,7 +2114,8 @@ int oxr_pt_resolve_rgba8(oxr_pt* h, const oxr_display* disp, uint8_t* out) {
 
 int oxr_pt_set_timing(oxr_pt* h, int enable) {
   if (!h) return fail(OXR_ERR_INVALID, "null handle");
-  h->eng.timing = enable != 0;
+  h->eng.timing = (enable & OXR_TIMING_EVENTS) != 0;
+  h->eng.count_trav = (enable & OXR_TIMING_COUNTS) != 0;
   return OXR_OK;
 }
 

@@ -144,11 +144,16 @@ struct alignas(256) PadCounter {
   uint32_t pad_[63];
 };
 constexpr uint32_t PAD_COUNTER_WORDS = sizeof(PadCounter) / sizeof(uint32_t);
+struct alignas(256) PadCounter64 {
+  unsigned long long v;
+  unsigned long long pad_[31];
+};
 
 struct Ctrl {
   PadCounter qcount[2];
   PadCounter scount;
   PadCounter work[16];  // traversal work counters: closest [0, 8), any-hit [8, 16), one per queue segment
+  PadCounter64 trav[2];  // counting refines: node and triangle fetches of closest-hit rays
   uint32_t error;
   unsigned long long rays_closest;
   unsigned long long rays_shadow;

@@ -220,7 +220,9 @@ struct Trav {
   // ray of the launch belongs to one scene, so they stay in SGPRs).
   // Node4Q: a plane q (byte) of axis k sits at org[k] + q * 2^e[k], so its
   // slab distance is q * (2^e[k] * inv[k]) + (org[k] * inv[k] - oi[k]).
-  OXR_D bool step(Stack& st, const Node4Q* __restrict__ nodes, const TriRec* __restrict__ tris) {
+  // COUNT: ntri counts the triangles fetched.
+  template <bool COUNT = false>
+  OXR_D bool step(Stack& st, const Node4Q* __restrict__ nodes, const TriRec* __restrict__ tris, uint32_t& ntri) {
     const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
     const uint4 q0 = np[0], q1 = np[1], q2 = np[2], ch = np[3];
     const float sx = ldexpf(inv.x, (int)(int8_t)(q0.w & 0xffu));
@@ -250,6 +252,7 @@ struct Trav {
     for (int c = 0; c < 4; ++c) {
       if (tn[c] != __builtin_huge_valf() && cnt[c] > 0) {
         const float4* tp = reinterpret_cast<const float4*>(tris + chl[c]);
+        if (COUNT) ntri += cnt[c];
         for (uint32_t k = 0; k < cnt[c]; ++k) {
           float4 a = tp[3 * k], b = tp[3 * k + 1], e = tp[3 * k + 2];
           float t, b1, b2;
@@ -375,20 +378,24 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, 
 // and keep 4.
 #define OXR_TRACE_WAVES(ALPHA) __attribute__((amdgpu_waves_per_eu((ALPHA) ? 4 : 7)))
 
-template <bool PATH, bool MULTI, bool ALPHA>
+// COUNT: adds the nodes and triangles the launch fetched to counts[0] and
+// counts[32] (two 256-byte lines), one atomic per wave and counter.
+template <bool PATH, bool MULTI, bool ALPHA, bool COUNT = false>
 __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(DevScene sc, const DevJob* __restrict__ jobs,
                                                      const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                      const uint32_t* __restrict__ qpid,
                                                      const uint4* __restrict__ meta, uint32_t fixed_count,
                                                      const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all,
                                                      uint4* __restrict__ hits, uint32_t* spill, uint32_t spill_cap,
-                                                     uint32_t* err, uint32_t nseg, const DevScene* asc) {
+                                                     uint32_t* err, uint32_t nseg, const DevScene* asc,
+                                                     unsigned long long* counts) {
   __shared__ uint32_t s_stack[STACK * TRACE_TB];
   Stack st;
   st.lds = s_stack + threadIdx.x;
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
+  uint32_t nnode = 0, ntri = 0;  // COUNT: this lane's fetches
   const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<false, ALPHA>& tr, uint32_t i) {
@@ -422,7 +429,8 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
     tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
   };
   auto step = [&](Trav<false, ALPHA>& tr) {
-    return MULTI ? tr.step(st, ray_nodes, ray_tris) : tr.step(st, sc.nodes, sc.tris);
+    if (COUNT) ++nnode;
+    return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri) : tr.template step<COUNT>(st, sc.nodes, sc.tris, ntri);
   };
   auto finish = [&](Trav<false, ALPHA>& tr, uint32_t i) {
     uint32_t prim = tr.best;
@@ -434,6 +442,16 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
                          __float_as_uint(tr.bv));
   };
   trace_loop<false, ALPHA>(count, work, nseg, st, setup, step, finish);
+  if (COUNT) {  // every lane of the wave leaves trace_loop together
+    for (int o = 32; o > 0; o >>= 1) {
+      nnode += __shfl_xor(nnode, o);
+      ntri += __shfl_xor(ntri, o);
+    }
+    if (lane_id() == 0u) {
+      atomicAdd(counts, (unsigned long long)nnode);
+      atomicAdd(counts + 32, (unsigned long long)ntri);
+    }
+  }
 }
 
 // Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
@@ -453,6 +471,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
+  uint32_t ntri_unused = 0;
   const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<true, ALPHA>& tr, uint32_t i) {
@@ -488,7 +507,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
     tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
   };
   auto step = [&](Trav<true, ALPHA>& tr) {
-    return MULTI ? tr.step(st, ray_nodes, ray_tris) : tr.step(st, sc.nodes, sc.tris);
+    return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri_unused) : tr.step(st, sc.nodes, sc.tris, ntri_unused);
   };
   auto finish = [&](Trav<true, ALPHA>& tr, uint32_t i) {
     bool occ = tr.hit_any;
@@ -1425,6 +1444,7 @@ __global__ void k_zero_ctrl(Ctrl* c) {
     c->error = 0;
     c->rays_closest = 0;
     c->rays_shadow = 0;
+    c->trav[0].v = c->trav[1].v = 0;
   }
   if (threadIdx.x < 16) c->work[threadIdx.x].v = 0;
 }
@@ -1668,15 +1688,19 @@ int stack_lds_entries() { return STACK; }
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
-             uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc) {
+             uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc, unsigned long long* counts) {
   if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
   // asc (the scene in device memory) selects the textured-alpha variant;
   // a mixed batch finds each path's scene through its job instead.
   const bool alpha = asc != nullptr;
 #define OXR_CLOSEST(P, M, A)                                                                                     \
   hipLaunchKernelGGL((k_closest<P, M, A>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, qpid, meta,      \
-                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err, nseg, asc)
-  if (!path && alpha)
+                     fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err, nseg, asc, nullptr)
+  if (counts && path && !multi && !alpha)
+    hipLaunchKernelGGL((k_closest<true, false, false, true>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays,
+                       qpid, meta, fixed_count, count_ptr, work, cull_all, hits, spill, spill_cap, err, nseg, asc,
+                       counts);
+  else if (!path && alpha)
     OXR_CLOSEST(false, false, true);
   else if (!path)
     OXR_CLOSEST(false, false, false);

@@ -46,7 +46,8 @@ int stack_lds_entries();
 void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
-             uint32_t spill_cap, uint32_t* err, uint32_t nseg = 1, const DevScene* asc = nullptr);
+             uint32_t spill_cap, uint32_t* err, uint32_t nseg = 1, const DevScene* asc = nullptr,
+             unsigned long long* counts = nullptr);
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
             const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,

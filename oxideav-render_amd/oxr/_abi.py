@@ -272,6 +272,8 @@ class PtStats(C.Structure):
         ("launches_anyhit", C.c_uint32),
         ("graph_replays", C.c_uint32),
         ("graph_captures", C.c_uint32),
+        ("closest_nodes", C.c_uint64),
+        ("closest_tris", C.c_uint64),
     ]
 
 

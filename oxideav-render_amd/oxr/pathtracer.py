@@ -237,8 +237,11 @@ class PathTracer:
               "set_guide")
 
     # -- instrumentation -----------------------------------------------------
-    def set_timing(self, enable=True):
-        check(self._L.oxr_pt_set_timing(self._h, 1 if enable else 0), "set_timing")
+    def set_timing(self, enable=True, count=False):
+        """enable: time every kernel with HIP events; count: count the BVH
+        nodes and triangles closest-hit rays fetch (stats closest_nodes /
+        closest_tris). Either runs the bounce loop without graphs."""
+        check(self._L.oxr_pt_set_timing(self._h, (1 if enable else 0) | (2 if count else 0)), "set_timing")
 
     def stats(self):
         s = A.PtStats()

@@ -260,6 +260,36 @@ def test_shading_without_extension_lobes_is_bit_identical(oxr_mod, make):
     assert np.array_equal(sa, sb)
 
 
+@pytest.mark.parametrize("bounces", [0, 3])
+def test_traversal_counts_match_the_oracle_restatement(oxr_mod, oracle, bounces):
+    """The counting closest-hit variant (the roofline's node and triangle
+    fetches) against the oracle's restatement of the device traversal
+    (oxrcpu_bvh4_count) over the exported BVH and the oracle's own path rays
+    of the same samples: ray counts equal; fetches within 1% (ties in
+    child distance may order children differently)."""
+    s = scenes.material_sweep(segments=32, rings=16, floor_half=6)
+    W, spp = 48, 4
+    pt = oxr_mod.PathTracer(0)
+    opts = oxr_mod.RenderOptions(width=W, height=W, ambient=s.ambient,
+                                 path_trace=oxr_mod.PathTraceOptions(samples_per_pixel=spp, max_bounces=bounces))
+    pt.sync(s, opts)
+    pt.set_timing(False, count=True)
+    pt.refine(spp)
+    st = pt.stats()
+    nodes, tris = pt.bvh_export()
+    pt.close()
+    rays = oracle.OracleScene(s).record_rays(W, W, 0, spp, 0, W, max_rays=1 << 21,
+                                             opts=oracle.options(W, W, max_bounces=bounces, ambient=s.ambient))
+    c = oracle.bvh4_count(nodes, tris, rays).astype(np.int64)
+    if bounces == 0:
+        assert st["closest_rays"] == len(rays) == W * W * spp
+    else:
+        assert abs(st["closest_rays"] - len(rays)) <= 0.001 * len(rays)
+    assert st["closest_nodes"] > 0 and st["closest_tris"] > 0
+    assert abs(st["closest_nodes"] - c[:, 0].sum()) <= 0.01 * c[:, 0].sum()
+    assert abs(st["closest_tris"] - c[:, 1].sum()) <= 0.01 * c[:, 1].sum()
+
+
 def test_refinement_is_additive_and_deterministic(oxr_mod):
     s = scenes.area_light_floor(0.5, 0.0)
     pt, a, _ = _render_gpu(oxr_mod, s, 9, 8, 4)
