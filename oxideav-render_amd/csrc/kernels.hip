@@ -528,18 +528,52 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
 // Camera rays (Sampler::new + pattern 0 + Camera::primary_ray,
 // src/pathtrace.rs:2005-2012, src/camera.rs:489-514).
 // ---------------------------------------------------------------------
+// Pixel offset of queue offset r in a band of `rows` whole rows of width
+// w, visited in 8x8 tiles (row-major tiles, row-major inside a tile;
+// narrower tiles at the right edge, shorter ones at the bottom): a bijection
+// of [0, rows * w).
+OXR_D uint32_t tile_order(uint32_t r, uint32_t w, uint32_t rows) {
+  const uint32_t tr = r / (8u * w);
+  const uint32_t ir = r - tr * 8u * w;
+  const uint32_t h = min(8u, rows - 8u * tr);
+  const uint32_t full = w / 8u;
+  const uint32_t c = ir / (8u * h);
+  uint32_t x, y;
+  if (c < full) {
+    const uint32_t j = ir - c * 8u * h;
+    y = 8u * tr + j / 8u;
+    x = 8u * c + j % 8u;
+  } else {
+    const uint32_t tw = w - 8u * full, j = ir - full * 8u * h;
+    y = 8u * tr + j / tw;
+    x = 8u * full + j % tw;
+  }
+  return y * w + x;
+}
+
+// Thread q writes queue entry q. Within a segment that covers whole rows,
+// the queue visits its pixels in 8x8 tiles, so a wave's 64 primary rays
+// (and, through the queue order, their later bounces) start close
+// together; the path id of a sample and pixel is unchanged, and so is every
+// result.
 __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_t P, float4* __restrict__ qray,
                          uint32_t* __restrict__ qpid, float4* __restrict__ beta, float4* __restrict__ rad,
                          float4* __restrict__ sigma, uint4* __restrict__ meta, uint32_t* __restrict__ pjob, Ctrl* ctrl) {
-  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid == 0) ctrl->qcount[0].v = P;
-  if (pid >= P) return;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) ctrl->qcount[0].v = P;
+  if (q >= P) return;
   uint32_t si = 0;
-  while (si + 1 < segs.n && segs.s[si + 1].path0 <= pid) ++si;
+  while (si + 1 < segs.n && segs.s[si + 1].path0 <= q) ++si;
   const Seg sg = segs.s[si];
   const DevFrame& fr = jobs[sg.job].fr;
+  const uint32_t lq = q - sg.path0;
+  const uint32_t s = lq / sg.npx;
+  uint32_t off = lq - s * sg.npx;
+  if (fr.width > 0u && sg.px0 % fr.width == 0u && sg.npx % fr.width == 0u)
+    off = tile_order(off, fr.width, sg.npx / fr.width);
+  const uint32_t pid = sg.path0 + s * sg.npx + off;
   uint32_t local = pid - sg.path0;
-  uint32_t s = local / sg.npx, px = sg.px0 + (local - s * sg.npx);
+  uint32_t px = sg.px0 + (local - s * sg.npx);
   uint32_t x = px % fr.width, y = px / fr.width;
   uint32_t index = sg.first + s;
   uint32_t ps = pixel_seed_of(fr.seed, x, y);
@@ -556,9 +590,9 @@ __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_
     d = normalise(add(add(scale(side, ndc_x * fr.half_w), scale(up, ndc_y * fr.half_h)), fwd));
     o = eye;
   }
-  qray[2 * pid] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
-  qray[2 * pid + 1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
-  qpid[pid] = pid;
+  qray[2 * q] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
+  qray[2 * q + 1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
+  qpid[q] = pid;
   beta[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   sigma[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
