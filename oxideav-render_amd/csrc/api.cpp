@@ -1178,7 +1178,7 @@ struct Engine {
     launch::prep(stream, d_ctrl, cur, 1u);
     hipEvent_t e4{};
     span_begin(e4);
-    launch::finalize(stream, P, nslots, pb);
+    launch::finalize(stream, d_jobs, segs, P, nslots, pb);
     for (const Reduce& r : red) launch::reduce(stream, r.npx, r.S, pb.out + r.path0, r.accum, r.sumsq);
     if (gb && gb->use.rec) launch::guide_collect(stream, P, gb->use, pb, gb->samp, gb->count, gb->samp_cap);
     span_end(3, e4);

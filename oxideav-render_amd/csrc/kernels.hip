@@ -528,52 +528,73 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
 // Camera rays (Sampler::new + pattern 0 + Camera::primary_ray,
 // src/pathtrace.rs:2005-2012, src/camera.rs:489-514).
 // ---------------------------------------------------------------------
-// Pixel offset of queue offset r in a band of `rows` whole rows of width
-// w, visited in 8x8 tiles (row-major tiles, row-major inside a tile;
-// narrower tiles at the right edge, shorter ones at the bottom): a bijection
-// of [0, rows * w).
+// Pixel offset of rank r in a band of `rows` whole rows of width w,
+// visited in TxT tiles (row-major tiles, row-major inside a tile; narrower
+// tiles at the right edge, shorter ones at the bottom): a bijection of
+// [0, rows * w).
+template <uint32_t T>
 OXR_D uint32_t tile_order(uint32_t r, uint32_t w, uint32_t rows) {
-  const uint32_t tr = r / (8u * w);
-  const uint32_t ir = r - tr * 8u * w;
-  const uint32_t h = min(8u, rows - 8u * tr);
-  const uint32_t full = w / 8u;
-  const uint32_t c = ir / (8u * h);
+  const uint32_t tr = r / (T * w);
+  const uint32_t ir = r - tr * T * w;
+  const uint32_t h = min(T, rows - T * tr);
+  const uint32_t full = w / T;
+  const uint32_t c = ir / (T * h);
   uint32_t x, y;
   if (c < full) {
-    const uint32_t j = ir - c * 8u * h;
-    y = 8u * tr + j / 8u;
-    x = 8u * c + j % 8u;
+    const uint32_t j = ir - c * T * h;
+    y = T * tr + j / T;
+    x = T * c + j % T;
   } else {
-    const uint32_t tw = w - 8u * full, j = ir - full * 8u * h;
-    y = 8u * tr + j / tw;
-    x = 8u * full + j % tw;
+    const uint32_t tw = w - T * full, j = ir - full * T * h;
+    y = T * tr + j / tw;
+    x = T * full + j % tw;
   }
   return y * w + x;
 }
 
-// Thread q writes queue entry q. Within a segment that covers whole rows,
-// the queue visits its pixels in 8x8 tiles, so a wave's 64 primary rays
-// (and, through the queue order, their later bounces) start close
-// together; the path id of a sample and pixel is unchanged, and so is every
-// result.
+// Segment of path p (segments are in path order).
+OXR_D uint32_t seg_of(const SegTable& segs, uint32_t p) {
+  uint32_t si = 0;
+  while (si + 1 < segs.n && segs.s[si + 1].path0 <= p) ++si;
+  return si;
+}
+
+// Path p of a segment is (sample s, pixel offset off). Where the segment
+// covers whole rows, consecutive paths walk 4x4-pixel tiles with 4 samples
+// of a pixel side by side (64 paths: 16 pixels x 4 samples) when S is a
+// multiple of 4, else 8x8-pixel tiles of one sample; so a wave's 64 primary
+// rays start close together, and, through the queue order, so do their
+// later bounces. Otherwise paths go sample-major in pixel order. Path state
+// is indexed by p; k_finalize writes each result at s * npx + off, where
+// k_reduce reads it.
+OXR_D void path_sample(const Seg& sg, uint32_t width, uint32_t lq, uint32_t& s, uint32_t& off) {
+  if (width > 0u && sg.px0 % width == 0u && sg.npx % width == 0u) {
+    const uint32_t rows = sg.npx / width;
+    if (sg.S % 4u == 0u) {
+      const uint32_t g = lq / (4u * sg.npx), rem = lq - g * 4u * sg.npx;
+      s = 4u * g + rem % 4u;
+      off = tile_order<4>(rem / 4u, width, rows);
+    } else {
+      s = lq / sg.npx;
+      off = tile_order<8>(lq - s * sg.npx, width, rows);
+    }
+  } else {
+    s = lq / sg.npx;
+    off = lq - s * sg.npx;
+  }
+}
+
 __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_t P, float4* __restrict__ qray,
                          uint32_t* __restrict__ qpid, float4* __restrict__ beta, float4* __restrict__ rad,
                          float4* __restrict__ sigma, uint4* __restrict__ meta, uint32_t* __restrict__ pjob, Ctrl* ctrl) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q == 0) ctrl->qcount[0].v = P;
-  if (q >= P) return;
-  uint32_t si = 0;
-  while (si + 1 < segs.n && segs.s[si + 1].path0 <= q) ++si;
-  const Seg sg = segs.s[si];
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid == 0) ctrl->qcount[0].v = P;
+  if (pid >= P) return;
+  const Seg sg = segs.s[seg_of(segs, pid)];
   const DevFrame& fr = jobs[sg.job].fr;
-  const uint32_t lq = q - sg.path0;
-  const uint32_t s = lq / sg.npx;
-  uint32_t off = lq - s * sg.npx;
-  if (fr.width > 0u && sg.px0 % fr.width == 0u && sg.npx % fr.width == 0u)
-    off = tile_order(off, fr.width, sg.npx / fr.width);
-  const uint32_t pid = sg.path0 + s * sg.npx + off;
-  uint32_t local = pid - sg.path0;
-  uint32_t px = sg.px0 + (local - s * sg.npx);
+  uint32_t s, off;
+  path_sample(sg, fr.width, pid - sg.path0, s, off);
+  uint32_t px = sg.px0 + off;
   uint32_t x = px % fr.width, y = px / fr.width;
   uint32_t index = sg.first + s;
   uint32_t ps = pixel_seed_of(fr.seed, x, y);
@@ -590,9 +611,9 @@ __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_
     d = normalise(add(add(scale(side, ndc_x * fr.half_w), scale(up, ndc_y * fr.half_h)), fwd));
     o = eye;
   }
-  qray[2 * q] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
-  qray[2 * q + 1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
-  qpid[q] = pid;
+  qray[2 * pid] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
+  qray[2 * pid + 1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
+  qpid[pid] = pid;
   beta[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   sigma[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1323,6 +1344,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 // Guide training samples of the recorded paths, once their radiance is
 // final: incident radiance estimate L_i = (L_final - L_mark) / beta per
 // channel, target = luminance(max(L_i, 0)), weight = target / pdf.
+// out: each path's final radiance, by path (k_finalize's copy in the path
+// radiance state).
 __global__ void k_guide_collect(uint32_t P, uint32_t nsel, uint32_t shift, const float4* __restrict__ rec,
                                 const float4* __restrict__ out, float4* __restrict__ samp, uint32_t* count,
                                 uint32_t cap) {
@@ -1428,9 +1451,11 @@ __global__ void k_material_eval(DevScene sc, uint32_t n, const uint32_t* __restr
 
 // Per-sample radiance: the pending NEE of the last vertex, then the
 // finite check of Integrator::sample (src/pathtrace.rs:2324-2328).
-__global__ void k_finalize(uint32_t P, uint32_t nslots, const float4* __restrict__ nee, uint8_t* __restrict__ vis,
-                           const float4* __restrict__ rad_s, const uint4* __restrict__ meta_s,
-                           float4* __restrict__ out) {
+// The path's final radiance: in rad_s (by path, for the guide's training
+// samples) and in out at its segment's s * npx + pixel offset (k_reduce).
+__global__ void k_finalize(const DevJob* __restrict__ jobs, SegTable segs, uint32_t P, uint32_t nslots,
+                           const float4* __restrict__ nee, uint8_t* __restrict__ vis, float4* __restrict__ rad_s,
+                           const uint4* __restrict__ meta_s, float4* __restrict__ out) {
   uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= P) return;
   uint4 meta = meta_s[pid];
@@ -1438,7 +1463,12 @@ __global__ void k_finalize(uint32_t P, uint32_t nslots, const float4* __restrict
   V3 radiance = f4xyz(rad_s[pid]);
   if (flags & PF_PENDING) add_pending(pid, nslots, nee, vis, radiance);
   if (!finite3(radiance)) radiance = v3(0.0f, 0.0f, 0.0f);
-  out[pid] = make_float4(radiance.x, radiance.y, radiance.z, (flags & PF_UNCOVERED) ? 0.0f : 1.0f);
+  const Seg sg = segs.s[seg_of(segs, pid)];
+  uint32_t s, off;
+  path_sample(sg, jobs[sg.job].fr.width, pid - sg.path0, s, off);
+  rad_s[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+  out[sg.path0 + (size_t)s * sg.npx + off] =
+      make_float4(radiance.x, radiance.y, radiance.z, (flags & PF_UNCOVERED) ? 0.0f : 1.0f);
 }
 
 // Accumulate S samples per pixel in sample order into the f64
@@ -1829,7 +1859,7 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
                    uint32_t* count, uint32_t cap) {
   uint32_t nsel = (uint32_t)(((uint64_t)P + (1ull << gu.shift) - 1ull) >> gu.shift);
-  hipLaunchKernelGGL(k_guide_collect, dim3((nsel + 255) / 256), dim3(256), 0, s, P, nsel, gu.shift, gu.rec, pb.out,
+  hipLaunchKernelGGL(k_guide_collect, dim3((nsel + 255) / 256), dim3(256), 0, s, P, nsel, gu.shift, gu.rec, pb.rad,
                      samp, count, cap);
 }
 
@@ -1845,9 +1875,10 @@ void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t
   hipLaunchKernelGGL(k_material_eval, dim3((n + 127) / 128), dim3(128), 0, s, sc, n, g, bary, width, dir, flip, out);
 }
 
-void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb) {
-  hipLaunchKernelGGL(k_finalize, dim3((P + 255) / 256), dim3(256), 0, s, P, nslots, pb.nee, pb.vis, pb.rad, pb.meta,
-                     pb.out);
+void finalize(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, uint32_t nslots,
+              const PathBuffers& pb) {
+  hipLaunchKernelGGL(k_finalize, dim3((P + 255) / 256), dim3(256), 0, s, jobs, segs, P, nslots, pb.nee, pb.vis, pb.rad,
+                     pb.meta, pb.out);
 }
 
 void reduce(hipStream_t s, uint32_t npx, uint32_t S, const float4* out, double* accum, double* sumsq) {

@@ -72,7 +72,8 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
 // appended at *count (entries past cap are dropped).
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
                    uint32_t* count, uint32_t cap);
-void finalize(hipStream_t s, uint32_t P, uint32_t nslots, const PathBuffers& pb);
+void finalize(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, uint32_t nslots,
+              const PathBuffers& pb);
 void reduce(hipStream_t s, uint32_t npx, uint32_t S, const float4* out, double* accum, double* sumsq);
 void fill_f64(hipStream_t s, double* p, size_t n);
 void zero_ctrl(hipStream_t s, Ctrl* c);
