@@ -853,7 +853,10 @@ struct Engine {
   uint32_t path_budget = 1u << 24;
   uint32_t max_budget = 1u << 24;
   int shade_waves = 2;  // plain single-scene shading: two waves per SIMD measured fastest
-  uint32_t xcd_segments = 1;  // persistent traversal queue split per XCD (1 or 8)
+  // Persistent traversal queue split per XCD (1 or 8). Eight keeps each run
+  // of neighbouring rays (pixel tiles) on one XCD and its L2: +13% on the
+  // material sweep, -2% on the 12.4M-triangle terrain.
+  uint32_t xcd_segments = 8;
   void* d_tmp = nullptr;
   size_t tmp_alloc = 0;
   bool timing = false;
