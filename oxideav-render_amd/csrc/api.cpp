@@ -1071,7 +1071,7 @@ struct Engine {
     hipEvent_t e2{};
     span_begin(e2);
     launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr,
-                  tex_variant, multi_jobs, ext_variant);
+                  tex_variant, multi_jobs, ext_variant, area_variant);
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);
@@ -1100,7 +1100,7 @@ struct Engine {
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
     key.alpha_sc = alpha_sc;
-    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u) | (ext_variant ? 4u : 0u);
+    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u) | (ext_variant ? 4u : 0u) | (area_variant ? 8u : 0u);
     if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
@@ -1142,14 +1142,17 @@ struct Engine {
   // scene's device copy through alpha_sc).
   bool tex_variant = false;
   bool ext_variant = true;
+  bool area_variant = true;  // emissive triangles or an environment map in the batch
   const DevScene* alpha_sc = nullptr;
   bool multi_jobs = true;  // the batch holds paths of more than one job
 
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
                  const std::vector<Reduce>& red, const DevScene& sc_any, bool multi, int& rc,
-                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr, bool ext = true) {
+                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr, bool ext = true,
+                 bool area = true) {
     tex_variant = tex;
     ext_variant = ext;
+    area_variant = area;
     alpha_sc = asc;
     multi_jobs = false;  // every segment of job slot 0: the single-job shading variant
     for (uint32_t q = 0; q < segs.n; ++q) multi_jobs = multi_jobs || segs.s[q].job != 0u;
@@ -1584,7 +1587,8 @@ int oxr_pt::refine(uint32_t n) {
       rc = OXR_OK;
       if (gb && gb->use.rec) glaunch::fill(eng.stream, reinterpret_cast<float*>(guide->d_rec), rec_floats);
       eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, false, rc, gb, scene.sc.has_tex != 0,
-                    scene.sc.has_alpha_tex ? scene.d_self : nullptr, scene.sc.has_ext != 0);
+                    scene.sc.has_alpha_tex ? scene.d_self : nullptr, scene.sc.has_ext != 0,
+                    scene.sc.num_emissive > 0 || job.fr.env.img != nullptr);
       if (rc) return rc;
     }
   }
@@ -1755,17 +1759,18 @@ struct oxr_srv {
       if ((rc = eng.ensure_spill(spill))) return rc;
       eng.begin_stats();
       rc = OXR_OK;
-      bool multi = false, tex = false, ext = false;
+      bool multi = false, tex = false, ext = false, area = false;
       const DevScene* asc = nullptr;  // a mixed batch reads each path's scene through its job
       for (auto& [id, S_] : took) {
         multi = multi || jobs[id].scene != jobs[took[0].first].scene;
         const SceneBuf& sb = *scenes[jobs[id].scene];
         tex = tex || sb.sc.has_tex;
         ext = ext || sb.sc.has_ext;
+        area = area || sb.sc.num_emissive > 0;  // server jobs have no environment map
         if (sb.sc.has_alpha_tex && !asc) asc = sb.d_self;
       }
       eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, multi, rc, nullptr,
-                    tex, asc, ext);
+                    tex, asc, ext, area);
       if (rc) return rc;
       if ((rc = eng.end_stats())) return rc;
       st.rounds++;

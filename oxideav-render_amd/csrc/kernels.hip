@@ -837,6 +837,10 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // MULTI: the batch mixes jobs (each path's job through pjob); else every
 // path belongs to job 0, whose scene and frame the wave reads with uniform
 // (scalar) loads instead of a per-lane dependent lookup.
+// AREA = false: no scene of the batch has an emissive triangle in its light
+// table and no job has an environment map, so the area-light sample, the
+// light-table MIS weight of an emitter hit (it has none: pl = 0, w = 1) and
+// the environment lookups are never taken and are left out.
 // EXT = false: no material of the batch has an extension lobe (clearcoat,
 // sheen, iridescence, transmission, diffuse transmission, anisotropy), so
 // no path ever enters a medium: the medium state stays zero and is neither
@@ -847,7 +851,7 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // atomic per wave and light made the counter the kernel's bottleneck.
 constexpr uint32_t STAGE_SLOTS = 4;
 
-template <int WAVES, bool GUIDE, bool TEX, bool MULTI, bool EXT>
+template <int WAVES, bool GUIDE, bool TEX, bool MULTI, bool EXT, bool AREA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
                                              const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
@@ -945,7 +949,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         // (src/pathtrace.rs:2038-2050).
         V3 le = v3(fr.ambient, fr.ambient, fr.ambient);
         float w = 1.0f;
-        if (fr.env.img) {
+        if (AREA && fr.env.img) {
           le = env_radiance(fr.env, d);
           w = fr.strategy == 0u ? power_heuristic(prev_pdf, fr.p_env * env_pdf(fr.env, d))
                                 : (fr.strategy == 1u ? 0.0f : 1.0f);
@@ -991,7 +995,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         if (k == 0) {
           radiance = add(radiance, mul(beta, em));
         } else {
-          uint32_t li = sc.l_lookup[g];
+          uint32_t li = AREA ? sc.l_lookup[g] : 0xffffffffu;
           float pl = li == 0xffffffffu ? 0.0f : tri_pdf(sc, fr.p_env, li, g, prev_pos, p);
           float w;
           if (pl <= 0.0f)
@@ -1130,7 +1134,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       uint32_t skip = NO_SKIP;
       F4 pl_u;
       const float pe = fr.p_env;
-      const bool area = alive && fr.strategy != 2u && (fr.env.img != nullptr || sc.num_emissive > 0);
+      const bool area = AREA && alive && fr.strategy != 2u && (fr.env.img != nullptr || sc.num_emissive > 0);
       if (area) pl_u = pattern(ps, index, 2 + 2 * k);
       if (area && pl_u.v[2] < pe) {
         V3 le;
@@ -1813,46 +1817,48 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 }
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, bool ext) {
+           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, bool ext, bool area) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
-#define OXR_SHADE(W, G, T, M, X)                                                                                 \
-  hipLaunchKernelGGL((k_shade<W, G, T, M, X>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur],    \
+#define OXR_SHADE(W, G, T, M, X, A)                                                                              \
+  hipLaunchKernelGGL((k_shade<W, G, T, M, X, A>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur], \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
                      pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
   // Mixed batches (the server) and textured or guided scenes run at one
   // wave per SIMD; the occupancy variants are for plain single-scene
-  // shading. Each has a variant without the extension lobes.
+  // shading. Plain and mixed batches have variants without the extension
+  // lobes and without area lights.
+#define OXR_SHADE_XA(W, M)                 \
+  do {                                     \
+    if (ext && area)                       \
+      OXR_SHADE(W, false, false, M, true, true);  \
+    else if (ext)                          \
+      OXR_SHADE(W, false, false, M, true, false); \
+    else if (area)                         \
+      OXR_SHADE(W, false, false, M, false, true); \
+    else                                   \
+      OXR_SHADE(W, false, false, M, false, false); \
+  } while (0)
   if (multi && tex)
-    OXR_SHADE(1, false, true, true, true);
-  else if (multi && ext)
-    OXR_SHADE(1, false, false, true, true);
+    OXR_SHADE(1, false, true, true, true, true);
   else if (multi)
-    OXR_SHADE(1, false, false, true, false);
+    OXR_SHADE_XA(1, true);
   else if (guide && tex)
-    OXR_SHADE(1, true, true, false, true);
+    OXR_SHADE(1, true, true, false, true, true);
   else if (guide && ext)
-    OXR_SHADE(1, true, false, false, true);
+    OXR_SHADE(1, true, false, false, true, true);
   else if (guide)
-    OXR_SHADE(1, true, false, false, false);
+    OXR_SHADE(1, true, false, false, false, true);
   else if (tex && ext)
-    OXR_SHADE(1, false, true, false, true);
+    OXR_SHADE(1, false, true, false, true, true);
   else if (tex)
-    OXR_SHADE(1, false, true, false, false);
-  else if (ext) {
-    if (waves >= 3)
-      OXR_SHADE(3, false, false, false, true);
-    else if (waves == 2)
-      OXR_SHADE(2, false, false, false, true);
-    else
-      OXR_SHADE(1, false, false, false, true);
-  } else {
-    if (waves >= 3)
-      OXR_SHADE(3, false, false, false, false);
-    else if (waves == 2)
-      OXR_SHADE(2, false, false, false, false);
-    else
-      OXR_SHADE(1, false, false, false, false);
-  }
+    OXR_SHADE(1, false, true, false, false, true);
+  else if (waves >= 3)
+    OXR_SHADE_XA(3, false);
+  else if (waves == 2)
+    OXR_SHADE_XA(2, false);
+  else
+    OXR_SHADE_XA(1, false);
+#undef OXR_SHADE_XA
 #undef OXR_SHADE
 }
 

@@ -66,8 +66,11 @@ void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t
 // multi: the batch mixes jobs (the server's rounds).
 // ext: a material of the batch has an extension lobe (clearcoat, sheen,
 // iridescence, transmission, diffuse transmission or anisotropy).
+// area: a scene of the batch has emissive triangles, or a job has an
+// environment map.
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex = false, bool multi = true, bool ext = true);
+           uint32_t cur, const DevGuideUse* guide, bool tex = false, bool multi = true, bool ext = true,
+           bool area = true);
 // Training samples (2 float4 each) of the guide-recorded paths of a batch,
 // appended at *count (entries past cap are dropped).
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,

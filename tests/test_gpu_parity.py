@@ -223,8 +223,10 @@ def test_direct_lighting_matches_oracle_per_pixel(oxr_mod, oracle, make, W, spp)
 
 
 @pytest.mark.parametrize("make,W,spp,b", [(scenes.cornell, 32, 64, 8), (scenes.reference_box, 32, 64, 6),
-                                          (scenes.glossy_caustic, 24, 32, 8)],
-                         ids=["cornell", "reference_box", "caustic"])
+                                          (scenes.glossy_caustic, 24, 32, 8),
+                                          (lambda: scenes.material_sweep(segments=32, rings=16, floor_half=6), 32, 32,
+                                           8)],
+                         ids=["cornell", "reference_box", "caustic", "sweep_small"])
 def test_full_bounce_images_agree_within_3_sigma(oxr_mod, oracle, make, W, spp, b):
     s = make()
     _, ag, sqg = _render_gpu(oxr_mod, s, W, spp, b, seed=5)
