@@ -12,8 +12,10 @@
 
 namespace oxr {
 
-// 128-byte wide node: child boxes as six float4 (lo/hi per axis, one
-// lane per child), then child references and leaf triangle counts.
+// 128-byte wide node, the builder's output (the device traverses its
+// quantised copy, Node4Q below): child boxes as six float4 (lo/hi per
+// axis, one lane per child), then child references and leaf triangle
+// counts.
 // count[i] == 0: inner child, child[i] is a node index;
 // count[i]  > 0: leaf of count[i] triangles starting at tris[child[i]];
 // child[i] == EMPTY: unused slot (its box is empty: lo = +inf, hi = -inf).

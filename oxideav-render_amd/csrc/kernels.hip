@@ -116,9 +116,9 @@ OXR_D void block_flush(uint32_t* s_cnt, uint32_t* s_base, const float4* sa, cons
 
 // ---------------------------------------------------------------------
 // BVH traversal (TraceScene::closest_hit_filtered / occluded_filtered,
-// src/trace.rs:405-450), 4-wide nodes, per-thread stack: a 32-entry ring
-// in LDS with overflow spilled to a per-thread global buffer whose size
-// the host derives from the tree depth.
+// src/trace.rs:405-450), 4-wide quantised nodes (Node4Q), per-thread
+// stack: a STACK-entry ring in LDS with overflow spilled to a per-thread
+// global buffer whose size the host derives from the tree depth.
 // ---------------------------------------------------------------------
 struct Stack {
   uint32_t* lds;    // &s[0][tid]; entry i at lds[i * TRACE_TB]
