@@ -222,6 +222,61 @@ def test_direct_lighting_matches_oracle_per_pixel(oxr_mod, oracle, make, W, spp)
     assert frac <= 2e-3, frac
 
 
+def _many_lights(n_lights):
+    """A Lambert floor and box lit by n_lights coloured point and spot
+    lights and a small emitter: n_lights + 1 NEE slots per path, so the
+    shading kernel stages its shadow rays in LDS up to 4 slots and appends
+    them per light beyond."""
+    from oxr import _abi as A
+    s = scenes.Scene(f"lights{n_lights}")
+    fl = s.add_material(scenes.lambert((0.7, 0.7, 0.7)))
+    t, nr = scenes.quad([[-3, 0, 3], [3, 0, 3], [3, 0, -3], [-3, 0, -3]])
+    s.add_tris(t, fl, normals=nr)
+    bx = s.add_material(A.material((0.6, 0.5, 0.4, 1.0), metallic=0.2, roughness=0.4))
+    t, nr = scenes.cuboid((-0.5, 0.0, -0.5), (0.5, 1.0, 0.5))
+    s.add_tris(t, bx, normals=nr)
+    em = s.add_material(scenes.lambert((0.0, 0.0, 0.0), emissive=(6.0, 6.0, 5.0)))
+    t, nr = scenes.quad([[-0.3, 2.5, 0.3], [0.3, 2.5, 0.3], [0.3, 2.5, -0.3], [-0.3, 2.5, -0.3]])
+    s.add_tris(t[:, [0, 1, 2, 6, 7, 8, 3, 4, 5]], em)  # facing down
+    for k in range(n_lights):
+        a = 2.0 * math.pi * k / n_lights
+        pos = (2.0 * math.cos(a), 2.0 + 0.3 * k, 2.0 * math.sin(a))
+        col = (1.0, 0.5 + 0.1 * k, 1.0 - 0.15 * k)
+        if k % 2:
+            d = [-pos[0], -pos[1], -pos[2]]
+            nd = math.sqrt(sum(v * v for v in d))
+            s.lights.append(A.light(A.OXR_LIGHT_SPOT, color=col, intensity=6.0, position=pos,
+                                    direction=tuple(v / nd for v in d), inner=0.3, outer=0.7))
+        else:
+            s.lights.append(A.light(A.OXR_LIGHT_POINT, color=col, intensity=5.0, position=pos))
+    s.camera = scenes.CameraSpec((0.0, 3.0, 4.5), (0.0, 0.4, 0.0), yfov_deg=50.0)
+    s.ambient = 0.2
+    return s
+
+
+@pytest.mark.parametrize("n_lights", [3, 5])
+def test_many_light_slots_match_oracle(oxr_mod, oracle, n_lights):
+    """Direct lighting per pixel and full bounces within 3 sigma with 4
+    (staged) and 6 (appended per light) NEE slots per path."""
+    s = _many_lights(n_lights)
+    W = 40
+    _, ag, _ = _render_gpu(oxr_mod, s, W, 8, 1)
+    ac, _ = _render_cpu(oracle, s, W, 8, 1)
+    assert np.array_equal(ag[..., 3], ac[..., 3])
+    cov = ac[..., 3] > 0
+    mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
+    mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
+    rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
+    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+    _, ag, sqg = _render_gpu(oxr_mod, s, 24, 32, 6, seed=3)
+    ac, sqc = _render_cpu(oracle, s, 24, 32, 6, seed=3)
+    cov = ac[..., 3] > 0
+    n = ac[..., 3][cov][:, None]
+    mg, mc = ag[..., :3][cov] / n, ac[..., :3][cov] / n
+    sigma = np.sqrt(np.maximum(sqc[cov] / n - mc ** 2, 0.0) / n) + 1e-6
+    assert (np.abs(mg - mc) / sigma > 3).mean() < 0.01
+
+
 @pytest.mark.parametrize("make,W,spp,b", [(scenes.cornell, 32, 64, 8), (scenes.reference_box, 32, 64, 6),
                                           (scenes.glossy_caustic, 24, 32, 8),
                                           (lambda: scenes.material_sweep(segments=32, rings=16, floor_half=6), 32, 32,
