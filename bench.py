@@ -68,35 +68,77 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(scene, cfg, ambient, target_s=12.0, single_s=6.0):
+def lease_threads():
+    """Host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where the host states its share that way."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(n, 1)
+
+
+def converged_error(acc_g, sq_g, n_g, acc_c, sq_c, n_c):
+    """GPU mean (n_g samples, seed 0) against the oracle's f64 mean (n_c
+    samples, another seed) over the same pixels, per pixel and channel:
+    z = |m_g - m_c| / sqrt(var_g / n_g + var_c / n_c), on pixels every
+    sample of both covered."""
+    full = (acc_g[..., 3] == n_g) & (acc_c[..., 3] == n_c)
+    mg, mc = acc_g[..., :3][full] / n_g, acc_c[..., :3][full] / n_c
+    vg = np.maximum(sq_g[full] / n_g - mg * mg, 0.0)
+    vc = np.maximum(sq_c[full] / n_c - mc * mc, 0.0)
+    sig = np.sqrt(vg / n_g + vc / n_c)
+    d = np.abs(mg - mc)
+    z = np.where(sig > 0, d / np.where(sig > 0, sig, 1.0), np.where(d > 0, np.inf, 0.0))
+    return {"pixels": int(full.sum()), "gpu_spp": int(n_g), "cpu_spp": int(n_c),
+            "max_z": float(z.max()) if z.size else None,
+            "frac_gt_3sigma": float((z > 3).mean()) if z.size else None,
+            "mean_gpu": float(mg.mean()) if mg.size else None, "mean_cpu": float(mc.mean()) if mc.size else None}
+
+
+def cpu_baseline(scene, cfg, ambient, gpu_accum=None, target_s=10.0, single_s=5.0, runs=3):
     """The oracle (C++ restatement of the reference's band-parallel
-    refine) on a bounded band of rows of the same frame: all usable cores
-    (at most 16), then one thread on a smaller band."""
+    refine) on a centred band of rows of the same frame, with every core of
+    the lease: the median of `runs` timed runs of at least `target_s`, then
+    one run on one thread over 4 rows. The oracle samples with seed 1, so
+    its last run is also an estimate independent of the GPU's: with
+    gpu_accum = (accum, sum_sq, spp) of the GPU frame (seed 0) the per-pixel
+    error over the band is reported (converged_error)."""
     import oracle as O
-    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
+    threads = lease_threads()
     os_ = O.OracleScene(scene)
     W, H = cfg["width"], cfg["height"]
-    opts = O.options(W, H, max_bounces=cfg["bounces"], ambient=ambient)
+    opts = O.options(W, H, max_bounces=cfg["bounces"], ambient=ambient, seed=1)
 
-    def timed(rows, nthreads, budget_s):
+    def timed(rows, nthreads, budget_s, nruns):
         y0 = max(0, H // 2 - rows // 2)
         t = time.perf_counter()
         os_.render(W, H, 0, 1, opts, y0=y0, y1=y0 + rows, threads=nthreads)
         dt1 = time.perf_counter() - t
-        spp = int(max(1, min(512, budget_s / max(dt1, 1e-3))))
-        t = time.perf_counter()
-        os_.render(W, H, 1, spp, opts, y0=y0, y1=y0 + rows, threads=nthreads)
-        dt = time.perf_counter() - t
-        return rows * W * spp / dt, spp, y0, dt
+        spp = int(max(1, min(1024, math.ceil(budget_s / max(dt1, 1e-3)))))
+        rates, last = [], None
+        for _ in range(nruns):
+            t = time.perf_counter()
+            last = os_.render(W, H, 1, spp, opts, y0=y0, y1=y0 + rows, threads=nthreads, sum_sq=True)
+            rates.append(rows * W * spp / (time.perf_counter() - t))
+        return float(np.median(rates)), rates, spp, y0, last
 
-    rows = min(H, 4 * threads * 2)
-    v, spp, y0, dt = timed(rows, threads, target_s)
-    v1, spp1, y01, dt1 = timed(4, 1, single_s)
-    return {"value": v, "unit": "paths/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
-            "hardware_threads": os.cpu_count(), "single_thread_paths_per_s": v1,
-            "sample": f"oracle refine({spp}) of rows {y0}..{y0 + rows} of the {W}x{H} frame "
-                      f"({rows * W * spp} paths, {dt:.1f} s, band-parallel f64 accumulate); one thread: "
-                      f"refine({spp1}) of rows {y01}..{y01 + 4} ({dt1:.1f} s)"}
+    rows = min(H, 8 * threads)
+    v, rates, spp, y0, (acc_c, sq_c) = timed(rows, threads, target_s, runs)
+    v1, _, spp1, y01, _ = timed(4, 1, single_s, 1)
+    out = {"value": v, "unit": "paths/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "hardware_threads": os.cpu_count(), "single_thread_paths_per_s": v1,
+           "runs_paths_per_s": rates,
+           "sample": f"oracle refine({spp}) of rows {y0}..{y0 + rows} of the {W}x{H} frame "
+                     f"({rows * W * spp} paths per run, median of {runs} runs of >= {target_s:.0f} s, "
+                     f"{threads} threads, band-parallel f64 accumulate, seed 1); one thread: "
+                     f"refine({spp1}) of rows {y01}..{y01 + 4} (one run)"}
+    if gpu_accum is not None:
+        ag, sg, n_g = gpu_accum
+        err = converged_error(ag[y0:y0 + rows], sg[y0:y0 + rows], n_g, acc_c, sq_c, spp)
+        err["rows"] = [y0, y0 + rows]
+        out["error_vs_gpu"] = err
+    return out
 
 
 def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sustained=128):
@@ -138,6 +180,90 @@ def serving_bench(n=512, lam=40.0, seed=0, n_soups=4, sustained_lam=10.0, n_sust
         o2.pop("server", None)
         out["at_lower_rate"] = o2
     srv.close()
+    return out
+
+
+def roofline_of(pt, spp, tot, steps, workload, W):
+    """The closest-hit roofline of one refine(spp) step: algorithmic bytes
+    per launch over the average HIP-event launch time (tot: the timed
+    step's stats scaled by `steps`), against 8 TB/s."""
+    bpr, n_node, n_tri, nr = bytes_per_closest_ray(pt, spp)
+    avg_ms = tot["ms_closest"] / tot["launches_closest"]
+    bytes_per_launch = bpr * tot["closest_rays"] / tot["launches_closest"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # Measured HBM bytes per launch: the PMC figure per closest ray
+    # (FETCH_SIZE + WRITE_SIZE, copy-calibrated, profiles/) times this
+    # run's rays per launch.
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"pathtrace_{workload}_{W}.json")
+    if os.path.exists(prof):
+        pj = json.load(open(prof))
+        if "hbm_bytes_per_closest_ray" in pj:
+            traffic = pj["hbm_bytes_per_closest_ray"] * tot["closest_rays"] / tot["launches_closest"]
+    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+         "algorithmic_bytes_per_launch": bytes_per_launch,
+         "kernel": "k_closest<true>", "avg_launch_ms": avg_ms,
+         "bytes_per_ray": bpr, "nodes_per_ray": n_node, "tris_per_ray": n_tri, "rays_counted": nr}
+    if traffic is not None:
+        r["hbm_traffic_gbs"] = traffic / (avg_ms * 1e-3) / 1e9
+        r["hbm_traffic_frac"] = r["hbm_traffic_gbs"] / HBM_PEAK_GBS
+        r["served_from_cache_frac"] = max(0.0, 1.0 - traffic / bytes_per_launch)
+    return r
+
+
+def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16):
+    """The large-scene config at its full size (12.4M triangles, 1920x1080,
+    8 bounces), bounded to a few steps of spp_per_step: paths/s, rays/s,
+    the closest-hit roofline from the device's own fetch counts, and the
+    error against the oracle (GPU frame at seed 0 vs the oracle's f64
+    accumulation of a centred crop at cpu_spp_factor x the samples, seed 1)."""
+    import oxr
+    scene, cfg = workload("terrain")
+    W, H = cfg["width"], cfg["height"]
+    pt = oxr.PathTracer(0)
+    opts = oxr.RenderOptions(width=W, height=H, ambient=getattr(scene, "ambient", 0.0),
+                             path_trace=oxr.PathTraceOptions(samples_per_pixel=cfg["spp"], max_bounces=cfg["bounces"]))
+    t = time.perf_counter()
+    pt.sync(scene, opts)
+    sync_s = time.perf_counter() - t
+    pt.refine(spp_per_step)
+    pt.reset()
+    pt.set_timing(False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pt.refine(spp_per_step)
+    dt = time.perf_counter() - t0
+    acc_g, sq_g = pt.accum(with_sum_sq=True)
+    pt.set_timing(True)
+    pt.refine(spp_per_step)
+    st = pt.stats()
+    pt.set_timing(False)
+    tot = {k: st[k] * steps for k in ("closest_rays", "shadow_rays", "ms_closest", "ms_anyhit", "ms_shade",
+                                      "ms_other")}
+    tot["launches_closest"] = st["launches_closest"] * steps
+    out = {"workload": f"terrain {W}x{H}, {scene.num_tris} tris, {cfg['bounces']} bounces, "
+                       f"{steps * spp_per_step} spp timed",
+           "paths_per_s": W * H * spp_per_step * steps / dt,
+           "rays_per_s": (tot["closest_rays"] + tot["shadow_rays"]) / dt, "ms_per_step": dt * 1e3 / steps,
+           "kernel_ms": {"closest": tot["ms_closest"], "anyhit": tot["ms_anyhit"], "shade": tot["ms_shade"],
+                         "other": tot["ms_other"]},
+           "scene_sync_s": sync_s, "bvh": pt.bvh_info()}
+    if tot["launches_closest"]:
+        out["roofline"] = roofline_of(pt, spp_per_step, tot, steps, "terrain", W)
+    pt.close()
+    import oracle as O
+    t = time.perf_counter()
+    os_ = O.OracleScene(scene)
+    y0, x0 = H // 2 - crop // 2, W // 2 - crop // 2
+    n_g = steps * spp_per_step
+    n_c = cpu_spp_factor * n_g
+    oo = O.options(W, H, max_bounces=cfg["bounces"], ambient=getattr(scene, "ambient", 0.0), seed=1)
+    ac, sc = os_.render_rect(W, H, 0, n_c, x0, x0 + crop, y0, y0 + crop, opts=oo, threads=lease_threads())
+    err = converged_error(acc_g[y0:y0 + crop, x0:x0 + crop], sq_g[y0:y0 + crop, x0:x0 + crop], n_g, ac, sc, n_c)
+    err["crop"] = [y0, y0 + crop, x0, x0 + crop]
+    err["oracle_s"] = time.perf_counter() - t
+    out["error"] = err
     return out
 
 
@@ -267,6 +393,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-guide", action="store_true", help="skip the guided vs plain time-to-MSE leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency leg")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the 12.4M-triangle terrain leg")
+    ap.add_argument("--terrain-only", action="store_true")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--guide-only", action="store_true")
     ap.add_argument("--guide-width", type=int, default=1024)
@@ -277,11 +405,11 @@ def main():
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
-    if a.serving_only or a.guide_only:
+    if a.serving_only or a.guide_only or a.terrain_only:
         import oxr
         if oxr.device_count() == 0:
             raise SystemExit("bench.py: no HIP device visible")
-        print(json.dumps(serving_bench() if a.serving_only else
+        print(json.dumps(serving_bench() if a.serving_only else terrain_leg() if a.terrain_only else
                          guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s,
                                         guide_cfg=json.loads(a.guide_config) if a.guide_config else None)), flush=True)
         return
@@ -316,6 +444,7 @@ def main():
     dt = t1 - t0
     paths = W * H * a.spp_per_step * a.steps
     value = paths / dt
+    gpu_accum = pt.accum(with_sum_sq=True) + (a.spp_per_step * a.steps,) if not a.no_cpu_baseline else None
     # One more step with HIP events around every kernel (eager launches):
     # the per-kernel breakdown and ray counts, scaled to the timed steps.
     pt.set_timing(True)
@@ -352,35 +481,22 @@ def main():
         "kernel_ms_per_step_source": "one extra step with per-kernel HIP events, x steps",
     }
     if not a.no_roofline and tot["launches_closest"]:
-        bpr, n_node, n_tri, nr = bytes_per_closest_ray(pt, a.spp_per_step)
-        avg_ms = tot["ms_closest"] / tot["launches_closest"]
-        bytes_per_launch = bpr * tot["closest_rays"] / tot["launches_closest"]
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        # Measured HBM bytes per launch: the PMC figure per closest ray
-        # (FETCH_SIZE + WRITE_SIZE, copy-calibrated, profiles/) times this
-        # run's rays per launch.
-        traffic = None
-        prof = os.path.join(ROOT, "profiles", f"pathtrace_{a.workload}_{W}.json")
-        if os.path.exists(prof):
-            pj = json.load(open(prof))
-            if "hbm_bytes_per_closest_ray" in pj:
-                traffic = pj["hbm_bytes_per_closest_ray"] * tot["closest_rays"] / tot["launches_closest"]
-        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                           "algorithmic_bytes_per_launch": bytes_per_launch,
-                           "kernel": "k_closest<true>", "avg_launch_ms": avg_ms,
-                           "bytes_per_ray": bpr, "nodes_per_ray": n_node, "tris_per_ray": n_tri,
-                           "rays_counted": nr}
+        out["roofline"] = roofline_of(pt, a.spp_per_step, tot, a.steps, a.workload, W)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scene, cfg, ambient)
+        out["cpu_baseline"] = cpu_baseline(scene, cfg, ambient, gpu_accum=gpu_accum)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["error"] = dict(out["cpu_baseline"].pop("error_vs_gpu"),
+                            what="per pixel and channel |GPU mean - oracle mean| / sigma of the difference over "
+                                 "the CPU leg's rows (oracle seed 1, GPU seed 0: independent estimates)")
     pt.close()
     # The other two parts of the metric, each reported beside the paths/s
     # value (an error in one is reported, not fatal to the line).
     guide_leg = lambda: guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s,
                                         guide_cfg=json.loads(a.guide_config) if a.guide_config else None)  # noqa: E731
-    for key, skip, fn in (("guided_time_to_mse", a.no_guide, guide_leg),
-                          ("serving", a.no_serving, serving_bench)):
+    legs = (("large_scene", a.no_terrain or a.workload == "terrain", terrain_leg),
+            ("guided_time_to_mse", a.no_guide, guide_leg),
+            ("serving", a.no_serving, serving_bench))
+    for key, skip, fn in legs:
         if skip:
             continue
         try:

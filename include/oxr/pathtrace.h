@@ -456,16 +456,24 @@ int oxr_sobol_owen_4d(int device, uint32_t n, const uint32_t* index,
                       const uint32_t* pattern_seed, float* out);
 
 /* Closest hit / occlusion of n host rays against the uploaded scene, all
- * candidates accepted (no alpha filter). flags bit 0: cull back faces of
- * single-sided materials (the primary-ray rule). */
+ * candidates accepted (no MASK or BLEND alpha filter), as
+ * TraceScene::closest_hit / occluded (src/trace.rs:396-450). flags bit 0:
+ * cull back faces of single-sided materials (the primary-ray rule). */
 int oxr_trace_closest(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint32_t flags,
                       oxr_hit* hits);
 int oxr_trace_any(oxr_pt* h, uint32_t n, const oxr_ray* rays, uint8_t* occluded);
 
-/* The same queries on rays already resident in device memory (the bench
- * path; `stream` is a hipStream_t or NULL for the handle's stream). */
+/* The same closest-hit query on rays already resident in device memory
+ * (`stream` is a hipStream_t or NULL for the handle's stream). It shares
+ * the handle's traversal buffers with refine, so on another stream it is
+ * ordered after the handle's queued work and before its later work. The
+ * call only enqueues; oxr_trace_device_check reports its outcome. */
 int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays,
                              void* d_hits, void* stream);
+/* Waits for the handle's stream and reports a traversal stack overflow of
+ * the last oxr_trace_closest_device call (OXR_ERR_DEVICE; its hits are
+ * then misses). Call it before the next refine or trace on the handle. */
+int oxr_trace_device_check(oxr_pt* h);
 
 /* Flattened BVH statistics: node and leaf counts, SAH cost, depth. */
 typedef struct oxr_bvh_info {

@@ -35,6 +35,8 @@ def lib():
         L.oxrcpu_emissive_table.argtypes = [C.c_void_p, up, fp, fp, up]
         L.oxrcpu_render.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp, dp]
+        L.oxrcpu_render_rect.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp, dp]
         L.oxrcpu_sample_one.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
                                         C.c_uint32, C.c_uint32, fp]
         L.oxrcpu_trace_closest.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(A.Ray), C.c_uint32, C.c_int,
@@ -78,10 +80,14 @@ def lib():
         L.oxrcpu_surface.argtypes = [C.c_void_p, C.c_uint32, fp, fp]
         L.oxrcpu_resolve_hdr.argtypes = [dp, C.c_uint32, C.c_uint32, C.POINTER(A.Display), fp]
         L.oxrcpu_resolve_rgba8.argtypes = [dp, C.c_uint32, C.c_uint32, C.POINTER(A.Display), bp]
+        L.oxrcpu_tone_map.argtypes = [C.c_uint32, C.c_uint32, fp, fp]
+        L.oxrcpu_linear_to_srgb_byte.argtypes = [C.c_uint32, fp, bp]
+        L.oxrcpu_srgb_to_linear.argtypes = [C.c_uint32, fp, fp]
         L.oxrcpu_record_rays.restype = C.c_uint32
         L.oxrcpu_record_rays.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
         L.oxrcpu_bvh4_count.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
+        L.oxrcpu_bvh4_stack_depth.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
         L.oxrcpu_tex_sample.restype = C.c_int
         L.oxrcpu_tex_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, fp, fp, fp]
         L.oxrcpu_tex_grad_lod.restype = C.c_int
@@ -141,6 +147,21 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle render failed")
         return (acc, sq) if sum_sq else acc
+
+    def render_rect(self, width, height, first, n, x0, x1, y0, y1, opts=None, threads=0, cam=None):
+        """Samples [first, first+n) of the pixels in columns [x0, x1) and
+        rows [y0, y1) of a width x height frame: (accum (h, w, 4),
+        sum_sq (h, w, 3)), f64 sums in sample order."""
+        if opts is None:
+            opts = options(width, height, ambient=getattr(self.scene, "ambient", 0.0))
+        cam = cam or self.scene.camera.resolve(width, height)
+        acc = np.zeros((y1 - y0, x1 - x0, 4), dtype=np.float64)
+        sq = np.zeros((y1 - y0, x1 - x0, 3), dtype=np.float64)
+        rc = lib().oxrcpu_render_rect(self.h, C.byref(cam), C.byref(opts), first, n, x0, x1, y0, y1, threads,
+                                      acc.ctypes.data_as(dp), sq.ctypes.data_as(dp))
+        if rc != 0:
+            raise RuntimeError("oracle render_rect failed")
+        return acc, sq
 
     def sample_one(self, width, height, x, y, index, opts=None, cam=None):
         if opts is None:
@@ -440,6 +461,30 @@ def resolve_rgba8(accum, samples, background=(0, 0, 0, 255), tone_map=0, exposur
     return out
 
 
+def tone_map(rgb, tm=0):
+    """ToneMap::apply (src/hdr.rs:44-66) on (n, 3) scene-linear colours."""
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    out = np.zeros_like(rgb)
+    lib().oxrcpu_tone_map(tm, len(rgb), rgb.ctypes.data_as(fp), out.ctypes.data_as(fp))
+    return out
+
+
+def linear_to_srgb_byte(c):
+    """linear_to_srgb_byte (src/hdr.rs:116-129): threshold binary search."""
+    c = np.ascontiguousarray(c, np.float32).ravel()
+    out = np.zeros(len(c), np.uint8)
+    lib().oxrcpu_linear_to_srgb_byte(len(c), c.ctypes.data_as(fp), out.ctypes.data_as(bp))
+    return out
+
+
+def srgb_to_linear(c):
+    """srgb_to_linear (src/hdr.rs:76-82)."""
+    c = np.ascontiguousarray(c, np.float32).ravel()
+    out = np.zeros(len(c), np.float32)
+    lib().oxrcpu_srgb_to_linear(len(c), c.ctypes.data_as(fp), out.ctypes.data_as(fp))
+    return out
+
+
 def bvh4_count(nodes, tris, rays):
     """(nodes fetched, triangles fetched) per ray on an exported device BVH."""
     rays = np.ascontiguousarray(rays, np.float32)
@@ -447,6 +492,16 @@ def bvh4_count(nodes, tris, rays):
     lib().oxrcpu_bvh4_count(nodes.ctypes.data, tris.ctypes.data, len(rays), rays.ctypes.data_as(fp),
                             counts.ctypes.data_as(up))
     return counts
+
+
+def bvh4_stack_depth(nodes, tris, rays):
+    """Most traversal-stack entries each ray holds at once on an exported
+    device BVH (closest-hit order)."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    depth = np.zeros(len(rays), np.uint32)
+    lib().oxrcpu_bvh4_stack_depth(nodes.ctypes.data, tris.ctypes.data, len(rays), rays.ctypes.data_as(fp),
+                                  depth.ctypes.data_as(up))
+    return depth
 
 
 # ---------------------------------------------------------------------------

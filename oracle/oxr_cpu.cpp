@@ -1659,12 +1659,11 @@ static void build_bvh(Scene& s) {
   struct Task {
     uint32_t node, lo, hi;
   };
-  s.nodes.push_back(BNode{});
-  std::vector<Task> stack{{0, 0, s.n}};
   const int NB = 16;
-  while (!stack.empty()) {
-    Task t = stack.back();
-    stack.pop_back();
+  // One node of the top-down build: bounds, the binned-SAH split (or a
+  // median split when binning cannot separate), and either a leaf or two
+  // child tasks pushed right then left.
+  auto split = [&](Task t, std::vector<BNode>& nodes, std::vector<uint32_t>& slots, std::vector<Task>& out) {
     BNode nd;
     float cmin[3] = {INF_F, INF_F, INF_F}, cmax[3] = {-INF_F, -INF_F, -INF_F};
     for (int k = 0; k < 3; ++k) {
@@ -1777,20 +1776,75 @@ static void build_bvh(Scene& s) {
                        });
     }
     if (cnt <= 4) {
-      nd.left_or_first = (uint32_t)s.slots.size();
+      nd.left_or_first = (uint32_t)slots.size();
       nd.count = cnt;
-      for (uint32_t i = t.lo; i < t.hi; ++i) s.slots.push_back(items[i].g);
-      s.nodes[t.node] = nd;
+      for (uint32_t i = t.lo; i < t.hi; ++i) slots.push_back(items[i].g);
+      nodes[t.node] = nd;
     } else {
-      uint32_t left = (uint32_t)s.nodes.size();
-      s.nodes.push_back(BNode{});
-      s.nodes.push_back(BNode{});
+      uint32_t left = (uint32_t)nodes.size();
+      nodes.push_back(BNode{});
+      nodes.push_back(BNode{});
       nd.left_or_first = left;
       nd.count = 0;
-      s.nodes[t.node] = nd;
-      stack.push_back({left + 1, mid, t.hi});
-      stack.push_back({left, t.lo, mid});
+      nodes[t.node] = nd;
+      out.push_back({left + 1, mid, t.hi});
+      out.push_back({left, t.lo, mid});
     }
+  };
+  // The top of the tree is built here until every open task holds at most
+  // n / (8 threads) triangles; those subtrees are then built by worker
+  // threads into their own arrays (they partition disjoint item ranges)
+  // and appended in task order. The tree is the one a serial build makes;
+  // only the node numbering differs.
+  s.nodes.push_back(BNode{});
+  const uint32_t nthreads = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+  const uint32_t grain = std::max<uint32_t>(4096, s.n / (8 * nthreads));
+  std::vector<Task> stack{{0, 0, s.n}}, frontier;
+  while (!stack.empty()) {
+    Task t = stack.back();
+    stack.pop_back();
+    if (t.hi - t.lo > grain || nthreads == 1)
+      split(t, s.nodes, s.slots, stack);
+    else
+      frontier.push_back(t);
+  }
+  struct Sub {
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> slots;
+  };
+  std::vector<Sub> subs(frontier.size());
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      size_t k = next.fetch_add(1);
+      if (k >= frontier.size()) break;
+      Sub& sb = subs[k];
+      sb.nodes.push_back(BNode{});
+      std::vector<Task> st{{0, frontier[k].lo, frontier[k].hi}};
+      while (!st.empty()) {
+        Task t = st.back();
+        st.pop_back();
+        split(t, sb.nodes, sb.slots, st);
+      }
+    }
+  };
+  {
+    std::vector<std::thread> ts;
+    for (uint32_t i = 0; i < nthreads && i < frontier.size(); ++i) ts.emplace_back(work);
+    for (auto& th : ts) th.join();
+  }
+  for (size_t k = 0; k < frontier.size(); ++k) {
+    const Sub& sb = subs[k];
+    const uint32_t base = (uint32_t)s.nodes.size(), slot0 = (uint32_t)s.slots.size();
+    // Local node i > 0 lands at base + i - 1; the local root replaces the
+    // frontier task's placeholder.
+    auto fix = [&](BNode nd) {
+      nd.left_or_first = nd.count ? nd.left_or_first + slot0 : base + nd.left_or_first - 1;
+      return nd;
+    };
+    s.nodes[frontier[k].node] = fix(sb.nodes[0]);
+    for (size_t i = 1; i < sb.nodes.size(); ++i) s.nodes.push_back(fix(sb.nodes[i]));
+    s.slots.insert(s.slots.end(), sb.slots.begin(), sb.slots.end());
   }
 }
 
@@ -2123,11 +2177,9 @@ struct Integrator {
     if (pt.clamp > 0.0f && m > pt.clamp) return scale(c, pt.clamp / m);
     return c;
   }
-  // pixel_spread (src/trace.rs:1138-1145): the cone angle of one pixel.
-  float pixel_spread() const {
-    if (cam.projection == OXR_PROJ_ORTHOGRAPHIC) return 0.0f;
-    return std::atan(2.0f * cam.half_h / (float)std::max(height, 1u));
-  }
+  // PathTracer::refine (src/pathtrace.rs:2537-2540): spread = 2 half_h /
+  // height, the same for both projections (no atan).
+  float pixel_spread() const { return 2.0f * cam.half_h / (float)std::max(height, 1u); }
   // Integrator::p_env (src/pathtrace.rs:1915-1921).
   float p_env() const { return ts.env ? (ts.l_tris.empty() ? 1.0f : 0.5f) : 0.0f; }
 
@@ -2640,6 +2692,48 @@ int oxrcpu_render(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* 
   return 0;
 }
 
+// Samples [first, first+n) of the pixels in columns [x0, x1) and rows
+// [y0, y1) only, into accum ((y1-y0) x (x1-x0) x 4) and sum_sq: per pixel
+// the same f64 sums in sample order as oxrcpu_render; rows go to threads.
+int oxrcpu_render_rect(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t first, uint32_t n,
+                       uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, int threads, double* accum, double* sum_sq) {
+  if (!h || !cam || !opts || !accum) return -1;
+  uint32_t W = std::max<uint32_t>(opts->width, 1), H = std::max<uint32_t>(opts->height, 1);
+  if (y1 > H || y0 >= y1 || x1 > W || x0 >= x1) return -1;
+  Integrator integ{h->s, *cam, W, H, std::isfinite(opts->ambient) ? fmaxr(opts->ambient, 0.0f) : 0.0f, *opts};
+  const uint32_t rows = y1 - y0, cols = x1 - x0;
+  std::atomic<uint32_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      uint32_t r = next.fetch_add(1);
+      if (r >= rows) break;
+      for (uint32_t c = 0; c < cols; ++c) {
+        double* px = accum + 4ull * ((size_t)r * cols + c);
+        double* sq = sum_sq ? sum_sq + 3ull * ((size_t)r * cols + c) : nullptr;
+        for (uint32_t si = 0; si < n; ++si) {
+          f3 v;
+          if (integ.sample(x0 + c, y0 + r, first + si, v)) {
+            for (int k = 0; k < 3; ++k) px[k] += (double)v[k];
+            px[3] += 1.0;
+            if (sq)
+              for (int k = 0; k < 3; ++k) sq[k] += (double)v[k] * (double)v[k];
+          }
+        }
+      }
+    }
+  };
+  int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  nt = std::max(1, (int)std::min<uint32_t>((uint32_t)nt, rows));
+  if (nt <= 1) {
+    work();
+  } else {
+    std::vector<std::thread> ts;
+    for (int i = 0; i < nt; ++i) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+  }
+  return 0;
+}
+
 // One sample of one pixel: writes rgb, returns 1 covered / 0 uncovered.
 int oxrcpu_sample_one(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t x, uint32_t y,
                       uint32_t index, float* rgb) {
@@ -2978,6 +3072,22 @@ void oxrcpu_resolve_rgba8(const double* accum, uint32_t npx, uint32_t samples, c
 }
 
 
+// The display-transform pieces of the resolve, one value at a time, for the
+// pins of src/hdr.rs:222-279: ToneMap::apply (src/hdr.rs:44-66),
+// linear_to_srgb_byte (:116-129) and srgb_to_linear (:76-82).
+void oxrcpu_tone_map(uint32_t tm, uint32_t n, const float* rgb, float* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    f3 c = tone_map(tm, {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]});
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = c[k];
+  }
+}
+void oxrcpu_linear_to_srgb_byte(uint32_t n, const float* c, uint8_t* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = linear_to_srgb_byte(c[i]);
+}
+void oxrcpu_srgb_to_linear(uint32_t n, const float* c, float* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = srgb_to_linear(c[i]);
+}
+
 // Records the closest-hit rays of samples [first, first+n) of rows
 // [y0, y1) (single thread, row-major, sample-minor) into out (8 floats per
 // ray: o, k, d, t_max), at most max_rays. Returns the number recorded.
@@ -3006,8 +3116,11 @@ done:
 // them; 48-byte triangles: v0, gid, v1, flags, v2, alpha), restating the
 // device's closest-hit order: leaves of a node first, inner children near
 // to far.
-// counts[2*i] = nodes fetched, counts[2*i+1] = triangles fetched.
-void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts) {
+// counts[2*i] = nodes fetched, counts[2*i+1] = triangles fetched;
+// depth[i] (when not null) = the most entries the ray's traversal stack
+// held at once (the device keeps 16 in LDS and spills the rest).
+static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts,
+                      uint32_t* depth) {
   for (uint32_t r = 0; r < n_rays; ++r) {
     const float* ry = rays + 8ull * r;
     f3 o = {ry[0], ry[1], ry[2]}, d = {ry[4], ry[5], ry[6]};
@@ -3016,7 +3129,7 @@ void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, c
     f3 inv = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
     f3 oi = {o[0] * inv[0], o[1] * inv[1], o[2] * inv[2]};
     std::vector<uint32_t> stack;
-    uint32_t node = 0, nn = 0, nt = 0;
+    uint32_t node = 0, nn = 0, nt = 0, dmax = 0;
     uint32_t best = 0xffffffffu;
     for (;;) {
       ++nn;
@@ -3073,13 +3186,24 @@ void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, c
           }
       for (int c = 0; c < 4; ++c)
         if (tn[c] != INF_F && tn[c] <= tmax * (1.0f + 1.0f / 65536.0f)) stack.push_back(ci[c]);
+      dmax = std::max(dmax, (uint32_t)stack.size());
       if (stack.empty()) break;
       node = stack.back();
       stack.pop_back();
     }
     counts[2 * r] = nn;
     counts[2 * r + 1] = nt;
+    if (depth) depth[r] = dmax;
   }
+}
+
+void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts) {
+  bvh4_walk(nodes, tris, n_rays, rays, counts, nullptr);
+}
+void oxrcpu_bvh4_stack_depth(const float* nodes, const float* tris, uint32_t n_rays, const float* rays,
+                             uint32_t* depth) {
+  std::vector<uint32_t> counts(2ull * std::max(n_rays, 1u));
+  bvh4_walk(nodes, tris, n_rays, rays, counts.data(), depth);
 }
 
 }  // extern "C"

@@ -570,11 +570,15 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
     }
   }
   // Punctual lights (zero-power lights contribute exactly nothing and are
-  // dropped).
+  // dropped). Each keeps its index in the full list: the BLEND coin of its
+  // shadow rays is keyed by it (src/pathtrace.rs:2147-2166).
   std::vector<DevLight> lights;
   for (uint32_t i = 0; i < d->num_lights; ++i) {
     DevLight dl;
-    if (prepare_light(d->lights[i], dl)) lights.push_back(dl);
+    if (prepare_light(d->lights[i], dl)) {
+      dl.index = i;
+      lights.push_back(dl);
+    }
   }
   if (lights.size() > 4096) return fail(OXR_ERR_INVALID, "too many punctual lights (%zu)", lights.size());
   // EmissiveLights::build (src/pathtrace.rs:1479-1543).
@@ -732,8 +736,11 @@ DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const Dev
   f.clamp = std::isfinite(opts.clamp) ? opts.clamp : 0.0f;
   f.ambient = std::isfinite(opts.ambient) ? std::fmax(opts.ambient, 0.0f) : 0.0f;
   f.nslots = sc.num_lights + 1;
-  // pixel_spread (src/trace.rs:1138-1145): the cone angle of one pixel row.
-  f.spread = cam.projection == OXR_PROJ_ORTHOGRAPHIC ? 0.0f : std::atan(2.0f * cam.half_h / (float)f.height);
+  // PathTracer::refine (src/pathtrace.rs:2537-2540): the cone spread is the
+  // image-plane height of one pixel row, 2 half_h / height, for both
+  // projections; the ray cone at the first hit is spread (orthographic) or
+  // spread * t (perspective).
+  f.spread = 2.0f * cam.half_h / (float)f.height;
   if (env && env->img) {
     f.env = *env;
     f.p_env = sc.num_emissive > 0 ? 0.5f : 1.0f;  // Integrator::p_env (src/pathtrace.rs:1915-1921)
@@ -865,6 +872,7 @@ struct Engine {
   double guide_infer_ms = 0.0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
+  hipEvent_t ev_order[2] = {nullptr, nullptr};  // orders a caller's stream against `stream`
   struct Span {
     int cat;
     hipEvent_t a, b;
@@ -886,6 +894,8 @@ struct Engine {
     dfree(d_spill);
     if (d_tmp) (void)hipFree(d_tmp);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (auto e : ev_order)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -2228,10 +2238,35 @@ int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays, void* d_
   int rc;
   if ((rc = e.ensure_spill(h->scene.spill_cap))) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : e.stream;
+  // The launch shares the handle's control block and spill buffer with
+  // refine: on a caller's stream it waits for the handle's stream, and the
+  // handle's stream then waits for it.
+  if (s != e.stream) {
+    for (auto& ev : e.ev_order)
+      if (!ev) HIPC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPC(hipEventRecord(e.ev_order[0], e.stream));
+    HIPC(hipStreamWaitEvent(s, e.ev_order[0], 0));
+  }
   launch::zero_ctrl(s, e.d_ctrl);
   launch::closest(s, e.grid_closest, h->scene.sc, nullptr, nullptr, false, false, (const float4*)d_rays, nullptr, nullptr, n,
                   nullptr, &e.d_ctrl->work[0].v, 0, (uint4*)d_hits, e.d_spill, e.spill_cap, &e.d_ctrl->error);
   HIPC(hipGetLastError());
+  if (s != e.stream) {
+    HIPC(hipEventRecord(e.ev_order[1], s));
+    HIPC(hipStreamWaitEvent(e.stream, e.ev_order[1], 0));
+  }
+  return OXR_OK;
+}
+
+int oxr_trace_device_check(oxr_pt* h) {
+  if (!h) return fail(OXR_ERR_INVALID, "null handle");
+  Engine& e = h->eng;
+  if (!e.d_ctrl) return OXR_OK;
+  HIPC(hipSetDevice(e.device));
+  Ctrl c{};
+  HIPC(hipMemcpyAsync(&c, e.d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, e.stream));
+  HIPC(hipStreamSynchronize(e.stream));
+  if (c.error) return fail(OXR_ERR_DEVICE, "traversal stack overflow");
   return OXR_OK;
 }
 

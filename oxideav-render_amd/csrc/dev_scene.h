@@ -44,7 +44,9 @@ struct alignas(16) DevLight {
   uint32_t has_range;
   float dir[3];
   float range;
-  float spot_scale, spot_offset, pad0, pad1;
+  float spot_scale, spot_offset;
+  uint32_t index;  // position in the scene's light list, zero-power lights included (the shadow-ray coin id)
+  uint32_t pad1;
 };
 static_assert(sizeof(DevLight) == 64, "DevLight is 64 bytes");
 
@@ -98,7 +100,7 @@ struct DevFrame {
   uint32_t nslots;  // NEE slots per path: punctual lights + 1 area
   float p_env;      // Integrator::p_env: 0, 1/2 or 1
   DevEnv env;
-  float spread;     // pixel_spread (src/trace.rs:1138-1145): the ray-cone angle
+  float spread;     // 2 half_h / height (src/pathtrace.rs:2537-2540): one pixel row on the image plane
   float pad_f[3];
 };
 

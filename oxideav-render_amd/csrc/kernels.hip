@@ -175,6 +175,11 @@ struct Trav {
   float tmin, tmax;
   bool negx, negy, negz, cull;
   uint32_t skip, path_seed, ray_id;
+  // Triangle flags the filter sees: all of them on path rays; only the
+  // double-sided bit on the parity entry points, which accept every
+  // candidate (TraceScene::closest_hit / occluded, src/trace.rs:396-450)
+  // apart from the optional back-face cull.
+  uint32_t fmask;
   uint32_t best;
   float bu, bv;
   uint32_t node;
@@ -259,7 +264,7 @@ struct Trav {
           bool front;
           if (!intersect_tri(wr, f4xyz(a), f4xyz(b), f4xyz(e), tmin, tmax, t, b1, b2, front)) continue;
           uint32_t gid = __float_as_uint(a.w);
-          uint32_t fl = __float_as_uint(b.w);
+          uint32_t fl = __float_as_uint(b.w) & fmask;
           float aw = e.w;
           if (ANY) {
             if (gid == skip) continue;
@@ -427,6 +432,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
     }
     if (ALPHA && !MULTI) tr.asc = asc;
     tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
+    tr.fmask = PATH ? ~0u : TRI_DOUBLE_SIDED;
   };
   auto step = [&](Trav<false, ALPHA>& tr) {
     if (COUNT) ++nnode;
@@ -498,13 +504,17 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
         uint32_t k = (m.z & 0xffffu);
         if (!((m.z >> 24) & PF_DONE)) k -= 1u;
         path_seed = hash_mix(m.w, m.y);
-        ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (j & 0x7fffu)));
+        // Punctual slot j is the j-th light kept at upload; the coin uses
+        // the light's index in the scene's full list.
+        const DevLight* lts = MULTI ? jobs[pjob[pid]].sc.lights : sc.lights;
+        ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (lts[j].index & 0x7fffu)));
       }
     } else {
       tmin = a.w;
     }
     if (ALPHA && !MULTI) tr.asc = asc;
     tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
+    tr.fmask = PATH ? ~0u : TRI_DOUBLE_SIDED;
   };
   auto step = [&](Trav<true, ALPHA>& tr) {
     return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri_unused) : tr.step(st, sc.nodes, sc.tris, ntri_unused);
@@ -1609,7 +1619,7 @@ __global__ void k_resolve_rgba8(uint32_t npx, uint32_t samples, const double* __
     return;
   }
   float a = clampr(o[3], 0.0f, 1.0f) * 255.0f;
-  out[i] = make_uchar4(srgb_byte(o[0]), srgb_byte(o[1]), srgb_byte(o[2]), (uint8_t)rintf(a));
+  out[i] = make_uchar4(srgb_byte(o[0]), srgb_byte(o[1]), srgb_byte(o[2]), (uint8_t)roundf(a));
 }
 
 // ---------------------------------------------------------------------

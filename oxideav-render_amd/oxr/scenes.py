@@ -780,6 +780,33 @@ def reference_box():
     return s
 
 
+
+def coplanar_overlaps(seed=5, n_base=40, copies=4, n_rotated=16):
+    """Triangles that overlap in the plane z = 0, for the equal-t
+    tie-break of closest hit (the lowest global id wins). n_base random
+    triangles with two-decimal corners in [0, 1]^2, each present `copies`
+    times with identical corners (so identical hit t) at randomly permuted
+    global ids, then n_rotated copies of base triangles with their vertices
+    rotated (same plane and region; t may differ in the last place).
+    One grey double-sided material. Returns (scene, corners (n, 3, 2) in
+    global-id order, base index per triangle (-1 for the rotated ones))."""
+    rng = np.random.default_rng(seed)
+    base = np.round(rng.random((n_base, 3, 2)), 2)
+    which = np.repeat(np.arange(n_base), copies)
+    perm = rng.permutation(len(which))
+    xy = base[which[perm]]
+    owner = which[perm]
+    rot = rng.choice(n_base, n_rotated, replace=False)
+    xy = np.concatenate([xy, np.roll(base[rot], 1, axis=1)])
+    owner = np.concatenate([owner, np.full(n_rotated, -1)])
+    pos = np.zeros((len(xy), 3, 3), F32)
+    pos[:, :, :2] = xy
+    sc = Scene("coplanar")
+    m = sc.add_material(A.material((0.5, 0.5, 0.5, 1.0), double_sided=True))
+    sc.add_tris(pos.reshape(-1, 9), m)
+    sc.camera = CameraSpec((0.5, 0.5, 2.0), (0.5, 0.5, 0.0), yfov_deg=40.0)
+    return sc, xy.astype(np.float64), owner
+
 # ---------------------------------------------------------------------------
 # Large-scene throughput: displaced terrain + emitters + instanced spheres.
 # ---------------------------------------------------------------------------

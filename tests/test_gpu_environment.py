@@ -96,7 +96,8 @@ def test_direct_lighting_with_environment_matches_oracle(which):
     mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
-    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+    print(f"direct lighting: max rel {rel.max():.3g}, > 1e-3: {(rel > 1e-3).mean():.2e}")
+    assert rel.max() <= 1e-3, rel.max()
 
 
 def test_full_bounce_with_environment_within_3_sigma():

@@ -217,9 +217,8 @@ def test_direct_lighting_matches_oracle_per_pixel(oxr_mod, oracle, make, W, spp)
     mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
-    frac = (rel > 1e-3).mean()
-    print(f"direct-lighting rel err: max {rel.max():.3g}, pixels > 1e-3: {frac:.2e}")
-    assert frac <= 2e-3, frac
+    print(f"direct-lighting rel err: max {rel.max():.3g}, pixels > 1e-3: {(rel > 1e-3).mean():.2e}")
+    assert rel.max() <= 1e-3, rel.max()
 
 
 def _many_lights(n_lights):
@@ -267,7 +266,7 @@ def test_many_light_slots_match_oracle(oxr_mod, oracle, n_lights):
     mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
-    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+    assert rel.max() <= 1e-3, rel.max()
     _, ag, sqg = _render_gpu(oxr_mod, s, 24, 32, 6, seed=3)
     ac, sqc = _render_cpu(oracle, s, 24, 32, 6, seed=3)
     cov = ac[..., 3] > 0
@@ -394,6 +393,39 @@ def test_resolve_matches_oracle_resolve(oxr_mod, oracle):
     assert list(img[0, 0]) == [200, 10, 77, 128]
     hdr = pt.hdr()
     assert np.array_equal(hdr, oracle.resolve_hdr(acc, 4, background=(200, 10, 77, 128)))
+
+
+def test_resolve_alpha_rounds_half_away_from_zero(oxr_mod, oracle):
+    """PathTracer::image (src/pathtrace.rs:2694-2716): alpha bytes use
+    f32::round. At 6 spp over a silhouette on a transparent background a
+    pixel with one covered sample has alpha 255/6 = 42.5, which rounds to
+    43; the device bytes equal the oracle's, which the resolve pins fix."""
+    s = scenes.Scene("disc")
+    m = s.add_material(scenes.lambert((0.8, 0.8, 0.8), emissive=(1.0, 0.5, 0.25)))
+    t, _ = scenes.uv_sphere(1.0, 24, 12)
+    s.add_tris(t, m)
+    s.camera = scenes.CameraSpec((0.0, 0.0, 4.0), (0.0, 0.0, 0.0), yfov_deg=40.0)
+    W, spp = 48, 6
+    pt = oxr_mod.PathTracer(0)
+    opts = oxr_mod.RenderOptions(width=W, height=W, ambient=0.0, background=(0, 0, 0, 0),
+                                 path_trace=oxr_mod.PathTraceOptions(samples_per_pixel=spp, max_bounces=1))
+    pt.sync(s, opts)
+    pt.refine(spp)
+    acc = pt.accum()
+    img = pt.image()
+    pt.close()
+    ref = oracle.resolve_rgba8(acc, spp, background=(0, 0, 0, 0))
+    assert np.array_equal(img, ref)
+    one = acc[..., 3] == 1
+    assert one.any(), "no pixel with exactly one covered sample"
+    assert (img[..., 3][one] == 43).all()
+    assert np.array_equal(pt_hdr_alpha(acc, spp), img[..., 3])
+
+
+def pt_hdr_alpha(acc, spp):
+    """Alpha bytes of the coverage fraction, f32 and half away from zero."""
+    f = np.clip((acc[..., 3] / spp).astype(np.float32), 0, 1) * np.float32(255.0)
+    return np.floor(f.astype(np.float64) + 0.5).astype(np.uint8)
 
 
 # ---- the reference's end-to-end closed forms, on the GPU --------------------

@@ -1,0 +1,257 @@
+"""Parity at the configs' full sizes and at the north star's bars, GPU
+against the oracle through the C ABI (marked `gpu`):
+
+* hit batches on the full material sweep (201,504 triangles) and the full
+  12.4M-triangle terrain: random rays plus the rays of the oracle's own
+  paths (camera, bounce), triangle ids and front faces bit-exact, t and
+  barycentrics within 1 ulp; on the terrain the oracle's restatement of the
+  device traversal shows rays whose stack outgrows the 16 LDS entries, so
+  the global spill path is part of the batch;
+* the equal-t tie-break on coplanar overlaps (lowest global id,
+  src/trace.rs:405-422), and the parity entry points accepting every
+  candidate (no MASK or BLEND filter);
+* converged images: the GPU at N spp (seed 0) against the oracle's f64
+  accumulation at 16 N spp with another seed (1) on a crop of each config's
+  frame; per pixel and channel z = |mean_gpu - mean_cpu| / sigma with
+  sigma^2 = var_gpu / N + var_cpu / (16 N) from the sums of squares; fewer
+  than 1% of the values beyond 3 sigma (0.27% for a normal difference).
+  Each case prints a CONVERGED line with max z and the fraction beyond
+  3 sigma."""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from oxr import _abi as A
+from oxr import scenes
+
+pytestmark = pytest.mark.gpu
+
+MISS = A.OXR_HIT_MISS
+
+
+@pytest.fixture(scope="module")
+def oxr_mod():
+    import oxr
+    if oxr.device_count() == 0:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+    return oxr
+
+
+def _compare_hits(g, c, what=""):
+    assert g.shape == c.shape
+    bad = np.nonzero(g[:, 1] != c[:, 1])[0]
+    assert len(bad) == 0, f"{what}: {len(bad)} triangle ids / front faces differ, first rays {bad[:8]}"
+    hit = c[:, 1] != MISS
+    dt = np.abs(g[hit, 0].astype(np.int64) - c[hit, 0].astype(np.int64))
+    assert dt.max(initial=0) <= 1, f"{what}: hit t differs by {dt.max()} ulp"
+    du = np.abs(g[hit, 2:4].astype(np.int64) - c[hit, 2:4].astype(np.int64))
+    assert du.max(initial=0) <= 1, f"{what}: barycentrics differ by {du.max()} ulp"
+    return int(hit.sum())
+
+
+def _random_rays(scene, n, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = scene.bounds()
+    c = (lo + hi) / 2
+    ext = (hi - lo) / 2 * 0.99
+    o = c + (rng.random((n, 3)) * 2 - 1) * ext
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[: n // 32] = 0.0
+    d[: n // 32, np.arange(n // 32) % 3] = 1.0
+    d[n // 32: n // 16, 1] = -0.0
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+# ---- coplanar overlaps and the unfiltered parity entry points ---------------
+
+def test_coplanar_overlaps_match_oracle_and_take_the_lowest_id(oxr_mod, oracle):
+    from test_oracle_trace import coplanar_expected, coplanar_rays
+    sc, xy, owner = scenes.coplanar_overlaps()
+    o, d, p = coplanar_rays(8192)
+    # Oblique rays over the same plane as well (t then differs by rounding
+    # between overlapping triangles; ids must still be bit-exact).
+    rng = np.random.default_rng(4)
+    m = 4096
+    o2 = np.concatenate([rng.random((m, 2)), np.full((m, 1), 1.0)], 1).astype(np.float32)
+    d2 = np.concatenate([rng.normal(scale=0.3, size=(m, 2)), np.full((m, 1), -1.0)], 1)
+    d2 = (d2 / np.linalg.norm(d2, axis=1, keepdims=True)).astype(np.float32)
+    rb = oracle.RayBatch(np.concatenate([o, o2]), np.concatenate([d, d2]))
+    n = len(rb)
+    pt = oxr_mod.PathTracer(0)
+    pt.upload(sc)
+    g = pt.trace_closest(rb.arr, n)
+    c = oracle.OracleScene(sc).trace_closest(rb.arr)
+    _compare_hits(g, c, "coplanar")
+    ties = 0
+    for i in range(len(p)):
+        want = coplanar_expected(xy, owner, p[i])
+        if want is None or want < 0:
+            continue
+        assert int(g[i, 1] & 0x7FFFFFFF) == want, (i, p[i])
+        ties += 1
+    assert ties > 500
+    pt.close()
+
+
+def test_parity_entry_points_accept_every_candidate(oxr_mod, oracle):
+    """oxr_trace_closest / oxr_trace_any accept every candidate, as
+    TraceScene::closest_hit and occluded (src/trace.rs:396-450): a MASK
+    quad below its cutoff and a BLEND quad still block, as in the oracle."""
+    s = scenes.Scene("alpha_quads")
+    fl = s.add_material(scenes.lambert((0.7, 0.7, 0.7)))
+    t, nr = scenes.quad([[-2, 0, 2], [2, 0, 2], [2, 0, -2], [-2, 0, -2]])
+    s.add_tris(t, fl, normals=nr)
+    mask = A.material((0.9, 0.5, 0.3, 0.2), double_sided=True)
+    mask.alpha_mode = A.OXR_ALPHA_MASK
+    mask.alpha_cutoff = 0.5
+    bm = A.material((0.3, 0.5, 0.9, 0.3), double_sided=True)
+    bm.alpha_mode = A.OXR_ALPHA_BLEND
+    for k, mat in enumerate((mask, bm)):
+        mi = s.add_material(mat)
+        x0 = -1.5 + 1.6 * k
+        t, nr = scenes.quad([[x0, 0.5, -1], [x0 + 1.2, 0.5, -1], [x0 + 1.2, 0.5, 1], [x0, 0.5, 1]])
+        s.add_tris(t, mi, normals=nr)
+    s.camera = scenes.CameraSpec((0.0, 3.0, 0.1), (0.0, 0.0, 0.0))
+    rng = np.random.default_rng(2)
+    n = 4096
+    o = np.stack([rng.uniform(-1.8, 1.8, n), np.full(n, 2.0), rng.uniform(-1.2, 1.2, n)], 1).astype(np.float32)
+    d = np.tile(np.array([0.0, -1.0, 0.0], np.float32), (n, 1))
+    rb = oracle.RayBatch(o, d)
+    pt = oxr_mod.PathTracer(0)
+    pt.upload(s)
+    os_ = oracle.OracleScene(s)
+    g = pt.trace_closest(rb.arr, n)
+    _compare_hits(g, os_.trace_closest(rb.arr), "alpha quads")
+    ids = g[:, 1] & 0x7FFFFFFF
+    assert ((ids == 2) | (ids == 3)).sum() > 500 and ((ids == 4) | (ids == 5)).sum() > 500
+    rb2 = oracle.RayBatch(o, d, 0.0, 1.9)
+    occ = pt.trace_any(rb2.arr, n)
+    assert np.array_equal(occ, os_.trace_any(rb2.arr))
+    assert occ.sum() > 1000
+    pt.close()
+
+
+# ---- full-size hit batches ---------------------------------------------------
+
+def _hit_batch(oxr_mod, oracle, s, pt, os_, W, H, rows, seed):
+    """Random rays, then the closest-hit rays of the oracle's own paths
+    (camera and bounce rays, 1 spp over `rows`, every other pixel); the
+    camera rays are traced once more with the primary back-face cull."""
+    o, d = _random_rays(s, 1 << 16, seed)
+    rb = oracle.RayBatch(o, d)
+    nhit = _compare_hits(pt.trace_closest(rb.arr, len(rb)), os_.trace_closest(rb.arr), "random rays")
+    rb2 = oracle.RayBatch(o, d, 0.0, 0.25 * float(np.linalg.norm(s.bounds()[1] - s.bounds()[0])))
+    assert np.array_equal(pt.trace_any(rb2.arr, len(rb2)), os_.trace_any(rb2.arr))
+    opts = oracle.options(W, H, max_bounces=8, ambient=getattr(s, "ambient", 0.0))
+    rec = os_.record_rays(W, H, 0, 1, rows[0], rows[1], x_step=2, max_rays=1 << 16, opts=opts)
+    rp = oracle.RayBatch(rec[:, 0:3], rec[:, 4:7])
+    nhit += _compare_hits(pt.trace_closest(rp.arr, len(rp)), os_.trace_closest(rp.arr), "path rays")
+    cam = rec[rec[:, 3] == 0]
+    rc = oracle.RayBatch(cam[:, 0:3], cam[:, 4:7])
+    _compare_hits(pt.trace_closest(rc.arr, len(rc), flags=1), os_.trace_closest(rc.arr, flags=1), "camera rays")
+    return rec, nhit, int((rec[:, 3] > 0).sum())
+
+
+@pytest.fixture(scope="module")
+def sweep_full(oxr_mod, oracle):
+    s = scenes.material_sweep()
+    pt = oxr_mod.PathTracer(0)
+    pt.upload(s)
+    yield s, pt, oracle.OracleScene(s)
+    pt.close()
+
+
+@pytest.fixture(scope="module")
+def terrain_full(oxr_mod, oracle):
+    print("building the 12.4M-triangle terrain (scene, device BVH, oracle BVH)", flush=True)
+    s = scenes.terrain()
+    pt = oxr_mod.PathTracer(0)
+    pt.upload(s)
+    os_ = oracle.OracleScene(s)
+    yield s, pt, os_
+    pt.close()
+
+
+def test_full_material_sweep_hit_batch(oxr_mod, oracle, sweep_full):
+    s, pt, os_ = sweep_full
+    assert s.num_tris == 201504
+    rec, nhit, nbounce = _hit_batch(oxr_mod, oracle, s, pt, os_, 1024, 1024, (480, 544), 11)
+    assert nhit > 50000 and nbounce > 10000
+
+
+@pytest.mark.timeout(900)
+def test_full_terrain_hit_batch_exercises_the_spill_stack(oxr_mod, oracle, terrain_full):
+    s, pt, os_ = terrain_full
+    assert s.num_tris == 12398608
+    rec, nhit, nbounce = _hit_batch(oxr_mod, oracle, s, pt, os_, 1920, 1080, (500, 564), 13)
+    assert nhit > 50000 and nbounce > 10000
+    nodes, tris = pt.bvh_export()
+    o, d = _random_rays(s, 1 << 16, 13)
+    rays = np.zeros((len(o), 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7], rays[:, 7] = o, d, np.inf
+    depth = oracle.bvh4_stack_depth(nodes, tris, np.concatenate([rays, rec]))
+    spilled = int((depth > 16).sum())
+    print(f"terrain: max stack depth {int(depth.max())}, rays beyond the 16 LDS entries: {spilled}")
+    assert spilled > 0
+
+
+# ---- converged images against a 16x longer f64 oracle run --------------------
+
+def _converged(oxr_mod, oracle, s, W, H, n, crop, bounces, pt=None, os_=None, name=""):
+    y0, y1, x0, x1 = crop
+    own = pt is None
+    if own:
+        pt = oxr_mod.PathTracer(0)
+    opts = oxr_mod.RenderOptions(width=W, height=H, ambient=getattr(s, "ambient", 0.0), background=(0, 0, 0, 255),
+                                 path_trace=oxr_mod.PathTraceOptions(samples_per_pixel=n, max_bounces=bounces, seed=0))
+    pt.sync(s, opts)
+    pt.refine(n)
+    ag, sg = pt.accum(with_sum_sq=True)
+    if own:
+        pt.close()
+    ag, sg = ag[y0:y1, x0:x1], sg[y0:y1, x0:x1]
+    os_ = os_ or oracle.OracleScene(s)
+    m = 16 * n
+    oo = oracle.options(W, H, spp=m, max_bounces=bounces, seed=1, ambient=getattr(s, "ambient", 0.0))
+    ac, sc = os_.render_rect(W, H, 0, m, x0, x1, y0, y1, opts=oo)
+    full = (ag[..., 3] == n) & (ac[..., 3] == m)
+    assert full.mean() > 0.9, f"{name}: crop not covered"
+    mg, mc = ag[..., :3][full] / n, ac[..., :3][full] / m
+    vg = np.maximum(sg[full] / n - mg ** 2, 0.0)
+    vc = np.maximum(sc[full] / m - mc ** 2, 0.0)
+    sigma = np.sqrt(vg / n + vc / m)
+    diff = np.abs(mg - mc)
+    z = np.where(sigma > 0, diff / np.where(sigma > 0, sigma, 1.0), np.where(diff > 0, np.inf, 0.0))
+    res = {"config": name, "gpu_spp": n, "cpu_spp": m, "crop": [y0, y1, x0, x1], "pixels": int(full.sum()),
+           "max_z": float(z.max()), "frac_gt_3sigma": float((z > 3).mean()),
+           "mean_gpu": float(mg.mean()), "mean_cpu": float(mc.mean()),
+           "rel_mean_diff": float(abs(mg.mean() - mc.mean()) / max(abs(mc.mean()), 1e-12))}
+    print("CONVERGED " + json.dumps(res), flush=True)
+    assert np.isfinite(z).all(), res
+    assert res["frac_gt_3sigma"] < 0.01, res
+    return res
+
+
+def test_converged_cornell_within_3_sigma(oxr_mod, oracle):
+    _converged(oxr_mod, oracle, scenes.cornell(), 256, 256, 64, (112, 144, 112, 144), 8, name="C0 cornell 256")
+
+
+def test_converged_material_sweep_within_3_sigma(oxr_mod, oracle, sweep_full):
+    s, pt, os_ = sweep_full
+    _converged(oxr_mod, oracle, s, 1024, 1024, 64, (496, 528, 496, 528), 8, pt=pt, os_=os_,
+               name="C1 material sweep 1024")
+
+
+def test_converged_glossy_caustic_within_3_sigma(oxr_mod, oracle):
+    _converged(oxr_mod, oracle, scenes.glossy_caustic(), 1024, 1024, 64, (560, 592, 496, 528), 8,
+               name="C4 glossy caustic 1024")
+
+
+@pytest.mark.timeout(900)
+def test_converged_terrain_within_3_sigma(oxr_mod, oracle, terrain_full):
+    s, pt, os_ = terrain_full
+    _converged(oxr_mod, oracle, s, 1920, 1080, 16, (524, 556, 944, 976), 8, pt=pt, os_=os_,
+               name="C2 terrain 1920x1080")

@@ -44,7 +44,8 @@ def test_prepared_graph_direct_lighting_matches_oracle():
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     assert mc.max() > 0.1
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
-    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+    print(f"direct lighting: max rel {rel.max():.3g}, > 1e-3: {(rel > 1e-3).mean():.2e}")
+    assert rel.max() <= 1e-3, rel.max()
 
 
 def test_prepared_graph_full_bounce_within_3_sigma():

@@ -89,7 +89,36 @@ def test_textured_direct_lighting_matches_oracle():
     mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
-    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+    print(f"textured direct lighting: max rel {rel.max():.3g}, > 1e-3: {(rel > 1e-3).mean():.2e}")
+    assert rel.max() <= 1e-3, rel.max()
+
+
+@pytest.mark.parametrize("projection", [0, 1], ids=["perspective", "orthographic"])
+def test_cone_level_at_the_first_hit_matches_oracle(projection):
+    """The first hit's ray-cone width (spread = 2 half_h / height, times t
+    for perspective; src/pathtrace.rs:2537-2554, 2083-2092) picks the same
+    trilinear level on the device as in the oracle: a white Lambert quad
+    with a random base-colour texture under one point light, per pixel."""
+    import oxr
+    from test_oracle_textures import _cone_scene
+    s = _cone_scene(projection)
+    W, spp = 48, 4
+    pt = oxr.PathTracer(0)
+    o = oxr.RenderOptions(width=W, height=W, ambient=0.0)
+    o.path_trace.samples_per_pixel = spp
+    o.path_trace.max_bounces = 1
+    pt.sync(s, o)
+    pt.refine(spp)
+    ag = pt.accum()
+    pt.close()
+    ac = O.OracleScene(s).render(W, W, 0, spp, O.options(W, W, max_bounces=1))
+    assert np.array_equal(ag[..., 3], ac[..., 3])
+    cov = ac[..., 3] > 0
+    assert cov.mean() > 0.9
+    mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
+    mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
+    rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
+    assert rel.max() <= 1e-3, rel.max()
 
 
 def test_textured_full_bounce_within_3_sigma():
@@ -102,7 +131,8 @@ def test_textured_full_bounce_within_3_sigma():
     assert (z > 3.0).mean() < 0.01
 
 
-def test_blend_shadow_coins_match_oracle():
+@pytest.mark.parametrize("zero_first", [False, True], ids=["lit", "zero_power_light_first"])
+def test_blend_shadow_coins_match_oracle(zero_first):
     """Stochastic BLEND (Integrator::accept, src/pathtrace.rs:1924-1934) on
     shadow rays: the coin's ray id is the vertex's bounce (k << 16 with
     0x8000 | j for punctual light j, 0xffff for the area set), so the
@@ -122,6 +152,11 @@ def test_blend_shadow_coins_match_oracle():
     em = s.add_material(scenes.lambert((0.0, 0.0, 0.0), emissive=(8.0, 8.0, 8.0)))
     t, nr = scenes.quad([[-0.3, 1.6, 0.3], [0.3, 1.6, 0.3], [0.3, 1.6, -0.3], [-0.3, 1.6, -0.3]])
     s.add_tris(t[:, [0, 1, 2, 6, 7, 8, 3, 4, 5]], em)  # facing down
+    if zero_first:
+        # Dropped at upload (it adds exactly zero), but the lit light keeps
+        # its index 1 in the BLEND coin of its shadow rays (src/pathtrace.rs:
+        # 2147-2166), as in the oracle.
+        s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=0.0, position=(-0.4, 2.0, 0.2), direction=(0, -1, 0)))
     s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=3.0, position=(0.4, 2.0, 0.2), direction=(0, -1, 0)))
     s.camera = scenes.CameraSpec((0.0, 2.5, 2.5), (0.0, 0.2, 0.0), yfov_deg=50.0)
     s.ambient = 0.0
@@ -140,4 +175,4 @@ def test_blend_shadow_coins_match_oracle():
     mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
-    assert (rel > 1e-3).mean() <= 2e-3, (rel > 1e-3).mean()
+    assert rel.max() <= 1e-3, rel.max()

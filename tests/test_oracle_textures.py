@@ -268,3 +268,71 @@ def test_texture_transform_matches_the_extension():
     vp = -s * 2 * u + c * 1 * v
     want = o.tex_sample(0, [[up, vp]], 0.0, srgb=True)[0, 0]
     assert o.material_eval([0], bary)[0, 0] == pytest.approx(want, rel=1e-6)
+
+
+
+def _cone_scene(projection):
+    """The unit quad, white Lambert with a seeded random 256x256 float
+    base-colour texture sampled trilinearly (so the base colour is a
+    continuous function of the cone level), lit by one point light."""
+    rng = np.random.default_rng(11)
+    px = np.ones((256, 256, 4), np.float32)
+    px[..., :3] = rng.random((256, 256, 3), dtype=np.float32)
+
+    def light(s):
+        s.lights.append(A.light(A.OXR_LIGHT_POINT, intensity=2.0, position=(0.5, 0.4, 1.0), direction=(0, 0, -1)))
+    sc = quad_scene(px, dict(mag=A.FILTER_LINEAR, min_filter=A.MIN_LINEAR_MIP_LINEAR), extra=light)
+    if projection == A.OXR_PROJ_ORTHOGRAPHIC:
+        sc.camera = S.CameraSpec((0.45, 0.5, 2.0), (0.5, 0.52, 0.0), projection=projection, ymag=0.45)
+    else:
+        sc.camera = S.CameraSpec((0.45, 0.5, 1.6), (0.5, 0.52, 0.0), yfov_deg=30.0)
+    sc.ambient = 0.0
+    return sc
+
+
+@pytest.mark.parametrize("projection", [A.OXR_PROJ_PERSPECTIVE, A.OXR_PROJ_ORTHOGRAPHIC])
+def test_first_hit_cone_width_is_the_references(projection):
+    """PathTracer::refine (src/pathtrace.rs:2537-2554) and the first hit of
+    Integrator::sample (:2083-2092): the ray cone's width is
+    spread = 2 half_h / height for an orthographic camera and spread * t for
+    a perspective one. The only term in this scene is the point light's
+    NEE on a white Lambert quad, base(w) / pi * I cos / d^2, so every
+    covered sample's radiance divided by the base colour evaluated at that
+    width (half_h from the camera definition here: tan(yfov / 2), or ymag)
+    is the grey irradiance factor, the same in all channels and equal to
+    its closed form."""
+    W = H = 24
+    sc = _cone_scene(projection)
+    o = O.OracleScene(sc)
+    opts = O.options(W, H, spp=1, max_bounces=1)
+    rays = o.record_rays(W, H, 0, 1, 0, H, opts=opts)
+    rays = rays[rays[:, 3] == 0]  # the primary rays, one per pixel
+    assert len(rays) == W * H
+    raw = o.trace_closest(O.RayBatch(rays[:, 0:3], rays[:, 4:7]).arr)
+    prim = raw[:, 1]
+    tuv = raw.view(np.float32)
+    if projection == A.OXR_PROJ_ORTHOGRAPHIC:
+        half_h = np.float32(0.45)
+    else:
+        half_h = np.float32(math.tan(math.radians(30.0) / 2.0))
+    spread = np.float32(2.0) * half_h / np.float32(H)
+    lp = np.array([0.5, 0.4, 1.0])
+    checked = covered = 0
+    for i in range(W * H):
+        if prim[i] == 0xFFFFFFFF:
+            continue
+        covered += 1
+        t, u, v = tuv[i, 0], tuv[i, 2], tuv[i, 3]
+        width = spread if projection == A.OXR_PROJ_ORTHOGRAPHIC else np.float32(spread * t)
+        bary = [[np.float32(1.0) - u - v, u, v]]
+        base = o.material_eval([prim[i]], bary, width=[width], dirs=[rays[i, 4:7]])[0, 0:3]
+        got = o.sample_one(W, H, i % W, i // W, 0, opts=opts)
+        p = rays[i, 0:3].astype(np.float64) + float(t) * rays[i, 4:7].astype(np.float64)
+        dl = lp - p
+        d2 = float(dl @ dl)
+        k = 2.0 * (dl[2] / math.sqrt(d2)) / (math.pi * d2)
+        np.testing.assert_allclose(got / base, [k, k, k], rtol=2e-5, err_msg=f"pixel {i}")
+        # A cone of zero width (mip level 0) gives a different base colour.
+        lvl0 = o.material_eval([prim[i]], bary, width=[0.0], dirs=[rays[i, 4:7]])[0, 0:3]
+        checked += int(not np.allclose(lvl0, base, rtol=1e-3))
+    assert covered > 0.9 * W * H and checked > 0.9 * covered

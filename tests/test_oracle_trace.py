@@ -59,11 +59,14 @@ def _random_rays(scene, n, seed):
 
 @pytest.mark.parametrize("make", [scenes.cornell, scenes.reference_box,
                                   lambda: scenes.material_sweep(segments=16, rings=8, floor_half=3),
-                                  lambda: scenes.triangle_soup(7, 3000)])
+                                  lambda: scenes.triangle_soup(7, 3000),
+                                  # Large enough that worker threads build subtrees the build stitches.
+                                  lambda: scenes.triangle_soup(8, 80000)],
+                         ids=["cornell", "reference_box", "sweep", "soup3k", "soup80k"])
 def test_bvh_traversal_equals_brute_force(oracle, make):
     s = make()
     os_ = oracle.OracleScene(s)
-    o, d = _random_rays(s, 4000, 11)
+    o, d = _random_rays(s, 4000 if s.num_tris < 10000 else 1500, 11)
     rb = _rays(o, d)
     a = os_.trace_closest(rb.arr)
     b = os_.trace_closest(rb.arr, brute=True)
@@ -147,3 +150,79 @@ def test_emissive_table(oracle):
     for i, g in enumerate(tris):
         assert lookup[g] == i
     assert (lookup[np.setdiff1d(np.arange(s.num_tris), tris)] == 0xFFFFFFFF).all()
+
+
+
+def coplanar_inside(xy, p, margin=1e-6):
+    """Per triangle (corners xy, (n, 3, 2), float64): 1 if point p is inside
+    by more than `margin` (edge-signed distance), 0 if outside by more than
+    it, -1 if within the margin of an edge."""
+    a, b, c = xy[:, 0], xy[:, 1], xy[:, 2]
+
+    def side(u, v):
+        e = v - u
+        ln = np.hypot(e[:, 0], e[:, 1])
+        return (e[:, 0] * (p[1] - u[:, 1]) - e[:, 1] * (p[0] - u[:, 0])) / np.maximum(ln, 1e-300)
+    s = np.stack([side(a, b), side(b, c), side(c, a)], 1)
+    area = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    s = s * np.sign(area)[:, None]
+    out = np.zeros(len(xy), np.int64)
+    out[(s > margin).all(1) & (area != 0)] = 1
+    out[(np.abs(s) <= margin).any(1) & (s > -margin).all(1)] = -1
+    return out
+
+
+def coplanar_expected(xy, owner, p):
+    """The id closest hit must return for the ray down -z through p when
+    the answer is fixed by the tie-break alone: p lies inside the copies of
+    exactly one base triangle (identical corners, so identical t) and in no
+    other triangle; the lowest of those copies' ids. None otherwise; -1
+    where p is in no triangle."""
+    ins = coplanar_inside(xy, p)
+    if (ins < 0).any():
+        return None
+    hit = np.nonzero(ins == 1)[0]
+    if len(hit) == 0:
+        return -1
+    owners = set(owner[hit].tolist())
+    if len(owners) != 1 or -1 in owners:
+        return None
+    return int(hit.min())
+
+
+def coplanar_rays(n, seed=9):
+    """Rays straight down -z onto the z = 0 plane from random points."""
+    rng = np.random.default_rng(seed)
+    p = rng.random((n, 2))
+    o = np.zeros((n, 3), np.float32)
+    o[:, :2] = p
+    o[:, 2] = 1.0
+    d = np.zeros((n, 3), np.float32)
+    d[:, 2] = -1.0
+    return o, d, p
+
+
+def test_coplanar_overlaps_take_the_lowest_id(oracle):
+    """Equal t on coplanar overlapping triangles: closest hit returns the
+    lowest global id (the oracle's stated tie-break for the unvendored
+    BVH's traversal-order choice). Checked where the tie-break alone fixes
+    the answer (copies with identical corners at permuted ids, closed form
+    by exact 2-D containment), and the BVH equals brute force on every ray,
+    including overlaps of different triangles whose t differ by rounding."""
+    sc, xy, owner = scenes.coplanar_overlaps()
+    os_ = oracle.OracleScene(sc)
+    o, d, p = coplanar_rays(8192)
+    rb = _rays(o, d)
+    h_bvh = os_.trace_closest(rb.arr)
+    h_bf = os_.trace_closest(rb.arr, brute=True)
+    assert np.array_equal(h_bvh, h_bf)
+    checked = ties = 0
+    for i in range(len(p)):
+        want = coplanar_expected(xy, owner, p[i])
+        if want is None:
+            continue
+        got = -1 if h_bvh[i, 1] == A.OXR_HIT_MISS else int(h_bvh[i, 1] & 0x7FFFFFFF)
+        assert got == want, (i, p[i], got, want)
+        checked += 1
+        ties += int(want >= 0)
+    assert checked > 1000 and ties > 500, (checked, ties)
