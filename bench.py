@@ -268,99 +268,109 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16):
 
 
 GUIDE_BENCH_CONFIG = {"max_steps": 160}
+CAUSTIC_REFERENCE = os.path.join(ROOT, "tests", "golden", "caustic_window_reference.npz")
 
 
-def guide_mse_bench(W=1024, target=1e-3, pass_spp=4, passes=80, max_s=30.0, guide_cfg=None):
-    """Time-to-MSE on the glossy-caustic scene, plain vs guided (online
-    training included in the guided time).
+def window_mse(acc, ref_mean, ref_var_mean):
+    """MSE of a window's mean image against the converged reference, per
+    pixel and channel, less the reference's own variance of the mean (so
+    the figure estimates the error against the exact expectation)."""
+    n = acc[..., 3:4]
+    m = np.where(n > 0, acc[..., :3] / np.maximum(n, 1.0), 0.0)
+    return float(((m - ref_mean) ** 2).mean() - ref_var_mean.mean())
 
-    Both estimators are unbiased, so the expected MSE of the image after N
-    samples per pixel is the mean per-pixel variance of that image:
-    MSE(N) = sum_k sigma_k^2 * s_k / N^2 over the passes k (s_k samples of
-    single-sample variance sigma_k^2, measured from the f64 sums and sums of
-    squares of each pass). The time to MSE `target` is the wall time of the
-    passes until MSE(N) <= target, continued at the last passes' variance
-    and time per sample when the run ends first. The two final images are
-    compared pixel by pixel (z-scores) as the unbiasedness check."""
+
+def guide_mse_bench(target=1e-3, pass_spp=256, max_s=40.0, guide_cfg=None, tile=8):
+    """Time to MSE `target`, plain vs guided, on a pixel-exact window of the
+    1024x1024 glossy-caustic frame: each method refines the window in
+    passes of pass_spp until the MSE of its image against the committed
+    f64 reference of the window (tests/golden/caustic_window_reference.npz:
+    plain sampling, another seed, at least 16x the samples plain needs here)
+    falls to `target`. Wall time of the refine calls; the guided time
+    includes its online training (train_ms) and inference (infer_ms). The
+    two final images are compared by tile means as the unbiasedness check."""
     import oxr
     from oxr import scenes
     from oxr.guide import Guide
+    ref = np.load(CAUSTIC_REFERENCE)
+    x0, y0, w, h = (int(v) for v in ref["window"])
+    W = int(ref["frame"][0])
     sc = scenes.glossy_caustic()
-
-    def opts(spp, seed):
-        o = oxr.RenderOptions(width=W, height=W, ambient=sc.ambient)
-        o.path_trace.samples_per_pixel = spp
-        o.path_trace.max_bounces = 8
-        o.path_trace.seed = seed
-        return o
-
     gcfg = dict(GUIDE_BENCH_CONFIG if guide_cfg is None else guide_cfg)
 
-    def run(guide):
+    def run(guided):
         pt = oxr.PathTracer(0)
-        pt.sync(sc, opts(1 << 20, 1 if guide else 2))
-        g = None
-        if guide:
-            g = Guide(0, **gcfg)
+        o = oxr.RenderOptions(width=W, height=W, ambient=sc.ambient)
+        o.path_trace.samples_per_pixel = 1 << 30
+        o.path_trace.max_bounces = int(ref["max_bounces"])
+        o.path_trace.seed = 0
+        pt.sync(sc, o)
+        pt.set_window(x0, y0, w, h)
+        g = Guide(0, **gcfg) if guided else None
+        if g:
             pt.set_guide(g, train=True)
-        pt.refine(pass_spp)  # warm-up: buffers, graph capture (the guide trains on it too)
-        pt.reset()
-        if g:  # start from an untrained guide: the training is part of the measured time
+        pt.refine(pass_spp)  # warm-up: buffers, graph capture
+        if g:  # start from an untrained guide: its training is part of the measured time
             pt.set_guide(None)
             g.close()
             g = Guide(0, **gcfg)
             pt.set_guide(g, train=True)
-        prev_a = np.zeros((W, W, 4))
-        prev_s = np.zeros((W, W, 3))
-        rows, t_tot = [], 0.0
-        for k in range(passes):
+        pt.reset()
+        t_tot, curve, hit = 0.0, [], None
+        while t_tot < max_s:
             t = time.perf_counter()
             pt.refine(pass_spp)
-            dt = time.perf_counter() - t
-            t_tot += dt
-            a, sq = pt.accum(with_sum_sq=True)
-            da, ds = a - prev_a, sq - prev_s
-            prev_a, prev_s = a, sq
-            n = np.maximum(da[..., 3:4], 1.0)
-            m = da[..., :3] / n
-            var = float((np.maximum(ds / n - m * m, 0.0) * n / np.maximum(n - 1.0, 1.0)).mean())
-            rows.append((dt, var))
-            if t_tot > max_s:
+            t_tot += time.perf_counter() - t
+            acc, sq = pt.accum(with_sum_sq=True)
+            mse = window_mse(acc, ref["mean"], ref["var_mean"])
+            curve.append((pt.samples(), round(t_tot, 4), mse))
+            if mse <= target:
+                hit = (pt.samples(), t_tot)
                 break
-        n = np.maximum(prev_a[..., 3:4], 1.0)
-        img = prev_a[..., :3] / n
-        var_img = np.maximum(prev_s / n - img * img, 0.0) / n
         st = g.stats() if g else None
+        infer_pass_ms = None
+        if g:
+            # Inference runs inside the replayed bounce graph: its device
+            # time comes from one more pass with per-kernel events.
+            pt.set_timing(True)
+            pt.refine(pass_spp)
+            infer_pass_ms = g.stats()["infer_ms"] - st["infer_ms"]
+            pt.set_timing(False)
         pt.close()
-        # Expected MSE after each pass; time to the target.
-        V, N, T, hit = 0.0, 0, 0.0, None
-        curve = []
-        for dt, var in rows:
-            V += var * pass_spp
-            N += pass_spp
-            T += dt
-            curve.append((N, round(T, 4), V / N ** 2))
-            if hit is None and V / N ** 2 <= target:
-                hit = (N, T, "measured")
-        tail = rows[len(rows) // 2:]
-        var_ss = float(np.mean([v for _, v in tail]))
-        t_spp = float(np.mean([dt for dt, _ in tail])) / pass_spp
-        if hit is None:
-            # V + var_ss * (M - N) <= target * M^2  ->  smallest M.
-            a_, b_, c_ = target, -var_ss, -(V - var_ss * N)
-            M = (-b_ + math.sqrt(b_ * b_ - 4 * a_ * c_)) / (2 * a_)
-            hit = (M, T + (M - N) * t_spp, "extrapolated")
-        return {"time_to_mse_s": hit[1], "spp": hit[0], "how": hit[2], "steady_var_per_sample": var_ss,
-                "s_per_spp": t_spp, "passes": len(rows), "curve": curve[::4], "guide": st}, img, var_img
+        if g:
+            g.close()
+        n = np.maximum(acc[..., 3:4], 1.0)
+        m = acc[..., :3] / n
+        var_m = np.maximum(sq / n - m * m, 0.0) / n
+        res = {"how": "measured" if hit else "not reached",
+               "time_to_mse_s": hit[1] if hit else None, "spp": hit[0] if hit else None,
+               "passes": len(curve), "final_mse": curve[-1][2], "final_spp": curve[-1][0],
+               "s_per_pass": t_tot / max(len(curve), 1), "curve": curve[:: max(1, len(curve) // 16)],
+               "guide": st}
+        if st:
+            res["train_ms"] = st["train_ms"]  # device time of the Adam steps within the measured passes
+            res["infer_ms"] = infer_pass_ms * len(curve)  # one timed pass's inference x the passes
+            res["infer_ms_per_pass"] = infer_pass_ms
+        return res, m, var_m
 
-    plain, pi, pv = run(False)
-    guided, gi, gv = run(True)
-    z = (gi - pi) / np.sqrt(pv + gv + 1e-30)
-    return {"workload": f"glossy_caustic {W}x{W}, 8 bounces, {pass_spp} spp per pass", "target_mse": target,
-            "guide_config": gcfg, "plain": plain, "guided": guided,
-            "guided_over_plain_time": guided["time_to_mse_s"] / plain["time_to_mse_s"],
-            "unbiased_check": {"mean_abs_z": float(np.abs(z).mean()), "frac_abs_z_gt_4": float((np.abs(z) > 4).mean()),
-                               "image_mean_plain": float(pi.mean()), "image_mean_guided": float(gi.mean())}}
+    plain, pm, pv = run(False)
+    guided, gm, gv = run(True)
+    # Unbiasedness: tile means (tile x tile pixels) of the two images.
+    th, tw = h // tile, w // tile
+    tmean = lambda a: a[:th * tile, :tw * tile].reshape(th, tile, tw, tile, 3).mean((1, 3))  # noqa: E731
+    tvar = lambda v: v[:th * tile, :tw * tile].reshape(th, tile, tw, tile, 3).sum((1, 3)) / tile ** 4  # noqa: E731
+    z = (tmean(gm) - tmean(pm)) / np.sqrt(tvar(gv) + tvar(pv) + 1e-30)
+    out = {"workload": f"glossy_caustic {W}x{W} frame, window {w}x{h} at ({x0}, {y0}), "
+                       f"{int(ref['max_bounces'])} bounces, {pass_spp} spp per pass",
+           "target_mse": target, "reference": {"spp": int(ref["spp"]), "seed": int(ref["seed"]),
+                                               "var_of_mean": float(ref["var_mean"].mean())},
+           "guide_config": gcfg, "plain": plain, "guided": guided,
+           "unbiased_check": {"tile": tile, "tiles": int(z.size), "max_abs_z": float(np.abs(z).max()),
+                              "frac_abs_z_gt_3": float((np.abs(z) > 3).mean()),
+                              "mean_plain": float(pm.mean()), "mean_guided": float(gm.mean())}}
+    if plain["time_to_mse_s"] and guided["time_to_mse_s"]:
+        out["guided_over_plain_time"] = guided["time_to_mse_s"] / plain["time_to_mse_s"]
+    return out
 
 
 def copy_calibration(mib):
@@ -397,10 +407,9 @@ def main():
     ap.add_argument("--terrain-only", action="store_true")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--guide-only", action="store_true")
-    ap.add_argument("--guide-width", type=int, default=1024)
-    ap.add_argument("--guide-passes", type=int, default=80)
+    ap.add_argument("--guide-pass-spp", type=int, default=256)
     ap.add_argument("--guide-config", default=None, help="JSON overrides of the guide configuration")
-    ap.add_argument("--guide-max-s", type=float, default=30.0)
+    ap.add_argument("--guide-max-s", type=float, default=40.0)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
@@ -410,7 +419,7 @@ def main():
         if oxr.device_count() == 0:
             raise SystemExit("bench.py: no HIP device visible")
         print(json.dumps(serving_bench() if a.serving_only else terrain_leg() if a.terrain_only else
-                         guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s,
+                         guide_mse_bench(pass_spp=a.guide_pass_spp, max_s=a.guide_max_s,
                                         guide_cfg=json.loads(a.guide_config) if a.guide_config else None)), flush=True)
         return
     if a.gpus != 1:
@@ -491,8 +500,8 @@ def main():
     pt.close()
     # The other two parts of the metric, each reported beside the paths/s
     # value (an error in one is reported, not fatal to the line).
-    guide_leg = lambda: guide_mse_bench(W=a.guide_width, passes=a.guide_passes, max_s=a.guide_max_s,
-                                        guide_cfg=json.loads(a.guide_config) if a.guide_config else None)  # noqa: E731
+    guide_leg = lambda: guide_mse_bench(pass_spp=a.guide_pass_spp, max_s=a.guide_max_s,  # noqa: E731
+                                        guide_cfg=json.loads(a.guide_config) if a.guide_config else None)
     legs = (("large_scene", a.no_terrain or a.workload == "terrain", terrain_leg),
             ("guided_time_to_mse", a.no_guide, guide_leg),
             ("serving", a.no_serving, serving_bench))

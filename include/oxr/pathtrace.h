@@ -308,6 +308,14 @@ int oxr_pt_upload_scene(oxr_pt* h, const oxr_scene_desc* scene);
 int oxr_pt_set_camera(oxr_pt* h, const oxr_camera* cam, int* reset_out);
 int oxr_pt_set_options(oxr_pt* h, const oxr_pt_options* opts, int* reset_out);
 int oxr_pt_reset(oxr_pt* h);
+/* Render window: refine traces only the frame pixels x in [x0, x0 + w),
+ * y in [y0, y0 + h), with the samples (pixel seeds, sample indices,
+ * camera rays) those pixels have in the whole-frame render, and the
+ * accumulator, read_accum and the resolves hold just those pixels
+ * (w x h, row-major). w = h = 0 restores the whole frame. A change resets
+ * the accumulation. No counterpart in the reference (its refine always
+ * covers the frame); it serves crop renders and convergence studies. */
+int oxr_pt_set_window(oxr_pt* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hgt);
 /* PathTracer::set_environment (src/pathtrace.rs:2449-2454): install an
  * equirectangular HDR environment (rgba: width*height*4 floats, scene-linear,
  * row 0 = +Y) scaled by `intensity`, or remove it (rgba = NULL). It replaces

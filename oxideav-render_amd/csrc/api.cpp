@@ -741,6 +741,9 @@ DevFrame make_frame(const oxr_camera& cam, const oxr_pt_options& opts, const Dev
   // projections; the ray cone at the first hit is spread (orthographic) or
   // spread * t (perspective).
   f.spread = 2.0f * cam.half_h / (float)f.height;
+  f.win_x0 = 0;
+  f.win_y0 = 0;
+  f.win_w = f.width;
   if (env && env->img) {
     f.env = *env;
     f.p_env = sc.num_emissive > 0 ? 0.5f : 1.0f;  // Integrator::p_env (src/pathtrace.rs:1915-1921)
@@ -1504,8 +1507,22 @@ struct oxr_pt {
     dfree(d_job);
   }
 
+  // Render window (x0, y0, w, h) inside the frame; w == 0: the whole frame.
+  uint32_t win[4] = {0, 0, 0, 0};
+  bool windowed() const { return win[2] != 0; }
+  uint64_t npx_frame() const { return (uint64_t)std::max(opts.width, 1u) * std::max(opts.height, 1u); }
+  // Pixels of the accumulator: the window's, or the frame's.
+  uint64_t npx_accum() const { return windowed() ? (uint64_t)win[2] * win[3] : npx_frame(); }
+  int check_window() const {
+    if (windowed() && ((uint64_t)win[0] + win[2] > std::max(opts.width, 1u) ||
+                       (uint64_t)win[1] + win[3] > std::max(opts.height, 1u)))
+      return fail(OXR_ERR_INVALID, "render window %ux%u at (%u, %u) exceeds the %ux%u frame", win[2], win[3], win[0],
+                  win[1], std::max(opts.width, 1u), std::max(opts.height, 1u));
+    return OXR_OK;
+  }
+
   int ensure_accum() {
-    size_t npx = (size_t)std::max(opts.width, 1u) * std::max(opts.height, 1u);
+    size_t npx = (size_t)npx_accum();
     if (npx != npx_alloc) {
       dfree(d_accum);
       dfree(d_sumsq);
@@ -1539,13 +1556,18 @@ int oxr_pt::refine(uint32_t n) {
   if (n == 0) return OXR_OK;
   int rc;
   HIPC(hipSetDevice(eng.device));
+  if ((rc = check_window())) return rc;
   if ((rc = ensure_accum())) return rc;
-  const uint32_t W = std::max(opts.width, 1u), H = std::max(opts.height, 1u);
-  const uint64_t npx = (uint64_t)W * H;
+  const uint64_t npx = npx_accum();
   if (npx > (1ull << 31)) return fail(OXR_ERR_INVALID, "image too large (%llu pixels)", (unsigned long long)npx);
   DevJob job;
   job.sc = scene.sc;
   job.fr = make_frame(cam, opts, scene.sc, env.env.img ? &env.env : nullptr);
+  if (windowed()) {
+    job.fr.win_x0 = win[0];
+    job.fr.win_y0 = win[1];
+    job.fr.win_w = win[2];
+  }
   job.accum = d_accum;
   job.sumsq = d_sumsq;
   HIPC(hipMemcpyAsync(d_job, &job, sizeof(DevJob), hipMemcpyHostToDevice, eng.stream));
@@ -1626,12 +1648,10 @@ int oxr_pt::train_guide() {
   int rc = guide->ensure_batch(B);
   if (rc) return rc;
   const uint32_t stride = n / B;  // start < stride keeps start + (B - 1) * stride < n
-  hipEvent_t a{}, b{};
-  if (eng.timing) {
-    a = eng.next_event();
-    b = eng.next_event();
-    (void)hipEventRecord(a, eng.stream);
-  }
+  // Training runs outside the bounce graph and ends in a synchronisation,
+  // so its device time is always taken (two events per pass).
+  hipEvent_t a = eng.next_event(), b = eng.next_event();
+  (void)hipEventRecord(a, eng.stream);
   uint32_t steps = guide->cfg.steps;
   if (guide->cfg.max_steps) steps = (uint32_t)std::min<uint64_t>(steps, guide->cfg.max_steps - guide->st.steps);
   const uint64_t s0 = guide->st.steps;
@@ -1644,16 +1664,14 @@ int oxr_pt::train_guide() {
     }
     if ((rc = guide->step(eng.stream, batch, B))) return rc;
   }
-  if (eng.timing) (void)hipEventRecord(b, eng.stream);
+  (void)hipEventRecord(b, eng.stream);
   HIPC(hipStreamSynchronize(eng.stream));
   float loss = 0.0f;
   HIPC(hipMemcpy(&loss, guide->d_scal + 1, sizeof(float), hipMemcpyDeviceToHost));
   guide->st.last_loss = loss;
-  if (eng.timing) {
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    guide->st.train_ms += ms;
-  }
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  guide->st.train_ms += ms;
   return OXR_OK;
 }
 
@@ -1882,6 +1900,24 @@ int oxr_pt_set_options(oxr_pt* h, const oxr_pt_options* o, int* reset_out) {
   return OXR_OK;
 }
 
+int oxr_pt_set_window(oxr_pt* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hgt) {
+  if (!h) return fail(OXR_ERR_INVALID, "null handle");
+  if ((w == 0) != (hgt == 0)) return fail(OXR_ERR_INVALID, "render window %ux%u: both sides or neither", w, hgt);
+  uint32_t nw[4] = {w ? x0 : 0u, w ? y0 : 0u, w, hgt};
+  if (std::memcmp(nw, h->win, sizeof(nw)) == 0) return OXR_OK;
+  uint32_t old[4];
+  std::memcpy(old, h->win, sizeof(old));
+  std::memcpy(h->win, nw, sizeof(nw));
+  int rc;
+  if (h->has_opts && (rc = h->check_window())) {
+    std::memcpy(h->win, old, sizeof(old));  // a rejected window leaves the tracer as it was
+    return rc;
+  }
+  HIPC(hipSetDevice(h->eng.device));
+  if ((rc = h->ensure_accum())) return rc;
+  return h->reset();
+}
+
 int oxr_pt_reset(oxr_pt* h) {
   if (!h) return fail(OXR_ERR_INVALID, "null handle");
   HIPC(hipSetDevice(h->eng.device));
@@ -1902,7 +1938,7 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
   const bool timing = e.timing, counting = e.count_trav;
   e.timing = false;
   e.count_trav = false;
-  const uint64_t npx = (uint64_t)std::max(h->opts.width, 1u) * std::max(h->opts.height, 1u);
+  const uint64_t npx = h->npx_accum();
   const uint32_t budgets[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25};
   const int waves[] = {1, 2};
   const uint32_t segs[] = {1u, 8u};
@@ -2092,7 +2128,7 @@ int oxr_pt_samples(oxr_pt* h, uint32_t* out) {
 
 int oxr_pt_read_accum(oxr_pt* h, double* accum, double* sum_sq) {
   if (!h || !accum) return fail(OXR_ERR_INVALID, "null argument");
-  size_t npx = (size_t)std::max(h->opts.width, 1u) * std::max(h->opts.height, 1u);
+  size_t npx = (size_t)h->npx_accum();
   if (!h->npx_alloc) {
     std::memset(accum, 0, npx * 4 * sizeof(double));
     if (sum_sq) std::memset(sum_sq, 0, npx * 3 * sizeof(double));
@@ -2107,7 +2143,7 @@ int oxr_pt_read_accum(oxr_pt* h, double* accum, double* sum_sq) {
 
 static int resolve_common(oxr_pt* h, const oxr_display* disp, bool rgba8, void* out) {
   if (!h || !disp || !out) return fail(OXR_ERR_INVALID, "null argument");
-  size_t npx = (size_t)std::max(h->opts.width, 1u) * std::max(h->opts.height, 1u);
+  size_t npx = (size_t)h->npx_accum();
   float bg[4];
   for (int k = 0; k < 3; ++k) bg[k] = srgb_to_linear((float)disp->background[k] / 255.0f);
   bg[3] = (float)disp->background[3] / 255.0f;

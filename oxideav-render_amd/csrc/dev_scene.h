@@ -101,7 +101,9 @@ struct DevFrame {
   float p_env;      // Integrator::p_env: 0, 1/2 or 1
   DevEnv env;
   float spread;     // 2 half_h / height (src/pathtrace.rs:2537-2540): one pixel row on the image plane
-  float pad_f[3];
+  // Render window: accumulator pixel i is frame pixel (win_x0 + i % win_w,
+  // win_y0 + i / win_w); the whole frame by default.
+  uint32_t win_x0, win_y0, win_w;
 };
 
 // Path flags (meta.z >> 24).

@@ -603,9 +603,9 @@ __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_
   const Seg sg = segs.s[seg_of(segs, pid)];
   const DevFrame& fr = jobs[sg.job].fr;
   uint32_t s, off;
-  path_sample(sg, fr.width, pid - sg.path0, s, off);
+  path_sample(sg, fr.win_w, pid - sg.path0, s, off);
   uint32_t px = sg.px0 + off;
-  uint32_t x = px % fr.width, y = px / fr.width;
+  uint32_t x = fr.win_x0 + px % fr.win_w, y = fr.win_y0 + px / fr.win_w;
   uint32_t index = sg.first + s;
   uint32_t ps = pixel_seed_of(fr.seed, x, y);
   F4 p0 = pattern(ps, index, 0);
@@ -1479,7 +1479,7 @@ __global__ void k_finalize(const DevJob* __restrict__ jobs, SegTable segs, uint3
   if (!finite3(radiance)) radiance = v3(0.0f, 0.0f, 0.0f);
   const Seg sg = segs.s[seg_of(segs, pid)];
   uint32_t s, off;
-  path_sample(sg, jobs[sg.job].fr.width, pid - sg.path0, s, off);
+  path_sample(sg, jobs[sg.job].fr.win_w, pid - sg.path0, s, off);
   rad_s[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
   out[sg.path0 + (size_t)s * sg.npx + off] =
       make_float4(radiance.x, radiance.y, radiance.z, (flags & PF_UNCOVERED) ? 0.0f : 1.0f);

@@ -42,6 +42,7 @@ def lib():
         "oxr_pt_set_camera": (C.c_int, [vp, C.POINTER(A.Camera), C.POINTER(C.c_int)]),
         "oxr_pt_set_options": (C.c_int, [vp, C.POINTER(A.PtOptions), C.POINTER(C.c_int)]),
         "oxr_pt_reset": (C.c_int, [vp]),
+        "oxr_pt_set_window": (C.c_int, [vp, u32, u32, u32, u32]),
         "oxr_pt_refine": (C.c_int, [vp, u32]),
         "oxr_pt_samples": (C.c_int, [vp, up]),
         "oxr_pt_read_accum": (C.c_int, [vp, dp, dp]),
@@ -98,7 +99,7 @@ def lib():
 
 EXPORTED = [
     "oxr_last_error", "oxr_device_count", "oxr_pt_create", "oxr_pt_destroy", "oxr_pt_upload_scene",
-    "oxr_pt_set_camera", "oxr_pt_set_options", "oxr_pt_reset", "oxr_pt_refine", "oxr_pt_samples",
+    "oxr_pt_set_camera", "oxr_pt_set_options", "oxr_pt_reset", "oxr_pt_set_window", "oxr_pt_refine", "oxr_pt_samples",
     "oxr_pt_read_accum", "oxr_pt_resolve_hdr", "oxr_pt_resolve_rgba8", "oxr_pt_set_timing", "oxr_pt_stats_get",
     "oxr_pt_autotune", "oxr_pt_set_environment", "oxr_pt_env_eval", "oxr_pt_tex_sample", "oxr_pt_material_eval", "oxr_pt_env_sample",
     "oxr_sobol_owen_4d", "oxr_trace_closest", "oxr_trace_any", "oxr_trace_closest_device", "oxr_trace_device_check", "oxr_pt_bvh_info",

@@ -108,6 +108,18 @@ class PathTracer:
     def reset(self):
         check(self._L.oxr_pt_reset(self._h), "reset")
 
+    def set_window(self, x0=0, y0=0, w=0, h=0):
+        """Trace only frame pixels [x0, x0 + w) x [y0, y0 + h), with the
+        samples they have in the whole-frame render; accum / hdr / image
+        then hold just that window. w = h = 0: the whole frame. Resets."""
+        check(self._L.oxr_pt_set_window(self._h, x0, y0, w, h), "set_window")
+        self._win = (x0, y0, w, h) if w else None
+
+    def _dims(self):
+        if getattr(self, "_win", None):
+            return self._win[2], self._win[3]
+        return max(self._opts.width, 1), max(self._opts.height, 1)
+
     def sync(self, scene, opts: RenderOptions):
         """Bring the tracer in line with `scene` + `opts`; returns True when
         the accumulation was reset (src/pathtrace.rs:2474-2508)."""
@@ -140,7 +152,7 @@ class PathTracer:
     def accum(self, with_sum_sq=False):
         """The accumulator: (H, W, 4) f64 sums (r, g, b, covered count)
         [and (H, W, 3) squared radiance]."""
-        W, H = max(self._opts.width, 1), max(self._opts.height, 1)
+        W, H = self._dims()
         acc = np.zeros((H, W, 4), np.float64)
         sq = np.zeros((H, W, 3), np.float64) if with_sum_sq else None
         dp = C.POINTER(C.c_double)
@@ -149,14 +161,14 @@ class PathTracer:
         return (acc, sq) if with_sum_sq else acc
 
     def hdr(self):
-        W, H = max(self._opts.width, 1), max(self._opts.height, 1)
+        W, H = self._dims()
         out = np.zeros((H, W, 4), np.float32)
         check(self._L.oxr_pt_resolve_hdr(self._h, C.byref(_display(self._opts)),
                                          out.ctypes.data_as(C.POINTER(C.c_float))), "resolve_hdr")
         return out
 
     def image(self):
-        W, H = max(self._opts.width, 1), max(self._opts.height, 1)
+        W, H = self._dims()
         out = np.zeros((H, W, 4), np.uint8)
         check(self._L.oxr_pt_resolve_rgba8(self._h, C.byref(_display(self._opts)),
                                            out.ctypes.data_as(C.POINTER(C.c_uint8))), "resolve_rgba8")

@@ -473,3 +473,27 @@ def test_gpu_empty_scene_is_background(oxr_mod):
     pt.sync(s, opts)
     pt.refine(2)
     assert (pt.image().reshape(-1, 4) == [200, 10, 77, 128]).all()
+
+
+def test_render_window_equals_the_frame_pixels(oxr_mod):
+    """oxr_pt_set_window: refine over a window traces those pixels with the
+    samples they have in the whole frame, so the window's accumulator (and
+    sums of squares) equal the frame's over the same pixels bit for bit."""
+    s = scenes.material_sweep(segments=32, rings=16, floor_half=6)
+    W, spp = 96, 8
+    pt, acc, sq = _render_gpu(oxr_mod, s, W, spp, 8)
+    for x0, y0, w, h in ((17, 30, 40, 23), (0, 0, 96, 5), (90, 91, 6, 5)):
+        pt.set_window(x0, y0, w, h)
+        pt.refine(3)
+        pt.refine(spp - 3)
+        a, q = pt.accum(with_sum_sq=True)
+        assert a.shape == (h, w, 4)
+        assert np.array_equal(a, acc[y0:y0 + h, x0:x0 + w]), (x0, y0, w, h)
+        assert np.array_equal(q, sq[y0:y0 + h, x0:x0 + w])
+    pt.set_window()
+    pt.refine(spp)
+    assert np.array_equal(pt.accum(), acc)
+    with pytest.raises(oxr_mod.BackendError):
+        pt.set_window(90, 90, 10, 10)
+        pt.refine(1)
+    pt.close()
