@@ -443,9 +443,113 @@ struct Collapser {
   }
 };
 
+// Linear BVH: items in Morton order of their centroids (10 bits per
+// axis), split top-down where the highest differing code bit flips
+// (Lauterbach et al., "Fast BVH Construction on GPUs", 2009; the radix-tree
+// split of Karras, HPG 2012); equal codes split at the middle. Boxes are
+// the unions of their items'. Linear in n after the sort: several times
+// faster than the binned SAH build, with a looser tree.
+struct Lbvh {
+  std::vector<Item>& items;
+  std::vector<BNode>& nodes;
+  std::vector<uint32_t> code;
+  uint32_t used = 1;
+  int max_leaf;
+
+  static uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | v << 16) & 0x030000ffu;
+    v = (v | v << 8) & 0x0300f00fu;
+    v = (v | v << 4) & 0x030c30c3u;
+    v = (v | v << 2) & 0x09249249u;
+    return v;
+  }
+
+  void sort() {
+    const uint32_t n = (uint32_t)items.size();
+    Box cb;
+    for (const Item& it : items) cb.grow_pt(it.c);
+    float s[3];
+    for (int k = 0; k < 3; ++k) {
+      float e = cb.hi[k] - cb.lo[k];
+      s[k] = (std::isfinite(e) && e > 0.0f) ? 1023.0f / e : 0.0f;
+    }
+    // (code, index) pairs, LSD radix sorted by two 16-bit digits: stable,
+    // so equal codes keep the items' input order.
+    std::vector<uint64_t> a(n), t(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t q[3];
+      for (int k = 0; k < 3; ++k) {
+        float v = (items[i].c[k] - cb.lo[k]) * s[k];
+        q[k] = (std::isfinite(v) && v > 0.0f) ? (uint32_t)std::fmin(v, 1023.0f) : 0u;
+      }
+      const uint32_t key = spread10(q[0]) << 2 | spread10(q[1]) << 1 | spread10(q[2]);
+      a[i] = (uint64_t)key << 32 | i;
+    }
+    std::vector<uint32_t> cnt(65537);
+    for (int pass = 0; pass < 2; ++pass) {
+      const int sh = 32 + 16 * pass;
+      std::fill(cnt.begin(), cnt.end(), 0u);
+      for (uint32_t i = 0; i < n; ++i) cnt[((a[i] >> sh) & 0xffff) + 1]++;
+      if (cnt[1] == n) continue;  // every code has this digit 0
+      for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+      for (uint32_t i = 0; i < n; ++i) t[cnt[(a[i] >> sh) & 0xffff]++] = a[i];
+      a.swap(t);
+    }
+    std::vector<Item> sorted(n);
+    code.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      sorted[i] = items[(uint32_t)a[i]];
+      code[i] = (uint32_t)(a[i] >> 32);
+    }
+    items.swap(sorted);
+  }
+
+  // Node `id` over items [lo, hi); returns its box.
+  Box build(uint32_t id, uint32_t lo, uint32_t hi) {
+    BNode& nd = nodes[id];
+    if (hi - lo <= (uint32_t)max_leaf) {
+      Box b;
+      for (uint32_t i = lo; i < hi; ++i) b.grow(items[i].b);
+      nd.b = b;
+      nd.first = lo;
+      nd.count = hi - lo;
+      return b;
+    }
+    uint32_t mid;
+    const uint32_t a = code[lo], z = code[hi - 1];
+    if (a == z) {
+      mid = lo + (hi - lo) / 2;
+    } else {
+      // First index whose code has the highest differing bit set.
+      const int bit = 31 - __builtin_clz(a ^ z);
+      const uint32_t m = 1u << bit;
+      uint32_t l = lo, h = hi - 1;  // code[l] has the bit clear, code[h] set
+      while (h - l > 1) {
+        uint32_t c = l + (h - l) / 2;
+        if (code[c] & m)
+          h = c;
+        else
+          l = c;
+      }
+      mid = h;
+    }
+    const uint32_t l = used;
+    used += 2;
+    nd.left = l;
+    nd.right = l + 1;
+    nd.count = 0;
+    Box b = build(l, lo, mid);
+    b.grow(build(l + 1, mid, hi));
+    nodes[id].b = b;
+    return b;
+  }
+};
+
 }  // namespace
 
-Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float* alpha, int max_leaf, int threads) {
+Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float* alpha, int max_leaf, int threads,
+                bool fast) {
   auto t0 = std::chrono::steady_clock::now();
   Bvh4 out;
   if (n == 0) return out;
@@ -471,8 +575,17 @@ Bvh4 build_bvh4(const float* pos, uint32_t n, const uint32_t* flags, const float
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
   nt = std::min(nt, 32);
   Builder b(items, bn, max_leaf, nt);
-  if (n >= Builder::BIG) b.tmp.resize(n);
-  if (n < 200000 || nt == 1) {
+  if (fast) {
+    Lbvh lb{items, bn, {}, 1u, max_leaf};
+    lb.sort();
+    lb.build(0, 0, n);
+    b.next_node = lb.used;
+  } else if (n >= Builder::BIG) {
+    b.tmp.resize(n);
+  }
+  if (fast) {
+    // built above
+  } else if (n < 200000 || nt == 1) {
     // One thread: build() still hands subtrees of more than 65536 items to
     // the queue; drain it here.
     b.build(0, 0, n);

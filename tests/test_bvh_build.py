@@ -34,7 +34,7 @@ int main(int argc, char** argv) {
       bmax[k] = std::fmax(bmax[k], pos[3 * i + k]);
     }
   std::vector<uint32_t> flags(n, 0); std::vector<float> alpha(n, 1.0f);
-  oxr::Bvh4 b = oxr::build_bvh4(pos.data(), n, flags.data(), alpha.data(), 4, atoi(argv[2]));
+  oxr::Bvh4 b = oxr::build_bvh4(pos.data(), n, flags.data(), alpha.data(), 4, atoi(argv[2]), argc > 4);
   // Leaf containment: every triangle of a leaf lies inside the leaf's box.
   long bad = 0, inleaf = 0;
   for (const auto& nd : b.nodes)
@@ -189,3 +189,24 @@ def test_quantised_nodes_contain_their_boxes(harness, case):
     assert qbad == 0
     assert culled == 0
 
+
+
+@pytest.mark.parametrize("n", [5, 1000, 260_000])
+def test_linear_bvh_every_triangle_in_one_leaf(harness, n):
+    """The fast (linear, Morton-order) build: every triangle in exactly one
+    leaf, every leaf box holds its triangles, the quantised nodes hold
+    their boxes; on a soup with many equal codes too (a dense clump)."""
+    d, exe = harness
+    pos = _soup(n, n + 1)
+    pos[: n // 3] = pos[: n // 3] * 1e-5 + np.float32(0.5)  # a clump: equal Morton codes, median splits
+    pf = d / f"l{n}.f32"
+    pos.tofile(pf)
+    gf = d / f"lg{n}.u32"
+    r = subprocess.run([str(exe), str(pf), "8", str(gf), "fast"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    nodes, ntris, depth, bad, inleaf, qbad, culled = map(int, r.stdout.split())
+    gids = np.fromfile(gf, dtype=np.uint32)
+    assert ntris == n and inleaf == n and bad == 0
+    assert qbad == 0 and culled == 0
+    assert np.array_equal(np.sort(gids), np.arange(n, dtype=np.uint32))
+    assert depth < 40

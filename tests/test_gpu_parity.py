@@ -493,7 +493,8 @@ def test_render_window_equals_the_frame_pixels(oxr_mod):
     pt.set_window()
     pt.refine(spp)
     assert np.array_equal(pt.accum(), acc)
-    with pytest.raises(oxr_mod.BackendError):
+    with pytest.raises(oxr_mod.InvalidOptions):  # outside the frame: an invalid argument
         pt.set_window(90, 90, 10, 10)
-        pt.refine(1)
+    pt.refine(1)  # the rejected window left the whole frame in place
+    assert pt.accum().shape == (W, W, 4)
     pt.close()

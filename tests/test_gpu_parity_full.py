@@ -13,7 +13,8 @@ against the oracle through the C ABI (marked `gpu`):
 * converged images: the GPU at N spp (seed 0) against the oracle's f64
   accumulation at 16 N spp with another seed (1) on a crop of each config's
   frame; per pixel and channel z = |mean_gpu - mean_cpu| / sigma with
-  sigma^2 = var_gpu / N + var_cpu / (16 N) from the sums of squares; fewer
+  sigma^2 = var / N + var / (16 N), var the per-sample variance pooled over
+  both runs' sums of squares (one estimator, two seeds); fewer
   than 1% of the values beyond 3 sigma (0.27% for a normal difference).
   Each case prints a CONVERGED line with max z and the fraction beyond
   3 sigma."""
@@ -220,9 +221,11 @@ def _converged(oxr_mod, oracle, s, W, H, n, crop, bounces, pt=None, os_=None, na
     full = (ag[..., 3] == n) & (ac[..., 3] == m)
     assert full.mean() > 0.9, f"{name}: crop not covered"
     mg, mc = ag[..., :3][full] / n, ac[..., :3][full] / m
-    vg = np.maximum(sg[full] / n - mg ** 2, 0.0)
-    vc = np.maximum(sc[full] / m - mc ** 2, 0.0)
-    sigma = np.sqrt(vg / n + vc / m)
+    # Same estimator on both sides (other seed): the per-sample variance is
+    # pooled over all n + m samples, so a short run that missed a rare
+    # large sample does not shrink sigma.
+    var = np.maximum((sg[full] + sc[full]) / (n + m) - ((ag[..., :3][full] + ac[..., :3][full]) / (n + m)) ** 2, 0.0)
+    sigma = np.sqrt(var / n + var / m)
     diff = np.abs(mg - mc)
     z = np.where(sigma > 0, diff / np.where(sigma > 0, sigma, 1.0), np.where(diff > 0, np.inf, 0.0))
     res = {"config": name, "gpu_spp": n, "cpu_spp": m, "crop": [y0, y1, x0, x1], "pixels": int(full.sum()),
@@ -253,5 +256,8 @@ def test_converged_glossy_caustic_within_3_sigma(oxr_mod, oracle):
 @pytest.mark.timeout(900)
 def test_converged_terrain_within_3_sigma(oxr_mod, oracle, terrain_full):
     s, pt, os_ = terrain_full
-    _converged(oxr_mod, oracle, s, 1920, 1080, 16, (524, 556, 944, 976), 8, pt=pt, os_=os_,
+    # The config's own 1024 spp: the terrain's many small bright emitters
+    # give a heavy-tailed per-sample distribution, which shorter runs do
+    # not average into the normal regime the 3-sigma count assumes.
+    _converged(oxr_mod, oracle, s, 1920, 1080, 1024, (532, 548, 952, 968), 8, pt=pt, os_=os_,
                name="C2 terrain 1920x1080")

@@ -1,0 +1,39 @@
+"""The converged reference of the glossy-caustic window that bench.py's
+time-to-MSE leg measures against (tests/golden/caustic_window_reference.npz,
+written by tests/golden/make_caustic_reference.py on the GPU: plain
+sampling, f64 accumulation) checked against the oracle's own f64 render of
+a crop of that window with another seed: per pixel and channel within
+3 sigma (sigma from the reference's variance of the mean and the oracle's
+per-sample variance), and the crop means within 3 sigma of each other."""
+import os
+
+import numpy as np
+import pytest
+
+from oxr import scenes
+
+REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "caustic_window_reference.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference window not generated")
+def test_caustic_window_reference_agrees_with_the_oracle(oracle):
+    ref = np.load(REF)
+    x0, y0, w, h = (int(v) for v in ref["window"])
+    W = int(ref["frame"][0])
+    s = scenes.glossy_caustic()
+    os_ = oracle.OracleScene(s)
+    # An 8 x 8 crop from the window's lower half, where the caustic edge is.
+    cx, cy = x0 + w // 2 - 4, y0 + h - 12
+    n = 4096
+    opts = oracle.options(W, W, spp=n, max_bounces=int(ref["max_bounces"]), seed=3, ambient=s.ambient)
+    acc, sq = os_.render_rect(W, W, 0, n, cx, cx + 8, cy, cy + 8, opts=opts)
+    assert (acc[..., 3] == n).all()
+    mc = acc[..., :3] / n
+    vc = np.maximum(sq / n - mc * mc, 0.0) / n
+    mr = ref["mean"][cy - y0:cy - y0 + 8, cx - x0:cx - x0 + 8]
+    vr = ref["var_mean"][cy - y0:cy - y0 + 8, cx - x0:cx - x0 + 8]
+    z = np.abs(mr - mc) / np.sqrt(vr + vc + 1e-30)
+    assert (z > 3).mean() < 0.02, ((z > 3).mean(), z.max())
+    zm = abs(mr.mean() - mc.mean()) / np.sqrt((vr.sum() + vc.sum()) / mr.size ** 2)
+    assert zm < 3, zm
+    assert mr.max() > 1.0  # the crop holds part of the caustic
