@@ -482,6 +482,18 @@ int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays,
  * the last oxr_trace_closest_device call (OXR_ERR_DEVICE; its hits are
  * then misses). Call it before the next refine or trace on the handle. */
 int oxr_trace_device_check(oxr_pt* h);
+/* oxr_trace_closest_device without allocating: no hipMalloc, hipFree,
+ * event or other allocating call. Its work counter and traversal-stack
+ * spill live in a fixed 64 MiB device array per GPU (set up when the
+ * library first uses the device), not in the handle's buffers, so it is
+ * ordered only on `stream` (NULL: the handle's stream) and may be captured
+ * into a HIP graph. Same hits as oxr_trace_closest_device. All calls on one
+ * GPU share the array: calls that overlap on one GPU race. A tree deeper
+ * than the array's spill capacity is rejected (OXR_ERR_INVALID). No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (HIP graph capture, tight serving loops). */
+int oxr_trace_closest_device_noalloc(oxr_pt* h, uint32_t n, const void* d_rays,
+                                     void* d_hits, void* stream);
 
 /* Flattened BVH statistics: node and leaf counts, SAH cost, depth. */
 typedef struct oxr_bvh_info {

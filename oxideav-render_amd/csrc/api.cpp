@@ -142,6 +142,16 @@ const Tables& tables() {
 std::mutex g_const_mu;
 bool g_const_done[256];
 
+// Fixed scratch of oxr_trace_closest_device_noalloc: its control block
+// (work counter, overflow flag) and its traversal-stack spill, so that
+// call allocates nothing. Every GPU the library runs on holds one copy
+// until the process ends; calls on one GPU share it (overlapping calls
+// race). g_scratch[d] is the device address of device d's copy: host code
+// uses it, never the array's name, which in host code is only a stand-in.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+void* g_scratch[256];
+
 int ensure_device(int device) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(OXR_ERR_DEVICE, "no HIP device available");
@@ -151,6 +161,11 @@ int ensure_device(int device) {
   if (!g_const_done[device]) {
     const Tables& t = tables();
     HIPC(launch::upload_constants(&t.sobol[0][0], t.sheen, t.srgb_th));
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(OXR_ERR_DEVICE, "device %d: address of the fixed scratch: %s", device, hipGetErrorString(e));
+    g_scratch[device] = p;
     g_const_done[device] = true;
   }
   return OXR_OK;
@@ -2291,6 +2306,35 @@ int oxr_trace_closest_device(oxr_pt* h, uint32_t n, const void* d_rays, void* d_
     HIPC(hipEventRecord(e.ev_order[1], s));
     HIPC(hipStreamWaitEvent(e.stream, e.ev_order[1], 0));
   }
+  return OXR_OK;
+}
+
+int oxr_trace_closest_device_noalloc(oxr_pt* h, uint32_t n, const void* d_rays, void* d_hits, void* stream) {
+  if (!h || (n && (!d_rays || !d_hits))) return fail(OXR_ERR_INVALID, "null argument");
+  if (!h->scene.valid) return fail(OXR_ERR_STATE, "no scene uploaded");
+  if (n == 0) return OXR_OK;
+  Engine& e = h->eng;
+  HIPC(hipSetDevice(e.device));
+  unsigned char* base = static_cast<unsigned char*>(g_scratch[e.device]);
+  if (!base) return fail(OXR_ERR_STATE, "device %d: fixed scratch not set up", e.device);
+  // Layout: the control block, then one spill stack of spill_cap entries
+  // per thread of the grid. The traversal is persistent, so a grid that
+  // fits the scratch gives the same hits as the full one.
+  const size_t ctrl_bytes = (sizeof(Ctrl) + 255) & ~(size_t)255;
+  const uint32_t cap = h->scene.spill_cap;
+  const size_t per_block = (size_t)launch::trace_block_threads() * cap * sizeof(uint32_t);
+  const size_t fit = (NOALLOC_SCRATCH_BYTES - ctrl_bytes) / per_block;
+  if (fit < 1)
+    return fail(OXR_ERR_INVALID, "traversal needs %zu bytes of scratch per block (tree depth %u); the fixed scratch holds %zu",
+                per_block + ctrl_bytes, h->scene.bvh.max_depth, NOALLOC_SCRATCH_BYTES);
+  const int grid = (int)std::min<size_t>((size_t)e.grid_closest, fit);
+  Ctrl* ctrl = reinterpret_cast<Ctrl*>(base);
+  uint32_t* spill = reinterpret_cast<uint32_t*>(base + ctrl_bytes);
+  hipStream_t s = stream ? (hipStream_t)stream : e.stream;
+  launch::zero_ctrl(s, ctrl);  // a kernel, not a memset: the call may be graph-captured
+  launch::closest(s, grid, h->scene.sc, nullptr, nullptr, false, false, (const float4*)d_rays, nullptr, nullptr, n,
+                  nullptr, &ctrl->work[0].v, 0, (uint4*)d_hits, spill, cap, &ctrl->error);
+  HIPC(hipGetLastError());
   return OXR_OK;
 }
 

@@ -61,6 +61,7 @@ def lib():
         "oxr_trace_any": (C.c_int, [vp, u32, C.POINTER(A.Ray), bp]),
         "oxr_trace_closest_device": (C.c_int, [vp, u32, vp, vp, vp]),
         "oxr_trace_device_check": (C.c_int, [vp]),
+        "oxr_trace_closest_device_noalloc": (C.c_int, [vp, u32, vp, vp, vp]),
         "oxr_pt_bvh_info": (C.c_int, [vp, C.POINTER(A.BvhInfo)]),
         "oxr_pt_bvh_export": (C.c_int, [vp, vp, vp]),
         "oxr_bsdf_eval": (C.c_int, [C.c_int, u32, C.POINTER(A.MaterialSample), fp, fp, bp, fp, fp, fp]),
@@ -102,7 +103,7 @@ EXPORTED = [
     "oxr_pt_set_camera", "oxr_pt_set_options", "oxr_pt_reset", "oxr_pt_set_window", "oxr_pt_refine", "oxr_pt_samples",
     "oxr_pt_read_accum", "oxr_pt_resolve_hdr", "oxr_pt_resolve_rgba8", "oxr_pt_set_timing", "oxr_pt_stats_get",
     "oxr_pt_autotune", "oxr_pt_set_environment", "oxr_pt_env_eval", "oxr_pt_tex_sample", "oxr_pt_material_eval", "oxr_pt_env_sample",
-    "oxr_sobol_owen_4d", "oxr_trace_closest", "oxr_trace_any", "oxr_trace_closest_device", "oxr_trace_device_check", "oxr_pt_bvh_info",
+    "oxr_sobol_owen_4d", "oxr_trace_closest", "oxr_trace_any", "oxr_trace_closest_device", "oxr_trace_device_check", "oxr_trace_closest_device_noalloc", "oxr_pt_bvh_info",
     "oxr_pt_bvh_export", "oxr_bsdf_eval", "oxr_bsdf_sample",
     "oxr_srv_create", "oxr_srv_destroy", "oxr_srv_add_scene", "oxr_srv_submit", "oxr_srv_step",
     "oxr_srv_job_status", "oxr_srv_read_accum", "oxr_srv_release", "oxr_srv_stats_get",
