@@ -42,20 +42,24 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def bytes_per_closest_ray(pt, spp):
-    """Algorithmic HBM bytes per closest-hit ray: 32 (ray) + 16 (hit) +
-    64 per 4-wide node (quantised child boxes) + 48 per triangle, with the
-    node and triangle fetches counted by the device itself (the counting
-    closest-hit variant) over every closest ray of one refine(spp) of the
-    workload; the accumulation is reset afterwards."""
+def traversal_counts(pt, spp):
+    """Node and triangle fetches per ray, counted by the device itself (the
+    counting closest-hit and occlusion variants) over every ray of one
+    refine(spp) of the workload; the accumulation is reset afterwards.
+    Algorithmic HBM bytes per closest ray: 32 (ray) + 16 (hit) + 64 per
+    4-wide node (quantised child boxes) + 48 per triangle; per occlusion
+    ray: 32 (ray) + 4 (slot) + 1 (visibility) + 64 per node + 48 per
+    triangle tested."""
     pt.set_timing(False, count=True)
     pt.refine(spp)
     s = pt.stats()
     pt.set_timing(False)
     pt.reset()
-    n = max(s["closest_rays"], 1)
-    n_node, n_tri = s["closest_nodes"] / n, s["closest_tris"] / n
-    return 32.0 + 16.0 + 64.0 * n_node + 48.0 * n_tri, n_node, n_tri, s["closest_rays"]
+    nc, ns = max(s["closest_rays"], 1), max(s["shadow_rays"], 1)
+    c_node, c_tri = s["closest_nodes"] / nc, s["closest_tris"] / nc
+    a_node, a_tri = s["anyhit_nodes"] / ns, s["anyhit_tris"] / ns
+    return {"closest": (32.0 + 16.0 + 64.0 * c_node + 48.0 * c_tri, c_node, c_tri, s["closest_rays"]),
+            "anyhit": (32.0 + 4.0 + 1.0 + 64.0 * a_node + 48.0 * a_tri, a_node, a_tri, s["shadow_rays"])}
 
 
 def cpu_model():
@@ -188,32 +192,42 @@ def serving_bench(n=512, lam=40.0, seed=0, rates=(32.0, 24.0, 16.0, 12.0, 8.0), 
 
 
 def roofline_of(pt, spp, tot, steps, workload, W):
-    """The closest-hit roofline of one refine(spp) step: algorithmic bytes
-    per launch over the average HIP-event launch time (tot: the timed
-    step's stats scaled by `steps`), against 8 TB/s."""
-    bpr, n_node, n_tri, nr = bytes_per_closest_ray(pt, spp)
-    avg_ms = tot["ms_closest"] / tot["launches_closest"]
-    bytes_per_launch = bpr * tot["closest_rays"] / tot["launches_closest"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    # Measured HBM bytes per launch: the PMC figure per closest ray
-    # (FETCH_SIZE + WRITE_SIZE, copy-calibrated, profiles/) times this
-    # run's rays per launch.
-    traffic = None
+    """Rooflines of the two traversal kernels over one refine(spp) step:
+    algorithmic bytes per launch (device-counted fetches) over the average
+    HIP-event launch time, against 8 TB/s. `traffic` is the measured HBM
+    bytes per launch (the copy-calibrated FETCH_SIZE + WRITE_SIZE per ray in
+    profiles/, times this run's rays per launch); where it is well below
+    the algorithmic bytes the BVH is served from L2 / MALL and `achieved` is
+    an effective bandwidth, not HBM's."""
+    counts = traversal_counts(pt, spp)
     prof = os.path.join(ROOT, "profiles", f"pathtrace_{workload}_{W}.json")
-    if os.path.exists(prof):
-        pj = json.load(open(prof))
-        if "hbm_bytes_per_closest_ray" in pj:
-            traffic = pj["hbm_bytes_per_closest_ray"] * tot["closest_rays"] / tot["launches_closest"]
-    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-         "algorithmic_bytes_per_launch": bytes_per_launch,
-         "kernel": "k_closest<true>", "avg_launch_ms": avg_ms,
-         "bytes_per_ray": bpr, "nodes_per_ray": n_node, "tris_per_ray": n_tri, "rays_counted": nr}
-    if traffic is not None:
-        r["hbm_traffic_gbs"] = traffic / (avg_ms * 1e-3) / 1e9
-        r["hbm_traffic_frac"] = r["hbm_traffic_gbs"] / HBM_PEAK_GBS
-        r["served_from_cache_frac"] = max(0.0, 1.0 - traffic / bytes_per_launch)
-    return r
+    pj = json.load(open(prof)) if os.path.exists(prof) else {}
+    out = {}
+    for kern, rays_key, ms_key, n_key, pkey in (
+            ("closest", "closest_rays", "ms_closest", "launches_closest", "hbm_bytes_per_closest_ray"),
+            ("anyhit", "shadow_rays", "ms_anyhit", "launches_anyhit", "hbm_bytes_per_shadow_ray")):
+        if not tot.get(n_key):
+            continue
+        bpr, n_node, n_tri, nr = counts[kern]
+        avg_ms = tot[ms_key] / tot[n_key]
+        bytes_per_launch = bpr * tot[rays_key] / tot[n_key]
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = pj[pkey] * tot[rays_key] / tot[n_key] if pkey in pj else None
+        r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+             "algorithmic_bytes_per_launch": bytes_per_launch,
+             "kernel": "k_closest<true>" if kern == "closest" else "k_anyhit<true>", "avg_launch_ms": avg_ms,
+             "bytes_per_ray": bpr, "nodes_per_ray": n_node, "tris_per_ray": n_tri, "rays_counted": nr}
+        if traffic is not None:
+            r["hbm_traffic_gbs"] = traffic / (avg_ms * 1e-3) / 1e9
+            r["hbm_traffic_frac"] = r["hbm_traffic_gbs"] / HBM_PEAK_GBS
+            r["kind"] = ("effective: the BVH is served from L2 / MALL (measured HBM traffic is "
+                         f"{traffic / bytes_per_launch:.2f}x the algorithmic bytes)"
+                         if traffic < 0.5 * bytes_per_launch else "HBM")
+        if kern in pj.get("counters", {}):
+            r["counters"] = pj["counters"][kern]
+        out[kern] = r
+    return out
 
 
 def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16):
@@ -246,6 +260,7 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16):
     tot = {k: st[k] * steps for k in ("closest_rays", "shadow_rays", "ms_closest", "ms_anyhit", "ms_shade",
                                       "ms_other")}
     tot["launches_closest"] = st["launches_closest"] * steps
+    tot["launches_anyhit"] = st["launches_anyhit"] * steps
     out = {"workload": f"terrain {W}x{H}, {scene.num_tris} tris, {cfg['bounces']} bounces, "
                        f"{steps * spp_per_step} spp timed",
            "paths_per_s": W * H * spp_per_step * steps / dt,
@@ -254,7 +269,10 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16):
                          "other": tot["ms_other"]},
            "scene_sync_s": sync_s, "bvh": pt.bvh_info()}
     if tot["launches_closest"]:
-        out["roofline"] = roofline_of(pt, spp_per_step, tot, steps, "terrain", W)
+        rl = roofline_of(pt, spp_per_step, tot, steps, "terrain", W)
+        out["roofline"] = rl["closest"]
+        if "anyhit" in rl:
+            out["roofline_anyhit"] = rl["anyhit"]
     pt.close()
     import oracle as O
     t = time.perf_counter()
@@ -466,6 +484,7 @@ def main():
     tot = {k: s[k] * a.steps for k in ("paths", "closest_rays", "shadow_rays", "ms_closest", "ms_anyhit",
                                        "ms_shade", "ms_other")}
     tot["launches_closest"] = s["launches_closest"] * a.steps
+    tot["launches_anyhit"] = s["launches_anyhit"] * a.steps
     rays = tot["closest_rays"] + tot["shadow_rays"]
     out = {
         "metric": METRIC,
@@ -494,7 +513,10 @@ def main():
         "kernel_ms_per_step_source": "one extra step with per-kernel HIP events, x steps",
     }
     if not a.no_roofline and tot["launches_closest"]:
-        out["roofline"] = roofline_of(pt, a.spp_per_step, tot, a.steps, a.workload, W)
+        rl = roofline_of(pt, a.spp_per_step, tot, a.steps, a.workload, W)
+        out["roofline"] = rl["closest"]
+        if "anyhit" in rl:
+            out["roofline_anyhit"] = rl["anyhit"]
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cfg, ambient, gpu_accum=gpu_accum)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -377,11 +377,13 @@ typedef struct oxr_pt_stats {
   uint32_t graph_captures;  /* of which (re)captured the graph first      */
   uint64_t closest_nodes;   /* node fetches of the closest-hit rays        */
   uint64_t closest_tris;    /* triangle fetches of the closest-hit rays    */
+  uint64_t anyhit_nodes;    /* node fetches of the occlusion rays          */
+  uint64_t anyhit_tris;     /* triangles the occlusion rays tested         */
 } oxr_pt_stats;
 /* enable: bit 0 (OXR_TIMING_EVENTS) times every kernel launch with HIP
- * events; bit 1 (OXR_TIMING_COUNTS) counts the closest-hit traversal work
- * of single-scene refines (a counting variant of the kernel). Either runs
- * the bounce loop eagerly instead of as a graph. */
+ * events; bit 1 (OXR_TIMING_COUNTS) counts the closest-hit and occlusion
+ * traversal work of single-scene refines (counting variants of the
+ * kernels). Either runs the bounce loop eagerly instead of as a graph. */
 #define OXR_TIMING_EVENTS 1
 #define OXR_TIMING_COUNTS 2
 int oxr_pt_set_timing(oxr_pt* h, int enable);
@@ -438,13 +440,34 @@ typedef struct oxr_srv_stats {
   uint32_t queued;
   uint64_t bytes_resident;      /* scenes + accumulators + wavefront buffers */
   double busy_ms;               /* wall time inside oxr_srv_step            */
+  uint32_t graph_captures;      /* bounce loops captured as HIP graphs      */
+  uint32_t graph_replays;       /* rounds whose bounce loop replayed one    */
+  uint64_t single_scene_rounds; /* rounds of one scene (single-scene kernels) */
+  uint32_t scenes;              /* resident scenes                          */
+  uint32_t pad;
 } oxr_srv_stats;
 
-/* mem_budget: bytes the server may keep resident for accumulators
- * (0 = 1/4 of the device). */
+/* mem_budget: bytes the server may keep resident: scenes, the shared
+ * wavefront buffers and job accumulators (0 = 1/4 of the device). A job is
+ * admitted while its accumulator fits (or when nothing else is active). */
 int oxr_srv_create(int device, uint64_t mem_budget, oxr_srv** out);
 int oxr_srv_destroy(oxr_srv* s);
 int oxr_srv_add_scene(oxr_srv* s, const oxr_scene_desc* scene, uint32_t* scene_id);
+/* A scene prepared on the host only (validation, BVH, light tables,
+ * materials): no device call, so any thread may build one while the
+ * server's owner keeps stepping; the object keeps its own copies of the
+ * scene's arrays. flags: OXR_BUILD_FAST builds a linear (Morton-order)
+ * BVH, several times faster than the default binned SAH and somewhat
+ * slower to trace: for scenes that arrive with a request. */
+typedef struct oxr_prebuilt oxr_prebuilt;
+#define OXR_BUILD_FAST 1u
+int oxr_scene_prebuild(const oxr_scene_desc* scene, uint32_t flags, oxr_prebuilt** out);
+int oxr_prebuilt_destroy(oxr_prebuilt* p);
+/* Uploads a prebuilt scene to the server's device (owner thread). */
+int oxr_srv_add_prebuilt(oxr_srv* s, const oxr_prebuilt* p, uint32_t* scene_id);
+/* Frees a scene as soon as no queued or active job uses it; its id is not
+ * reused and no longer accepted. */
+int oxr_srv_remove_scene(oxr_srv* s, uint32_t scene_id);
 int oxr_srv_submit(oxr_srv* s, uint32_t scene_id, const oxr_camera* cam, const oxr_pt_options* opts,
                    uint32_t* job_id);
 /* Admit what fits, then run one round over the active jobs (no-op when

@@ -88,6 +88,10 @@ def lib():
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
         L.oxrcpu_bvh4_count.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
         L.oxrcpu_bvh4_stack_depth.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
+        L.oxrcpu_bvh4_count_any.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, fp, up]
+        L.oxrcpu_record_shadow_rays.restype = C.c_uint32
+        L.oxrcpu_record_shadow_rays.argtypes = [C.c_void_p, C.POINTER(A.Camera), C.POINTER(A.PtOptions), C.c_uint32,
+                                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
         L.oxrcpu_tex_sample.restype = C.c_int
         L.oxrcpu_tex_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, fp, fp, fp]
         L.oxrcpu_tex_grad_lod.restype = C.c_int
@@ -239,6 +243,17 @@ class OracleScene:
         out = np.zeros((max_rays, 8), np.float32)
         m = lib().oxrcpu_record_rays(self.h, C.byref(cam), C.byref(opts), first, n, y0, y1, x_step, max_rays,
                                      out.ctypes.data_as(fp))
+        return out[:m]
+
+    def record_shadow_rays(self, width, height, first, n, y0, y1, x_step=1, max_rays=1 << 20, opts=None):
+        """Occlusion rays the oracle's paths trace (8 floats: o, skip id bits,
+        d, t_max)."""
+        if opts is None:
+            opts = options(width, height, ambient=getattr(self.scene, "ambient", 0.0))
+        cam = self.scene.camera.resolve(width, height)
+        out = np.zeros((max_rays, 8), np.float32)
+        m = lib().oxrcpu_record_shadow_rays(self.h, C.byref(cam), C.byref(opts), first, n, y0, y1, x_step, max_rays,
+                                            out.ctypes.data_as(fp))
         return out[:m]
 
     def tex_sample(self, tex, uv, lod=None, srgb=False):
@@ -491,6 +506,16 @@ def bvh4_count(nodes, tris, rays):
     counts = np.zeros((len(rays), 2), np.uint32)
     lib().oxrcpu_bvh4_count(nodes.ctypes.data, tris.ctypes.data, len(rays), rays.ctypes.data_as(fp),
                             counts.ctypes.data_as(up))
+    return counts
+
+
+def bvh4_count_any(nodes, tris, rays):
+    """(nodes fetched, triangles tested) per occlusion ray (o, skip id bits,
+    d, t_max) on an exported device BVH, in the device's any-hit order."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    counts = np.zeros((len(rays), 2), np.uint32)
+    lib().oxrcpu_bvh4_count_any(nodes.ctypes.data, tris.ctypes.data, len(rays), rays.ctypes.data_as(fp),
+                                counts.ctypes.data_as(up))
     return counts
 
 

@@ -2171,6 +2171,7 @@ struct Integrator {
   float ambient;
   oxr_pt_options pt;
   std::vector<std::array<float, 8>>* record = nullptr;  // closest-hit rays, when recording
+  std::vector<std::array<float, 8>>* record_shadow = nullptr;  // occlusion rays (o, skip bits, d, t_max)
 
   f3 clampc(f3 c) const {
     float m = max3(c);
@@ -2196,6 +2197,11 @@ struct Integrator {
     }
   }
   bool visible(f3 o, f3 d, float t_max, const Sampler& s, uint32_t ray, uint32_t skip) const {
+    if (record_shadow) {
+      float sk;
+      std::memcpy(&sk, &skip, 4);
+      record_shadow->push_back({o[0], o[1], o[2], sk, d[0], d[1], d[2], t_max});
+    }
     return !occluded(ts, o, d, 0.0f, t_max, [&](const Hit& c) { return c.g != skip && accept(c, s, ray, false); });
   }
   // Integrator::emission (src/pathtrace.rs:1951-1966): the factor times
@@ -3091,12 +3097,16 @@ void oxrcpu_srgb_to_linear(uint32_t n, const float* c, float* out) {
 // Records the closest-hit rays of samples [first, first+n) of rows
 // [y0, y1) (single thread, row-major, sample-minor) into out (8 floats per
 // ray: o, k, d, t_max), at most max_rays. Returns the number recorded.
-uint32_t oxrcpu_record_rays(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t first,
-                            uint32_t n, uint32_t y0, uint32_t y1, uint32_t x_step, uint32_t max_rays, float* out) {
+static uint32_t record_rays(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t first,
+                            uint32_t n, uint32_t y0, uint32_t y1, uint32_t x_step, uint32_t max_rays, float* out,
+                            bool shadow) {
   uint32_t W = std::max<uint32_t>(opts->width, 1), H = std::max<uint32_t>(opts->height, 1);
   std::vector<std::array<float, 8>> rec;
   Integrator integ{h->s, *cam, W, H, std::isfinite(opts->ambient) ? fmaxr(opts->ambient, 0.0f) : 0.0f, *opts};
-  integ.record = &rec;
+  if (shadow)
+    integ.record_shadow = &rec;
+  else
+    integ.record = &rec;
   for (uint32_t y = y0; y < std::min(y1, H); ++y)
     for (uint32_t x = 0; x < W; x += std::max(x_step, 1u))
       for (uint32_t si = 0; si < n; ++si) {
@@ -3110,6 +3120,16 @@ done:
     for (int k = 0; k < 8; ++k) out[8 * i + k] = rec[i][k];
   return m;
 }
+uint32_t oxrcpu_record_rays(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t first,
+                            uint32_t n, uint32_t y0, uint32_t y1, uint32_t x_step, uint32_t max_rays, float* out) {
+  return record_rays(h, cam, opts, first, n, y0, y1, x_step, max_rays, out, false);
+}
+// The occlusion rays of the same paths (8 floats: o, skip id bits, d, t_max).
+uint32_t oxrcpu_record_shadow_rays(oxrcpu_scene* h, const oxr_camera* cam, const oxr_pt_options* opts, uint32_t first,
+                                   uint32_t n, uint32_t y0, uint32_t y1, uint32_t x_step, uint32_t max_rays,
+                                   float* out) {
+  return record_rays(h, cam, opts, first, n, y0, y1, x_step, max_rays, out, true);
+}
 
 // Traversal cost of rays on the device's flattened 4-wide BVH (64-byte
 // nodes with 8-bit quantised child boxes, decoded as the device decodes
@@ -3119,12 +3139,18 @@ done:
 // counts[2*i] = nodes fetched, counts[2*i+1] = triangles fetched;
 // depth[i] (when not null) = the most entries the ray's traversal stack
 // held at once (the device keeps 16 in LDS and spills the rest).
+// any: the occlusion order instead: the ray (o, skip id bits, d, t_max)
+// ends at the first triangle hit in range (the skip id excluded), and the
+// triangles counted are those tested up to it.
 static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts,
-                      uint32_t* depth) {
+                      uint32_t* depth, bool any = false) {
   for (uint32_t r = 0; r < n_rays; ++r) {
     const float* ry = rays + 8ull * r;
     f3 o = {ry[0], ry[1], ry[2]}, d = {ry[4], ry[5], ry[6]};
     float tmin = 0.0f, tmax = ry[7];
+    uint32_t skip;
+    std::memcpy(&skip, &ry[3], 4);
+    bool done_any = false;
     PRay pr = prepare_ray(o, d);
     f3 inv = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
     f3 oi = {o[0] * inv[0], o[1] * inv[1], o[2] * inv[2]};
@@ -3159,7 +3185,7 @@ static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, co
         bool hit = NU[12 + c] != 0xffffffffu && tnear <= tfar * (1.0f + 1.0f / 65536.0f);
         tn[c] = hit ? tnear : INF_F;
       }
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < 4 && !done_any; ++c) {
         const uint32_t cnt = (NU[10] >> (8 * c)) & 0xffu;
         if (tn[c] != INF_F && cnt > 0) {
           for (uint32_t k = 0; k < cnt; ++k) {
@@ -3170,6 +3196,11 @@ static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, co
             bool front;
             if (!intersect_tri(pr, v9, tmin, tmax, t, b1, b2, front)) continue;
             uint32_t gid = reinterpret_cast<const uint32_t*>(T)[3];
+            if (any) {
+              if (gid == skip) continue;
+              done_any = true;
+              break;
+            }
             if (best != 0xffffffffu && !(t < tmax || (t == tmax && gid < best))) continue;
             tmax = t;
             best = gid;
@@ -3177,6 +3208,7 @@ static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, co
           tn[c] = INF_F;
         }
       }
+      if (done_any) break;
       uint32_t ci[4] = {NU[12], NU[13], NU[14], NU[15]};
       for (int a = 0; a < 4; ++a)
         for (int b = a + 1; b < 4; ++b)
@@ -3199,6 +3231,10 @@ static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, co
 
 void oxrcpu_bvh4_count(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts) {
   bvh4_walk(nodes, tris, n_rays, rays, counts, nullptr);
+}
+void oxrcpu_bvh4_count_any(const float* nodes, const float* tris, uint32_t n_rays, const float* rays,
+                           uint32_t* counts) {
+  bvh4_walk(nodes, tris, n_rays, rays, counts, nullptr, true);
 }
 void oxrcpu_bvh4_stack_depth(const float* nodes, const float* tris, uint32_t n_rays, const float* rays,
                              uint32_t* depth) {

@@ -385,6 +385,63 @@ bool prepare_light(const oxr_light& l, DevLight& d) {
 // =====================================================================
 // Scenes resident on the device.
 // =====================================================================
+// A scene prepared on the host, ready to upload: everything
+// SceneBuf::upload_host copies to the device. The triangle arrays either
+// point into the caller's scene description (a direct upload) or into the
+// object's own copies (own(): a prebuilt scene that outlives the call).
+struct HostScene {
+  uint32_t n = 0;
+  float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+  std::vector<DevMaterial> mats;
+  std::vector<float4> tex_pool;
+  std::vector<DevTexLevel> tex_levels;
+  std::vector<DevTexChain> tex_chains;
+  std::vector<int32_t> chain_of;
+  std::vector<DevTexBind> binds;
+  std::vector<uint2> mattex;
+  bool any_attr = false;
+  std::vector<uint32_t> attr;
+  const float* attr_src[4] = {nullptr, nullptr, nullptr, nullptr};  // uv0, uv1, tangents, colors
+  const float* pos = nullptr;
+  const float* nrm = nullptr;
+  std::vector<float> own_arrays[6];  // pos, nrm, uv0, uv1, tangents, colors
+  std::vector<uint32_t> info;
+  bool has_blend = false, has_alpha_tex = false;
+  std::vector<DevLight> lights;
+  std::vector<uint32_t> ltris, lookup;
+  std::vector<float> pmf, cdf;
+  std::vector<Node4Q> qnodes;
+  std::vector<TriRec> tris;
+  uint32_t num_nodes = 0, num_leaves = 0, max_depth = 0;
+  double sah_cost = 0.0, build_ms = 0.0;
+
+  static constexpr size_t ATTR_W[4] = {6, 6, 12, 12};  // floats per triangle
+
+  // Copies the caller's triangle arrays, so the object outlives the call.
+  void own() {
+    auto keep = [&](const float*& src, std::vector<float>& dst, size_t count) {
+      if (!src) return;
+      dst.assign(src, src + count);
+      src = dst.data();
+    };
+    keep(pos, own_arrays[0], 9ull * n);
+    keep(nrm, own_arrays[1], 9ull * n);
+    for (int k = 0; k < 4; ++k) keep(attr_src[k], own_arrays[2 + k], ATTR_W[k] * n);
+  }
+  size_t device_bytes() const {
+    size_t b = qnodes.size() * sizeof(Node4Q) + tris.size() * sizeof(TriRec) + 36ull * n * (nrm ? 2 : 1) +
+               info.size() * 4 + mats.size() * sizeof(DevMaterial) + lights.size() * sizeof(DevLight) +
+               (ltris.size() + pmf.size() + cdf.size() + lookup.size()) * 4 + tex_pool.size() * sizeof(float4) +
+               attr.size() * 4;
+    for (int k = 0; k < 4; ++k)
+      if (attr_src[k]) b += ATTR_W[k] * 4ull * n;
+    return b;
+  }
+};
+constexpr size_t HostScene::ATTR_W[4];
+
+int prepare_scene(const oxr_scene_desc* d, bool fast, HostScene& hs);
+
 struct SceneBuf {
   DevScene sc{};
   Node4Q* d_nodes = nullptr;
@@ -445,16 +502,16 @@ struct SceneBuf {
     bytes = 0;
   }
   int upload(const oxr_scene_desc* d);
+  int upload_host(const HostScene& hs);
 };
 
-int SceneBuf::upload(const oxr_scene_desc* d) {
+int prepare_scene(const oxr_scene_desc* d, bool fast, HostScene& hs) {
   if (!d) return fail(OXR_ERR_INVALID, "null scene");
   if (d->num_tris > 0 && (!d->positions || !d->material_ids))
     return fail(OXR_ERR_INVALID, "positions and material_ids are required");
   if (d->num_tris >= (1u << 31)) return fail(OXR_ERR_INVALID, "too many triangles (%u)", d->num_tris);
   if (d->num_materials > 0 && !d->materials) return fail(OXR_ERR_INVALID, "null materials");
   if (d->num_lights > 0 && !d->lights) return fail(OXR_ERR_INVALID, "null lights");
-  release();
   const uint32_t n = d->num_tris;
   {
     float l[3] = {INF_F, INF_F, INF_F}, h[3] = {-INF_F, -INF_F, -INF_F};
@@ -465,8 +522,8 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
       h[k % 3] = std::max(h[k % 3], v);
     }
     for (int a = 0; a < 3; ++a) {
-      lo[a] = l[a] <= h[a] ? l[a] : 0.0f;
-      hi[a] = l[a] <= h[a] ? h[a] : 0.0f;
+      hs.lo[a] = l[a] <= h[a] ? l[a] : 0.0f;
+      hs.hi[a] = l[a] <= h[a] ? h[a] : 0.0f;
     }
   }
   // Materials (+ the fallback for out-of-range ids).
@@ -629,7 +686,7 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   }
   if (!cdf.empty()) cdf.back() = 1.0f;
   // BVH over the soup.
-  Bvh4 bvh4 = build_bvh4(d->positions, n, tflags.data(), alpha.data(), 4, 0);
+  Bvh4 bvh4 = build_bvh4(d->positions, n, tflags.data(), alpha.data(), 4, 0, fast);
   if (bvh4.nodes.empty()) {
     Node4 root;
     for (int i = 0; i < 4; ++i) {
@@ -640,6 +697,52 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
     }
     bvh4.nodes.push_back(root);
   }
+  hs.n = n;
+  hs.mats = std::move(mats);
+  hs.tex_pool = std::move(tb.pool);
+  hs.tex_levels = std::move(tb.levels);
+  hs.tex_chains = std::move(tb.chains);
+  hs.chain_of = std::move(tb.chain_of);
+  hs.binds = std::move(binds);
+  hs.mattex = std::move(mattex);
+  hs.any_attr = any_attr;
+  hs.attr = std::move(attr);
+  for (int k = 0; k < 4; ++k) hs.attr_src[k] = n > 0 ? attr_src[k] : nullptr;
+  hs.pos = d->positions;
+  hs.nrm = d->normals;
+  hs.info = std::move(info);
+  hs.has_blend = has_blend;
+  hs.has_alpha_tex = has_alpha_tex;
+  hs.lights = std::move(lights);
+  hs.ltris = std::move(ltris);
+  hs.lookup = std::move(lookup);
+  hs.pmf = std::move(pmf);
+  hs.cdf = std::move(cdf);
+  hs.qnodes = quantize_nodes(bvh4.nodes);
+  hs.tris = std::move(bvh4.tris);
+  hs.num_nodes = (uint32_t)bvh4.nodes.size();
+  hs.num_leaves = bvh4.num_leaves;
+  hs.max_depth = bvh4.max_depth;
+  hs.sah_cost = bvh4.sah_cost;
+  hs.build_ms = bvh4.build_ms;
+  return OXR_OK;
+}
+
+int SceneBuf::upload(const oxr_scene_desc* d) {
+  HostScene hs;
+  int rc = prepare_scene(d, false, hs);
+  if (rc) return rc;
+  return upload_host(hs);
+}
+
+int SceneBuf::upload_host(const HostScene& hs) {
+  release();
+  const uint32_t n = hs.n;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = hs.lo[a];
+    hi[a] = hs.hi[a];
+  }
+  int rc;
   auto up = [&](auto** dst, const auto* srcp, size_t count) -> int {
     int r = dalloc(dst, count);
     if (r) return r;
@@ -650,36 +753,34 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
     }
     return OXR_OK;
   };
-  const std::vector<Node4Q> qnodes = quantize_nodes(bvh4.nodes);
-  if ((rc = up(&d_nodes, qnodes.data(), qnodes.size()))) return rc;
-  if ((rc = up(&d_tris, bvh4.tris.data(), bvh4.tris.size()))) return rc;
-  if ((rc = up(&d_pos, d->positions, 9ull * n))) return rc;
-  if (d->normals) {
-    if ((rc = up(&d_nrm, d->normals, 9ull * n))) return rc;
+  if ((rc = up(&d_nodes, hs.qnodes.data(), hs.qnodes.size()))) return rc;
+  if ((rc = up(&d_tris, hs.tris.data(), hs.tris.size()))) return rc;
+  if ((rc = up(&d_pos, hs.pos, 9ull * n))) return rc;
+  if (hs.nrm) {
+    if ((rc = up(&d_nrm, hs.nrm, 9ull * n))) return rc;
   }
-  if ((rc = up(&d_info, info.data(), info.size()))) return rc;
-  if ((rc = up(&d_mats, mats.data(), mats.size()))) return rc;
-  if ((rc = up(&d_lights, lights.data(), lights.size()))) return rc;
-  if ((rc = up(&d_ltris, ltris.data(), ltris.size()))) return rc;
-  if ((rc = up(&d_lpmf, pmf.data(), pmf.size()))) return rc;
-  if ((rc = up(&d_lcdf, cdf.data(), cdf.size()))) return rc;
-  if ((rc = up(&d_llookup, lookup.data(), lookup.size()))) return rc;
-  const bool has_tex = !binds.empty() || any_attr;
-  if (!tb.chains.empty()) {
-    if ((rc = up(&d_chains, tb.chains.data(), tb.chains.size()))) return rc;
-    if ((rc = up(&d_levels, tb.levels.data(), tb.levels.size()))) return rc;
-    if ((rc = up(&d_texels, tb.pool.data(), tb.pool.size()))) return rc;
+  if ((rc = up(&d_info, hs.info.data(), hs.info.size()))) return rc;
+  if ((rc = up(&d_mats, hs.mats.data(), hs.mats.size()))) return rc;
+  if ((rc = up(&d_lights, hs.lights.data(), hs.lights.size()))) return rc;
+  if ((rc = up(&d_ltris, hs.ltris.data(), hs.ltris.size()))) return rc;
+  if ((rc = up(&d_lpmf, hs.pmf.data(), hs.pmf.size()))) return rc;
+  if ((rc = up(&d_lcdf, hs.cdf.data(), hs.cdf.size()))) return rc;
+  if ((rc = up(&d_llookup, hs.lookup.data(), hs.lookup.size()))) return rc;
+  const bool has_tex = !hs.binds.empty() || hs.any_attr;
+  if (!hs.tex_chains.empty()) {
+    if ((rc = up(&d_chains, hs.tex_chains.data(), hs.tex_chains.size()))) return rc;
+    if ((rc = up(&d_levels, hs.tex_levels.data(), hs.tex_levels.size()))) return rc;
+    if ((rc = up(&d_texels, hs.tex_pool.data(), hs.tex_pool.size()))) return rc;
   }
-  if (!binds.empty()) {
-    if ((rc = up(&d_binds, binds.data(), binds.size()))) return rc;
-    if ((rc = up(&d_mattex, mattex.data(), mattex.size()))) return rc;
+  if (!hs.binds.empty()) {
+    if ((rc = up(&d_binds, hs.binds.data(), hs.binds.size()))) return rc;
+    if ((rc = up(&d_mattex, hs.mattex.data(), hs.mattex.size()))) return rc;
   }
-  const size_t attr_w[4] = {6, 6, 12, 12};
   float** attr_dst[4] = {&d_uv[0], &d_uv[1], &d_tan, &d_col};
   for (int k = 0; k < 4; ++k)
-    if (attr_src[k] && n > 0 && (rc = up(attr_dst[k], attr_src[k], attr_w[k] * n))) return rc;
-  if (any_attr && (rc = up(&d_attr, attr.data(), attr.size()))) return rc;
-  chain_of = tb.chain_of;
+    if (hs.attr_src[k] && n > 0 && (rc = up(attr_dst[k], hs.attr_src[k], HostScene::ATTR_W[k] * n))) return rc;
+  if (hs.any_attr && (rc = up(&d_attr, hs.attr.data(), hs.attr.size()))) return rc;
+  chain_of = hs.chain_of;
   sc.nodes = d_nodes;
   sc.tris = d_tris;
   sc.pos = d_pos;
@@ -692,16 +793,16 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   sc.l_cdf = d_lcdf;
   sc.l_lookup = d_llookup;
   sc.num_tris = n;
-  sc.num_mats = (uint32_t)mats.size();
-  sc.num_lights = (uint32_t)lights.size();
-  sc.num_emissive = (uint32_t)ltris.size();
-  sc.has_blend = has_blend ? 1u : 0u;
+  sc.num_mats = (uint32_t)hs.mats.size();
+  sc.num_lights = (uint32_t)hs.lights.size();
+  sc.num_emissive = (uint32_t)hs.ltris.size();
+  sc.has_blend = hs.has_blend ? 1u : 0u;
   sc.has_tex = has_tex ? 1u : 0u;
-  sc.has_alpha_tex = has_alpha_tex ? 1u : 0u;
+  sc.has_alpha_tex = hs.has_alpha_tex ? 1u : 0u;
   // Any factor that is not exactly zero (NaN included) selects the
   // extension-lobe shading; a texture only scales a factor.
   bool has_ext = false;
-  for (const DevMaterial& m : mats)
+  for (const DevMaterial& m : hs.mats)
     has_ext = has_ext || !(m.transmission == 0.0f) || !(m.clearcoat == 0.0f) || !(m.sheen[0] == 0.0f) ||
               !(m.sheen[1] == 0.0f) || !(m.sheen[2] == 0.0f) || !(m.iridescence == 0.0f) || !(m.dt == 0.0f) ||
               !(m.aniso == 0.0f);
@@ -717,14 +818,14 @@ int SceneBuf::upload(const oxr_scene_desc* d) {
   sc.tex.col = d_col;
   sc.tex.attr = d_attr;
   if ((rc = up(&d_self, &sc, 1))) return rc;
-  bvh.num_nodes = (uint32_t)bvh4.nodes.size();
-  bvh.num_leaves = bvh4.num_leaves;
-  bvh.num_tris = (uint32_t)bvh4.tris.size();
-  bvh.max_depth = bvh4.max_depth;
-  bvh.sah_cost = bvh4.sah_cost;
-  bvh.build_ms = bvh4.build_ms;
+  bvh.num_nodes = hs.num_nodes;
+  bvh.num_leaves = hs.num_leaves;
+  bvh.num_tris = (uint32_t)hs.tris.size();
+  bvh.max_depth = hs.max_depth;
+  bvh.sah_cost = hs.sah_cost;
+  bvh.build_ms = hs.build_ms;
   // Stack bound: each level leaves at most 3 siblings behind.
-  spill_cap = (uint32_t)std::max(4, 4 * (int)bvh4.max_depth + 8 - launch::stack_lds_entries());
+  spill_cap = (uint32_t)std::max(4, 4 * (int)hs.max_depth + 8 - launch::stack_lds_entries());
   valid = true;
   return OXR_OK;
 }
@@ -906,7 +1007,7 @@ struct Engine {
 
   ~Engine() {
     if (device >= 0) (void)hipSetDevice(device);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (auto& g : graphs_cached) (void)hipGraphExecDestroy(g.second);
     free_paths();
     dfree(d_ctrl);
     dfree(d_spill);
@@ -1075,9 +1176,14 @@ struct Engine {
   };
   bool graphs = true;
   uint32_t gen = 0;  // bumped when path or spill buffers move
-  hipGraphExec_t gexec = nullptr;
-  GraphKey gkey{};
+  // Captured bounce loops, most recent first (at most GRAPH_CACHE): a
+  // server alternates between a few round shapes (one per scene, mixed).
+  static constexpr size_t GRAPH_CACHE = 8;
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs_cached;
   uint64_t graph_captures = 0, graph_replays = 0;
+  // Serving rounds: the shading grid is the full persistent grid whatever
+  // the round's path count, so the captured loop does not depend on it.
+  bool fixed_shade_grid = false;
 
   void bounce(uint32_t k, uint32_t cur, const DevJob* d_jobs, uint32_t P, uint32_t nslots, int grid_shade,
               const DevScene& sc_any, bool multi, const GuideBatch* gb) {
@@ -1106,7 +1212,7 @@ struct Engine {
     launch::anyhit(stream, alpha_sc ? grid_any_alpha : grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray,
                    pb.sidx, pb.meta, nslots, 0,
                    &d_ctrl->scount.v, &d_ctrl->work[8].v, pb.vis, d_spill, spill_cap, &d_ctrl->error,
-                   xcd_segments, alpha_sc);
+                   xcd_segments, alpha_sc, count_trav ? &d_ctrl->trav[2].v : nullptr);
     span_end(1, e3);
   }
 
@@ -1118,7 +1224,7 @@ struct Engine {
     std::memset(&key, 0, sizeof(key));
     key.max_bounces = max_bounces;
     key.nslots = nslots;
-    key.P = P;
+    key.P = gb ? P : 0u;
     key.gen = gen;
     key.grid_shade = grid_shade;
     key.shade_waves = shade_waves;
@@ -1129,9 +1235,18 @@ struct Engine {
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
     key.alpha_sc = alpha_sc;
     key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u) | (ext_variant ? 4u : 0u) | (area_variant ? 8u : 0u);
-    if (!gexec || std::memcmp(&key, &gkey, sizeof(key)) != 0) {
-      if (gexec) (void)hipGraphExecDestroy(gexec);
-      gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    for (size_t q = 0; q < graphs_cached.size(); ++q)
+      if (std::memcmp(&key, &graphs_cached[q].first, sizeof(key)) == 0) {
+        gexec = graphs_cached[q].second;
+        std::rotate(graphs_cached.begin(), graphs_cached.begin() + q, graphs_cached.begin() + q + 1);
+        break;
+      }
+    if (!gexec) {
+      if (graphs_cached.size() >= GRAPH_CACHE) {
+        (void)hipGraphExecDestroy(graphs_cached.back().second);
+        graphs_cached.pop_back();
+      }
       if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
       uint32_t cur = 0;
       for (uint32_t k = 0; k <= max_bounces; ++k, cur ^= 1u)
@@ -1147,11 +1262,10 @@ struct Engine {
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) {
         (void)hipGetLastError();
-        gexec = nullptr;
         graphs = false;
         return false;
       }
-      gkey = key;
+      graphs_cached.insert(graphs_cached.begin(), {key, gexec});
       graph_captures++;
       stats.graph_captures++;
     }
@@ -1184,7 +1298,8 @@ struct Engine {
     alpha_sc = asc;
     multi_jobs = false;  // every segment of job slot 0: the single-job shading variant
     for (uint32_t q = 0; q < segs.n; ++q) multi_jobs = multi_jobs || segs.s[q].job != 0u;
-    const int grid_shade = (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
+    const int grid_shade =
+        fixed_shade_grid ? num_cu * 8 : (int)std::min<uint64_t>((P + 255) / 256, (uint64_t)num_cu * 8);
     hipEvent_t ea{};
     span_begin(ea);
     launch::camera(stream, d_jobs, segs, P, pb, d_ctrl);
@@ -1230,6 +1345,8 @@ struct Engine {
     stats.shadow_rays = c.rays_shadow;
     stats.closest_nodes = c.trav[0].v;
     stats.closest_tris = c.trav[1].v;
+    stats.anyhit_nodes = c.trav[2].v;
+    stats.anyhit_tris = c.trav[3].v;
     for (auto& s : spans) {
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, s.a, s.b);
@@ -1711,7 +1828,10 @@ struct oxr_srv {
   std::vector<Job> jobs;
   std::vector<uint32_t> queue;   // queued job ids (FIFO)
   std::vector<uint32_t> active;  // active job ids
-  uint64_t mem_budget = 0, mem_used = 0;
+  std::vector<uint8_t> doomed;   // scenes to free once no job uses them
+  uint64_t mem_budget = 0, mem_used = 0;  // mem_used: accumulators
+  uint64_t scene_bytes = 0;               // resident scenes
+  uint32_t nslots_all = 1;                // NEE slots of the widest resident scene
   DevJob* d_jobs = nullptr;      // device job table, MAX_SEGS slots
   std::vector<uint32_t> free_slots;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -1732,11 +1852,40 @@ struct oxr_srv {
     dfree(d_jobs);
   }
 
+  int add(std::unique_ptr<SceneBuf> sb, uint32_t* id) {
+    scene_bytes += sb->bytes;
+    nslots_all = std::max(nslots_all, sb->sc.num_lights + 1);
+    scenes.push_back(std::move(sb));
+    doomed.push_back(0);
+    *id = (uint32_t)scenes.size() - 1;
+    return OXR_OK;
+  }
+
+  bool in_use(uint32_t scene) const {
+    for (uint32_t id : queue)
+      if (jobs[id].scene == scene) return true;
+    for (uint32_t id : active)
+      if (jobs[id].scene == scene) return true;
+    return false;
+  }
+
+  // Frees the scenes marked for removal that no queued or active job uses
+  // (between rounds: nothing is in flight on the device then).
+  void reap() {
+    for (size_t i = 0; i < scenes.size(); ++i)
+      if (doomed[i] && scenes[i] && !in_use((uint32_t)i)) {
+        scene_bytes -= scenes[i]->bytes;
+        scenes[i].reset();
+      }
+  }
+
+  // Admission by HBM footprint: resident scenes, the shared wavefront
+  // buffers and the accumulators of the active jobs, against the budget.
   int admit() {
     while (!queue.empty() && !free_slots.empty()) {
       Job& j = jobs[queue.front()];
       size_t need = j.npx * 7 * sizeof(double);
-      if (mem_used + need > mem_budget && !active.empty()) break;  // wait for memory
+      if (mem_used + scene_bytes + eng.path_bytes() + need > mem_budget && !active.empty()) break;  // wait for memory
       int rc;
       if ((rc = dalloc(&j.d_accum, 4 * j.npx)) || (rc = dalloc(&j.d_sumsq, 3 * j.npx))) {
         dfree(j.d_accum);
@@ -1771,32 +1920,44 @@ struct oxr_srv {
     uint32_t done = 0;
     if (!active.empty()) {
       // Share the path budget: every active job contributes whole sample
-      // passes over its pixels (at least one), round-robin from `rr`.
+      // passes over its pixels (at least one), round-robin from `rr`. The
+      // round leads with the jobs of one scene (the next in turn), so it
+      // runs the single-scene kernels; jobs of other scenes fill it only
+      // when the lead scene's work leaves more than half of it empty.
       SegTable segs{};
       std::vector<Engine::Reduce> red;
       std::vector<std::pair<uint32_t, uint32_t>> took;  // (job id, samples)
-      uint32_t P = 0, nslots = 1, max_b = 0, spill = 4;
+      uint32_t P = 0, nslots = nslots_all, max_b = 0, spill = 4;
       size_t na = active.size();
       uint32_t budget = eng.path_budget;
-      for (size_t q = 0; q < na && segs.n < (uint32_t)MAX_SEGS; ++q) {
-        Job& j = jobs[active[(rr + q) % na]];
-        uint32_t left = j.opts.samples_per_pixel - j.samples;
-        uint32_t npx = (uint32_t)j.npx;
-        uint32_t fair = std::max<uint32_t>(1u, (uint32_t)(budget / std::max<size_t>(na, 1) / std::max(npx, 1u)));
-        uint32_t S = std::min(left, fair);
-        if (P > 0 && (uint64_t)P + (uint64_t)npx * S > budget) {
-          S = (uint32_t)((budget - P) / std::max(npx, 1u));
-          if (S == 0) continue;
+      const uint32_t lead = jobs[active[rr % na]].scene;
+      size_t n_lead = 0;
+      for (uint32_t id : active) n_lead += jobs[id].scene == lead ? 1 : 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && P >= budget / 2) break;
+        for (size_t q = 0; q < na && segs.n < (uint32_t)MAX_SEGS; ++q) {
+          Job& j = jobs[active[(rr + q) % na]];
+          if ((j.scene == lead) != (pass == 0)) continue;
+          uint32_t left = j.opts.samples_per_pixel - j.samples;
+          uint32_t npx = (uint32_t)j.npx;
+          size_t share = pass == 0 ? n_lead : na;
+          uint32_t fair = std::max<uint32_t>(1u, (uint32_t)(budget / std::max<size_t>(share, 1) / std::max(npx, 1u)));
+          uint32_t S = std::min(left, fair);
+          if (P > 0 && (uint64_t)P + (uint64_t)npx * S > budget) {
+            S = (uint32_t)((budget - P) / std::max(npx, 1u));
+            if (S == 0) continue;
+          }
+          segs.s[segs.n++] = Seg{j.slot, P, npx, 0u, j.samples, S};
+          red.push_back({P, npx, S, j.d_accum, j.d_sumsq});
+          took.push_back({j.id, S});
+          P += npx * S;
+          max_b = std::max(max_b, std::min(j.opts.max_bounces, 65534u));
+          spill = std::max(spill, scenes[j.scene]->spill_cap);
         }
-        segs.s[segs.n++] = Seg{j.slot, P, npx, 0u, j.samples, S};
-        red.push_back({P, npx, S, j.d_accum, j.d_sumsq});
-        took.push_back({j.id, S});
-        P += npx * S;
-        const DevScene& sc = scenes[j.scene]->sc;
-        nslots = std::max(nslots, sc.num_lights + 1);
-        max_b = std::max(max_b, std::min(j.opts.max_bounces, 65534u));
-        spill = std::max(spill, scenes[j.scene]->spill_cap);
       }
+      // Paths end at their own job's bounce limit; up to 8 the loop always
+      // runs 8 bounces so that rounds of different mixes share one graph.
+      if (max_b <= 8u) max_b = 8u;
       rr = na ? (rr + 1) % (uint32_t)na : 0;
       if ((rc = eng.ensure_paths(std::max(P, 1u), nslots))) return rc;
       if ((rc = eng.ensure_spill(spill))) return rc;
@@ -1812,12 +1973,17 @@ struct oxr_srv {
         area = area || sb.sc.num_emissive > 0;  // server jobs have no environment map
         if (sb.sc.has_alpha_tex && !asc) asc = sb.d_self;
       }
-      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, scenes[jobs[took[0].first].scene]->sc, multi, rc, nullptr,
-                    tex, asc, ext, area);
+      // A mixed round reads each path's scene through its job: the scene
+      // argument is unused, so it stays out of the graph key.
+      eng.run_batch(d_jobs, segs, P, nslots, max_b, red, multi ? DevScene{} : scenes[jobs[took[0].first].scene]->sc,
+                    multi, rc, nullptr, tex, asc, ext, area);
       if (rc) return rc;
       if ((rc = eng.end_stats())) return rc;
       st.rounds++;
       st.paths += P;
+      st.graph_captures = (uint32_t)eng.graph_captures;
+      st.graph_replays = (uint32_t)eng.graph_replays;
+      if (!multi) st.single_scene_rounds++;
       double t = now_ms();
       for (auto& [id, S] : took) {
         Job& j = jobs[id];
@@ -1833,6 +1999,7 @@ struct oxr_srv {
         }
       }
     }
+    reap();
     st.busy_ms += now_ms() - t_start;
     if (n_active) *n_active = (uint32_t)active.size();
     if (n_done) *n_done = done;
@@ -2412,6 +2579,7 @@ int oxr_srv_create(int device, uint64_t mem_budget, oxr_srv** out) {
   *out = nullptr;
   oxr_srv* s = new oxr_srv();
   int rc = s->eng.init(device);
+  s->eng.fixed_shade_grid = true;
   if (!rc) rc = dalloc(&s->d_jobs, MAX_SEGS);
   if (rc) {
     delete s;
@@ -2436,15 +2604,54 @@ int oxr_srv_add_scene(oxr_srv* s, const oxr_scene_desc* scene, uint32_t* scene_i
   auto sb = std::make_unique<SceneBuf>();
   int rc = sb->upload(scene);
   if (rc) return rc;
-  s->scenes.push_back(std::move(sb));
-  *scene_id = (uint32_t)s->scenes.size() - 1;
+  return s->add(std::move(sb), scene_id);
+}
+
+struct oxr_prebuilt {
+  HostScene hs;
+};
+
+int oxr_scene_prebuild(const oxr_scene_desc* scene, uint32_t flags, oxr_prebuilt** out) {
+  if (!scene || !out) return fail(OXR_ERR_INVALID, "null argument");
+  *out = nullptr;
+  auto p = std::make_unique<oxr_prebuilt>();
+  int rc = prepare_scene(scene, (flags & OXR_BUILD_FAST) != 0, p->hs);
+  if (rc) return rc;
+  p->hs.own();
+  *out = p.release();
+  return OXR_OK;
+}
+
+int oxr_prebuilt_destroy(oxr_prebuilt* p) {
+  delete p;
+  return OXR_OK;
+}
+
+int oxr_srv_add_prebuilt(oxr_srv* s, const oxr_prebuilt* p, uint32_t* scene_id) {
+  if (!s || !p || !scene_id) return fail(OXR_ERR_INVALID, "null argument");
+  HIPC(hipSetDevice(s->eng.device));
+  auto sb = std::make_unique<SceneBuf>();
+  int rc = sb->upload_host(p->hs);
+  if (rc) return rc;
+  return s->add(std::move(sb), scene_id);
+}
+
+int oxr_srv_remove_scene(oxr_srv* s, uint32_t scene_id) {
+  if (!s) return fail(OXR_ERR_INVALID, "null server");
+  if (scene_id >= s->scenes.size() || !s->scenes[scene_id] || s->doomed[scene_id])
+    return fail(OXR_ERR_INVALID, "unknown scene %u", scene_id);
+  s->doomed[scene_id] = 1;
+  HIPC(hipSetDevice(s->eng.device));
+  HIPC(hipStreamSynchronize(s->eng.stream));
+  s->reap();
   return OXR_OK;
 }
 
 int oxr_srv_submit(oxr_srv* s, uint32_t scene_id, const oxr_camera* cam, const oxr_pt_options* opts,
                    uint32_t* job_id) {
   if (!s || !cam || !opts || !job_id) return fail(OXR_ERR_INVALID, "null argument");
-  if (scene_id >= s->scenes.size()) return fail(OXR_ERR_INVALID, "unknown scene %u", scene_id);
+  if (scene_id >= s->scenes.size() || !s->scenes[scene_id] || s->doomed[scene_id])
+    return fail(OXR_ERR_INVALID, "unknown scene %u", scene_id);
   if (opts->samples_per_pixel == 0) return fail(OXR_ERR_INVALID, "samples_per_pixel must be >= 1");
   uint64_t npx = (uint64_t)std::max(opts->width, 1u) * std::max(opts->height, 1u);
   if (npx > s->eng.path_budget) return fail(OXR_ERR_INVALID, "job larger than one round (%llu pixels)",
@@ -2513,9 +2720,9 @@ int oxr_srv_stats_get(oxr_srv* s, oxr_srv_stats* out) {
   *out = s->st;
   out->active = (uint32_t)s->active.size();
   out->queued = (uint32_t)s->queue.size();
-  uint64_t b = s->mem_used + s->eng.path_bytes();
-  for (auto& sc : s->scenes) b += sc->bytes;
-  out->bytes_resident = b;
+  out->bytes_resident = s->mem_used + s->eng.path_bytes() + s->scene_bytes;
+  out->scenes = 0;
+  for (auto& sc : s->scenes) out->scenes += sc ? 1u : 0u;
   return OXR_OK;
 }
 

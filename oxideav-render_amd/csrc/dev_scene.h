@@ -157,7 +157,7 @@ struct Ctrl {
   PadCounter qcount[2];
   PadCounter scount;
   PadCounter work[16];  // traversal work counters: closest [0, 8), any-hit [8, 16), one per queue segment
-  PadCounter64 trav[2];  // counting refines: node and triangle fetches of closest-hit rays
+  PadCounter64 trav[4];  // counting refines: node and triangle fetches of closest-hit [0, 1] and occlusion [2, 3] rays
   uint32_t error;
   unsigned long long rays_closest;
   unsigned long long rays_shadow;

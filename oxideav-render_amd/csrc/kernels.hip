@@ -257,8 +257,9 @@ struct Trav {
     for (int c = 0; c < 4; ++c) {
       if (tn[c] != __builtin_huge_valf() && cnt[c] > 0) {
         const float4* tp = reinterpret_cast<const float4*>(tris + chl[c]);
-        if (COUNT) ntri += cnt[c];
+        if (COUNT && !ANY) ntri += cnt[c];
         for (uint32_t k = 0; k < cnt[c]; ++k) {
+          if (COUNT && ANY) ++ntri;  // occlusion rays stop at the first accepted triangle
           float4 a = tp[3 * k], b = tp[3 * k + 1], e = tp[3 * k + 2];
           float t, b1, b2;
           bool front;
@@ -463,21 +464,23 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
 // Persistent occlusion kernel. PATH: record = (o, skip gid), (d, t_max),
 // sidx[i] = path * nslots + slot; writes vis[slot] = 2 when unoccluded.
 // Else: record = oxr_ray, writes occ[i] (1 = occluded). MULTI as k_closest.
-template <bool PATH, bool MULTI, bool ALPHA>
+// COUNT: as k_closest (nodes fetched, triangles tested up to the first
+// accepted one).
+template <bool PATH, bool MULTI, bool ALPHA, bool COUNT = false>
 __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevScene sc, const DevJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ pjob, const float4* __restrict__ rays,
                                                     const uint32_t* __restrict__ sidx, const uint4* __restrict__ meta,
                                                     uint32_t nslots, uint32_t fixed_count, const uint32_t* count_ptr,
                                                     uint32_t* work, uint8_t* __restrict__ out, uint32_t* spill,
                                                     uint32_t spill_cap, uint32_t* err, uint32_t nseg,
-                                                    const DevScene* asc) {
+                                                    const DevScene* asc, unsigned long long* counts) {
   __shared__ uint32_t s_stack[STACK * TRACE_TB];
   Stack st;
   st.lds = s_stack + threadIdx.x;
   st.spill = spill + (size_t)(blockIdx.x * TRACE_TB + threadIdx.x) * spill_cap;
   st.cap = spill_cap;
   const uint32_t count = count_ptr ? *count_ptr : fixed_count;
-  uint32_t ntri_unused = 0;
+  uint32_t nnode = 0, ntri = 0;  // COUNT: this lane's fetches
   const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
   auto setup = [&](Trav<true, ALPHA>& tr, uint32_t i) {
@@ -517,7 +520,8 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
     tr.fmask = PATH ? ~0u : TRI_DOUBLE_SIDED;
   };
   auto step = [&](Trav<true, ALPHA>& tr) {
-    return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri_unused) : tr.step(st, sc.nodes, sc.tris, ntri_unused);
+    if (COUNT) ++nnode;
+    return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri) : tr.template step<COUNT>(st, sc.nodes, sc.tris, ntri);
   };
   auto finish = [&](Trav<true, ALPHA>& tr, uint32_t i) {
     bool occ = tr.hit_any;
@@ -532,6 +536,16 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
     }
   };
   trace_loop<true, ALPHA>(count, work, nseg, st, setup, step, finish);
+  if (COUNT) {  // every lane of the wave leaves trace_loop together
+    for (int o = 32; o > 0; o >>= 1) {
+      nnode += __shfl_xor(nnode, o);
+      ntri += __shfl_xor(ntri, o);
+    }
+    if (lane_id() == 0u) {
+      atomicAdd(counts, (unsigned long long)nnode);
+      atomicAdd(counts + 32, (unsigned long long)ntri);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -1522,7 +1536,7 @@ __global__ void k_zero_ctrl(Ctrl* c) {
     c->error = 0;
     c->rays_closest = 0;
     c->rays_shadow = 0;
-    c->trav[0].v = c->trav[1].v = 0;
+    c->trav[0].v = c->trav[1].v = c->trav[2].v = c->trav[3].v = 0;
   }
   if (threadIdx.x < 16) c->work[threadIdx.x].v = 0;
 }
@@ -1796,13 +1810,16 @@ void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, co
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots,
             uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill,
-            uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc) {
+            uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc, unsigned long long* counts) {
   if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
   const bool alpha = asc != nullptr;
 #define OXR_ANYHIT(P, M, A)                                                                                      \
   hipLaunchKernelGGL((k_anyhit<P, M, A>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta,      \
-                     nslots, fixed_count, count_ptr, work, out, spill, spill_cap, err, nseg, asc)
-  if (!path && alpha)
+                     nslots, fixed_count, count_ptr, work, out, spill, spill_cap, err, nseg, asc, nullptr)
+  if (counts && path && !multi && !alpha)
+    hipLaunchKernelGGL((k_anyhit<true, false, false, true>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays,
+                       sidx, meta, nslots, fixed_count, count_ptr, work, out, spill, spill_cap, err, nseg, asc, counts);
+  else if (!path && alpha)
     OXR_ANYHIT(false, false, true);
   else if (!path)
     OXR_ANYHIT(false, false, false);

@@ -51,7 +51,7 @@ void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, co
 void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, const uint32_t* pjob, bool path,
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots, uint32_t fixed_count,
             const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill, uint32_t spill_cap,
-            uint32_t* err, uint32_t nseg = 1, const DevScene* asc = nullptr);
+            uint32_t* err, uint32_t nseg = 1, const DevScene* asc = nullptr, unsigned long long* counts = nullptr);
 void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl);
 void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev);
 // guide: null for the plain estimator.

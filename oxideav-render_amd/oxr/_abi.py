@@ -274,6 +274,8 @@ class PtStats(C.Structure):
         ("graph_captures", C.c_uint32),
         ("closest_nodes", C.c_uint64),
         ("closest_tris", C.c_uint64),
+        ("anyhit_nodes", C.c_uint64),
+        ("anyhit_tris", C.c_uint64),
     ]
 
 
@@ -303,8 +305,12 @@ class JobStatus(C.Structure):
 class SrvStats(C.Structure):
     """oxr_srv_stats."""
     _fields_ = [("rounds", C.c_uint64), ("paths", C.c_uint64), ("jobs_done", C.c_uint64), ("active", C.c_uint32),
-                ("queued", C.c_uint32), ("bytes_resident", C.c_uint64), ("busy_ms", C.c_double)]
+                ("queued", C.c_uint32), ("bytes_resident", C.c_uint64), ("busy_ms", C.c_double),
+                ("graph_captures", C.c_uint32), ("graph_replays", C.c_uint32), ("single_scene_rounds", C.c_uint64),
+                ("scenes", C.c_uint32), ("pad", C.c_uint32)]
 
+
+OXR_BUILD_FAST = 1
 
 OXR_JOB_QUEUED = 0
 OXR_JOB_ACTIVE = 1

@@ -61,6 +61,10 @@ def lib():
         "oxr_trace_any": (C.c_int, [vp, u32, C.POINTER(A.Ray), bp]),
         "oxr_trace_closest_device": (C.c_int, [vp, u32, vp, vp, vp]),
         "oxr_trace_device_check": (C.c_int, [vp]),
+        "oxr_scene_prebuild": (C.c_int, [vp, u32, vp]),
+        "oxr_prebuilt_destroy": (C.c_int, [vp]),
+        "oxr_srv_add_prebuilt": (C.c_int, [vp, vp, vp]),
+        "oxr_srv_remove_scene": (C.c_int, [vp, u32]),
         "oxr_trace_closest_device_noalloc": (C.c_int, [vp, u32, vp, vp, vp]),
         "oxr_pt_bvh_info": (C.c_int, [vp, C.POINTER(A.BvhInfo)]),
         "oxr_pt_bvh_export": (C.c_int, [vp, vp, vp]),
@@ -107,6 +111,7 @@ EXPORTED = [
     "oxr_pt_bvh_export", "oxr_bsdf_eval", "oxr_bsdf_sample",
     "oxr_srv_create", "oxr_srv_destroy", "oxr_srv_add_scene", "oxr_srv_submit", "oxr_srv_step",
     "oxr_srv_job_status", "oxr_srv_read_accum", "oxr_srv_release", "oxr_srv_stats_get",
+    "oxr_scene_prebuild", "oxr_prebuilt_destroy", "oxr_srv_add_prebuilt", "oxr_srv_remove_scene",
     "oxr_guide_default_config", "oxr_guide_create", "oxr_guide_destroy", "oxr_guide_num_params",
     "oxr_guide_get_params", "oxr_guide_set_params", "oxr_guide_infer", "oxr_guide_pdf", "oxr_guide_sample",
     "oxr_guide_gradient", "oxr_guide_train", "oxr_guide_stats_get", "oxr_guide_collected", "oxr_pt_set_guide",

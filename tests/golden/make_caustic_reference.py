@@ -47,7 +47,7 @@ def main():
         k = min(a.pass_spp, a.spp - done)
         pt.refine(k)
         done += k
-        if done % (16 * a.pass_spp) == 0 or done == a.spp:
+        if done % (128 * a.pass_spp) == 0 or done == a.spp:
             print(f"{done} / {a.spp} spp, {time.time() - t:.1f} s", flush=True)
     acc, sq = pt.accum(with_sum_sq=True)
     pt.close()

@@ -346,6 +346,32 @@ def test_traversal_counts_match_the_oracle_restatement(oxr_mod, oracle, bounces)
     assert abs(st["closest_tris"] - c[:, 1].sum()) <= 0.01 * c[:, 1].sum()
 
 
+def test_occlusion_counts_match_the_oracle_restatement(oxr_mod, oracle):
+    """The counting any-hit variant (the occlusion roofline's node and
+    triangle fetches) against the oracle's restatement of the device's
+    occlusion order (oxrcpu_bvh4_count_any: leaves first, inner children
+    near to far, stop at the first triangle hit) over the oracle's own
+    occlusion rays of the same samples: ray counts and fetches within 1%."""
+    s = scenes.material_sweep(segments=32, rings=16, floor_half=6)
+    W, spp, bounces = 48, 4, 3
+    pt = oxr_mod.PathTracer(0)
+    opts = oxr_mod.RenderOptions(width=W, height=W, ambient=s.ambient,
+                                 path_trace=oxr_mod.PathTraceOptions(samples_per_pixel=spp, max_bounces=bounces))
+    pt.sync(s, opts)
+    pt.set_timing(False, count=True)
+    pt.refine(spp)
+    st = pt.stats()
+    nodes, tris = pt.bvh_export()
+    pt.close()
+    rays = oracle.OracleScene(s).record_shadow_rays(W, W, 0, spp, 0, W, max_rays=1 << 21,
+                                                    opts=oracle.options(W, W, max_bounces=bounces, ambient=s.ambient))
+    c = oracle.bvh4_count_any(nodes, tris, rays).astype(np.int64)
+    assert abs(st["shadow_rays"] - len(rays)) <= 0.001 * len(rays)
+    assert st["anyhit_nodes"] > 0 and st["anyhit_tris"] > 0
+    assert abs(st["anyhit_nodes"] - c[:, 0].sum()) <= 0.01 * c[:, 0].sum()
+    assert abs(st["anyhit_tris"] - c[:, 1].sum()) <= 0.01 * c[:, 1].sum()
+
+
 def test_refinement_is_additive_and_deterministic(oxr_mod):
     s = scenes.area_light_floor(0.5, 0.0)
     pt, a, _ = _render_gpu(oxr_mod, s, 9, 8, 4)
