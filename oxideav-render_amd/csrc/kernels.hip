@@ -907,6 +907,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
   uint32_t pid_n = i0 < count ? qpid[i0] : 0u;
   uint4 h_n = i0 < count ? hits[i0] : make_uint4(0, HIT_MISS, 0, 0);
   float4 rd_n = i0 < count ? qray[2 * i0 + 1] : make_float4(0, 0, 1, 0);
+  // A thread past the queue's end shades nothing but still reads a job's
+  // scene (the material loads below are unconditional). It takes the job
+  // of the queue's first path, which is rendering, not a job-table slot
+  // that may hold a finished job whose scene the server has freed.
+  const uint32_t pid_any = (MULTI && count > 0u) ? qpid[0] : 0u;
   for (uint32_t r = 0; r < rounds; ++r) {
     uint32_t i = i0 + r * stride;
     bool valid = i < count;
@@ -923,7 +928,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     }
     bool alive = valid;
     uint32_t staged = 0, nstaged = 0;  // entries this wave staged this round (wave-uniform)
-    const DevJob& job = jobs[MULTI ? (valid ? pjob[pid] : 0u) : 0u];
+    const DevJob& job = jobs[MULTI ? pjob[valid ? pid : pid_any] : 0u];
     const DevScene& sc = job.sc;
     const DevFrame& fr = job.fr;
     uint4 meta = valid ? meta_s[pid] : make_uint4(0, 0, 0, 0);

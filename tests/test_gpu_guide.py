@@ -140,7 +140,13 @@ def test_adam_step_matches_oracle():
 
 def test_training_learns_a_direction_field():
     """Synthetic target: at x, radiance arrives from the direction of a
-    fixed point; the trained mixture concentrates on it."""
+    fixed point, within a cone about 0.05 rad wide; the trained mixture
+    concentrates on it. The bar: at the source direction the guide's
+    density beats that of a uniform spread over a 30-degree cone,
+    1 / (2 pi (1 - cos 30 deg)) = 1.19 sr^-1 (15x the uniform sphere's
+    1 / (4 pi)): the target is six times narrower than that cone, so a
+    mixture that has learned where the radiance comes from clears it,
+    while one that spreads over a hemisphere (0.16 sr^-1) does not."""
     from oxr.guide import Guide
     cfg = small_cfg(lr=1e-2, seed=5)
     g = Guide(0, config=cfg)
@@ -163,7 +169,8 @@ def test_training_learns_a_direction_field():
     after = g.pdf(test, unit(src - test))
     assert l1 < l0 - 2.0
     assert np.median(after) > 20.0 * np.median(before)
-    assert np.median(after) > 1.0  # a uniform sphere is 1 / (4 pi) = 0.08
+    cone30 = 1.0 / (2.0 * math.pi * (1.0 - math.cos(math.radians(30.0))))
+    assert np.median(after) > cone30, (np.median(after), cone30)
 
 
 # ---------------------------------------------------------------------------
@@ -215,11 +222,15 @@ def test_guided_render_converges_to_plain():
     assert g.stats()["steps"] == 64 and g.stats()["collected"] > 0
     mp, vp = _mean_sigma(plain_acc, plain_sq)
     mg, vg = _mean_sigma(guided_acc, guided_sq)
-    z = (mg - mp) / np.sqrt(vp + vg + 1e-30)
-    # Unbiased: image means agree, per-pixel errors look Gaussian.
+    # Unbiased: image means agree, and so do the means of 8x8-pixel tiles
+    # (z-scores of tile means: 64 pixels each, closer to normal than single
+    # pixels of a caustic, whose samples are Bernoulli-like).
     tot = (mg.mean() - mp.mean()) / math.sqrt((vp.sum() + vg.sum()) / mp.size ** 2)
     assert abs(tot) < 4.0, tot
-    assert np.mean(np.abs(z) > 4.0) < 0.005
+    tile = lambda a: a.reshape(H // 8, 8, W // 8, 8, 3).sum((1, 3))  # noqa: E731
+    zt = (tile(mg) - tile(mp)) / np.sqrt(tile(vp) + tile(vg) + 1e-30)
+    assert np.abs(zt).max() < 5.0, np.abs(zt).max()
+    assert np.mean(np.abs(zt) > 3.0) < 0.05
 
 
 def _pass_variance(pt, passes, spp):

@@ -82,8 +82,15 @@ def test_mix_equals_alone(scenes, monkeypatch):
 
 def test_small_budget_queues_and_admits(scenes):
     from oxr.serve import Server
-    # Room for about two 64x64 accumulators at a time (7 doubles / pixel).
-    srv = Server(0, mem_budget=2 * 64 * 64 * 7 * 8 + 1024)
+    # The budget covers the resident scenes, the wavefront buffers and the
+    # accumulators: room for the scenes and about two 64x64 accumulators
+    # (7 doubles / pixel) before the first round allocates wavefronts.
+    probe = Server(0)
+    for s in scenes:
+        probe.add_scene(s)
+    base = probe.stats()["bytes_resident"]
+    probe.close()
+    srv = Server(0, mem_budget=base + 2 * 64 * 64 * 7 * 8 + 1024)
     ids = [srv.add_scene(s) for s in scenes]
     mix = [(0, 64, 64, 8, 8), (1, 64, 64, 8, 4), (2, 64, 64, 8, 6), (0, 64, 64, 6, 3)]
     jobs = []
@@ -125,4 +132,77 @@ def test_request_loop_reports_latency(scenes):
     assert out["completed"] == 24
     assert out["p50_ms"] <= out["p99_ms"]
     assert out["server"]["jobs_done"] == 24
+    srv.close()
+
+
+def test_prebuilt_fast_scene_job_equals_refine_and_is_freed(scenes):
+    """A scene prepared off the owner thread with the linear (fast) BVH and
+    uploaded with add_prebuilt renders the same bits as PathTracer.refine of
+    the same scene with the SAH BVH (hits do not depend on the tree); once
+    its job is released, remove_scene frees it and the id is refused."""
+    import oxr
+    from oxr import scenes as S
+    from oxr.serve import Prebuilt, Server
+    soup = S.triangle_soup(11, 30000, n_emit=3)
+    srv = Server(0)
+    pb = Prebuilt(soup, fast=True)
+    sid = srv.add_prebuilt(pb)
+    pb.close()
+    assert srv.stats()["scenes"] == 1
+    w, h, spp, b = 96, 80, 8, 6
+    job = srv.submit(sid, soup.camera.resolve(w, h), _opts(w, h, spp, b, soup.ambient))
+    for _ in range(100):
+        srv.step()
+        if srv.status(job).state == 2:
+            break
+    acc, sq = srv.accum(job, with_sum_sq=True)
+    ref, rsq = _alone(soup, _opts(w, h, spp, b, soup.ambient))
+    assert np.array_equal(acc, ref)
+    assert np.array_equal(sq, rsq)
+    srv.release(job)
+    srv.remove_scene(sid)
+    assert srv.stats()["scenes"] == 0
+    with pytest.raises(oxr.InvalidOptions):
+        srv.submit(sid, soup.camera.resolve(w, h), _opts(w, h, spp, b, soup.ambient))
+    srv.close()
+
+
+def test_rounds_after_a_removed_scene_keep_rendering(scenes):
+    """A finished job's slot in the device job table keeps its (stale)
+    entry until another job takes the slot. Here the job in slot 0 renders
+    a large prebuilt soup, finishes, and its scene is freed while two jobs
+    of another scene (slots 1 and 2) keep rendering in shared rounds; the
+    shading threads past each queue's end must not read the freed scene
+    through slot 0. Both jobs finish and equal refine alone."""
+    from oxr import scenes as S
+    from oxr.serve import Prebuilt, Server
+    srv = Server(0)
+    sid_a = srv.add_scene(scenes[0])
+    soup = S.triangle_soup(21, 1_500_000, n_emit=2)
+    pb = Prebuilt(soup, fast=True)
+    sid_s = srv.add_prebuilt(pb)
+    pb.close()
+    j0 = srv.submit(sid_s, soup.camera.resolve(64, 64), _opts(64, 64, 2, 4, soup.ambient))
+    srv.step()  # j0 takes slot 0
+    mix = [(160, 96, 48, 6), (128, 128, 40, 8)]
+    jobs = [srv.submit(sid_a, scenes[0].camera.resolve(w, h), _opts(w, h, spp, b, scenes[0].ambient))
+            for w, h, spp, b in mix]
+    for _ in range(200):
+        srv.step()
+        if srv.status(j0).state == 2:
+            break
+    assert srv.status(j0).state == 2
+    srv.release(j0)
+    srv.remove_scene(sid_s)
+    assert srv.stats()["scenes"] == 1
+    for _ in range(2000):
+        srv.step()
+        if all(srv.status(j).state == 2 for j in jobs):
+            break
+    for j, (w, h, spp, b) in zip(jobs, mix):
+        assert srv.status(j).state == 2
+        acc = srv.accum(j)
+        ref, _ = _alone(scenes[0], _opts(w, h, spp, b, scenes[0].ambient))
+        assert np.array_equal(acc, ref), (w, h, spp, b)
+        srv.release(j)
     srv.close()
