@@ -3142,6 +3142,10 @@ uint32_t oxrcpu_record_shadow_rays(oxrcpu_scene* h, const oxr_camera* cam, const
 // any: the occlusion order instead: the ray (o, skip id bits, d, t_max)
 // ends at the first triangle hit in range (the skip id excluded), and the
 // triangles counted are those tested up to it.
+// Direction components below 2^-64 in magnitude (zero included) take the
+// reciprocal +-2^64, as on the device: an infinite one turns the node's
+// slab offset into NaN and the axis then culls nothing.
+static float slab_rcp(float d) { return std::fabs(d) < 0x1p-64f ? std::copysign(0x1p64f, d) : 1.0f / d; }
 static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, const float* rays, uint32_t* counts,
                       uint32_t* depth, bool any = false) {
   for (uint32_t r = 0; r < n_rays; ++r) {
@@ -3152,7 +3156,7 @@ static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, co
     std::memcpy(&skip, &ry[3], 4);
     bool done_any = false;
     PRay pr = prepare_ray(o, d);
-    f3 inv = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    f3 inv = {slab_rcp(d[0]), slab_rcp(d[1]), slab_rcp(d[2])};
     f3 oi = {o[0] * inv[0], o[1] * inv[1], o[2] * inv[2]};
     std::vector<uint32_t> stack;
     uint32_t node = 0, nn = 0, nt = 0, dmax = 0;

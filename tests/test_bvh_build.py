@@ -72,15 +72,28 @@ int main(int argc, char** argv) {
       }
     }
   }
+  // The device's slab test on the decoded boxes never culls a child that
+  // an exact test (double precision, axis-parallel components handled as
+  // "origin inside the slab") keeps, over random rays: a quarter of them
+  // axis-parallel with one or two zero components, some with the origin on
+  // a vertex's plane. The device takes the reciprocal +-2^64 for components
+  // below 2^-64 (slab_rcp).
   srand(7);
   const size_t nn = b.nodes.size() < 4000 ? b.nodes.size() : 4000;
-  for (int r = 0; r < 300; ++r) {
+  for (int r = 0; r < 600; ++r) {
     float o[3], d[3], inv[3], oi[3];
     for (int k = 0; k < 3; ++k) {
       o[k] = bmin[k] + (bmax[k] - bmin[k]) * (1.4f * rand() / (float)RAND_MAX - 0.2f);
       d[k] = 2.0f * rand() / (float)RAND_MAX - 1.0f;
-      if (r % 10 == 0 && k == r % 3) d[k] = 0.0f;  // axis-parallel rays too
-      inv[k] = 1.0f / d[k];
+    }
+    if (r % 4 == 0) {
+      const int k0 = (r / 4) % 3;
+      d[k0] = (r % 12 == 0) ? -0.0f : 0.0f;
+      if (r % 8 == 0) d[(k0 + 1) % 3] = 0.0f;
+      if (r % 16 == 0) o[k0] = pos[9 * (rand() % n) + k0];
+    }
+    for (int k = 0; k < 3; ++k) {
+      inv[k] = std::fabs(d[k]) < 0x1p-64f ? std::copysign(0x1p64f, d[k]) : 1.0f / d[k];
       oi[k] = o[k] * inv[k];
     }
     for (size_t i = 0; i < nn; ++i) {
@@ -96,17 +109,24 @@ int main(int argc, char** argv) {
       }
       for (int c = 0; c < 4; ++c) {
         if (nd.child[c] == oxr::NODE_EMPTY) continue;
-        float n0 = 0.0f, f0 = 1e30f, n1 = 0.0f, f1 = 1e30f;
+        double n0 = 0.0, f0 = 1e30;
+        bool inside = true;
+        float n1 = 0.0f, f1 = 1e30f;
         for (int k = 0; k < 3; ++k) {
-          float a = std::fma(lo[k][c], inv[k], -oi[k]), bb = std::fma(hi[k][c], inv[k], -oi[k]);
+          if (d[k] == 0.0f) {
+            if (o[k] < lo[k][c] || o[k] > hi[k][c]) inside = false;
+          } else {
+            double a = ((double)lo[k][c] - o[k]) / d[k], bb = ((double)hi[k][c] - o[k]) / d[k];
+            n0 = std::fmax(n0, std::fmin(a, bb));
+            f0 = std::fmin(f0, std::fmax(a, bb));
+          }
           float qa = std::fma((float)((wl[k] >> (8 * c)) & 0xffu), sk[k], ok[k]);
           float qb = std::fma((float)((wh[k] >> (8 * c)) & 0xffu), sk[k], ok[k]);
           bool neg = std::signbit(inv[k]);
-          n0 = std::fmax(n0, neg ? bb : a); f0 = std::fmin(f0, neg ? a : bb);
           n1 = std::fmax(n1, neg ? qb : qa); f1 = std::fmin(f1, neg ? qa : qb);
         }
         const float eps = 1.0f + 1.0f / 65536.0f;
-        if (n0 <= f0 * eps && !(n1 <= f1 * eps)) ++culled;
+        if (inside && n0 <= f0 && !(n1 <= f1 * eps)) ++culled;
       }
     }
   }
@@ -210,3 +230,82 @@ def test_linear_bvh_every_triangle_in_one_leaf(harness, n):
     assert qbad == 0 and culled == 0
     assert np.array_equal(np.sort(gids), np.arange(n, dtype=np.uint32))
     assert depth < 40
+
+
+DUMP = r'''
+#include "bvh.h"
+#include <cstdio>
+#include <vector>
+int main(int argc, char** argv) {
+  FILE* f = fopen(argv[1], "rb");
+  fseek(f, 0, SEEK_END); long sz = ftell(f); fseek(f, 0, SEEK_SET);
+  std::vector<float> pos(sz / 4);
+  if (fread(pos.data(), 1, sz, f) != (size_t)sz) return 2;
+  fclose(f);
+  uint32_t n = pos.size() / 9;
+  std::vector<uint32_t> flags(n, 0); std::vector<float> alpha(n, 1.0f);
+  oxr::Bvh4 b = oxr::build_bvh4(pos.data(), n, flags.data(), alpha.data(), 4, 8, false);
+  std::vector<oxr::Node4Q> q = oxr::quantize_nodes(b.nodes);
+  FILE* o = fopen(argv[2], "wb"); fwrite(q.data(), sizeof(oxr::Node4Q), q.size(), o); fclose(o);
+  o = fopen(argv[3], "wb"); fwrite(b.tris.data(), sizeof(oxr::TriRec), b.tris.size(), o); fclose(o);
+  return 0;
+}
+'''
+
+
+def test_axis_parallel_rays_walk_as_few_nodes_as_their_neighbours(tmp_path, oracle):
+    """Rays with an exactly zero direction component (flat ground, mirror
+    bounces) on the device tree of a 2 x 192^2-triangle heightfield, walked
+    in the device's closest-hit and occlusion orders (oxrcpu_bvh4_count /
+    _any, whose slab reciprocal is the device's): the same nodes and
+    triangles as the rays with that component set to +-1e-9 (a direction
+    change of 1e-9 radians), and a small mean. An infinite reciprocal makes
+    the node's slab offset NaN and the axis cull nothing: such a ray then
+    fetches every node its other two slabs overlap (hundreds to thousands
+    here)."""
+    src = tmp_path / "d.cpp"
+    src.write_text(DUMP)
+    exe = tmp_path / "d"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{CSRC}", str(src), os.path.join(CSRC, "bvh.cpp"),
+                    "-o", str(exe)], check=True)
+    import oxr.scenes as S
+    g = 192
+    xs = np.linspace(0.0, 1.0, g + 1, dtype=np.float32)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    Y = (0.1 * S.fbm(X, Z, 3)).astype(np.float32)
+    P = np.stack([X, Y, Z], -1)
+    a, b_, c, d = P[:-1, :-1], P[:-1, 1:], P[1:, :-1], P[1:, 1:]
+    pos = np.concatenate([np.stack([a, c, b_], 2).reshape(-1, 9), np.stack([b_, c, d], 2).reshape(-1, 9)])
+    pos = np.ascontiguousarray(pos, np.float32)
+    pf, nf, tf = tmp_path / "p.f32", tmp_path / "n.bin", tmp_path / "t.bin"
+    pos.tofile(pf)
+    subprocess.run([str(exe), str(pf), str(nf), str(tf)], check=True, timeout=120)
+    nodes = np.fromfile(nf, np.float32)
+    tris = np.fromfile(tf, np.float32)
+    rng = np.random.default_rng(5)
+    m = 3000
+    o = np.stack([rng.uniform(0.05, 0.95, m), rng.uniform(0.0, 0.3, m), rng.uniform(0.05, 0.95, m)], 1)
+    k0 = np.arange(m) % 3
+    d = rng.normal(size=(m, 3))
+    d[np.arange(m), k0] = 0.0
+    sel = np.arange(0, m, 7)
+    d[sel, (k0[sel] + 1) % 3] = 0.0  # two zero components
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((m, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    near = rays.copy()
+    near[np.arange(m), 4 + k0] = np.where(np.arange(m) % 2 == 0, 1e-9, -1e-9)
+    near[sel, 4 + (k0[sel] + 1) % 3] = 1e-9
+    shadow = rays.copy()
+    shadow[:, 3] = np.frombuffer(np.uint32(0xFFFFFFFF).tobytes(), np.float32)[0]
+    shadow[:, 7] = 0.5
+    shadow_near = near.copy()
+    shadow_near[:, 3] = shadow[:, 3]
+    shadow_near[:, 7] = 0.5
+    for walk, r0, r1 in ((oracle.bvh4_count, rays, near), (oracle.bvh4_count_any, shadow, shadow_near)):
+        c0 = walk(nodes, tris, r0).astype(np.int64)
+        c1 = walk(nodes, tris, r1).astype(np.int64)
+        assert np.abs(c0 - c1).max() <= 4, np.abs(c0 - c1).max()
+        assert c0[:, 0].mean() < 80 and c0[:, 0].max() < 400, (c0[:, 0].mean(), c0[:, 0].max())

@@ -316,14 +316,22 @@ def test_shading_without_extension_lobes_is_bit_identical(oxr_mod, make):
     assert np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("bounces", [0, 3])
-def test_traversal_counts_match_the_oracle_restatement(oxr_mod, oracle, bounces):
+@pytest.mark.parametrize("bounces,camera", [(0, "perspective"), (3, "perspective"), (0, "axis_ortho"),
+                                            (3, "axis_ortho")])
+def test_traversal_counts_match_the_oracle_restatement(oxr_mod, oracle, bounces, camera):
     """The counting closest-hit variant (the roofline's node and triangle
     fetches) against the oracle's restatement of the device traversal
     (oxrcpu_bvh4_count) over the exported BVH and the oracle's own path rays
     of the same samples: ray counts equal; fetches within 1% (ties in
-    child distance may order children differently)."""
+    child distance may order children differently). axis_ortho: an
+    orthographic camera looking down -z, so every camera ray has two zero
+    direction components (and mirror bounces off axis-aligned faces keep
+    them): the slab reciprocal of such components (+-2^64 on both sides)
+    decides whether those axes cull; an infinite one fetches several times
+    the nodes."""
     s = scenes.material_sweep(segments=32, rings=16, floor_half=6)
+    if camera == "axis_ortho":
+        s.camera = scenes.CameraSpec((0.0, 1.0, 9.0), (0.0, 1.0, 0.0), projection=A.OXR_PROJ_ORTHOGRAPHIC, ymag=3.0)
     W, spp = 48, 4
     pt = oxr_mod.PathTracer(0)
     opts = oxr_mod.RenderOptions(width=W, height=W, ambient=s.ambient,
