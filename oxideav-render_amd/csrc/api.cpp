@@ -801,12 +801,15 @@ int SceneBuf::upload_host(const HostScene& hs) {
   sc.has_alpha_tex = hs.has_alpha_tex ? 1u : 0u;
   // Any factor that is not exactly zero (NaN included) selects the
   // extension-lobe shading; a texture only scales a factor.
-  bool has_ext = false;
-  for (const DevMaterial& m : hs.mats)
+  // Iridescence alone selects the full variant (2); the other extension
+  // lobes the variant without the thin-film Fresnel (1).
+  bool has_ext = false, has_irid = false;
+  for (const DevMaterial& m : hs.mats) {
     has_ext = has_ext || !(m.transmission == 0.0f) || !(m.clearcoat == 0.0f) || !(m.sheen[0] == 0.0f) ||
-              !(m.sheen[1] == 0.0f) || !(m.sheen[2] == 0.0f) || !(m.iridescence == 0.0f) || !(m.dt == 0.0f) ||
-              !(m.aniso == 0.0f);
-  sc.has_ext = has_ext ? 1u : 0u;
+              !(m.sheen[1] == 0.0f) || !(m.sheen[2] == 0.0f) || !(m.dt == 0.0f) || !(m.aniso == 0.0f);
+    has_irid = has_irid || !(m.iridescence == 0.0f);
+  }
+  sc.has_ext = has_irid ? 2u : (has_ext ? 1u : 0u);
   sc.tex.chains = d_chains;
   sc.tex.levels = d_levels;
   sc.tex.texels = d_texels;
@@ -1234,7 +1237,7 @@ struct Engine {
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);
     if (gb) std::memcpy(&key.gb, gb, sizeof(GuideBatch));
     key.alpha_sc = alpha_sc;
-    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u) | (ext_variant ? 4u : 0u) | (area_variant ? 8u : 0u);
+    key.tex = (tex_variant ? 1u : 0u) | (multi_jobs ? 2u : 0u) | ((uint32_t)ext_variant << 2) | (area_variant ? 16u : 0u);
     hipGraphExec_t gexec = nullptr;
     for (size_t q = 0; q < graphs_cached.size(); ++q)
       if (std::memcmp(&key, &graphs_cached[q].first, sizeof(key)) == 0) {
@@ -1283,14 +1286,14 @@ struct Engine {
   // the textured-alpha traversal (whose single-scene launches read the
   // scene's device copy through alpha_sc).
   bool tex_variant = false;
-  bool ext_variant = true;
+  int ext_variant = 2;  // 0: no extension lobe; 1: no iridescence; 2: every lobe
   bool area_variant = true;  // emissive triangles or an environment map in the batch
   const DevScene* alpha_sc = nullptr;
   bool multi_jobs = true;  // the batch holds paths of more than one job
 
   void run_batch(const DevJob* d_jobs, const SegTable& segs, uint32_t P, uint32_t nslots, uint32_t max_bounces,
                  const std::vector<Reduce>& red, const DevScene& sc_any, bool multi, int& rc,
-                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr, bool ext = true,
+                 const GuideBatch* gb = nullptr, bool tex = false, const DevScene* asc = nullptr, int ext = 2,
                  bool area = true) {
     tex_variant = tex;
     ext_variant = ext;
@@ -1751,7 +1754,7 @@ int oxr_pt::refine(uint32_t n) {
       rc = OXR_OK;
       if (gb && gb->use.rec) glaunch::fill(eng.stream, reinterpret_cast<float*>(guide->d_rec), rec_floats);
       eng.run_batch(d_job, segs, P, nslots, job.fr.max_bounces, red, scene.sc, false, rc, gb, scene.sc.has_tex != 0,
-                    scene.sc.has_alpha_tex ? scene.d_self : nullptr, scene.sc.has_ext != 0,
+                    scene.sc.has_alpha_tex ? scene.d_self : nullptr, (int)scene.sc.has_ext,
                     scene.sc.num_emissive > 0 || job.fr.env.img != nullptr);
       if (rc) return rc;
     }
@@ -1963,13 +1966,14 @@ struct oxr_srv {
       if ((rc = eng.ensure_spill(spill))) return rc;
       eng.begin_stats();
       rc = OXR_OK;
-      bool multi = false, tex = false, ext = false, area = false;
+      bool multi = false, tex = false, area = false;
+      int ext = 0;
       const DevScene* asc = nullptr;  // a mixed batch reads each path's scene through its job
       for (auto& [id, S_] : took) {
         multi = multi || jobs[id].scene != jobs[took[0].first].scene;
         const SceneBuf& sb = *scenes[jobs[id].scene];
         tex = tex || sb.sc.has_tex;
-        ext = ext || sb.sc.has_ext;
+        ext = std::max(ext, (int)sb.sc.has_ext);
         area = area || sb.sc.num_emissive > 0;  // server jobs have no environment map
         if (sb.sc.has_alpha_tex && !asc) asc = sb.d_self;
       }

@@ -264,13 +264,19 @@ struct Microfacet {
   }
 };
 
-// EXT = false: the scene has no material with clearcoat, sheen,
-// iridescence, transmission, diffuse transmission or anisotropy. Those
-// factors are then compile-time zeros: every branch they guard takes the
-// way a zero factor takes at run time, and their lobe probabilities, which
-// a zero factor makes exactly 0, are constants, so the code and the state
-// they need drop out and the results stay bit for bit those of EXT = true.
-template <bool EXT>
+// EXT = 0: the scene has no material with clearcoat, sheen, iridescence,
+// transmission, diffuse transmission or anisotropy. Those factors are then
+// compile-time zeros: every branch they guard takes the way a zero factor
+// takes at run time, and their lobe probabilities, which a zero factor
+// makes exactly 0, are constants, so the code and the state they need drop
+// out and the results stay bit for bit those of EXT = 2.
+// EXT = 1: the same for iridescence alone (every material's iridescence
+// factor is 0). The thin-film Fresnel is the heaviest part of the BSDF:
+// with it inlined at every Fresnel evaluation the shading kernel of a
+// scene with other extension lobes and emissive triangles spilled about
+// 400 bytes a lane at two waves per SIMD, against about 160 without it.
+// EXT = 2: every lobe.
+template <int EXT>
 struct BsdfT {
   V3 v, n, ng, ncc, base;
   float metallic;
@@ -342,9 +348,9 @@ struct BsdfT {
       sheen = inside ? v3(0.0f, 0.0f, 0.0f) : m.sheen_color;
       sheen_rough = m.sheen_roughness;
       sheen_alpha = fmaxr(m.sheen_roughness * m.sheen_roughness, 1.0e-3f);
-      irid = inside ? 0.0f : clampr(m.iridescence, 0.0f, 1.0f);
-      irid_ior = isfinite(m.iridescence_ior) ? fmaxr(m.iridescence_ior, 1.0f) : 1.3f;
-      irid_thick = isfinite(m.iridescence_thickness) ? fmaxr(m.iridescence_thickness, 0.0f) : 400.0f;
+      irid = (EXT < 2 || inside) ? 0.0f : clampr(m.iridescence, 0.0f, 1.0f);
+      irid_ior = EXT < 2 ? 0.0f : (isfinite(m.iridescence_ior) ? fmaxr(m.iridescence_ior, 1.0f) : 1.3f);
+      irid_thick = EXT < 2 ? 0.0f : (isfinite(m.iridescence_thickness) ? fmaxr(m.iridescence_thickness, 0.0f) : 400.0f);
       dt = clampr(m.diffuse_transmission, 0.0f, 1.0f);
       dt_color = v3(clampr(m.diffuse_transmission_color.x, 0.0f, 1.0f),
                     clampr(m.diffuse_transmission_color.y, 0.0f, 1.0f),
@@ -405,7 +411,7 @@ struct BsdfT {
   OXR_D void fresnel_pair(float c, V3& fd_out, V3& fm_out) const {
     V3 fd = scale(fresnel_d(c), specular);
     V3 fm = schlick(base, c);
-    if (!EXT || irid <= 0.0f) {
+    if (EXT < 2 || irid <= 0.0f) {
       fd_out = fd;
       fm_out = fm;
       return;
@@ -586,7 +592,7 @@ struct BsdfT {
   }
 };
 
-using Bsdf = BsdfT<true>;
+using Bsdf = BsdfT<2>;
 
 // ---- spherical triangles (pathtrace.rs:1749-1871), f64 inside ----------
 struct D3 {

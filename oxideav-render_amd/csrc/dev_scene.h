@@ -69,7 +69,7 @@ struct DevScene {
   uint32_t has_blend;
   uint32_t has_tex;        // textures, vertex colours or tangents: the TEX shading variant
   uint32_t has_alpha_tex;  // MASK / BLEND alpha varies over triangles: the ALPHA traversal variant
-  uint32_t has_ext;        // a material has an extension lobe: the EXT shading variant
+  uint32_t has_ext;        // shading variant: 0 no extension lobe, 1 extension lobes but no iridescence, 2 iridescence
   DevTextures tex;
 };
 

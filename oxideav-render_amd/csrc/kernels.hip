@@ -881,14 +881,15 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // EXT = false: no material of the batch has an extension lobe (clearcoat,
 // sheen, iridescence, transmission, diffuse transmission, anisotropy), so
 // no path ever enters a medium: the medium state stays zero and is neither
-// read nor written, and the BSDF drops those lobes (BsdfT<false>).
+// read nor written, and the BSDF drops those lobes (BsdfT<0>). EXT = 1:
+// no material of the batch has iridescence (BsdfT<1>); EXT = 2: all lobes.
 // With at most STAGE_SLOTS shadow slots per path (punctual lights + the
 // area sample) a block stages its round's shadow rays and next rays in LDS
 // and appends each queue with one atomic per round (block_flush); one
 // atomic per wave and light made the counter the kernel's bottleneck.
 constexpr uint32_t STAGE_SLOTS = 4;
 
-template <int WAVES, bool GUIDE, bool TEX, bool MULTI, bool EXT, bool AREA>
+template <int WAVES, bool GUIDE, bool TEX, bool MULTI, int EXT, bool AREA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
                                              const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
@@ -1059,6 +1060,89 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
       }
     }
     if (alive && k >= fr.max_bounces) alive = false;
+    // The area set's NEE sample (src/pathtrace.rs:2173-2279) is drawn here,
+    // before the material and the BSDF exist: it needs only the hit point,
+    // and its f64 spherical-triangle sampling then does not hold registers
+    // beside the BSDF's lobes (which spilled 400 bytes a lane in the
+    // extension + area variant). The BSDF evaluation, MIS weight and shadow
+    // ray follow once the BSDF is built. Every operation is the one the
+    // combined order runs, on the same operands, so results are unchanged.
+    bool a_have = false;
+    V3 a_l = v3(0, 0, 0), a_le = v3(0, 0, 0);
+    float a_pl = 0.0f, a_tmax = 0.0f;
+    uint32_t a_skip = NO_SKIP;
+    if (AREA && alive && fr.strategy != 2u && (fr.env.img != nullptr || sc.num_emissive > 0)) {
+      const float pe = fr.p_env;
+      const F4 pl_u = pattern(ps, index, 2 + 2 * k);
+      if (pl_u.v[2] < pe) {
+        float epdf;
+        if (env_sample(fr.env, pl_u.v[0], pl_u.v[1], a_l, a_le, epdf)) {
+          a_pl = pe * epdf;
+          a_tmax = __builtin_huge_valf();
+          a_have = !is_black(a_le) && a_pl > 0.0f;
+        }
+      } else if (sc.num_emissive > 0) {
+        float us = pe > 0.0f ? clampr((pl_u.v[2] - pe) / (1.0f - pe), 0.0f, 0.999999f) : pl_u.v[2];
+        // EmissiveLights::pick: partition_point(c <= u).min(len - 1).
+        uint32_t lo = 0, hi = sc.num_emissive;
+        while (lo < hi) {
+          uint32_t mid2 = (lo + hi) >> 1;
+          if (sc.l_cdf[mid2] <= us)
+            lo = mid2 + 1;
+          else
+            hi = mid2;
+        }
+        uint32_t li = lo < sc.num_emissive - 1 ? lo : sc.num_emissive - 1;
+        uint32_t lg = sc.l_tris[li];
+        const float* tv = sc.pos + 9ull * lg;
+        V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
+        bool have = false;
+        V3 q;
+        if (triangle_solid_angle(p, a, b, c) >= 1.0e-4f) {
+          V3 ls;
+          if (sample_spherical_triangle(p, a, b, c, pl_u.v[0], pl_u.v[1], ls)) {
+            V3 n = cross(sub(b, a), sub(c, a));
+            float dn = dot(ls, n);
+            if (dn != 0.0f) {
+              float tt = dot(sub(a, p), n) / dn;
+              if (tt > 0.0f && isfinite(tt)) {
+                q = add(p, scale(ls, tt));
+                have = true;
+              }
+            }
+          }
+        } else {
+          float su = sqrtf(pl_u.v[0]);
+          float w0 = 1.0f - su, w1 = su * (1.0f - pl_u.v[1]), w2 = su * pl_u.v[1];
+          q = v3(a.x * w0 + b.x * w1 + c.x * w2, a.y * w0 + b.y * w1 + c.y * w2, a.z * w0 + b.z * w1 + c.z * w2);
+          have = true;
+        }
+        V3 lbary;
+        if (have && barycentric_of(q, a, b, c, lbary)) {
+          V3 to = sub(q, p);
+          float dist2 = dot(to, to);
+          V3 cr = cross(sub(b, a), sub(c, a));
+          float len = sqrtf(dot(cr, cr));
+          V3 lng = len > 0.0f ? v3(cr.x / len, cr.y / len, cr.z / len) : v3(0.0f, 0.0f, 1.0f);
+          const uint32_t lmid = sc.tri_info[lg] & TI_MAT_MASK;
+          const DevMaterial& LM = sc.mats[lmid];
+          if (dist2 > 1.0e-12f) {
+            float dist = sqrtf(dist2);
+            a_l = scale(to, 1.0f / dist);
+            float cos_l = -dot(lng, a_l);
+            a_pl = tri_pdf(sc, fr.p_env, li, lg, p, q);
+            bool lds = (LM.flags & MAT_DS) != 0;
+            if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && a_pl > 0.0f && isfinite(a_pl)) {
+              a_le = (LM.flags & MAT_UNLIT) ? v3(0, 0, 0) : ld3(LM.emissive);
+              if (TEX && !is_black(a_le)) a_le = mul(a_le, tex_emission_factor(sc.tex, lmid, lg, lbary));
+              a_tmax = fmaxr(dist * (1.0f - 1.0e-4f), 0.0f);
+              a_skip = lg;
+              a_have = !is_black(a_le);
+            }
+          }
+        }
+      }
+    }
     MatSample ms;
     load_mat(*M, sn, ms);
     if (TEX && alive) {
@@ -1167,102 +1251,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         any_shadow = true;
       }
     }
-    // NEE: one sample of the area set — the environment map with
-    // probability P_env, else an emissive triangle (src/pathtrace.rs:2173-2279).
+    // NEE: the area set's sample drawn above, now with the BSDF
+    // (src/pathtrace.rs:2173-2279).
     {
       bool emit = false;
-      V3 o2, l, contrib;
-      float tmax = 0.0f;
-      uint32_t skip = NO_SKIP;
-      F4 pl_u;
-      const float pe = fr.p_env;
-      const bool area = AREA && alive && fr.strategy != 2u && (fr.env.img != nullptr || sc.num_emissive > 0);
-      if (area) pl_u = pattern(ps, index, 2 + 2 * k);
-      if (area && pl_u.v[2] < pe) {
-        V3 le;
-        float epdf;
-        if (env_sample(fr.env, pl_u.v[0], pl_u.v[1], l, le, epdf)) {
-          float pl = pe * epdf;
-          V3 f;
-          float pb;
-          bsdf.eval(l, f, pb);
-          if (!is_black(f) && !is_black(le) && pl > 0.0f) {
-            if (GUIDE && gmix > 0.0f) pb = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pb;
-            float w = fr.strategy == 0u ? power_heuristic(pl, pb) : 1.0f;
-            o2 = spawn(p, ng, l);
-            tmax = __builtin_huge_valf();
-            contrib = clampc(scale(mul(mul(beta, f), le), w / pl), fr.clamp);
-            emit = true;
-          }
-        }
-      } else if (area && sc.num_emissive > 0) {
-        float us = pe > 0.0f ? clampr((pl_u.v[2] - pe) / (1.0f - pe), 0.0f, 0.999999f) : pl_u.v[2];
-        // EmissiveLights::pick: partition_point(c <= u).min(len - 1).
-        uint32_t lo = 0, hi = sc.num_emissive;
-        while (lo < hi) {
-          uint32_t mid2 = (lo + hi) >> 1;
-          if (sc.l_cdf[mid2] <= us)
-            lo = mid2 + 1;
-          else
-            hi = mid2;
-        }
-        uint32_t li = lo < sc.num_emissive - 1 ? lo : sc.num_emissive - 1;
-        uint32_t lg = sc.l_tris[li];
-        const float* tv = sc.pos + 9ull * lg;
-        V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
-        bool have = false;
-        V3 q;
-        if (triangle_solid_angle(p, a, b, c) >= 1.0e-4f) {
-          V3 ls;
-          if (sample_spherical_triangle(p, a, b, c, pl_u.v[0], pl_u.v[1], ls)) {
-            V3 n = cross(sub(b, a), sub(c, a));
-            float dn = dot(ls, n);
-            if (dn != 0.0f) {
-              float tt = dot(sub(a, p), n) / dn;
-              if (tt > 0.0f && isfinite(tt)) {
-                q = add(p, scale(ls, tt));
-                have = true;
-              }
-            }
-          }
-        } else {
-          float su = sqrtf(pl_u.v[0]);
-          float w0 = 1.0f - su, w1 = su * (1.0f - pl_u.v[1]), w2 = su * pl_u.v[1];
-          q = v3(a.x * w0 + b.x * w1 + c.x * w2, a.y * w0 + b.y * w1 + c.y * w2, a.z * w0 + b.z * w1 + c.z * w2);
-          have = true;
-        }
-        V3 bary;
-        if (have && barycentric_of(q, a, b, c, bary)) {
-          V3 to = sub(q, p);
-          float dist2 = dot(to, to);
-          V3 cr = cross(sub(b, a), sub(c, a));
-          float len = sqrtf(dot(cr, cr));
-          V3 lng = len > 0.0f ? v3(cr.x / len, cr.y / len, cr.z / len) : v3(0.0f, 0.0f, 1.0f);
-          const DevMaterial& LM = sc.mats[sc.tri_info[lg] & TI_MAT_MASK];
-          if (dist2 > 1.0e-12f) {
-            float dist = sqrtf(dist2);
-            l = scale(to, 1.0f / dist);
-            float cos_l = -dot(lng, l);
-            float pl = tri_pdf(sc, fr.p_env, li, lg, p, q);
-            bool lds = (LM.flags & MAT_DS) != 0;
-            if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && pl > 0.0f && isfinite(pl)) {
-              V3 f;
-              float pb;
-              bsdf.eval(l, f, pb);
-              V3 le = (LM.flags & MAT_UNLIT) ? v3(0, 0, 0) : ld3(LM.emissive);
-              if (TEX && !is_black(le))
-                le = mul(le, tex_emission_factor(sc.tex, sc.tri_info[lg] & TI_MAT_MASK, lg, bary));
-              if (!is_black(f) && !is_black(le)) {
-                if (GUIDE && gmix > 0.0f) pb = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pb;
-                float w = fr.strategy == 0u ? power_heuristic(pl, pb) : 1.0f;
-                o2 = spawn(p, ng, l);
-                tmax = fmaxr(dist * (1.0f - 1.0e-4f), 0.0f);
-                contrib = clampc(scale(mul(mul(beta, f), le), w / pl), fr.clamp);
-                skip = lg;
-                emit = true;
-              }
-            }
-          }
+      V3 o2, contrib;
+      const V3 l = a_l;
+      const float tmax = a_tmax;
+      const uint32_t skip = a_skip;
+      if (AREA && alive && a_have) {
+        V3 f;
+        float pb;
+        bsdf.eval(l, f, pb);
+        if (!is_black(f)) {
+          if (GUIDE && gmix > 0.0f) pb = gmix * guide_pdf(GL, l) + (1.0f - gmix) * pb;
+          float w = fr.strategy == 0u ? power_heuristic(a_pl, pb) : 1.0f;
+          o2 = spawn(p, ng, l);
+          contrib = clampc(scale(mul(mul(beta, f), a_le), w / a_pl), fr.clamp);
+          emit = true;
         }
       }
       const float4 ra = make_float4(o2.x, o2.y, o2.z, __uint_as_float(skip));
@@ -1862,7 +1868,7 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 }
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, bool ext, bool area) {
+           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, int ext, bool area) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
 #define OXR_SHADE(W, G, T, M, X, A)                                                                              \
   hipLaunchKernelGGL((k_shade<W, G, T, M, X, A>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur], \
@@ -1872,31 +1878,35 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
   // wave per SIMD; the occupancy variants are for plain single-scene
   // shading. Plain and mixed batches have variants without the extension
   // lobes and without area lights.
-#define OXR_SHADE_XA(W, M)                 \
-  do {                                     \
-    if (ext && area)                       \
-      OXR_SHADE(W, false, false, M, true, true);  \
-    else if (ext)                          \
-      OXR_SHADE(W, false, false, M, true, false); \
-    else if (area)                         \
-      OXR_SHADE(W, false, false, M, false, true); \
-    else                                   \
-      OXR_SHADE(W, false, false, M, false, false); \
+#define OXR_SHADE_XA(W, M)                           \
+  do {                                               \
+    if (ext >= 2 && area)                            \
+      OXR_SHADE(W, false, false, M, 2, true);        \
+    else if (ext >= 2)                               \
+      OXR_SHADE(W, false, false, M, 2, false);       \
+    else if (ext == 1 && area)                       \
+      OXR_SHADE(W, false, false, M, 1, true);        \
+    else if (ext == 1)                               \
+      OXR_SHADE(W, false, false, M, 1, false);       \
+    else if (area)                                   \
+      OXR_SHADE(W, false, false, M, 0, true);        \
+    else                                             \
+      OXR_SHADE(W, false, false, M, 0, false);       \
   } while (0)
   if (multi && tex)
-    OXR_SHADE(1, false, true, true, true, true);
+    OXR_SHADE(1, false, true, true, 2, true);
   else if (multi)
     OXR_SHADE_XA(1, true);
   else if (guide && tex)
-    OXR_SHADE(1, true, true, false, true, true);
+    OXR_SHADE(1, true, true, false, 2, true);
   else if (guide && ext)
-    OXR_SHADE(1, true, false, false, true, true);
+    OXR_SHADE(1, true, false, false, 2, true);
   else if (guide)
-    OXR_SHADE(1, true, false, false, false, true);
+    OXR_SHADE(1, true, false, false, 0, true);
   else if (tex && ext)
-    OXR_SHADE(1, false, true, false, true, true);
+    OXR_SHADE(1, false, true, false, 2, true);
   else if (tex)
-    OXR_SHADE(1, false, true, false, false, true);
+    OXR_SHADE(1, false, true, false, 0, true);
   else if (waves >= 3)
     OXR_SHADE_XA(3, false);
   else if (waves == 2)

@@ -69,7 +69,7 @@ void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t
 // area: a scene of the batch has emissive triangles, or a job has an
 // environment map.
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex = false, bool multi = true, bool ext = true,
+           uint32_t cur, const DevGuideUse* guide, bool tex = false, bool multi = true, int ext = 2,
            bool area = true);
 // Training samples (2 float4 each) of the guide-recorded paths of a batch,
 // appended at *count (entries past cap are dropped).

@@ -316,6 +316,23 @@ def test_shading_without_extension_lobes_is_bit_identical(oxr_mod, make):
     assert np.array_equal(sa, sb)
 
 
+@pytest.mark.parametrize("make", [scenes.glossy_caustic, lambda: scenes.terrain(grid=48, n_emit=200, n_spheres=300)],
+                         ids=["caustic", "terrain_small"])
+def test_shading_without_iridescence_is_bit_identical(oxr_mod, make):
+    """Scenes with extension lobes (transmission; clearcoat, sheen) but no
+    iridescent material shade with the variant that leaves the thin-film
+    Fresnel out; one unused iridescent material in the table selects the
+    variant with every lobe. Both accumulate the same bits."""
+    s = make()
+    _, a, sa = _render_gpu(oxr_mod, s, 40, 8, 8)
+    full = make()
+    full.add_material(A.material((1.0, 1.0, 1.0, 1.0), iridescence=1.0))
+    assert full.num_tris == s.num_tris
+    _, b, sb = _render_gpu(oxr_mod, full, 40, 8, 8)
+    assert np.array_equal(a, b)
+    assert np.array_equal(sa, sb)
+
+
 @pytest.mark.parametrize("bounces,camera", [(0, "perspective"), (3, "perspective"), (0, "axis_ortho"),
                                             (3, "axis_ortho")])
 def test_traversal_counts_match_the_oracle_restatement(oxr_mod, oracle, bounces, camera):
