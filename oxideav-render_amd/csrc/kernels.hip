@@ -688,12 +688,11 @@ OXR_D V3 clampc(V3 c, float cl) {
 }
 
 // Light-sampling pdf of emissive triangle g (Integrator::tri_pdf,
-// src/pathtrace.rs:1972-1991).
-OXR_D float tri_pdf(const DevScene& sc, float p_env, uint32_t li, uint32_t g, V3 p, V3 q) {
-  const float* tv = sc.pos + 9ull * g;
-  V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
+// src/pathtrace.rs:1972-1991), given its corners and its solid angle from
+// p (the area sample computes both before it picks a direction; tri_pdf
+// below computes them).
+OXR_D float tri_pdf_of(const DevScene& sc, float p_env, uint32_t li, V3 a, V3 b, V3 c, float omega, V3 p, V3 q) {
   float sel = (1.0f - p_env) * sc.l_pmf[li];
-  float omega = triangle_solid_angle(p, a, b, c);
   if (omega >= 1.0e-4f) return sel / omega;
   V3 cr = cross(sub(b, a), sub(c, a));
   float len = sqrtf(dot(cr, cr));
@@ -703,6 +702,11 @@ OXR_D float tri_pdf(const DevScene& sc, float p_env, uint32_t li, uint32_t g, V3
   float cos = fabsf(dot(cr, to) / (len * sqrtf(d2)));
   if (cos <= 0.0f) return 0.0f;
   return sel * d2 / (0.5f * len * cos);
+}
+OXR_D float tri_pdf(const DevScene& sc, float p_env, uint32_t li, uint32_t g, V3 p, V3 q) {
+  const float* tv = sc.pos + 9ull * g;
+  V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
+  return tri_pdf_of(sc, p_env, li, a, b, c, triangle_solid_angle(p, a, b, c), p, q);
 }
 
 // ---------------------------------------------------------------------
@@ -883,13 +887,16 @@ OXR_D V3 spawn(V3 p, V3 ng, V3 dir) { return dot(dir, ng) >= 0.0f ? offset_ray_o
 // no path ever enters a medium: the medium state stays zero and is neither
 // read nor written, and the BSDF drops those lobes (BsdfT<0>). EXT = 1:
 // no material of the batch has iridescence (BsdfT<1>); EXT = 2: all lobes.
+// PUNCT = false: the batch has no punctual light (nslots == 1), so the
+// punctual NEE loop, whose inlined BSDF evaluation held the most registers
+// of the extension + area variant, is left out.
 // With at most STAGE_SLOTS shadow slots per path (punctual lights + the
 // area sample) a block stages its round's shadow rays and next rays in LDS
 // and appends each queue with one atomic per round (block_flush); one
 // atomic per wave and light made the counter the kernel's bottleneck.
 constexpr uint32_t STAGE_SLOTS = 4;
 
-template <int WAVES, bool GUIDE, bool TEX, bool MULTI, int EXT, bool AREA>
+template <int WAVES, bool GUIDE, bool TEX, bool MULTI, int EXT, bool AREA, bool PUNCT = true>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_shade(const DevJob* __restrict__ jobs,
                                              const uint32_t* __restrict__ pjob, uint32_t nslots, const float4* __restrict__ qray,
                                              const uint32_t* __restrict__ qpid, const uint4* __restrict__ hits,
@@ -1098,7 +1105,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
         V3 a = ld3(tv), b = ld3(tv + 3), c = ld3(tv + 6);
         bool have = false;
         V3 q;
-        if (triangle_solid_angle(p, a, b, c) >= 1.0e-4f) {
+        // One solid angle serves the choice of sampler and the pdf below
+        // (tri_pdf recomputes the same value from the same operands).
+        const float omega = triangle_solid_angle(p, a, b, c);
+        if (omega >= 1.0e-4f) {
           V3 ls;
           if (sample_spherical_triangle(p, a, b, c, pl_u.v[0], pl_u.v[1], ls)) {
             V3 n = cross(sub(b, a), sub(c, a));
@@ -1130,7 +1140,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             float dist = sqrtf(dist2);
             a_l = scale(to, 1.0f / dist);
             float cos_l = -dot(lng, a_l);
-            a_pl = tri_pdf(sc, fr.p_env, li, lg, p, q);
+            a_pl = tri_pdf_of(sc, fr.p_env, li, a, b, c, omega, p, q);
             bool lds = (LM.flags & MAT_DS) != 0;
             if ((cos_l > 0.0f || (lds && cos_l < 0.0f)) && a_pl > 0.0f && isfinite(a_pl)) {
               a_le = (LM.flags & MAT_UNLIT) ? v3(0, 0, 0) : ld3(LM.emissive);
@@ -1189,7 +1199,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     bool any_shadow = false;
     // Uniform trip count (nslots - 1 covers every scene of the batch) so
     // every lane reaches the appends.
-    for (uint32_t j = 0; j + 1u < nslots; ++j) {
+    for (uint32_t j = 0; PUNCT && j + 1u < nslots; ++j) {
       bool emit = false;
       V3 o2, l, contrib;
       float tmax = 0.0f;
@@ -1870,28 +1880,39 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
            uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, int ext, bool area) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
-#define OXR_SHADE(W, G, T, M, X, A)                                                                              \
-  hipLaunchKernelGGL((k_shade<W, G, T, M, X, A>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur], \
+#define OXR_SHADE_P(W, G, T, M, X, A, P)                                                                         \
+  hipLaunchKernelGGL((k_shade<W, G, T, M, X, A, P>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur], \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
                      pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
+#define OXR_SHADE(W, G, T, M, X, A) OXR_SHADE_P(W, G, T, M, X, A, true)
   // Mixed batches (the server) and textured or guided scenes run at one
   // wave per SIMD; the occupancy variants are for plain single-scene
   // shading. Plain and mixed batches have variants without the extension
   // lobes and without area lights.
-#define OXR_SHADE_XA(W, M)                           \
-  do {                                               \
-    if (ext >= 2 && area)                            \
-      OXR_SHADE(W, false, false, M, 2, true);        \
-    else if (ext >= 2)                               \
-      OXR_SHADE(W, false, false, M, 2, false);       \
-    else if (ext == 1 && area)                       \
-      OXR_SHADE(W, false, false, M, 1, true);        \
-    else if (ext == 1)                               \
-      OXR_SHADE(W, false, false, M, 1, false);       \
-    else if (area)                                   \
-      OXR_SHADE(W, false, false, M, 0, true);        \
-    else                                             \
-      OXR_SHADE(W, false, false, M, 0, false);       \
+#define OXR_SHADE_XA(W, M)                                 \
+  do {                                                     \
+    if (ext >= 2 && area)                                  \
+      OXR_SHADE(W, false, false, M, 2, true);              \
+    else if (ext >= 2)                                     \
+      OXR_SHADE(W, false, false, M, 2, false);             \
+    else if (ext == 1 && area)                             \
+      OXR_SHADE(W, false, false, M, 1, true);              \
+    else if (ext == 1)                                     \
+      OXR_SHADE(W, false, false, M, 1, false);             \
+    else if (area)                                         \
+      OXR_SHADE(W, false, false, M, 0, true);              \
+    else                                                   \
+      OXR_SHADE(W, false, false, M, 0, false);             \
+  } while (0)
+  // Single-scene batches lit only by the area set.
+#define OXR_SHADE_AREA_ONLY(W)                             \
+  do {                                                     \
+    if (ext >= 2)                                          \
+      OXR_SHADE_P(W, false, false, false, 2, true, false); \
+    else if (ext == 1)                                     \
+      OXR_SHADE_P(W, false, false, false, 1, true, false); \
+    else                                                   \
+      OXR_SHADE_P(W, false, false, false, 0, true, false); \
   } while (0)
   if (multi && tex)
     OXR_SHADE(1, false, true, true, 2, true);
@@ -1907,14 +1928,22 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
     OXR_SHADE(1, false, true, false, 2, true);
   else if (tex)
     OXR_SHADE(1, false, true, false, 0, true);
+  else if (area && nslots == 1u && waves >= 3)
+    OXR_SHADE_AREA_ONLY(3);
+  else if (area && nslots == 1u && waves == 2)
+    OXR_SHADE_AREA_ONLY(2);
+  else if (area && nslots == 1u)
+    OXR_SHADE_AREA_ONLY(1);
   else if (waves >= 3)
     OXR_SHADE_XA(3, false);
   else if (waves == 2)
     OXR_SHADE_XA(2, false);
   else
     OXR_SHADE_XA(1, false);
+#undef OXR_SHADE_AREA_ONLY
 #undef OXR_SHADE_XA
 #undef OXR_SHADE
+#undef OXR_SHADE_P
 }
 
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
