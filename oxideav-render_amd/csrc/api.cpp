@@ -986,6 +986,10 @@ struct Engine {
   // of neighbouring rays (pixel tiles) on one XCD and its L2: +13% on the
   // material sweep, -2% on the 12.4M-triangle terrain.
   uint32_t xcd_segments = 8;
+  // Idle lanes at which a traversal wave takes new rays (1..64), for the
+  // closest-hit and the occlusion kernel: 52-56 and 48 measured fastest on
+  // the material sweep and the terrain (+3-5% over 64).
+  uint32_t refill_closest = 56, refill_any = 48;
   void* d_tmp = nullptr;
   size_t tmp_alloc = 0;
   bool timing = false;
@@ -1039,6 +1043,14 @@ struct Engine {
     }
     if (const char* gph = std::getenv("OXR_HIP_GRAPHS")) graphs = std::atoi(gph) != 0;
     if (const char* x = std::getenv("OXR_XCD")) xcd_segments = std::atoi(x) >= 8 ? 8u : 1u;
+    if (const char* r = std::getenv("OXR_REFILL_CLOSEST")) {
+      int v = std::atoi(r);
+      if (v >= 1 && v <= 64) refill_closest = (uint32_t)v;
+    }
+    if (const char* r = std::getenv("OXR_REFILL_ANY")) {
+      int v = std::atoi(r);
+      if (v >= 1 && v <= 64) refill_any = (uint32_t)v;
+    }
     if (const char* w = std::getenv("OXR_SHADE_WAVES")) {
       int v = std::atoi(w);
       if (v >= 1 && v <= 3) shade_waves = v;
@@ -1196,7 +1208,7 @@ struct Engine {
     launch::closest(stream, alpha_sc ? grid_closest_alpha : grid_closest, sc_any, d_jobs, pb.pjob, true, multi,
                     pb.qray[cur], pb.qpid[cur], pb.meta, 0,
                     &d_ctrl->qcount[cur].v, &d_ctrl->work[0].v, 0, pb.hits, d_spill, spill_cap, &d_ctrl->error,
-                    xcd_segments, alpha_sc, count_trav ? &d_ctrl->trav[0].v : nullptr);
+                    xcd_segments | (refill_closest << 8), alpha_sc, count_trav ? &d_ctrl->trav[0].v : nullptr);
     span_end(0, e1);
     if (gb && k < gb->use.kmax) {  // the guide serves the first kmax bounces
       hipEvent_t eg{};
@@ -1215,7 +1227,7 @@ struct Engine {
     launch::anyhit(stream, alpha_sc ? grid_any_alpha : grid_any, sc_any, d_jobs, pb.pjob, true, multi, pb.sray,
                    pb.sidx, pb.meta, nslots, 0,
                    &d_ctrl->scount.v, &d_ctrl->work[8].v, pb.vis, d_spill, spill_cap, &d_ctrl->error,
-                   xcd_segments, alpha_sc, count_trav ? &d_ctrl->trav[2].v : nullptr);
+                   xcd_segments | (refill_any << 8), alpha_sc, count_trav ? &d_ctrl->trav[2].v : nullptr);
     span_end(1, e3);
   }
 
@@ -1231,7 +1243,7 @@ struct Engine {
     key.gen = gen;
     key.grid_shade = grid_shade;
     key.shade_waves = shade_waves;
-    key.nseg = xcd_segments;
+    key.nseg = xcd_segments | (refill_closest << 8) | (refill_any << 16);
     key.jobs = d_jobs;
     key.sc = sc_any;
     key.guided = (gb != nullptr ? 1u : 0u) | (multi ? 2u : 0u);

@@ -326,24 +326,29 @@ struct Trav {
 
 // Persistent traversal loop shared by the closest-hit and occlusion
 // kernels. A wave refills from the shared counter (one atomic per refill)
-// once REFILL of its lanes are idle; the first rays of every wave are
+// once `refill` of its lanes are idle; the first rays of every wave are
 // assigned by wave index, so a launch whose rays fit in one round makes no
-// atomic at all.
-// A wave takes new rays only once all 64 lanes are idle: rays stay in
-// runs of 64 neighbours from the queue (coherent node and triangle
-// fetches). Refilling single idle lanes mixes far-apart rays into a wave
-// and measured 1.4x slower on the material sweep.
-constexpr uint32_t REFILL = 64;
+// atomic at all. The idle lanes take the next rays of the queue together,
+// so rays enter a wave in runs of queue neighbours (coherent node and
+// triangle fetches). Waiting for all 64 lanes holds a wave on its slowest
+// ray; refilling at a few idle lanes mixes far-apart rays (at 1 idle lane
+// 1.4x slower on the material sweep). The engine's defaults, 56 (closest
+// hit) and 48 (occlusion), measured 3-5% faster than 64 on the material
+// sweep and the terrain; 8-40 were slower.
 
 // nseg > 1: the queue is split into nseg contiguous segments and block b
 // works on segment b % nseg. Workgroups are placed on the XCDs round-robin,
 // so with nseg = 8 each XCD traces its own run of neighbouring rays and
 // keeps their BVH nodes in its own L2. `work` holds one counter per segment;
 // gridDim.x is a multiple of nseg.
+// segw = nseg | refill << 8: a wave takes new rays once `refill` of its
+// lanes are idle (64 when the field is 0).
 template <bool ANY, bool ALPHA, class Setup, class Step, class Finish>
-OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, Setup setup, Step step,
+OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t segw, Stack& st, Setup setup, Step step,
                       Finish finish) {
   const uint32_t lane = lane_id();
+  const uint32_t nseg = segw & 0xffu;
+  const uint32_t refill = (segw >> 8) ? (segw >> 8) : 64u;
   const uint32_t seg = blockIdx.x % nseg;
   const uint32_t nwaves = (gridDim.x / nseg) * (TRACE_TB / 64);
   const uint32_t wave = (blockIdx.x / nseg) * (TRACE_TB / 64) + threadIdx.x / 64;
@@ -369,7 +374,7 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t nseg, Stack& st, 
       if (nfree == 64u) break;
       continue;
     }
-    if (nfree >= REFILL) {
+    if (nfree >= refill) {
       int leader = __ffsll((unsigned long long)freem) - 1;
       uint32_t base = 0;
       if ((int)lane == leader) base = atomicAdd(wk, nfree) + nwaves * 64u + s0;
@@ -1815,7 +1820,12 @@ void closest(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, co
              bool multi, const float4* rays, const uint32_t* qpid, const uint4* meta, uint32_t fixed_count,
              const uint32_t* count_ptr, uint32_t* work, uint32_t cull_all, uint4* hits, uint32_t* spill,
              uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc, unsigned long long* counts) {
-  if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
+  {
+    uint32_t ns = nseg & 0xffu, rf = (nseg >> 8) & 0xffu;
+    if (ns < 1 || ns > 8 || grid % (int)ns != 0) ns = 1;
+    if (rf == 0 || rf > 64) rf = 64;
+    nseg = ns | (rf << 8);
+  }
   // asc (the scene in device memory) selects the textured-alpha variant;
   // a mixed batch finds each path's scene through its job instead.
   const bool alpha = asc != nullptr;
@@ -1845,7 +1855,12 @@ void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, con
             bool multi, const float4* rays, const uint32_t* sidx, const uint4* meta, uint32_t nslots,
             uint32_t fixed_count, const uint32_t* count_ptr, uint32_t* work, uint8_t* out, uint32_t* spill,
             uint32_t spill_cap, uint32_t* err, uint32_t nseg, const DevScene* asc, unsigned long long* counts) {
-  if (nseg < 1 || nseg > 8 || grid % (int)nseg != 0) nseg = 1;
+  {
+    uint32_t ns = nseg & 0xffu, rf = (nseg >> 8) & 0xffu;
+    if (ns < 1 || ns > 8 || grid % (int)ns != 0) ns = 1;
+    if (rf == 0 || rf > 64) rf = 64;
+    nseg = ns | (rf << 8);
+  }
   const bool alpha = asc != nullptr;
 #define OXR_ANYHIT(P, M, A)                                                                                      \
   hipLaunchKernelGGL((k_anyhit<P, M, A>), dim3(grid), dim3(TRACE_TB), 0, s, sc, jobs, pjob, rays, sidx, meta,      \

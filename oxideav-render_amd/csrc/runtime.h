@@ -37,7 +37,8 @@ int trace_block_threads();
 int stack_lds_entries();
 // work: nseg counters (closest: Ctrl::work, any-hit: Ctrl::work + 8);
 // nseg = 8 splits the queue into per-XCD segments (1 when the grid is not
-// a multiple of it).
+// a multiple of it). Bits 8-15 of nseg: the idle lanes at which a wave
+// takes new rays (0 = 64).
 // path: rays of the wavefront (else host rays of the parity entry points);
 // multi: the launch mixes scenes (each ray's scene from its job), else all
 // rays trace `sc`. asc non-null: the textured-alpha variant (MASK / BLEND
