@@ -1900,10 +1900,11 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
                      pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
 #define OXR_SHADE(W, G, T, M, X, A) OXR_SHADE_P(W, G, T, M, X, A, true)
-  // Mixed batches (the server) and textured or guided scenes run at one
-  // wave per SIMD; the occupancy variants are for plain single-scene
-  // shading. Plain and mixed batches have variants without the extension
-  // lobes and without area lights.
+  // Mixed batches (the server), textured scenes and guided scenes with
+  // punctual lights run at one wave per SIMD; the occupancy variants are
+  // for single-scene shading (guided: lit by the area set only). Plain and
+  // mixed batches have variants without the extension lobes (or without
+  // iridescence) and without area lights.
 #define OXR_SHADE_XA(W, M)                                 \
   do {                                                     \
     if (ext >= 2 && area)                                  \
@@ -1918,6 +1919,15 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
       OXR_SHADE(W, false, false, M, 0, true);              \
     else                                                   \
       OXR_SHADE(W, false, false, M, 0, false);             \
+  } while (0)
+  // Guided single-scene batches lit only by the area set, without
+  // iridescence (two waves per SIMD spill 28 bytes a lane at EXT = 1).
+#define OXR_SHADE_GUIDE_AREA_ONLY(W)                       \
+  do {                                                     \
+    if (ext == 1)                                          \
+      OXR_SHADE_P(W, true, false, false, 1, true, false);  \
+    else                                                   \
+      OXR_SHADE_P(W, true, false, false, 0, true, false);  \
   } while (0)
   // Single-scene batches lit only by the area set.
 #define OXR_SHADE_AREA_ONLY(W)                             \
@@ -1935,6 +1945,12 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
     OXR_SHADE_XA(1, true);
   else if (guide && tex)
     OXR_SHADE(1, true, true, false, 2, true);
+  else if (guide && area && nslots == 1u && ext >= 2)
+    OXR_SHADE_P(1, true, false, false, 2, true, false);
+  else if (guide && area && nslots == 1u && waves >= 2)
+    OXR_SHADE_GUIDE_AREA_ONLY(2);
+  else if (guide && area && nslots == 1u)
+    OXR_SHADE_GUIDE_AREA_ONLY(1);
   else if (guide && ext)
     OXR_SHADE(1, true, false, false, 2, true);
   else if (guide)
@@ -1956,6 +1972,7 @@ void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslo
   else
     OXR_SHADE_XA(1, false);
 #undef OXR_SHADE_AREA_ONLY
+#undef OXR_SHADE_GUIDE_AREA_ONLY
 #undef OXR_SHADE_XA
 #undef OXR_SHADE
 #undef OXR_SHADE_P
