@@ -266,39 +266,53 @@ struct Trav {
       bool hit = chl[c] != NODE_EMPTY && tnear <= tfar * SLAB_EPS;
       tn[c] = hit ? tnear : __builtin_huge_valf();
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (tn[c] != __builtin_huge_valf() && cnt[c] > 0) {
-        const float4* tp = reinterpret_cast<const float4*>(tris + chl[c]);
-        if (COUNT && !ANY) ntri += cnt[c];
-        for (uint32_t k = 0; k < cnt[c]; ++k) {
-          if (COUNT && ANY) ++ntri;  // occlusion rays stop at the first accepted triangle
-          float4 a = tp[3 * k], b = tp[3 * k + 1], e = tp[3 * k + 2];
-          float t, b1, b2;
-          bool front;
-          if (!intersect_tri(wr, f4xyz(a), f4xyz(b), f4xyz(e), tmin, tmax, t, b1, b2, front)) continue;
-          uint32_t gid = __float_as_uint(a.w);
-          uint32_t fl = __float_as_uint(b.w) & fmask;
-          float aw = e.w;
-          if (ANY) {
-            if (gid == skip) continue;
-            if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
-            if (!accept_tri(fl, aw, front, false, path_seed, ray_id, gid)) continue;
-            hit_any = true;
-            return true;
-          } else {
-            if (best != HIT_MISS && !(t < tmax || (t == tmax && gid < (best & 0x7fffffffu)))) continue;
-            if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
-            if (!accept_tri(fl, aw, front, cull, path_seed, ray_id, gid)) continue;
-            tmax = t;
-            best = gid | (front ? 0u : HIT_BACK);
-            bu = b1;
-            bv = b2;
-          }
-        }
-        tn[c] = __builtin_huge_valf();
+    // The triangles of the hit leaf children, as one run in child order:
+    // e0..e3 are the running ends. One loop over the run (rather than one
+    // per child) lets a wave test lanes whose leaves sit under different
+    // children together; each lane tests the same triangles in the same
+    // order as child by child.
+    uint32_t e0, e1, e2, e3;
+    {
+      const uint32_t n0 = (tn[0] != __builtin_huge_valf()) ? cnt[0] : 0u;
+      const uint32_t n1 = (tn[1] != __builtin_huge_valf()) ? cnt[1] : 0u;
+      const uint32_t n2 = (tn[2] != __builtin_huge_valf()) ? cnt[2] : 0u;
+      const uint32_t n3 = (tn[3] != __builtin_huge_valf()) ? cnt[3] : 0u;
+      e0 = n0;
+      e1 = e0 + n1;
+      e2 = e1 + n2;
+      e3 = e2 + n3;
+    }
+    if (COUNT && !ANY) ntri += e3;
+    for (uint32_t j = 0; j < e3; ++j) {
+      if (COUNT && ANY) ++ntri;  // occlusion rays stop at the first accepted triangle
+      const uint32_t ti = j < e0 ? chl[0] + j : (j < e1 ? chl[1] + (j - e0) : (j < e2 ? chl[2] + (j - e1) : chl[3] + (j - e2)));
+      const float4* tp = reinterpret_cast<const float4*>(tris + ti);
+      float4 a = tp[0], b = tp[1], e = tp[2];
+      float t, b1, b2;
+      bool front;
+      if (!intersect_tri(wr, f4xyz(a), f4xyz(b), f4xyz(e), tmin, tmax, t, b1, b2, front)) continue;
+      uint32_t gid = __float_as_uint(a.w);
+      uint32_t fl = __float_as_uint(b.w) & fmask;
+      float aw = e.w;
+      if (ANY) {
+        if (gid == skip) continue;
+        if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
+        if (!accept_tri(fl, aw, front, false, path_seed, ray_id, gid)) continue;
+        hit_any = true;
+        return true;
+      } else {
+        if (best != HIT_MISS && !(t < tmax || (t == tmax && gid < (best & 0x7fffffffu)))) continue;
+        if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
+        if (!accept_tri(fl, aw, front, cull, path_seed, ray_id, gid)) continue;
+        tmax = t;
+        best = gid | (front ? 0u : HIT_BACK);
+        bu = b1;
+        bv = b2;
       }
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (cnt[c] > 0) tn[c] = __builtin_huge_valf();  // leaves are never pushed
     // Inner children near to far (the nearest is popped first). Occlusion
     // rays too: the nearest blockers end them sooner (-17% any-hit time on
     // the terrain).
