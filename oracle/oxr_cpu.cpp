@@ -3138,7 +3138,8 @@ uint32_t oxrcpu_record_shadow_rays(oxrcpu_scene* h, const oxr_camera* cam, const
 // to far.
 // counts[2*i] = nodes fetched, counts[2*i+1] = triangles fetched;
 // depth[i] (when not null) = the most entries the ray's traversal stack
-// held at once (the device keeps 16 in LDS and spills the rest).
+// held at once (the device keeps 16 in LDS and spills the rest; the
+// nearest child to visit next is held in a register, not on the stack).
 // any: the occlusion order instead: the ray (o, skip id bits, d, t_max)
 // ends at the first triangle hit in range (the skip id excluded), and the
 // triangles counted are those tested up to it.
@@ -3220,9 +3221,12 @@ static void bvh4_walk(const float* nodes, const float* tris, uint32_t n_rays, co
             std::swap(tn[a], tn[b]);
             std::swap(ci[a], ci[b]);
           }
+      size_t before = stack.size();
       for (int c = 0; c < 4; ++c)
         if (tn[c] != INF_F && tn[c] <= tmax * (1.0f + 1.0f / 65536.0f)) stack.push_back(ci[c]);
-      dmax = std::max(dmax, (uint32_t)stack.size());
+      // The device visits the nearest child next from a register and
+      // stacks only the others.
+      dmax = std::max(dmax, (uint32_t)(stack.size() - (stack.size() > before ? 1 : 0)));
       if (stack.empty()) break;
       node = stack.back();
       stack.pop_back();
