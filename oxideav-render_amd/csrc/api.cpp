@@ -2148,7 +2148,6 @@ int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out) {
     if (b > e.max_budget) continue;
     for (int w : waves)
     for (uint32_t x : segs) {
-      if (w > 1 && h->guide) continue;  // the guided kernel has one variant
       e.path_budget = b;
       e.shade_waves = w;
       e.xcd_segments = x;

@@ -140,18 +140,27 @@ struct Stack {
   uint32_t sp, bottom, spn;
   bool overflow;
 
-  OXR_D void push(uint32_t x) {
-    if (sp == STACK) {
-      if (spn >= cap) {
-        overflow = true;
-        return;
-      }
-      spill[spn++] = lds[(bottom & (STACK - 1)) * TRACE_TB];
-      bottom = (bottom + 1) & (STACK - 1);
-      --sp;
+  // Pushes the last k (0..3) of c0, c1, c2 in that order (the children to
+  // visit sorted far to near are a suffix). Room is made first by moving
+  // bottom entries to the global spill, which leaves the stack as k single
+  // pushes would; no pushed entry can reach the bottom (k < STACK).
+  OXR_D void push_suffix(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t k) {
+    if (sp + k > STACK) {
+      do {
+        if (spn >= cap) {
+          overflow = true;
+          return;
+        }
+        spill[spn++] = lds[(bottom & (STACK - 1)) * TRACE_TB];
+        bottom = (bottom + 1) & (STACK - 1);
+        --sp;
+      } while (sp + k > STACK);
     }
-    lds[((bottom + sp) & (STACK - 1)) * TRACE_TB] = x;
-    ++sp;
+    const uint32_t e0 = k == 3u ? c0 : (k == 2u ? c1 : c2), e1 = k == 3u ? c1 : c2;
+    if (k >= 1u) lds[((bottom + sp) & (STACK - 1)) * TRACE_TB] = e0;
+    if (k >= 2u) lds[((bottom + sp + 1u) & (STACK - 1)) * TRACE_TB] = e1;
+    if (k == 3u) lds[((bottom + sp + 2u) & (STACK - 1)) * TRACE_TB] = c2;
+    sp += k;
   }
   OXR_D bool pop(uint32_t& x) {
     if (sp > 0) {
@@ -251,16 +260,19 @@ struct Trav {
     const float oz = fmaf(__uint_as_float(q0.z), inv.z, -oi.z);
     uint32_t chl[4] = {ch.x, ch.y, ch.z, ch.w};
     uint32_t cnt[4] = {q2.z & 0xffu, (q2.z >> 8) & 0xffu, (q2.z >> 16) & 0xffu, q2.z >> 24};
+    // Near and far byte planes of all four children, chosen once per axis
+    // by the ray's direction sign (a select on the packed words instead of
+    // one per child and plane).
+    const uint32_t pnx = negx ? q1.y : q1.x, pfx = negx ? q1.x : q1.y;
+    const uint32_t pny = negy ? q1.w : q1.z, pfy = negy ? q1.z : q1.w;
+    const uint32_t pnz = negz ? q2.y : q2.x, pfz = negz ? q2.x : q2.y;
     float tn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const uint32_t sh = 8u * c;
-      float ax = fmaf((float)((q1.x >> sh) & 0xffu), sx, ox), bx = fmaf((float)((q1.y >> sh) & 0xffu), sx, ox);
-      float ay = fmaf((float)((q1.z >> sh) & 0xffu), sy, oy), by = fmaf((float)((q1.w >> sh) & 0xffu), sy, oy);
-      float az = fmaf((float)((q2.x >> sh) & 0xffu), sz, oz), bz = fmaf((float)((q2.y >> sh) & 0xffu), sz, oz);
-      float nx = negx ? bx : ax, fx = negx ? ax : bx;
-      float ny = negy ? by : ay, fy = negy ? ay : by;
-      float nz = negz ? bz : az, fz = negz ? az : bz;
+      float nx = fmaf((float)((pnx >> sh) & 0xffu), sx, ox), fx = fmaf((float)((pfx >> sh) & 0xffu), sx, ox);
+      float ny = fmaf((float)((pny >> sh) & 0xffu), sy, oy), fy = fmaf((float)((pfy >> sh) & 0xffu), sy, oy);
+      float nz = fmaf((float)((pnz >> sh) & 0xffu), sz, oz), fz = fmaf((float)((pfz >> sh) & 0xffu), sz, oz);
       float tnear = fmaxf(fmaxf(fmaxf(tmin, nx), ny), nz);
       float tfar = fminf(fminf(fminf(tmax, fx), fy), fz);
       bool hit = chl[c] != NODE_EMPTY && tnear <= tfar * SLAB_EPS;
@@ -318,14 +330,17 @@ struct Trav {
     // the terrain).
     {
       uint32_t ci[4] = {chl[0], chl[1], chl[2], chl[3]};
-#define OXR_CSWAP(i, j)  \
-  if (tn[i] < tn[j]) {   \
-    float tt = tn[i];    \
-    tn[i] = tn[j];       \
-    tn[j] = tt;          \
-    uint32_t cc = ci[i]; \
-    ci[i] = ci[j];       \
-    ci[j] = cc;          \
+  // Compare-swap as selects (as a branch, each of the five cost an exec
+  // mask save and restore).
+#define OXR_CSWAP(i, j)                        \
+  {                                            \
+    const bool sw = tn[i] < tn[j];             \
+    const float ti = tn[i], tj = tn[j];        \
+    const uint32_t cii = ci[i], cij = ci[j];   \
+    tn[i] = sw ? tj : ti;                      \
+    tn[j] = sw ? ti : tj;                      \
+    ci[i] = sw ? cij : cii;                    \
+    ci[j] = sw ? cii : cij;                    \
   }
       OXR_CSWAP(0, 1) OXR_CSWAP(2, 3) OXR_CSWAP(0, 2) OXR_CSWAP(1, 3) OXR_CSWAP(1, 2)
 #undef OXR_CSWAP
@@ -334,11 +349,13 @@ struct Trav {
       // the others go on the stack. Pushing it and popping it back (the
       // same order) put an LDS store and load between a node's test and
       // the next node's fetch.
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        if (tn[c] != __builtin_huge_valf() && tn[c] <= tmax * SLAB_EPS) st.push(ci[c]);
+      const float lim = tmax * SLAB_EPS;
+      const uint32_t k = (tn[0] != __builtin_huge_valf() && tn[0] <= lim ? 1u : 0u) +
+                         (tn[1] != __builtin_huge_valf() && tn[1] <= lim ? 1u : 0u) +
+                         (tn[2] != __builtin_huge_valf() && tn[2] <= lim ? 1u : 0u);
+      st.push_suffix(ci[0], ci[1], ci[2], k);
       if (st.overflow) return true;
-      if (tn[3] != __builtin_huge_valf() && tn[3] <= tmax * SLAB_EPS) {
+      if (tn[3] != __builtin_huge_valf() && tn[3] <= lim) {
         node = ci[3];
         return false;
       }
