@@ -295,7 +295,11 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16, error_spp=1
     return out
 
 
-GUIDE_BENCH_CONFIG = {"max_steps": 160}
+# Online training budget of the guided leg: 80 Adam steps of 16384
+# samples. On the caustic window, 40-160 steps (16k-64k samples each) all
+# reach MSE 1e-3 in 86-92k spp, so the extra steps only add time; at 20
+# steps the guide is undertrained and needs 6x the samples of plain.
+GUIDE_BENCH_CONFIG = {"max_steps": 80, "batch": 16384}
 CAUSTIC_REFERENCE = os.path.join(ROOT, "tests", "golden", "caustic_window_reference.npz")
 
 
