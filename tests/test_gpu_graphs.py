@@ -90,3 +90,22 @@ def test_xcd_split_traversal_equals_whole_queue(monkeypatch):
         out.append(pt.accum(with_sum_sq=True))
         pt.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("refill", [("64", "64"), ("8", "8"), ("33", "17")])
+def test_traversal_refill_threshold_leaves_the_image_unchanged(monkeypatch, refill):
+    """The idle-lane count at which a traversal wave takes new rays changes
+    only which rays share a wave: the image (accumulator and sums of
+    squares) is bit for bit the default's, on a scene with area lights and
+    extension lobes and on the material sweep."""
+    from oxr import scenes as S
+    for sc in (S.material_sweep(segments=16, rings=8, floor_half=6), S.glossy_caustic(segments=24, rings=16)):
+        out = []
+        for rc, ra in (("56", "48"), refill):
+            monkeypatch.setenv("OXR_REFILL_CLOSEST", rc)
+            monkeypatch.setenv("OXR_REFILL_ANY", ra)
+            pt = _tracer(sc, 160, 120, 8, bounces=6)
+            pt.refine(8)
+            out.append(pt.accum(with_sum_sq=True))
+            pt.close()
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
