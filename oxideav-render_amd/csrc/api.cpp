@@ -1220,7 +1220,7 @@ struct Engine {
     hipEvent_t e2{};
     span_begin(e2);
     launch::shade(stream, grid_shade, shade_waves, d_jobs, nslots, pb, d_ctrl, cur, gb ? &gb->use : nullptr,
-                  tex_variant, multi_jobs, ext_variant, area_variant);
+                  tex_variant, multi_jobs, ext_variant, area_variant, k);
     span_end(2, e2);
     hipEvent_t e3{};
     span_begin(e3);

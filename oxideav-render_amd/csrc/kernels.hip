@@ -666,9 +666,12 @@ OXR_D void path_sample(const Seg& sg, uint32_t width, uint32_t lq, uint32_t& s, 
   }
 }
 
+// Throughput, radiance, previous position and medium of a camera path are
+// not written here: the bounce-0 shading launch starts every path from
+// beta = 1, zero radiance and no medium without reading them, and writes
+// all of them at its end.
 __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_t P, float4* __restrict__ qray,
-                         uint32_t* __restrict__ qpid, float4* __restrict__ beta, float4* __restrict__ rad,
-                         float4* __restrict__ sigma, uint4* __restrict__ meta, uint32_t* __restrict__ pjob, Ctrl* ctrl) {
+                         uint32_t* __restrict__ qpid, uint4* __restrict__ meta, uint32_t* __restrict__ pjob, Ctrl* ctrl) {
   const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid == 0) ctrl->qcount[0].v = P;
   if (pid >= P) return;
@@ -696,9 +699,6 @@ __global__ void k_camera(const DevJob* __restrict__ jobs, SegTable segs, uint32_
   qray[2 * pid] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
   qray[2 * pid + 1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
   qpid[pid] = pid;
-  beta[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  sigma[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   meta[pid] = make_uint4(x | (y << 16), index, 0xffu << 16, ps);
   pjob[pid] = sg.job;
 }
@@ -951,7 +951,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                                              float4* __restrict__ prev_s, float4* __restrict__ sigma_s,
                                              uint4* __restrict__ meta_s, float4* __restrict__ nee,
                                              uint8_t* __restrict__ vis, Ctrl* ctrl, uint32_t cur,
-                                             DevGuideUse gu) {
+                                             DevGuideUse gu, uint32_t kb) {
   __shared__ float4 s_sa[STAGE_SLOTS * 256], s_sb[STAGE_SLOTS * 256], s_na[256], s_nb[256];
   __shared__ uint32_t s_si[STAGE_SLOTS * 256], s_np[256], s_cnt[8], s_base[8];
   // Uniform over the grid: every thread reaches each block_flush.
@@ -1002,12 +1002,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     uint32_t hero = (meta.z >> 16) & 0xffu;
     uint32_t flags = meta.z >> 24;
     uint32_t ps = meta.w, index = meta.y;
-    float4 b4 = valid ? beta_s[pid] : make_float4(0, 0, 0, 0);
+    // kb: the launch's bounce (every queue entry of a launch has it). At
+    // bounce 0 the path starts from beta = 1 and nothing else, unread.
+    const bool later = valid && kb > 0u;
+    float4 b4 = later ? beta_s[pid] : (valid ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : make_float4(0, 0, 0, 0));
     V3 beta = f4xyz(b4);
     float prev_pdf = b4.w;
-    V3 radiance = valid ? f4xyz(rad_s[pid]) : v3(0, 0, 0);
-    float4 sg = (EXT && valid) ? sigma_s[pid] : make_float4(0, 0, 0, 0);
-    float4 pv4 = valid ? prev_s[pid] : make_float4(0, 0, 0, 0);
+    V3 radiance = later ? f4xyz(rad_s[pid]) : v3(0, 0, 0);
+    float4 sg = (EXT && later) ? sigma_s[pid] : make_float4(0, 0, 0, 0);
+    float4 pv4 = later ? prev_s[pid] : make_float4(0, 0, 0, 0);
     V3 prev_pos = f4xyz(pv4);
     // Roulette throughput scale of guided paths: the product of
     // p_mixture / p_bsdf over the guided vertices so far (1 unguided).
@@ -1924,8 +1927,8 @@ void anyhit(hipStream_t s, int grid, const DevScene& sc, const DevJob* jobs, con
 }
 
 void camera(hipStream_t s, const DevJob* jobs, const SegTable& segs, uint32_t P, const PathBuffers& pb, Ctrl* ctrl) {
-  hipLaunchKernelGGL(k_camera, dim3((P + 255) / 256), dim3(256), 0, s, jobs, segs, P, pb.qray[0], pb.qpid[0], pb.beta,
-                     pb.rad, pb.sigma, pb.meta, pb.pjob, ctrl);
+  hipLaunchKernelGGL(k_camera, dim3((P + 255) / 256), dim3(256), 0, s, jobs, segs, P, pb.qray[0], pb.qpid[0],
+                     pb.meta, pb.pjob, ctrl);
 }
 
 void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
@@ -1933,12 +1936,12 @@ void prep(hipStream_t s, Ctrl* ctrl, uint32_t cur, uint32_t count_shadow_prev) {
 }
 
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, int ext, bool area) {
+           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, int ext, bool area, uint32_t kb) {
   DevGuideUse gu = guide ? *guide : DevGuideUse{nullptr, nullptr, 0.0f, 0u, 0u};
 #define OXR_SHADE_P(W, G, T, M, X, A, P)                                                                         \
   hipLaunchKernelGGL((k_shade<W, G, T, M, X, A, P>), dim3(grid), dim3(256), 0, s, jobs, pb.pjob, nslots, pb.qray[cur], \
                      pb.qpid[cur], pb.hits, pb.qray[cur ^ 1u], pb.qpid[cur ^ 1u], pb.sray, pb.sidx, pb.beta, pb.rad, \
-                     pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu)
+                     pb.prev, pb.sigma, pb.meta, pb.nee, pb.vis, ctrl, cur, gu, kb)
 #define OXR_SHADE(W, G, T, M, X, A) OXR_SHADE_P(W, G, T, M, X, A, true)
   // Mixed batches (the server), textured scenes and guided scenes with
   // punctual lights run at one wave per SIMD; the occupancy variants are

@@ -68,10 +68,10 @@ void material_eval(hipStream_t s, const DevScene& sc, uint32_t n, const uint32_t
 // ext: a material of the batch has an extension lobe (clearcoat, sheen,
 // iridescence, transmission, diffuse transmission or anisotropy).
 // area: a scene of the batch has emissive triangles, or a job has an
-// environment map.
+// environment map. kb: the bounce of every entry of the launch's queue
+// (bounce 0 reads no path state but the metadata).
 void shade(hipStream_t s, int grid, int waves, const DevJob* jobs, uint32_t nslots, const PathBuffers& pb, Ctrl* ctrl,
-           uint32_t cur, const DevGuideUse* guide, bool tex = false, bool multi = true, int ext = 2,
-           bool area = true);
+           uint32_t cur, const DevGuideUse* guide, bool tex, bool multi, int ext, bool area, uint32_t kb);
 // Training samples (2 float4 each) of the guide-recorded paths of a batch,
 // appended at *count (entries past cap are dropped).
 void guide_collect(hipStream_t s, uint32_t P, const DevGuideUse& gu, const PathBuffers& pb, float4* samp,
