@@ -164,10 +164,16 @@ OXR_D V3 offset_ray_origin(V3 p, V3 n) { return v3(offset1(p.x, n.x), offset1(p.
 // v0's weight is 1 - b1 - b2. Front face <=> det > 0 (the ray runs
 // against the CCW-front normal). Valid hits have t in [t_min, t_max].
 // ---------------------------------------------------------------------
+// The axis permutation (kx, ky, kz) is packed in one word, 2 bits each:
+// the traversal loops keep it live across every triangle test, and three
+// separate registers were spilled at 7 waves per SIMD.
 struct WRay {
   V3 o;
   float Sx, Sy, Sz;
-  int kx, ky, kz;
+  uint32_t kp;  // kx | ky << 2 | kz << 4
+  OXR_D int kx() const { return (int)(kp & 3u); }
+  OXR_D int ky() const { return (int)((kp >> 2) & 3u); }
+  OXR_D int kz() const { return (int)(kp >> 4); }
 };
 
 OXR_D WRay prepare_wray(V3 o, V3 d) {
@@ -189,9 +195,7 @@ OXR_D WRay prepare_wray(V3 o, V3 d) {
     kx = ky;
     ky = t;
   }
-  r.kx = kx;
-  r.ky = ky;
-  r.kz = kz;
+  r.kp = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
   r.Sx = comp(d, kx) / dz;
   r.Sy = comp(d, ky) / dz;
   r.Sz = 1.0f / dz;
@@ -201,13 +205,14 @@ OXR_D WRay prepare_wray(V3 o, V3 d) {
 OXR_D bool intersect_tri(const WRay& r, V3 v0, V3 v1, V3 v2, float t_min, float t_max, float& t_out, float& b1,
                          float& b2, bool& front) {
   V3 A = sub(v0, r.o), B = sub(v1, r.o), C = sub(v2, r.o);
-  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-  float Ax = comp(A, r.kx) - r.Sx * Akz;
-  float Ay = comp(A, r.ky) - r.Sy * Akz;
-  float Bx = comp(B, r.kx) - r.Sx * Bkz;
-  float By = comp(B, r.ky) - r.Sy * Bkz;
-  float Cx = comp(C, r.kx) - r.Sx * Ckz;
-  float Cy = comp(C, r.ky) - r.Sy * Ckz;
+  const int kx = r.kx(), ky = r.ky(), kz = r.kz();
+  float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
+  float Ax = comp(A, kx) - r.Sx * Akz;
+  float Ay = comp(A, ky) - r.Sy * Akz;
+  float Bx = comp(B, kx) - r.Sx * Bkz;
+  float By = comp(B, ky) - r.Sy * Bkz;
+  float Cx = comp(C, kx) - r.Sx * Ckz;
+  float Cy = comp(C, ky) - r.Sy * Ckz;
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;

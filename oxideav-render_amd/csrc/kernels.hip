@@ -179,48 +179,51 @@ struct Stack {
 // Acceptance of a candidate (Integrator::accept, src/pathtrace.rs:1924-1934)
 // from the per-triangle flags: primary back-face culling of single-sided
 // materials, MASK cutoff, stochastic BLEND coin.
-OXR_D bool accept_tri(uint32_t flags, float alpha, bool front, bool cull, uint32_t path_seed, uint32_t ray_id,
-                      uint32_t gid) {
+OXR_D bool accept_tri(uint32_t flags, float alpha, bool front, bool cull, uint32_t coin_key, uint32_t gid) {
   if (cull && !front && !(flags & TRI_DOUBLE_SIDED)) return false;
   if (flags & TRI_MASK_REJECT) return false;
-  if (flags & TRI_BLEND) return coin(path_seed, ray_id, gid) < alpha;
+  if (flags & TRI_BLEND) return to_unit(hash_mix(coin_key, gid)) < alpha;  // coin(path_seed, ray_id, gid)
   return true;
 }
 
 // One ray's traversal state. step() visits one node (its leaves are
 // tested right away) and returns true once the ray is finished.
-template <bool ANY, bool ALPHA = false>
+// PATH: rays of the wavefront (t_min 0, every triangle flag seen); else
+// rays of the parity entry points.
+template <bool ANY, bool ALPHA = false, bool PATH = true>
 struct Trav {
   const DevScene* asc;  // ALPHA: the scene whose textured alpha the filter evaluates
   V3 inv, oi;
   WRay wr;
   float tmin, tmax;
   bool negx, negy, negz, cull;
-  uint32_t skip, path_seed, ray_id;
+  // skip: the global id an occlusion ray ignores. coin_key:
+  // hash_mix(path_seed, ray_id), the ray's part of the BLEND coin
+  // coin(path_seed, ray_id, gid) (one register instead of two, and no
+  // per-leaf rehash).
+  uint32_t skip, coin_key;
   // Triangle flags the filter sees: all of them on path rays; only the
   // double-sided bit on the parity entry points, which accept every
   // candidate (TraceScene::closest_hit / occluded, src/trace.rs:396-450)
   // apart from the optional back-face cull.
-  uint32_t fmask;
+  static constexpr uint32_t fmask = PATH ? ~0u : TRI_DOUBLE_SIDED;
   uint32_t best;
   float bu, bv;
   uint32_t node;
   bool hit_any;
 
-  OXR_D void setup(V3 o, V3 d, float tmin_, float tmax_, bool cull_, uint32_t skip_, uint32_t seed_, uint32_t rid_,
-                   Stack& st) {
+  OXR_D void setup(V3 o, V3 d, float tmin_, float tmax_, bool cull_, uint32_t skip_, uint32_t coin_key_, Stack& st) {
     inv = v3(slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z));
     oi = mul(o, inv);
     negx = signbit(inv.x);
     negy = signbit(inv.y);
     negz = signbit(inv.z);
     wr = prepare_wray(o, d);
-    tmin = tmin_;
+    tmin = PATH ? 0.0f : tmin_;
     tmax = tmax_;
     cull = cull_;
     skip = skip_;
-    path_seed = seed_;
-    ray_id = rid_;
+    coin_key = coin_key_;
     best = HIT_MISS;
     bu = bv = 0.0f;
     node = 0;
@@ -309,13 +312,13 @@ struct Trav {
       if (ANY) {
         if (gid == skip) continue;
         if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
-        if (!accept_tri(fl, aw, front, false, path_seed, ray_id, gid)) continue;
+        if (!accept_tri(fl, aw, front, false, coin_key, gid)) continue;
         hit_any = true;
         return true;
       } else {
         if (best != HIT_MISS && !(t < tmax || (t == tmax && gid < (best & 0x7fffffffu)))) continue;
         if (ALPHA && !tex_alpha_filter(fl, gid, t, b1, b2, aw)) continue;
-        if (!accept_tri(fl, aw, front, cull, path_seed, ray_id, gid)) continue;
+        if (!accept_tri(fl, aw, front, cull, coin_key, gid)) continue;
         tmax = t;
         best = gid | (front ? 0u : HIT_BACK);
         bu = b1;
@@ -383,7 +386,7 @@ struct Trav {
 // gridDim.x is a multiple of nseg.
 // segw = nseg | refill << 8: a wave takes new rays once `refill` of its
 // lanes are idle (64 when the field is 0).
-template <bool ANY, bool ALPHA, class Setup, class Step, class Finish>
+template <bool ANY, bool ALPHA, bool PATH, class Setup, class Step, class Finish>
 OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t segw, Stack& st, Setup setup, Step step,
                       Finish finish) {
   const uint32_t lane = lane_id();
@@ -395,7 +398,7 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t segw, Stack& st, 
   const uint32_t s0 = (uint32_t)((uint64_t)count * seg / nseg);
   const uint32_t s1 = (uint32_t)((uint64_t)count * (seg + 1) / nseg);
   uint32_t* wk = work + (size_t)seg * PAD_COUNTER_WORDS;
-  Trav<ANY, ALPHA> tr;
+  Trav<ANY, ALPHA, PATH> tr;
   tr.asc = nullptr;
   uint32_t ri = s0 + wave * 64u + lane;
   bool active = ri < s1;
@@ -462,9 +465,9 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
   uint32_t nnode = 0, ntri = 0;  // COUNT: this lane's fetches
   const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
-  auto setup = [&](Trav<false, ALPHA>& tr, uint32_t i) {
+  auto setup = [&](Trav<false, ALPHA, PATH>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
-    uint32_t path_seed = 0, ray_id = 0;
+    uint32_t coin_key = 0;
     float tmin;
     bool cull;
     if (PATH) {
@@ -480,24 +483,22 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
       }
       tmin = 0.0f;
       cull = k == 0;
-      ray_id = k << 16;
       if (blend) {
         uint4 m = meta[pid];
-        path_seed = hash_mix(m.w, m.y);
+        coin_key = hash_mix(hash_mix(m.w, m.y), k << 16);
       }
     } else {
       tmin = a.w;
       cull = cull_all != 0;
     }
     if (ALPHA && !MULTI) tr.asc = asc;
-    tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, path_seed, ray_id, st);
-    tr.fmask = PATH ? ~0u : TRI_DOUBLE_SIDED;
+    tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, cull, NO_SKIP, coin_key, st);
   };
-  auto step = [&](Trav<false, ALPHA>& tr) {
+  auto step = [&](Trav<false, ALPHA, PATH>& tr) {
     if (COUNT) ++nnode;
     return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri) : tr.template step<COUNT>(st, sc.nodes, sc.tris, ntri);
   };
-  auto finish = [&](Trav<false, ALPHA>& tr, uint32_t i) {
+  auto finish = [&](Trav<false, ALPHA, PATH>& tr, uint32_t i) {
     uint32_t prim = tr.best;
     if (st.overflow) {
       atomicOr(err, ERR_STACK);
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_closest(Dev
     hits[i] = make_uint4(prim == HIT_MISS ? 0u : __float_as_uint(tr.tmax), prim, __float_as_uint(tr.bu),
                          __float_as_uint(tr.bv));
   };
-  trace_loop<false, ALPHA>(count, work, nseg, st, setup, step, finish);
+  trace_loop<false, ALPHA, PATH>(count, work, nseg, st, setup, step, finish);
   if (COUNT) {  // every lane of the wave leaves trace_loop together
     for (int o = 32; o > 0; o >>= 1) {
       nnode += __shfl_xor(nnode, o);
@@ -541,10 +542,10 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
   uint32_t nnode = 0, ntri = 0;  // COUNT: this lane's fetches
   const Node4Q* ray_nodes = sc.nodes;
   const TriRec* ray_tris = sc.tris;
-  auto setup = [&](Trav<true, ALPHA>& tr, uint32_t i) {
+  auto setup = [&](Trav<true, ALPHA, PATH>& tr, uint32_t i) {
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     float tmin = 0.0f;
-    uint32_t skip = NO_SKIP, path_seed = 0, ray_id = 0;
+    uint32_t skip = NO_SKIP, coin_key = 0;
     if (PATH) {
       skip = __float_as_uint(a.w);
       uint32_t slot = sidx[i];
@@ -564,24 +565,23 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
         // to the vertex before.
         uint32_t k = (m.z & 0xffffu);
         if (!((m.z >> 24) & PF_DONE)) k -= 1u;
-        path_seed = hash_mix(m.w, m.y);
         // Punctual slot j is the j-th light kept at upload; the coin uses
         // the light's index in the scene's full list.
         const DevLight* lts = MULTI ? jobs[pjob[pid]].sc.lights : sc.lights;
-        ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (lts[j].index & 0x7fffu)));
+        const uint32_t ray_id = (k << 16) | (j == nslots - 1 ? 0xffffu : (0x8000u | (lts[j].index & 0x7fffu)));
+        coin_key = hash_mix(hash_mix(m.w, m.y), ray_id);
       }
     } else {
       tmin = a.w;
     }
     if (ALPHA && !MULTI) tr.asc = asc;
-    tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, false, skip, path_seed, ray_id, st);
-    tr.fmask = PATH ? ~0u : TRI_DOUBLE_SIDED;
+    tr.setup(f4xyz(a), f4xyz(b), tmin, b.w, false, skip, coin_key, st);
   };
-  auto step = [&](Trav<true, ALPHA>& tr) {
+  auto step = [&](Trav<true, ALPHA, PATH>& tr) {
     if (COUNT) ++nnode;
     return MULTI ? tr.step(st, ray_nodes, ray_tris, ntri) : tr.template step<COUNT>(st, sc.nodes, sc.tris, ntri);
   };
-  auto finish = [&](Trav<true, ALPHA>& tr, uint32_t i) {
+  auto finish = [&](Trav<true, ALPHA, PATH>& tr, uint32_t i) {
     bool occ = tr.hit_any;
     if (st.overflow) {
       atomicOr(err, ERR_STACK);
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(TRACE_TB) OXR_TRACE_WAVES(ALPHA) void k_anyhit(DevS
       out[i] = occ ? 1 : 0;
     }
   };
-  trace_loop<true, ALPHA>(count, work, nseg, st, setup, step, finish);
+  trace_loop<true, ALPHA, PATH>(count, work, nseg, st, setup, step, finish);
   if (COUNT) {  // every lane of the wave leaves trace_loop together
     for (int o = 32; o > 0; o >>= 1) {
       nnode += __shfl_xor(nnode, o);
