@@ -31,8 +31,8 @@ __constant__ float c_srgb_th[255];
 constexpr int TRACE_TB = 128;
 // LDS traversal stack entries per lane (a ring; deeper entries spill to the
 // global stack). 16 entries = 8 KB per block leaves the traversal kernels
-// register-limited at 6 waves per SIMD; 32 entries capped them at 5 by LDS
-// and measured 2-4% slower (material sweep, terrain).
+// limited by registers (7 waves per SIMD), not LDS; 32 entries capped them
+// at 5 by LDS and measured 2-4% slower (material sweep, terrain).
 constexpr int STACK = 16;
 constexpr float SLAB_EPS = 1.0f + 1.0f / 65536.0f;
 
@@ -439,10 +439,12 @@ OXR_D void trace_loop(uint32_t count, uint32_t* work, uint32_t segw, Stack& st, 
 // bounce index k in the t_min slot (t_min = 0); else t_min as given.
 // MULTI: the launch mixes scenes (server rounds); each ray finds its scene
 // through its path's job. Otherwise every ray traces `sc`.
-// Occupancy of the traversal kernels: 7 waves per SIMD (72 VGPRs, a few
-// spills) measured 4% faster than the register-limited 6 on the material
-// sweep and 3% on the terrain; the textured-alpha variants hold more state
-// and keep 4.
+// Occupancy of the traversal kernels: 7 waves per SIMD (72 VGPRs) measured
+// 4% faster than the register-limited 6 on the material sweep and 3% on
+// the terrain. At 72 VGPRs the path kernels spill only on cold paths (the
+// BLEND coin, the stack's global spill, refills); 8 waves (64 VGPRs) put
+// reloads back on the leaf test. The textured-alpha variants hold more
+// state and keep 4.
 #define OXR_TRACE_WAVES(ALPHA) __attribute__((amdgpu_waves_per_eu((ALPHA) ? 4 : 7)))
 
 // COUNT: adds the nodes and triangles the launch fetched to counts[0] and
