@@ -1411,6 +1411,7 @@ struct oxr_guide {
   float* d_m = nullptr;
   float* d_v = nullptr;
   float* d_grad = nullptr;
+  GradCopies gcopy{};  // zero between gradient evaluations
   uint4* d_frag = nullptr;
   uint32_t t = 0;  // Adam steps
   float4* d_samp = nullptr;
@@ -1440,6 +1441,7 @@ struct oxr_guide {
     dfree(d_m);
     dfree(d_v);
     dfree(d_grad);
+    dfree(gcopy.p);
     dfree(d_frag);
     dfree(d_samp);
     dfree(d_batch);
@@ -1481,13 +1483,23 @@ struct oxr_guide {
     }
     net.tmask = T - 1u;
     net.log2_table = cfg.log2_table;
+    gcopy = GradCopies{};
+    for (int l = 0; l < G_COPY_LEVELS && ((net.dense_mask >> l) & 1u); ++l) {
+      const uint64_t r = net.res[l] + 1ull;
+      if (r * r * r > G_COPY_MAX_ENTRIES) break;
+      gcopy.off[l] = gcopy.stride;
+      gcopy.stride += (uint32_t)(2 * r * r * r);
+      gcopy.nlev = (uint32_t)l + 1u;
+    }
     bool empty = true;
     for (int a = 0; a < 3; ++a) empty = empty && !(cfg.aabb_max[a] > cfg.aabb_min[a]);
     if (!empty) set_box(cfg.aabb_min, cfg.aabb_max);
     if ((rc = dalloc(&d_params, n_total)) || (rc = dalloc(&d_m, n_total)) || (rc = dalloc(&d_v, n_total)) ||
         (rc = dalloc(&d_grad, n_total)) || (rc = dalloc(&d_frag, (size_t)NFRAG * 64)) || (rc = dalloc(&d_count, 4)) ||
-        (rc = dalloc(&d_scal, 4)))
+        (rc = dalloc(&d_scal, 4)) ||
+        (gcopy.nlev && (rc = dalloc(&gcopy.p, (size_t)G_GRAD_COPIES * gcopy.stride))))
       return rc;
+    if (gcopy.nlev) glaunch::fill(stream, gcopy.p, (uint64_t)G_GRAD_COPIES * gcopy.stride);
     // Initialisation: grid U(-1e-4, 1e-4); hidden weights Glorot-uniform;
     // last layer x0.1 (broad, near-uniform lobes at first); biases 0.
     std::vector<float> h(n_total, 0.0f);
@@ -1559,7 +1571,7 @@ struct oxr_guide {
     uint32_t ns = splits_for(n);
     int rc = ensure_train(n, ns);
     if (rc) return rc;
-    glaunch::train_grad(s, net, n, samp, d_scal, d_grad, d_grad + n_grid, d_xs, d_ds, d_part, d_bpart, ns,
+    glaunch::train_grad(s, net, n, samp, d_scal, d_grad, gcopy, d_grad + n_grid, d_xs, d_ds, d_part, d_bpart, ns,
                         d_scal + 1);
     HIPC(hipGetLastError());
     return OXR_OK;

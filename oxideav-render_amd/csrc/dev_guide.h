@@ -51,6 +51,21 @@ struct DevGuideNet {
   float lo[3], inv[3];
 };
 
+// Gradient copies of the coarse dense levels 0 .. nlev-1 (nlev <=
+// G_COPY_LEVELS): G_GRAD_COPIES copies of `stride` floats; level l's
+// dense table ((res_l + 1)^3 entries x 2 features) starts at float off[l]
+// of each copy. Levels qualify while (res_l + 1)^3 <= G_COPY_MAX_ENTRIES.
+constexpr int G_GRAD_COPIES = 8;
+constexpr int G_COPY_LEVELS = 4;
+constexpr uint32_t G_COPY_MAX_ENTRIES = 1u << 15;
+
+struct GradCopies {
+  float* p;
+  uint32_t nlev;
+  uint32_t stride;
+  uint32_t off[G_COPY_LEVELS];
+};
+
 // Per-query guide output of the inference kernel: 8 x (mu, kappa), then
 // the 8 lobe weights. 160 bytes.
 struct GuideLobes {

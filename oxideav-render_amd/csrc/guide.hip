@@ -247,10 +247,16 @@ OXR_D void store4(uint16_t* base, int f0, int q, f32x4_t a) {
   for (int j = 0; j < 4; ++j) base[(f0 + j) * 16 + q] = (uint16_t)bf16_bits(a[j]);
 }
 
+// The first gc.nlev levels (coarse and dense: few entries, each hit by
+// many samples of a batch) take their gradient atomics in G_GRAD_COPIES
+// copies, chosen by block, which k_guide_fold then sums into ggrid:
+// atomics to one address serialise, and on a single table these levels
+// took most of the atomic time.
 __global__ __launch_bounds__(GUIDE_TB) __attribute__((amdgpu_waves_per_eu(2))) void k_guide_fwdbwd(DevGuideNet n, uint32_t count,
                                                            const float4* __restrict__ samp,
                                                            const float* __restrict__ inv_wsum,
-                                                           float* __restrict__ ggrid, uint16_t* __restrict__ xs,
+                                                           float* __restrict__ ggrid, GradCopies gc,
+                                                           uint16_t* __restrict__ xs,
                                                            uint16_t* __restrict__ ds, float* __restrict__ loss) {
   __shared__ GuideShared<NFRAG> sh;
   stage<NFRAG>(n, sh);
@@ -378,9 +384,15 @@ __global__ __launch_bounds__(GUIDE_TB) __attribute__((amdgpu_waves_per_eu(2))) v
         uint32_t idx[8];
         float w[8];
         g_corners(n, sh.res[l], (n.dense_mask >> l) & 1u, p, idx, w);
+        // Copied levels are below G_COPY_LEVELS <= 4, so only lane group 0
+        // (levels 0..3, l == lv) can meet one and the offset index is a
+        // constant.
+        float* gl = (g == 0 && (uint32_t)lv < gc.nlev)
+                        ? gc.p + (size_t)(blockIdx.x % G_GRAD_COPIES) * gc.stride + gc.off[lv]
+                        : ggrid + (size_t)l * T * 2;
 #pragma unroll
         for (int cc = 0; cc < 8; ++cc) {
-          float* e = ggrid + ((size_t)l * T + idx[cc]) * 2;
+          float* e = gl + (size_t)idx[cc] * 2;
           if (df0 != 0.0f) atomicAdd(e, w[cc] * df0);
           if (df1 != 0.0f) atomicAdd(e + 1, w[cc] * df1);
         }
@@ -480,6 +492,30 @@ __global__ void k_guide_wreduce(uint32_t nsplit, const float4* __restrict__ part
       for (int gq = 0; gq < 4; ++gq) s += bpart[((size_t)sp * N_WTILES + tile) * 64 + (o & 15) + 16 * gq];
   }
   gmlp[k] = s;
+}
+
+// Sums the gradient copies into ggrid and clears them. Float i of a copy
+// is float i - off[l] of level l's dense table for the last l with
+// off[l] <= i.
+__global__ void k_guide_fold(uint32_t tmask, float* __restrict__ ggrid, GradCopies gc) {
+  const size_t T = (size_t)tmask + 1u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < gc.stride; i += gridDim.x * blockDim.x) {
+    uint32_t l = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < G_COPY_LEVELS; ++k)
+      if (k < gc.nlev && i >= gc.off[k]) l = k;
+    float sum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < G_GRAD_COPIES; ++c) {
+      float* e = gc.p + (size_t)c * gc.stride + i;
+      const float v = *e;
+      if (v != 0.0f) {
+        sum += v;
+        *e = 0.0f;
+      }
+    }
+    if (sum != 0.0f) ggrid[(size_t)l * T * 2 + (i - gc.off[l])] += sum;
+  }
 }
 
 // Adam over [grid | mlp]; the grid part is lazy (zero gradient: no
@@ -608,13 +644,17 @@ void infer(hipStream_t s, const DevGuideNet& n, uint32_t fixed_count, const uint
 }
 
 void train_grad(hipStream_t s, const DevGuideNet& n, uint32_t count, const float4* samp, float* inv_wsum,
-                float* ggrid, float* gmlp, uint16_t* xs, uint16_t* ds, float4* part, float* bpart, uint32_t nsplit,
-                float* loss) {
+                float* ggrid, const GradCopies& gc, float* gmlp, uint16_t* xs, uint16_t* ds, float4* part,
+                float* bpart, uint32_t nsplit, float* loss) {
   k_guide_wsum<<<1, 1024, 0, s>>>(count, samp, inv_wsum, loss);
   uint32_t groups = count / 16u;
   int grid = (int)std::min<uint32_t>((groups + 3u) / 4u, (uint32_t)cu_count() * 4u);
   if (grid < 1) grid = 1;
-  k_guide_fwdbwd<<<grid, GUIDE_TB, 0, s>>>(n, count, samp, inv_wsum, ggrid, xs, ds, loss);
+  k_guide_fwdbwd<<<grid, GUIDE_TB, 0, s>>>(n, count, samp, inv_wsum, ggrid, gc, xs, ds, loss);
+  if (gc.nlev) {
+    int fgrid = (int)std::min<uint32_t>((gc.stride + 255u) / 256u, (uint32_t)cu_count() * 4u);
+    k_guide_fold<<<fgrid, 256, 0, s>>>(n.tmask, ggrid, gc);
+  }
   k_guide_wgrad<<<dim3(N_WTILES, nsplit), 64, 0, s>>>(count, xs, ds, part, bpart);
   k_guide_wreduce<<<(G_NMLP + 255) / 256, 256, 0, s>>>(nsplit, part, bpart, gmlp);
 }

