@@ -4,9 +4,12 @@ world-space triangle soup, material table, textures and lights that
 `oxr_pt_upload_scene` takes, plus the scene's camera instances and the
 camera that Camera::resolve picks among them (src/camera.rs:177-475).
 
-It restates PreparedScene::build for static scenes (src/prepare.rs:
-1098-1237): preorder node traversal from the roots (`preorder_nodes`,
-:1282-1296), world transforms, `flatten_primitive` (:1352-1517: index
+It restates PreparedScene::build (src/prepare.rs:1098-1237): the pose of
+an animation at a time (or the rest pose), morph weights by the precedence
+animation > node > mesh, skinned primitives placed by their joint palette
+in world space, KHR_materials_variants, preorder node traversal from the
+roots (`preorder_nodes`, :1282-1296), world transforms,
+`flatten_primitive` (:1352-1517: index
 streams, strips and fans, garbage indices dropped, winding flipped under a
 mirroring world transform, per-corner normals or flat ones, UV sets,
 tangents, COLOR_0), `face_tangents` (:1522-1575) for normal-mapped
@@ -16,12 +19,21 @@ primitives without tangents, `PreparedMaterial::from_material` /
 flattened (they count as items) but trace nothing, as in TraceScene.
 
 The scene-graph types mirror oxideav-mesh3d's Scene3D, which is a
-crates.io dependency not vendored with the reference: node transforms
-compose as glTF's T * R * S with (x, y, z, w) quaternions, primitives
-transform with the world matrix (positions), its inverse transpose
-(normals) and its upper 3x3 (tangents), and strips and fans triangulate
-as the glTF specification defines. Animation, skinning, morph targets and
-material variants (which that crate evaluates) are not restated.
+crates.io dependency not vendored with the reference, so what that crate
+evaluates is restated from the glTF 2.0 specification:
+- node transforms compose as T * R * S with (x, y, z, w) quaternions;
+- primitives transform with the world matrix (positions), its inverse
+  transpose (normals) and its upper 3x3 (tangents, whose handedness w
+  flips under a mirroring matrix: B = w (N x T) must map to M B);
+- strips and fans triangulate as section 3.7.2.1 defines;
+- animation samplers follow Appendix C (STEP, LINEAR with slerp for
+  rotations, CUBICSPLINE Hermite splines), clamped outside the keys;
+- morph targets add weighted POSITION / NORMAL / TANGENT deltas (3.7.2.2);
+- skinning is linear blending of the joint matrices world(joint) x
+  inverse bind matrix, which puts the vertices in world space; the mesh
+  node's own transform is then ignored (3.7.3.2);
+- a variant picks the material of the first KHR_materials_variants
+  mapping that lists it.
 """
 import math
 from dataclasses import dataclass, field
@@ -93,7 +105,19 @@ class Material:
 
 
 @dataclass
+class MorphTarget:
+    """Per-vertex deltas of one morph target (POSITION, NORMAL, TANGENT
+    xyz); each is optional."""
+    positions: Optional[np.ndarray] = None
+    normals: Optional[np.ndarray] = None
+    tangents: Optional[np.ndarray] = None
+
+
+@dataclass
 class Primitive:
+    """`joints` / `weights`: JOINTS_n / WEIGHTS_n side by side, (n, 4k)
+    each. `variants`: KHR_materials_variants mappings, (material, [variant
+    indices]) in the file's order."""
     positions: np.ndarray
     topology: str = "triangles"
     normals: Optional[np.ndarray] = None
@@ -102,11 +126,16 @@ class Primitive:
     colors: List[np.ndarray] = field(default_factory=list)
     indices: Optional[np.ndarray] = None
     material: Optional[int] = None
+    targets: List[MorphTarget] = field(default_factory=list)
+    joints: Optional[np.ndarray] = None
+    weights: Optional[np.ndarray] = None
+    variants: List[tuple] = field(default_factory=list)
 
 
 @dataclass
 class Mesh:
     primitives: List[Primitive]
+    weights: List[float] = field(default_factory=list)   # default morph weights
 
 
 @dataclass
@@ -142,6 +171,50 @@ class Node:
     translation: tuple = (0.0, 0.0, 0.0)
     rotation: tuple = (0.0, 0.0, 0.0, 1.0)     # quaternion x, y, z, w
     scale: tuple = (1.0, 1.0, 1.0)
+    skin: Optional[int] = None
+    weights: List[float] = field(default_factory=list)   # morph weights overriding the mesh's
+
+
+@dataclass
+class Skin:
+    """Joint nodes and their inverse bind matrices (4x4, column vectors;
+    identity when absent)."""
+    joints: List[int]
+    inverse_bind_matrices: Optional[list] = None
+    skeleton: Optional[int] = None
+
+
+@dataclass
+class AnimationSampler:
+    """Keyframe times and outputs, one row per key (three per key for
+    CUBICSPLINE: in-tangent, value, out-tangent). interpolation: 'STEP',
+    'LINEAR' or 'CUBICSPLINE'."""
+    times: np.ndarray
+    values: np.ndarray
+    interpolation: str = "LINEAR"
+
+
+@dataclass
+class Channel:
+    node: int
+    path: str                    # 'translation', 'rotation', 'scale' or 'weights'
+    sampler: AnimationSampler
+
+
+@dataclass
+class Animation:
+    channels: List[Channel] = field(default_factory=list)
+    name: Optional[str] = None
+
+
+@dataclass
+class Pose:
+    """Per node: the animated translation / rotation / scale / morph
+    weights (nodes and properties no channel drives are absent)."""
+    translation: dict = field(default_factory=dict)
+    rotation: dict = field(default_factory=dict)
+    scale: dict = field(default_factory=dict)
+    weights: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -153,6 +226,8 @@ class Scene3D:
     textures: List[Texture] = field(default_factory=list)
     lights: List[Light] = field(default_factory=list)
     cameras: List[Camera] = field(default_factory=list)
+    skins: List[Skin] = field(default_factory=list)
+    animations: List[Animation] = field(default_factory=list)
 
 
 @dataclass
@@ -180,6 +255,7 @@ class DrawItem:
     material: int
     bounds_min: np.ndarray
     bounds_max: np.ndarray
+    skinned: bool = False       # placed by its joint palette (world is then the identity)
 
     def primitive_count(self):
         per = {"triangles": 3, "lines": 2, "points": 1}[self.topology]
@@ -211,12 +287,20 @@ def _quat_matrix(q):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]], np.float64)
 
 
-def local_matrix(node):
+def local_matrix(node, pose=None, i=None):
+    """T * R * S of the node, with the pose's animated properties in place
+    of the node's own. A node given by a matrix is not animated (glTF
+    allows channels only on T/R/S nodes): its matrix stands."""
     if node.matrix is not None:
         return np.asarray(node.matrix, np.float64).reshape(4, 4)
+    t, r, sc = node.translation, node.rotation, node.scale
+    if pose is not None:
+        t = pose.translation.get(i, t)
+        r = pose.rotation.get(i, r)
+        sc = pose.scale.get(i, sc)
     m = np.eye(4)
-    m[:3, :3] = _quat_matrix(node.rotation) * np.asarray(node.scale, np.float64)[None, :]
-    m[:3, 3] = node.translation
+    m[:3, :3] = _quat_matrix(r) * np.asarray(sc, np.float64)[None, :]
+    m[:3, 3] = t
     return m
 
 
@@ -234,15 +318,16 @@ def preorder_nodes(scene):
     return out
 
 
-def world_transforms(scene):
-    """World matrix of every node reachable from the roots (first visit)."""
+def world_transforms(scene, pose=None):
+    """World matrix of every node reachable from the roots (first visit),
+    in the rest pose or in `pose`."""
     worlds = [None] * len(scene.nodes)
     stack = [(r, np.eye(4)) for r in reversed(scene.roots)]
     while stack:
         i, parent = stack.pop()
         if i < 0 or i >= len(worlds) or worlds[i] is not None:
             continue
-        worlds[i] = parent @ local_matrix(scene.nodes[i])
+        worlds[i] = parent @ local_matrix(scene.nodes[i], pose, i)
         for c in reversed(scene.nodes[i].children):
             stack.append((c, worlds[i]))
     return [None if w is None else w.astype(F32) for w in worlds]
@@ -277,9 +362,16 @@ def _normalise(v):
     return out.astype(F32)
 
 
+def _with(prim, **kw):
+    d = dict(prim.__dict__)
+    d.update(kw)
+    return Primitive(**d)
+
+
 def transformed(prim, world):
     """The primitive in world space: positions by the world matrix, normals
-    by its inverse transpose, tangent directions by its upper 3x3."""
+    by its inverse transpose, tangent directions by its upper 3x3, and the
+    tangent handedness flipped when the matrix mirrors."""
     w = np.asarray(world, F32)
     p = np.asarray(prim.positions, F32).reshape(-1, 3)
     pos = (p @ w[:3, :3].T + w[:3, 3]).astype(F32)
@@ -289,9 +381,178 @@ def transformed(prim, world):
     tan = None
     if prim.tangents is not None:
         t = np.asarray(prim.tangents, F32).reshape(-1, 4)
-        tan = np.concatenate([t[:, :3] @ w[:3, :3].T, t[:, 3:4]], 1).astype(F32)
-    return Primitive(positions=pos, topology=prim.topology, normals=nrm, tangents=tan, uvs=prim.uvs,
-                     colors=prim.colors, indices=prim.indices, material=prim.material)
+        hand = F32(-1.0) if det3(w) < 0.0 else F32(1.0)
+        tan = np.concatenate([t[:, :3] @ w[:3, :3].T, t[:, 3:4] * hand], 1).astype(F32)
+    return _with(prim, positions=pos, normals=nrm, tangents=tan)
+
+
+# ---------------------------------------------------------------------------
+# Animation, morph targets, skinning, variants (glTF 2.0: Appendix C,
+# sections 3.7.2.2, 3.7.3 and 3.11; KHR_materials_variants).
+# ---------------------------------------------------------------------------
+def _slerp(a, b, u):
+    """Appendix C: spherical interpolation along the shorter arc,
+    sin((1-u) g)/sin g a + s sin(u g)/sin g b with cos g = |a.b| and
+    s = sign(a.b); normalised linear interpolation when g is tiny."""
+    d = float(np.dot(a, b))
+    s = -1.0 if d < 0.0 else 1.0
+    ad = min(abs(d), 1.0)
+    if ad > 1.0 - 1.0e-6:
+        q = (1.0 - u) * a + s * u * b
+    else:
+        g = math.acos(ad)
+        q = (math.sin((1.0 - u) * g) * a + s * math.sin(u * g) * b) / math.sin(g)
+    n = math.sqrt(float(np.dot(q, q)))
+    return q / n if n > 0.0 else a
+
+
+def sample_channel(sampler, t, rotation=False):
+    """The output of an animation sampler at time t (Appendix C), as a
+    float64 vector, or None when the sampler is malformed (no keys, or
+    outputs that are not a whole number of rows per key). Before the first
+    key and after the last the end values hold; keys that do not increase
+    hold the earlier value. Rotations come out normalised."""
+    times = np.asarray(sampler.times, np.float64).ravel()
+    n = len(times)
+    cubic = sampler.interpolation == "CUBICSPLINE"
+    vals = np.asarray(sampler.values, np.float64)
+    rows = n * (3 if cubic else 1)
+    if n == 0 or vals.size == 0 or vals.size % rows:
+        return None
+    vals = vals.reshape(rows, -1)
+
+    def value(k):
+        return vals[3 * k + 1] if cubic else vals[k]
+
+    def out(v):
+        if rotation:
+            nn = math.sqrt(float(np.dot(v, v)))
+            return v / nn if nn > 0.0 else np.array([0.0, 0.0, 0.0, 1.0])
+        return v
+    if not (t > times[0]):
+        return out(value(0).copy())
+    if t >= times[-1]:
+        return out(value(n - 1).copy())
+    k = int(np.searchsorted(times, t, side="right")) - 1
+    td = times[k + 1] - times[k]
+    if not (td > 0.0) or sampler.interpolation == "STEP":
+        return out(value(k).copy())
+    u = (t - times[k]) / td
+    if cubic:
+        u2, u3 = u * u, u * u * u
+        b_k, a_k1 = vals[3 * k + 2], vals[3 * (k + 1)]
+        v = ((2 * u3 - 3 * u2 + 1) * value(k) + td * (u3 - 2 * u2 + u) * b_k + (-2 * u3 + 3 * u2) * value(k + 1)
+             + td * (u3 - u2) * a_k1)
+        return out(v)
+    if rotation:
+        return _slerp(value(k), value(k + 1), u)
+    return (1.0 - u) * value(k) + u * value(k + 1)
+
+
+def sample_pose(anim, t, n_nodes):
+    """Animation::sample_pose: every channel on a node in range, sampled at
+    t; a channel whose output has the wrong width for its property is
+    skipped, and of two channels on one property the later wins."""
+    pose = Pose()
+    width = {"translation": 3, "rotation": 4, "scale": 3}
+    for ch in anim.channels:
+        if not (0 <= ch.node < n_nodes):
+            continue
+        v = sample_channel(ch.sampler, t, rotation=ch.path == "rotation")
+        if v is None:
+            continue
+        if ch.path == "weights":
+            pose.weights[ch.node] = [float(x) for x in v]
+        elif ch.path in width and len(v) == width[ch.path]:
+            getattr(pose, ch.path)[ch.node] = tuple(float(x) for x in v)
+    return pose
+
+
+def morphed(prim, weights):
+    """Primitive::morphed: base attributes plus the weighted deltas of each
+    target (a missing or non-finite weight counts as 0; deltas of the wrong
+    length are skipped; tangent deltas move xyz only)."""
+    if not prim.targets or not len(weights):
+        return prim
+    pos = np.array(prim.positions, F32).reshape(-1, 3)
+    n = len(pos)
+    nrm = None if prim.normals is None else np.array(prim.normals, F32).reshape(-1, 3)
+    tan = None if prim.tangents is None else np.array(prim.tangents, F32).reshape(-1, 4)
+    for w, tg in zip(weights, prim.targets):
+        w = F32(w) if math.isfinite(w) else F32(0.0)
+        if w == 0.0:
+            continue
+        if tg.positions is not None and len(tg.positions) == n:
+            pos += w * np.asarray(tg.positions, F32).reshape(-1, 3)
+        if nrm is not None and tg.normals is not None and len(tg.normals) == n:
+            nrm += w * np.asarray(tg.normals, F32).reshape(-1, 3)
+        if tan is not None and tg.tangents is not None and len(tg.tangents) == n:
+            tan[:, :3] += w * np.asarray(tg.tangents, F32).reshape(n, -1)[:, :3]
+    return _with(prim, positions=pos, normals=nrm, tangents=tan)
+
+
+def joint_matrices(scene, nid, worlds):
+    """Scene3D::joint_matrices_with: world(joint j) x inverse bind matrix j
+    for the node's skin, or None when the node has no valid skin (a joint
+    outside the node list). A joint that the roots do not reach takes its
+    local matrix as its world matrix."""
+    node = scene.nodes[nid]
+    if node.skin is None or not (0 <= node.skin < len(scene.skins)):
+        return None
+    skin = scene.skins[node.skin]
+    ibms = skin.inverse_bind_matrices or []
+    pal = []
+    for j, jn in enumerate(skin.joints):
+        if not (0 <= jn < len(scene.nodes)):
+            return None
+        w = worlds[jn] if worlds[jn] is not None else local_matrix(scene.nodes[jn])
+        ibm = np.asarray(ibms[j], np.float64).reshape(4, 4) if j < len(ibms) else np.eye(4)
+        pal.append(np.asarray(w, np.float64) @ ibm)
+    return np.asarray(pal, F32).reshape(-1, 4, 4) if pal else None
+
+
+def has_influences(prim):
+    n = len(np.asarray(prim.positions).reshape(-1, 3))
+    return (prim.joints is not None and prim.weights is not None
+            and np.asarray(prim.joints).reshape(n, -1).shape == np.asarray(prim.weights).reshape(n, -1).shape)
+
+
+def skinned(prim, palette):
+    """Primitive::skinned: each vertex by the weighted sum of its joints'
+    palette matrices (linear blend skinning) into world space; normals by
+    that sum's inverse transpose, tangents by its upper 3x3 with the
+    handedness flipped where it mirrors. Influences naming a joint outside
+    the palette, or with a non-finite weight, count as 0."""
+    p = np.asarray(prim.positions, F32).reshape(-1, 3)
+    n = len(p)
+    J = np.asarray(prim.joints).reshape(n, -1).astype(np.int64)
+    W = np.asarray(prim.weights, F32).reshape(n, -1)
+    ok = (J >= 0) & (J < len(palette)) & np.isfinite(W)
+    Wv = np.where(ok, W, F32(0.0)).astype(F32)
+    M = np.einsum("nk,nkij->nij", Wv, palette[np.where(ok, J, 0)]).astype(F32)
+    R = M[:, :3, :3]
+    pos = (np.einsum("nij,nj->ni", R, p) + M[:, :3, 3]).astype(F32)
+    nrm = None
+    if prim.normals is not None:
+        nv = np.asarray(prim.normals, F32).reshape(-1, 3)
+        nm = np.stack([normal_matrix(r) for r in R]) if n else np.zeros((0, 3, 3), F32)
+        nrm = np.einsum("nij,nj->ni", nm, nv).astype(F32)
+    tan = None
+    if prim.tangents is not None:
+        t = np.asarray(prim.tangents, F32).reshape(-1, 4)
+        hand = np.array([F32(-1.0) if det3(r) < 0.0 else F32(1.0) for r in R], F32).reshape(-1, 1)
+        tan = np.concatenate([np.einsum("nij,nj->ni", R, t[:, :3]), t[:, 3:4] * hand], 1).astype(F32)
+    return _with(prim, positions=pos, normals=nrm, tangents=tan)
+
+
+def material_for_variant(prim, variant):
+    """KHR_materials_variants: the material of the first mapping that lists
+    `variant`, else the primitive's own."""
+    if variant is not None:
+        for mat, vs in prim.variants:
+            if variant in vs:
+                return mat
+    return prim.material
 
 
 def triangle_indices(prim, seq):
@@ -464,8 +725,12 @@ def light_from_scene(light, world):
 # ---------------------------------------------------------------------------
 # PreparedScene::build.
 # ---------------------------------------------------------------------------
-def build(scene, use_scene_lights=True, fallback_light=LightSpec(), generate_tangents=True):
-    """PreparedScene::build (src/prepare.rs:1098-1237) for the rest pose."""
+def build(scene, use_scene_lights=True, fallback_light=LightSpec(), generate_tangents=True, time=None,
+          animation=None, material_variant=None):
+    """PreparedScene::build (src/prepare.rs:1098-1237). With a finite
+    `time`, animation `animation` (default 0; none when out of range) is
+    sampled into the pose the scene is built in; otherwise the rest pose.
+    `material_variant` selects a KHR_materials_variants variant."""
     ntex = len(scene.textures)
     materials, mat_tex = [], {}
     for i, m in enumerate(scene.materials):
@@ -475,7 +740,12 @@ def build(scene, use_scene_lights=True, fallback_light=LightSpec(), generate_tan
             mat_tex[i] = (binds, m)
     default_material = len(materials)
     materials.append(fallback_material())
-    worlds = world_transforms(scene)
+    pose = None
+    if time is not None and math.isfinite(time):
+        ai = 0 if animation is None else animation
+        if 0 <= ai < len(scene.animations):
+            pose = sample_pose(scene.animations[ai], float(time), len(scene.nodes))
+    worlds = world_transforms(scene, pose)
     items, lights, cameras = [], [], []
     for nid in preorder_nodes(scene):
         node, world = scene.nodes[nid], worlds[nid]
@@ -487,14 +757,26 @@ def build(scene, use_scene_lights=True, fallback_light=LightSpec(), generate_tan
             cameras.append((nid, node.camera, scene.cameras[node.camera], world))
         if node.mesh is None or not (0 <= node.mesh < len(scene.meshes)):
             continue
-        for pi, prim in enumerate(scene.meshes[node.mesh].primitives):
-            mi = prim.material if (prim.material is not None and 0 <= prim.material < default_material) \
-                else default_material
+        mesh = scene.meshes[node.mesh]
+        if pose is not None and nid in pose.weights:
+            weights = pose.weights[nid]
+        else:
+            weights = node.weights if len(node.weights) else mesh.weights
+        palette = joint_matrices(scene, nid, worlds) if node.skin is not None else None
+        for pi, prim in enumerate(mesh.primitives):
+            mv = material_for_variant(prim, material_variant)
+            mi = mv if (mv is not None and 0 <= mv < default_material) else default_material
             normal_ref = mat_tex.get(mi, ({}, None))[0].get("normal")
             needs = generate_tangents and normal_ref is not None
-            it = flatten_primitive(transformed(prim, world), nid, node.mesh, pi, world, mi, needs,
+            m = morphed(prim, weights)
+            if palette is not None and has_influences(m):
+                baked, item_world, sk = skinned(m, palette), np.eye(4, dtype=F32), True
+            else:
+                baked, item_world, sk = transformed(m, world), world, False
+            it = flatten_primitive(baked, nid, node.mesh, pi, item_world, mi, needs,
                                    normal_ref.effective_uv_set() if normal_ref else 0)
             if it is not None:
+                it.skinned = sk
                 items.append(it)
     scene_lights = use_scene_lights and len(lights) > 0
     if not scene_lights:
@@ -710,3 +992,52 @@ def showcase_graph(res=32):
               Light("point", color=(0.6, 0.7, 1.0), intensity=4.0)]
     return Scene3D(nodes=nodes, roots=[0], meshes=meshes, materials=materials, textures=textures, lights=lights,
                    cameras=[Camera(yfov=0.9)])
+
+
+def skinned_morph_beam():
+    """The reference's animated test scene (src/testscenes.rs:576-680): a
+    2 x 0.2 strip along +X (21 vertex columns) skinned to two joints, root
+    at the origin and `elbow` at x = 1, with weights ramping linearly over
+    x in [0.5, 1.5], and one morph target lifting every vertex by 0.3 in Y.
+    Animation 0 turns the elbow from 0 to 90 degrees about +Z and drives the
+    morph weight from 0 to 1 over t in [0, 1]. Nodes: 0 elbow, 1 root, 2 the
+    mesh, 3 a directional light along -Z, 4 a camera at (1, 0.5, 4) looking
+    down -Z (yfov 0.9)."""
+    cols = 21
+    pos, joints, weights = [], [], []
+    for c in range(cols):
+        x = 2.0 * c / (cols - 1)
+        w1 = min(max(x - 0.5, 0.0), 1.0)
+        for y in (-0.1, 0.1):
+            pos.append((x, y, 0.0))
+            joints.append((0, 1, 0, 0))
+            weights.append((1.0 - w1, w1, 0.0, 0.0))
+    idx = []
+    for c in range(cols - 1):
+        a = 2 * c
+        idx += [a, a + 2, a + 3, a, a + 3, a + 1]
+    n = len(pos)
+    prim = Primitive(positions=np.array(pos, F32), indices=np.array(idx, np.uint32),
+                     normals=np.tile(np.array([0.0, 0.0, 1.0], F32), (n, 1)), material=0,
+                     targets=[MorphTarget(positions=np.tile(np.array([0.0, 0.3, 0.0], F32), (n, 1)))],
+                     joints=np.array(joints, np.uint16), weights=np.array(weights, F32))
+    ibm1 = np.eye(4)
+    ibm1[0, 3] = -1.0
+    half = math.sqrt(0.5)
+    bend = Animation(name="bend", channels=[
+        Channel(0, "rotation", AnimationSampler(np.array([0.0, 1.0]), np.array([[0.0, 0.0, 0.0, 1.0],
+                                                                              [0.0, 0.0, half, half]]))),
+        Channel(2, "weights", AnimationSampler(np.array([0.0, 1.0]), np.array([[0.0], [1.0]]))),
+    ])
+    nodes = [
+        Node(translation=(1.0, 0.0, 0.0)),          # elbow
+        Node(children=[0]),                         # root
+        Node(mesh=0, skin=0),
+        Node(light=0),
+        Node(camera=0, translation=(1.0, 0.5, 4.0)),
+    ]
+    return Scene3D(nodes=nodes, roots=[1, 2, 3, 4], meshes=[Mesh([prim], weights=[0.0])],
+                   materials=[Material(base_color=(0.2, 0.6, 0.9, 1.0), metallic=0.0, roughness=0.6)],
+                   lights=[Light("directional", intensity=3.0)], cameras=[Camera(yfov=0.9)],
+                   skins=[Skin(joints=[1, 0], inverse_bind_matrices=[np.eye(4), ibm1], skeleton=1)],
+                   animations=[bend])

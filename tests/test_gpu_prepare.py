@@ -4,7 +4,10 @@ generated tangents, an emissive fan, spot and point lights on transformed
 nodes, seen through its camera node) goes through the prepare layer
 (src/prepare.rs:1098-1237, src/camera.rs:177-332) into `sync`, and its
 direct-lighting image matches the oracle's per pixel, its full-bounce
-image within 3 sigma."""
+image within 3 sigma. The reference's skinned, morphed beam goes through
+the animated pipeline: rays at its rest and bent tips hit as its
+scanline test asserts (tests/scanline_pbr.rs:252-270), and its posed
+direct lighting matches the oracle per pixel."""
 import numpy as np
 import pytest
 
@@ -20,9 +23,9 @@ def _scene():
     return s
 
 
-def _pair(W, spp, bounces):
+def _pair(W, spp, bounces, s=None):
     import oxr
-    s = _scene()
+    s = _scene() if s is None else s
     pt = oxr.PathTracer(0)
     o = oxr.RenderOptions(width=W, height=W, ambient=s.ambient)
     o.path_trace.samples_per_pixel = spp
@@ -56,3 +59,27 @@ def test_prepared_graph_full_bounce_within_3_sigma():
     var = (np.maximum(sg / n - mg ** 2, 0) + np.maximum(sc / n - mc ** 2, 0)) / n
     z = np.abs(mg - mc) / np.sqrt(var + 1e-20)
     assert (z > 3.0).mean() < 0.01
+
+
+def test_animated_beam_tips_and_direct_lighting():
+    import oxr
+    from oxr import _abi as A
+    for t, hit, miss in ((0.0, (1.9, 0.0), (0.72, 0.95)), (1.0, (0.72, 0.95), (1.9, 0.0))):
+        s = P.to_scene(P.build(P.skinned_morph_beam(), time=t), "beam", scene_camera=0)
+        pt = oxr.PathTracer(0)
+        pt.upload(s)
+        o = np.array([[hit[0], hit[1], 4.0], [miss[0], miss[1], 4.0]], np.float32)
+        d = np.array([[0.0, 0.0, -1.0]] * 2, np.float32)
+        h = pt.trace_closest(O.RayBatch(o, d).arr, 2)
+        pt.close()
+        assert h[0, 1] != A.OXR_HIT_MISS and h[1, 1] == A.OXR_HIT_MISS, t
+    s = P.to_scene(P.build(P.skinned_morph_beam(), time=0.5), "beam", scene_camera=0)
+    s.ambient = 0.05
+    ag, _, ac, _ = _pair(96, 4, 1, s)
+    assert np.array_equal(ag[..., 3], ac[..., 3])
+    cov = ac[..., 3] > 0
+    assert cov.sum() > 200
+    mg = ag[..., :3][cov] / ag[..., 3][cov][:, None]
+    mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
+    rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
+    assert rel.max() <= 1e-3, rel.max()

@@ -1,10 +1,14 @@
 """The prepare layer (src/prepare.rs:1098-1575) on the CPU: the
-reference's own prepare tests (src/prepare.rs:1599-1776, animation
-aside), the glTF rules that fix strips, fans and T * R * S, closed forms
-for tangents and the normal matrix, the material and texture-binding
-rules, and prepared lights checked through the oracle's integrator
-(irradiance of a Lambert plane under the prepared spot and fallback
-lights)."""
+reference's own prepare tests (src/prepare.rs:1599-1776), the glTF rules
+that fix strips, fans and T * R * S, closed forms for tangents and the
+normal matrix, the material and texture-binding rules, prepared lights
+checked through the oracle's integrator (irradiance of a Lambert plane
+under the prepared spot and fallback lights), and the animated pipeline:
+the reference's skinned, morphed beam (src/testscenes.rs:576-680) in
+closed form at t = 0, 0.5 and 1 with the coverage its scanline test
+asserts (tests/scanline_pbr.rs:252-270), and Appendix C's samplers,
+morph weight precedence, skinned normals and variants against closed
+forms."""
 import math
 
 import numpy as np
@@ -426,3 +430,169 @@ def test_auto_frame_and_orbit():
     np.testing.assert_allclose(_vec(fb, "eye")[:2], centre[:2], atol=1e-6)
     empty = P.PreparedCamera(P.build(P.Scene3D())).resolve(10, 10)
     np.testing.assert_allclose(_vec(empty, "eye"), [0, 0, 0.6 / math.tan(math.radians(30))], rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Animation, morph targets, skinning and variants.
+# ---------------------------------------------------------------------------
+def test_animation_time_moves_geometry():
+    """src/prepare.rs:1735-1756."""
+    s = tri_scene()
+    s.animations = [P.Animation(channels=[P.Channel(0, "translation", P.AnimationSampler(
+        np.array([0.0, 1.0]), np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0]])))])]
+    rest = P.build(s)
+    assert list(rest.items[0].positions[0]) == [0.0, 0.0, 0.0]
+    half = P.build(s, time=0.5)
+    assert abs(half.items[0].positions[0][0] - 5.0) < 1e-5
+    # A non-finite time, or an animation index out of range, is the rest pose.
+    for kw in (dict(time=float("nan")), dict(time=0.5, animation=3)):
+        assert list(P.build(s, **kw).items[0].positions[0]) == [0.0, 0.0, 0.0]
+
+
+def test_sampler_interpolation_closed_forms():
+    S = P.AnimationSampler
+    keys = np.array([1.0, 2.0, 4.0])
+    vals = np.array([[0.0], [10.0], [30.0]])
+    lin = S(keys, vals)
+    # Clamped outside the keys; linear inside each interval.
+    for t, want in ((0.0, 0.0), (1.0, 0.0), (1.25, 2.5), (3.0, 20.0), (4.0, 30.0), (9.0, 30.0)):
+        assert P.sample_channel(lin, t)[0] == pytest.approx(want, abs=1e-12)
+    step = S(keys, vals, "STEP")
+    assert [P.sample_channel(step, t)[0] for t in (1.5, 2.0, 3.99, 4.5)] == [0.0, 10.0, 10.0, 30.0]
+    # Cubic Hermite reproduces a cubic from its values and derivatives:
+    # f(t) = t^3 on [0, 2], f'(0) = 0, f'(2) = 12 (tangents are per unit time).
+    cub = S(np.array([0.0, 2.0]), np.array([[5.0], [0.0], [0.0], [12.0], [8.0], [7.0]]), "CUBICSPLINE")
+    for t in (0.0, 0.5, 1.0, 1.7, 2.0):
+        assert P.sample_channel(cub, t)[0] == pytest.approx(t ** 3, abs=1e-12)
+    # Slerp from identity to 90 degrees about Z: 45 degrees at u = 0.5 and
+    # 22.5 degrees at u = 0.25 (not the normalised lerp, which gives 26.6).
+    h = math.sqrt(0.5)
+    rot = S(np.array([0.0, 1.0]), np.array([[0.0, 0.0, 0.0, 1.0], [0.0, 0.0, h, h]]))
+    for u, ang in ((0.5, 45.0), (0.25, 22.5)):
+        q = P.sample_channel(rot, u, rotation=True)
+        a = math.radians(ang) / 2
+        assert q == pytest.approx([0.0, 0.0, math.sin(a), math.cos(a)], abs=1e-12)
+    # The shorter arc: -q is the same rotation, and the result does not change.
+    rot_neg = S(np.array([0.0, 1.0]), np.array([[0.0, 0.0, 0.0, 1.0], [0.0, 0.0, -h, -h]]))
+    assert P.sample_channel(rot_neg, 0.5, rotation=True) == pytest.approx(P.sample_channel(rot, 0.5, True),
+                                                                          abs=1e-12)
+    # Malformed outputs (not a whole row count per key) sample to nothing.
+    assert P.sample_channel(S(np.array([0.0, 1.0]), np.zeros(3)), 0.5) is None
+
+
+def _beam_vertex(prep, x, y):
+    """Prepared position of the beam's rest vertex (x, y)."""
+    scene = P.skinned_morph_beam()
+    rest = scene.meshes[0].primitives[0].positions
+    idx = np.asarray(scene.meshes[0].primitives[0].indices)
+    it = prep.items[0]
+    k = int(np.argmin(np.abs(rest[:, 0] - x) + np.abs(rest[:, 1] - y)))
+    return it.positions[int(np.nonzero(idx == k)[0][0])]
+
+
+def _covered(prep, x, y):
+    """Whether (x, y) in z = 0 lies inside one of the prepared triangles."""
+    t = prep.items[0].positions.reshape(-1, 3, 3)[:, :, :2].astype(np.float64)
+    p = np.array([x, y])
+
+    def cross(a, b, c):
+        return (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    pp = np.broadcast_to(p, (len(t), 2))
+    s0, s1, s2 = cross(t[:, 0], t[:, 1], pp), cross(t[:, 1], t[:, 2], pp), cross(t[:, 2], t[:, 0], pp)
+    return bool(np.any(((s0 >= 0) & (s1 >= 0) & (s2 >= 0)) | ((s0 <= 0) & (s1 <= 0) & (s2 <= 0))))
+
+
+def test_skinned_morph_beam_closed_form():
+    """The reference's beam (src/testscenes.rs:576-680). At time t the elbow
+    has turned 90t degrees about +Z and the morph lifts by 0.3t, so a
+    vertex of rest position (x, y) with elbow weight w lands at
+    (1 - w) p + w (E R (p - E)) with p = (x, y + 0.3t) and E = (1, 0)."""
+    scene = P.skinned_morph_beam()
+    for t in (0.0, 0.5, 1.0):
+        prep = P.build(scene, time=t)
+        assert len(prep.items) == 1 and prep.items[0].skinned
+        assert np.array_equal(prep.items[0].world, np.eye(4, dtype=np.float32))
+        a = math.radians(90.0 * t)
+        for x, y in ((2.0, 0.1), (2.0, -0.1), (1.0, 0.1), (0.5, -0.1), (0.3, 0.1), (1.6, -0.1)):
+            w = min(max(x - 0.5, 0.0), 1.0)
+            px, py = x, y + 0.3 * t
+            bx, by = math.cos(a) * (px - 1.0) - math.sin(a) * py + 1.0, math.sin(a) * (px - 1.0) + math.cos(a) * py
+            want = ((1 - w) * px + w * bx, (1 - w) * py + w * by, 0.0)
+            assert _beam_vertex(prep, x, y) == pytest.approx(want, abs=2e-6), (t, x, y)
+        # Normals stay +Z: a rotation about Z, and scale-free blends.
+        assert np.allclose(prep.items[0].normals, [0.0, 0.0, 1.0], atol=1e-6)
+    # tests/scanline_pbr.rs:252-270: the rest tip is covered at t = 0 and
+    # left at t = 1, where the bent and morphed tip is covered instead.
+    p0, p1 = P.build(scene, time=0.0), P.build(scene, time=1.0)
+    assert _covered(p0, 1.9, 0.0) and not _covered(p0, 0.72, 0.95)
+    assert not _covered(p1, 1.9, 0.0) and _covered(p1, 0.72, 0.95)
+    # Without a time the rest pose is the t = 0 pose here.
+    assert np.array_equal(P.build(scene).items[0].positions, p0.items[0].positions)
+    # A skinned mesh ignores its node's transform (glTF 3.7.3.2).
+    scene.nodes[2].translation = (5.0, 5.0, 5.0)
+    assert np.array_equal(P.build(scene, time=1.0).items[0].positions, p1.items[0].positions)
+
+
+def test_morph_weight_precedence_and_linearity():
+    prim = P.Primitive(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32),
+                       targets=[P.MorphTarget(positions=np.array([[0, 0, 1]] * 3, np.float32)),
+                                P.MorphTarget(positions=np.array([[1, 0, 0], [0, 0, 0], [0, 2, 0]], np.float32))])
+    s = P.Scene3D(meshes=[P.Mesh([prim], weights=[0.5, 0.25])], nodes=[P.Node(mesh=0)], roots=[0])
+    base = prim.positions.astype(np.float64)
+    d0, d1 = prim.targets[0].positions, prim.targets[1].positions
+
+    def first(prep):
+        return prep.items[0].positions
+    assert np.allclose(first(P.build(s)), base + 0.5 * d0 + 0.25 * d1)           # the mesh's weights
+    s.nodes[0].weights = [1.0, -1.0]
+    assert np.allclose(first(P.build(s)), base + d0 - d1)                        # the node's override them
+    s.animations = [P.Animation(channels=[P.Channel(0, "weights", P.AnimationSampler(
+        np.array([0.0, 2.0]), np.array([[0.0, 0.0], [2.0, 4.0]])))])]
+    assert np.allclose(first(P.build(s, time=0.5)), base + 0.5 * d0 + 1.0 * d1)  # the animation's win
+    assert np.allclose(first(P.build(s)), base + d0 - d1)                        # rest: the node's
+
+
+def test_skinned_normals_take_the_inverse_transpose():
+    """One joint scaled by 2 along X: positions scale, and a normal
+    (1, 1, 0)/sqrt(2) becomes (1/2, 1, 0) normalised, the inverse
+    transpose's image."""
+    r = 1.0 / math.sqrt(2.0)
+    prim = P.Primitive(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32),
+                       normals=np.array([[r, r, 0]] * 3, np.float32),
+                       tangents=np.array([[r, -r, 0, 1]] * 3, np.float32),
+                       joints=np.zeros((3, 4), np.uint16), weights=np.array([[1, 0, 0, 0]] * 3, np.float32))
+    s = P.Scene3D(meshes=[P.Mesh([prim])], nodes=[P.Node(mesh=0, skin=0), P.Node(scale=(2.0, 1.0, 1.0))],
+                  roots=[0, 1], skins=[P.Skin(joints=[1])])
+    it = P.build(s).items[0]
+    assert it.skinned
+    assert np.allclose(it.positions, [[0, 0, 0], [2, 0, 0], [0, 1, 0]])
+    n = np.array([0.5, 1.0, 0.0]) / math.sqrt(1.25)
+    assert np.allclose(it.normals, n, atol=1e-6)
+    # Without influences the primitive is placed by its node, not skinned.
+    s.meshes[0].primitives[0].weights = None
+    it2 = P.build(s).items[0]
+    assert not it2.skinned and np.allclose(it2.positions, [[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+
+
+def test_mirrored_world_flips_tangent_handedness():
+    """B = w (N x T) must transform as M B: under scale (-1, 1, 1) a tangent
+    (1, 0, 0, +1) with normal +Z (bitangent +Y) becomes (-1, 0, 0, -1)."""
+    prim = P.Primitive(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32),
+                       normals=np.array([[0, 0, 1]] * 3, np.float32),
+                       tangents=np.array([[1, 0, 0, 1]] * 3, np.float32))
+    s = P.Scene3D(meshes=[P.Mesh([prim])], nodes=[P.Node(mesh=0, scale=(-1.0, 1.0, 1.0))], roots=[0])
+    it = P.build(s).items[0]
+    for k in range(3):
+        t, n = it.tangents[k], it.normals[k]
+        assert list(t) == [-1.0, 0.0, 0.0, -1.0]
+        assert np.allclose(t[3] * np.cross(n, t[:3]), [0.0, 1.0, 0.0])
+
+
+def test_material_variants():
+    prim = P.Primitive(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), material=0,
+                       variants=[(1, [0, 2]), (2, [1]), (7, [3])])
+    mats = [P.Material(base_color=(c, c, c, 1.0)) for c in (0.1, 0.2, 0.3)]
+    s = P.Scene3D(meshes=[P.Mesh([prim])], nodes=[P.Node(mesh=0)], roots=[0], materials=mats)
+    got = {v: P.build(s, material_variant=v).items[0].material for v in (None, 0, 1, 2, 3, 9)}
+    # Variant 3 maps to a material that does not exist: the fallback.
+    assert got == {None: 0, 0: 1, 1: 2, 2: 1, 3: 3, 9: 0}
