@@ -596,12 +596,52 @@ def face_tangents(pos, nrm, uv):
     return out
 
 
+def _perpendicular(n):
+    a = np.where((np.abs(n[:, 0]) < 0.9)[:, None], np.array([1, 0, 0], F32), np.array([0, 1, 0], F32))
+    return _normalise(np.cross(a, n))
+
+
+def compute_tangents(prim, uv_set):
+    """Primitive::compute_tangents (oxideav-mesh3d; the reference's notes
+    call it vertex-averaged per-triangle UV gradients, src/prepare.rs:
+    29-32), after Lengyel, "Computing Tangent Space Basis Vectors for an
+    Arbitrary Mesh" (2001): each triangle's dP/du and dP/dv are summed
+    into its three vertices; a vertex's tangent is its dP/du sum made
+    orthogonal to the vertex normal and normalised, and w is the sign of
+    (N x T) . (dP/dv sum). Triangles with a degenerate UV map add nothing;
+    a vertex that gets nothing (or a sum parallel to its normal) takes a
+    perpendicular of the normal, w = +1. None without vertex normals or
+    that UV set."""
+    p = np.asarray(prim.positions, F32).reshape(-1, 3)
+    nv = len(p)
+    if prim.normals is None or len(prim.normals) != nv or uv_set >= len(prim.uvs) or len(prim.uvs[uv_set]) != nv:
+        return None
+    nrm = np.asarray(prim.normals, F32).reshape(-1, 3)
+    uv = np.asarray(prim.uvs[uv_set], F32).reshape(-1, 2)
+    seq = list(range(nv)) if prim.indices is None else [int(i) for i in np.asarray(prim.indices).ravel()]
+    tri = np.array([t for t in triangle_indices(prim, seq) if all(0 <= i < nv for i in t)], np.int64).reshape(-1, 3)
+    tan1 = np.zeros((nv, 3), F32)
+    tan2 = np.zeros((nv, 3), F32)
+    if len(tri):
+        e1, e2 = p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]]
+        d1, d2 = uv[tri[:, 1]] - uv[tri[:, 0]], uv[tri[:, 2]] - uv[tri[:, 0]]
+        det = (d1[:, 0] * d2[:, 1] - d2[:, 0] * d1[:, 1]).astype(F32)
+        good = np.abs(det) > 1.0e-12
+        r = np.where(good, F32(1.0) / np.where(good, det, F32(1.0)), F32(0.0)).astype(F32)[:, None]
+        sdir = ((e1 * d2[:, 1:2] - e2 * d1[:, 1:2]) * r).astype(F32)
+        tdir = ((e2 * d1[:, 0:1] - e1 * d2[:, 0:1]) * r).astype(F32)
+        for k in range(3):
+            np.add.at(tan1, tri[:, k], sdir)
+            np.add.at(tan2, tri[:, k], tdir)
+    t = _normalise(tan1 - nrm * (nrm * tan1).sum(1, keepdims=True))
+    bad = (t * t).sum(1) < 0.5
+    t[bad] = _perpendicular(nrm[bad])
+    w = np.where((np.cross(nrm, t) * tan2).sum(1) < 0.0, F32(-1.0), F32(1.0))
+    return np.concatenate([t, w[:, None]], 1).astype(F32)
+
+
 def flatten_primitive(prim, node, mesh, pi, world, material, needs_tangents, tangent_uv_set):
-    """src/prepare.rs:1352-1517. One departure: for a normal-mapped
-    primitive that has vertex normals but no tangents the reference first
-    asks oxideav-mesh3d's Primitive::compute_tangents (:1371-1377, not
-    vendored); here such a primitive takes the per-face tangents that the
-    reference uses when that call yields none (parity unpinned there)."""
+    """src/prepare.rs:1352-1517."""
     pos_all = np.asarray(prim.positions, F32).reshape(-1, 3)
     nv = len(pos_all)
     if nv == 0:
@@ -610,6 +650,8 @@ def flatten_primitive(prim, node, mesh, pi, world, material, needs_tangents, tan
     tangents_src = None if prim.tangents is None or len(prim.tangents) != nv else np.asarray(prim.tangents, F32)
     topo = ("triangles" if prim.topology in TOPOLOGIES[:3] else "lines" if prim.topology in TOPOLOGIES[3:6]
             else "points")
+    if tangents_src is None and needs_tangents and topo == "triangles" and normals_src is not None:
+        tangents_src = compute_tangents(prim, tangent_uv_set)
     seq = list(range(nv)) if prim.indices is None else [int(i) for i in np.asarray(prim.indices).ravel()]
     flip = det3(world) < 0.0
     finite = np.all(np.isfinite(pos_all), axis=1)

@@ -596,3 +596,31 @@ def test_material_variants():
     got = {v: P.build(s, material_variant=v).items[0].material for v in (None, 0, 1, 2, 3, 9)}
     # Variant 3 maps to a material that does not exist: the fallback.
     assert got == {None: 0, 0: 1, 1: 2, 2: 1, 3: 3, 9: 0}
+
+
+def test_compute_tangents_vertex_averaged_closed_form():
+    """Lengyel's vertex-averaged tangents: a vertex shared by two triangles
+    whose dP/du are (1, 0, 0) and (1, 1, 0) gets their normalised sum
+    (2, 1, 0)/sqrt(5) (per-face tangents would give each corner its own
+    triangle's); flipping v flips w; a tilted normal makes the tangent
+    orthogonal to it by Gram-Schmidt."""
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0]], np.float32)
+    uv = np.array([[0, 0], [1, 0], [0, 1], [-1, 1]], np.float32)
+    z = np.array([[0, 0, 1]] * 4, np.float32)
+    prim = P.Primitive(positions=pos, normals=z, uvs=[uv], indices=np.array([0, 1, 2, 0, 2, 3], np.uint32))
+    t = P.compute_tangents(prim, 0)
+    r5, r2 = 1 / math.sqrt(5), 1 / math.sqrt(2)
+    want = [[2 * r5, r5, 0, 1], [1, 0, 0, 1], [2 * r5, r5, 0, 1], [r2, r2, 0, 1]]
+    assert np.allclose(t, want, atol=1e-6)
+    flipped = P.Primitive(positions=pos, normals=z, uvs=[uv * np.array([1, -1], np.float32)],
+                          indices=prim.indices)
+    assert np.allclose(P.compute_tangents(flipped, 0)[:, 3], -1.0)
+    tilt = np.array([[0.6, 0, 0.8]] * 4, np.float32)
+    one = P.Primitive(positions=pos[:3], normals=tilt[:3], uvs=[uv[:3]])
+    assert np.allclose(P.compute_tangents(one, 0), [[0.8, 0, -0.6, 1]] * 3, atol=1e-6)
+    # Without vertex normals, or without the UV set, there is nothing to average.
+    assert P.compute_tangents(P.Primitive(positions=pos, uvs=[uv]), 0) is None
+    assert P.compute_tangents(prim, 1) is None
+    # flatten_primitive takes them for a normal-mapped primitive with normals.
+    it = P.flatten_primitive(prim, 0, 0, 0, np.eye(4, dtype=np.float32), 0, True, 0)
+    assert np.allclose(it.tangents[0], want[0], atol=1e-6) and np.allclose(it.tangents[3], want[0], atol=1e-6)
