@@ -376,26 +376,45 @@ __global__ __launch_bounds__(GUIDE_TB) __attribute__((amdgpu_waves_per_eu(2))) v
       dx0[t] = mfma16(F[(BW1 + 2 * t) * 64 + ln], db[0], dx0[t]);
       dx0[t] = mfma16(F[(BW1 + 2 * t + 1) * 64 + ln], db[1], dx0[t]);
     }
-    if (wn != 0.0f) {
+    // Grid gradient. Every lane of the wave runs this (the merge below reads
+    // the other lanes of its group); a zero-weight sample's gradient is 0.
 #pragma unroll
-      for (int lv = 0; lv < 4; ++lv) {
-        int l = 4 * g + lv;
-        float df0 = dx0[lv >> 1][2 * (lv & 1)], df1 = dx0[lv >> 1][2 * (lv & 1) + 1];
-        uint32_t idx[8];
-        float w[8];
-        g_corners(n, sh.res[l], (n.dense_mask >> l) & 1u, p, idx, w);
-        // Copied levels are below G_COPY_LEVELS <= 4, so only lane group 0
-        // (levels 0..3, l == lv) can meet one and the offset index is a
-        // constant.
-        float* gl = (g == 0 && (uint32_t)lv < gc.nlev)
-                        ? gc.p + (size_t)(blockIdx.x % G_GRAD_COPIES) * gc.stride + gc.off[lv]
-                        : ggrid + (size_t)l * T * 2;
+    for (int lv = 0; lv < 4; ++lv) {
+      int l = 4 * g + lv;
+      float df0 = dx0[lv >> 1][2 * (lv & 1)], df1 = dx0[lv >> 1][2 * (lv & 1) + 1];
+      uint32_t idx[8];
+      float w[8];
+      g_corners(n, sh.res[l], (n.dense_mask >> l) & 1u, p, idx, w);
+      // Copied levels are below G_COPY_LEVELS <= 4, so only lane group 0
+      // (levels 0..3, l == lv) can meet one and the offset index is a
+      // constant.
+      float* gl = (g == 0 && (uint32_t)lv < gc.nlev)
+                      ? gc.p + (size_t)(blockIdx.x % G_GRAD_COPIES) * gc.stride + gc.off[lv]
+                      : ggrid + (size_t)l * T * 2;
 #pragma unroll
-        for (int cc = 0; cc < 8; ++cc) {
-          float* e = gl + (size_t)idx[cc] * 2;
-          if (df0 != 0.0f) atomicAdd(e, w[cc] * df0);
-          if (df1 != 0.0f) atomicAdd(e + 1, w[cc] * df1);
+      for (int cc = 0; cc < 8; ++cc) {
+        // The group's 16 samples whose corner cc falls on the same entry of
+        // level l: the lowest of those lanes adds their sum, the others
+        // nothing (same-address atomics serialise; samples of one region
+        // share the coarse and middle levels' entries).
+        const float c0 = w[cc] * df0, c1 = w[cc] * df1;
+        float s0 = c0, s1 = c1;
+        bool lead = true;
+#pragma unroll
+        for (int sft = 1; sft < 16; ++sft) {
+          const int oq = (q + sft) & 15;
+          const int src = (lane & ~15) | oq;
+          const uint32_t oi = (uint32_t)__shfl((int)idx[cc], src);
+          const float o0 = __shfl(c0, src), o1 = __shfl(c1, src);
+          if (oi == idx[cc]) {
+            s0 += o0;
+            s1 += o1;
+            lead = lead && oq > q;
+          }
         }
+        float* e = gl + (size_t)idx[cc] * 2;
+        if (lead && s0 != 0.0f) atomicAdd(e, s0);
+        if (lead && s1 != 0.0f) atomicAdd(e + 1, s1);
       }
     }
   }
