@@ -532,17 +532,34 @@ def skinned(prim, palette):
     M = np.einsum("nk,nkij->nij", Wv, palette[np.where(ok, J, 0)]).astype(F32)
     R = M[:, :3, :3]
     pos = (np.einsum("nij,nj->ni", R, p) + M[:, :3, 3]).astype(F32)
+    det, nm = _normal_matrices(R)
     nrm = None
     if prim.normals is not None:
         nv = np.asarray(prim.normals, F32).reshape(-1, 3)
-        nm = np.stack([normal_matrix(r) for r in R]) if n else np.zeros((0, 3, 3), F32)
         nrm = np.einsum("nij,nj->ni", nm, nv).astype(F32)
     tan = None
     if prim.tangents is not None:
         t = np.asarray(prim.tangents, F32).reshape(-1, 4)
-        hand = np.array([F32(-1.0) if det3(r) < 0.0 else F32(1.0) for r in R], F32).reshape(-1, 1)
+        hand = np.where(det < 0.0, F32(-1.0), F32(1.0)).astype(F32).reshape(-1, 1)
         tan = np.concatenate([np.einsum("nij,nj->ni", R, t[:, :3]), t[:, 3:4] * hand], 1).astype(F32)
     return _with(prim, positions=pos, normals=nrm, tangents=tan)
+
+
+def _normal_matrices(R):
+    """det3 and normal_matrix of each 3x3 in R (n, 3, 3), vectorised with
+    the same f32 operations."""
+    R = np.asarray(R, F32)
+    m = lambda i, j: R[:, i, j]
+    c = lambda r0, c0, r1, c1: m(r0, c0) * m(r1, c1) - m(r0, c1) * m(r1, c0)
+    det = (m(0, 0) * (m(1, 1) * m(2, 2) - m(1, 2) * m(2, 1)) - m(0, 1) * (m(1, 0) * m(2, 2) - m(1, 2) * m(2, 0))
+           + m(0, 2) * (m(1, 0) * m(2, 1) - m(1, 1) * m(2, 0))).astype(F32)
+    ok = np.isfinite(det) & (np.abs(det) > 1.0e-20)
+    inv = np.where(ok, F32(1.0) / np.where(ok, det, F32(1.0)), F32(0.0)).astype(F32)
+    nm = np.stack([np.stack([c(1, 1, 2, 2) * inv, -c(1, 0, 2, 2) * inv, c(1, 0, 2, 1) * inv], -1),
+                   np.stack([-c(0, 1, 2, 2) * inv, c(0, 0, 2, 2) * inv, -c(0, 0, 2, 1) * inv], -1),
+                   np.stack([c(0, 1, 1, 2) * inv, -c(0, 0, 1, 2) * inv, c(0, 0, 1, 1) * inv], -1)], 1).astype(F32)
+    nm[~ok] = np.eye(3, dtype=F32)
+    return det, nm
 
 
 def material_for_variant(prim, variant):
