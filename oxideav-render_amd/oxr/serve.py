@@ -99,7 +99,9 @@ class Prebuilt:
     """A scene prepared on the host (BVH, light tables, materials) by
     oxr_scene_prebuild: no device call, so any thread may build one while
     the server's owner thread keeps stepping. fast=True builds a linear
-    (Morton-order) BVH, several times faster than the binned SAH build."""
+    (Morton-order) BVH with single-triangle leaves: about twice as fast to
+    build as the binned SAH, and within 5% of its trace speed on random
+    soups."""
 
     def __init__(self, scene, fast=True):
         self._L = lib()
