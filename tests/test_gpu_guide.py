@@ -110,6 +110,28 @@ def test_gradient_matches_oracle(guide_and_params):
     assert np.allclose(gg2, gg, rtol=1e-5, atol=1e-9) and np.allclose(gm2, gm, rtol=1e-5, atol=1e-9)
 
 
+def test_gradient_of_a_concentrated_batch_matches_oracle(guide_and_params):
+    """All samples inside one small region: the 16 samples of a lane group
+    share their grid entries at every level, so nearly every contribution
+    goes through the merge (the lowest lane of each set of equal entries
+    adds the set's sum). Same bars as the spread-out batch."""
+    cfg, g, grid, mlp = guide_and_params
+    rng = np.random.default_rng(12)
+    n = 1024
+    pos = (np.array([0.31, -0.42, 0.17]) + rng.uniform(-2e-3, 2e-3, (n, 3))).astype(np.float32)
+    dirs = unit(rng.normal(size=(n, 3)))
+    w = rng.uniform(0.0, 2.0, n).astype(np.float32)
+    loss, gg, gm = g.gradient(pos, dirs, w)
+    l_o, gg_o, gm_o = O.guide_loss_grad(cfg, grid, mlp, pos, dirs, w, mode=1)
+    assert abs(loss - l_o) <= 1e-4 * abs(l_o)
+    assert np.linalg.norm(gm - gm_o) <= 2e-2 * np.linalg.norm(gm_o)
+    assert np.linalg.norm(gg - gg_o) <= 2e-2 * np.linalg.norm(gg_o)
+    assert np.array_equal(gg == 0, gg_o == 0)
+    # Few entries carry the whole batch: their sums are large, and they match.
+    top = np.argsort(-np.abs(gg_o))[:64]
+    np.testing.assert_allclose(gg[top], gg_o[top], rtol=2e-2, atol=1e-3 * np.abs(gg_o).max())
+
+
 def test_adam_step_matches_oracle():
     from oxr.guide import Guide
     cfg = small_cfg(lr=1e-3, eps=1e-8)
