@@ -5,8 +5,10 @@ Names, argument meaning and error behaviour follow the reference:
 `PathTracer::{new, sync, refine, samples, is_converged, reset,
 invalidate_scene, image, hdr}` (src/pathtrace.rs:2420-2735) and
 `PathTraceRenderer::{render, render_hdr}` (:2745-2782). The scene handed
-to `sync` is a prepared soup (`oxr.scenes.Scene`), the output of the
-reference's prepare layer.
+to `sync` is either a scene graph (`oxr.prepare.Scene3D`), which `sync`
+prepares as the reference does (re-preparing when the animation time,
+animation, variant or lights change), or an already prepared soup
+(`oxr.scenes.Scene`).
 """
 import ctypes as C
 from dataclasses import dataclass, field, replace
@@ -32,6 +34,12 @@ class PathTraceOptions:
 
 @dataclass
 class RenderOptions:
+    """The path tracer's part of RenderOptions (src/options.rs:326-414).
+    The fields after `path_trace` apply to a scene graph handed to
+    `PathTracer.sync`: its preparation (PrepareOptions::from_render_options,
+    src/prepare.rs:137-146) and its camera (Camera::resolve: `scene_camera`,
+    else auto-frame or orbit with `projection`, `fov_deg`, `camera` =
+    (elevation_deg, azimuth_deg, distance) and `camera_target_offset`)."""
     width: int = 512
     height: int = 512
     background: tuple = (0, 0, 0, 0)
@@ -39,6 +47,16 @@ class RenderOptions:
     tone_map: int = CLAMP
     exposure: float = 1.0
     path_trace: PathTraceOptions = field(default_factory=PathTraceOptions)
+    time: float = None
+    animation: int = None
+    material_variant: int = None
+    use_scene_lights: bool = True
+    light: object = None             # prepare.LightSpec; None = its default
+    scene_camera: int = None
+    projection: int = A.OXR_PROJ_PERSPECTIVE
+    fov_deg: float = 60.0
+    camera: tuple = None
+    camera_target_offset: tuple = (0.0, 0.0, 0.0)
 
     def validate(self):
         """RenderOptions::validate's path-trace checks (src/options.rs:723-741)."""
@@ -89,6 +107,9 @@ class PathTracer:
         self._h = h
         self._device = device
         self._scene = None
+        self._graph = None      # the scene graph the uploaded soup was prepared from
+        self._prep_key = None
+        self._prepared = None
         self._desc = None
         self._dirty = True
         self._opts = RenderOptions()
@@ -122,15 +143,36 @@ class PathTracer:
 
     def sync(self, scene, opts: RenderOptions):
         """Bring the tracer in line with `scene` + `opts`; returns True when
-        the accumulation was reset (src/pathtrace.rs:2474-2508)."""
+        the accumulation was reset (src/pathtrace.rs:2474-2508). A scene
+        graph is prepared again when this tracer is dirty, the graph is
+        another object, or its preparation inputs (time, animation,
+        variant, scene lights, fallback light) changed."""
+        from . import prepare as P
         reset = False
-        if self._dirty or self._scene is not scene:
-            self._desc = scene.desc()
-            check(self._L.oxr_pt_upload_scene(self._h, C.byref(self._desc)), "upload_scene")
-            self._scene = scene
-            self._dirty = False
-            reset = True
-        cam = scene.camera.resolve(max(opts.width, 1), max(opts.height, 1))
+        if isinstance(scene, P.Scene3D):
+            prep_key = (opts.time if opts.time is not None and np.isfinite(opts.time) else None, opts.animation,
+                        opts.material_variant, bool(opts.use_scene_lights),
+                        opts.light if opts.light is not None else P.LightSpec())
+            if self._dirty or self._graph is not scene or self._prep_key != prep_key:
+                self._prepared = P.build(scene, use_scene_lights=prep_key[3], fallback_light=prep_key[4],
+                                         time=prep_key[0], animation=opts.animation,
+                                         material_variant=opts.material_variant)
+                soup = P.to_scene(self._prepared)
+                self._desc = soup.desc()
+                check(self._L.oxr_pt_upload_scene(self._h, C.byref(self._desc)), "upload_scene")
+                self._scene, self._graph, self._prep_key = soup, scene, prep_key
+                self._dirty = False
+                reset = True
+            cam = P.PreparedCamera(self._prepared, opts.scene_camera, opts.projection, opts.fov_deg, opts.camera,
+                                   opts.camera_target_offset).resolve(max(opts.width, 1), max(opts.height, 1))
+        else:
+            if self._dirty or self._scene is not scene:
+                self._desc = scene.desc()
+                check(self._L.oxr_pt_upload_scene(self._h, C.byref(self._desc)), "upload_scene")
+                self._scene, self._graph = scene, None
+                self._dirty = False
+                reset = True
+            cam = scene.camera.resolve(max(opts.width, 1), max(opts.height, 1))
         r1 = C.c_int(0)
         r2 = C.c_int(0)
         check(self._L.oxr_pt_set_options(self._h, C.byref(_pt_opts(opts)), C.byref(r1)), "set_options")
@@ -284,7 +326,7 @@ class PathTracer:
         """Upload a scene without options (for the trace entry points)."""
         self._desc = scene.desc()
         check(self._L.oxr_pt_upload_scene(self._h, C.byref(self._desc)), "upload_scene")
-        self._scene = scene
+        self._scene, self._graph = scene, None
         self._dirty = False
 
     def trace_closest(self, rays, n, flags=0):

@@ -83,3 +83,38 @@ def test_animated_beam_tips_and_direct_lighting():
     mc = ac[..., :3][cov] / ac[..., 3][cov][:, None]
     rel = np.abs(mg - mc) / np.maximum(np.abs(mc), 1e-3)
     assert rel.max() <= 1e-3, rel.max()
+
+
+def test_sync_prepares_scene_graphs_with_their_options():
+    """PathTracer.sync over a scene graph (src/pathtrace.rs:2474-2508):
+    prepared as an explicit prepare call would be (same accumulators bit for
+    bit), prepared again and reset exactly when a preparation input (here
+    the animation time) changes, untouched when nothing changed; and the
+    Renderer takes a scene graph too."""
+    import oxr
+    graph = P.skinned_morph_beam()
+    W = 48
+
+    def opts(**kw):
+        o = oxr.RenderOptions(width=W, height=W, ambient=0.05, scene_camera=0, **kw)
+        o.path_trace.samples_per_pixel = 4
+        o.path_trace.max_bounces = 2
+        return o
+    pt = oxr.PathTracer(0)
+    assert pt.sync(graph, opts(time=1.0))
+    assert not pt.sync(graph, opts(time=1.0))
+    pt.refine(4)
+    bent = pt.accum()
+    assert pt.sync(graph, opts(time=0.5))
+    assert pt.samples() == 0
+    pt.refine(4)
+    half = pt.accum()
+    pt.close()
+    assert not np.array_equal(bent, half)
+    ref = oxr.PathTracer(0)
+    ref.sync(P.to_scene(P.build(graph, time=0.5), scene_camera=0), opts(time=0.5))
+    ref.refine(4)
+    assert np.array_equal(half, ref.accum())
+    ref.close()
+    img = oxr.PathTraceRenderer(0).render(graph, opts(time=1.0))
+    assert img.shape[:2] == (W, W) and img[..., 3].max() > 0
