@@ -230,7 +230,7 @@ def roofline_of(pt, spp, tot, steps, workload, W):
     return out
 
 
-def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16, error_spp=1024):
+def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16, error_spp=1024, autotune=True):
     """The large-scene config at its full size (12.4M triangles, 1920x1080,
     8 bounces), bounded to a few steps of spp_per_step: paths/s, rays/s,
     the rooflines of both traversal kernels from the device's own fetch
@@ -238,7 +238,8 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16, error_spp=1
     error_spp samples (seed 0) against the oracle's f64 accumulation of a
     centred crop at cpu_spp_factor x the samples (seed 1). The terrain's
     small bright emitters make single samples heavy-tailed; at the config's
-    1024 spp the 3-sigma count behaves as for a normal difference."""
+    1024 spp the 3-sigma count behaves as for a normal difference. The
+    launch configuration is autotuned first, as for the headline workload."""
     import oxr
     scene, cfg = workload("terrain")
     W, H = cfg["width"], cfg["height"]
@@ -248,6 +249,7 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16, error_spp=1
     t = time.perf_counter()
     pt.sync(scene, opts)
     sync_s = time.perf_counter() - t
+    tune = pt.autotune(spp=spp_per_step) if autotune else None
     pt.refine(spp_per_step)
     pt.reset()
     pt.set_timing(False)
@@ -269,7 +271,7 @@ def terrain_leg(spp_per_step=4, steps=2, crop=16, cpu_spp_factor=16, error_spp=1
            "rays_per_s": (tot["closest_rays"] + tot["shadow_rays"]) / dt, "ms_per_step": dt * 1e3 / steps,
            "kernel_ms": {"closest": tot["ms_closest"], "anyhit": tot["ms_anyhit"], "shade": tot["ms_shade"],
                          "other": tot["ms_other"]},
-           "scene_sync_s": sync_s, "bvh": pt.bvh_info()}
+           "scene_sync_s": sync_s, "autotune": tune, "bvh": pt.bvh_info()}
     if tot["launches_closest"]:
         rl = roofline_of(pt, spp_per_step, tot, steps, "terrain", W)
         out["roofline"] = rl["closest"]
@@ -538,7 +540,8 @@ def main():
     # value (an error in one is reported, not fatal to the line).
     guide_leg = lambda: guide_mse_bench(pass_spp=a.guide_pass_spp, max_s=a.guide_max_s,  # noqa: E731
                                         guide_cfg=json.loads(a.guide_config) if a.guide_config else None)
-    legs = (("large_scene", a.no_terrain or a.workload == "terrain", terrain_leg),
+    legs = (("large_scene", a.no_terrain or a.workload == "terrain",
+             lambda: terrain_leg(autotune=not a.no_autotune)),
             ("guided_time_to_mse", a.no_guide, guide_leg),
             ("serving", a.no_serving, serving_bench))
     for key, skip, fn in legs:

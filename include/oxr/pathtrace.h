@@ -395,7 +395,9 @@ int oxr_pt_stats_get(oxr_pt* h, oxr_pt_stats* out);
  * memory cap) x shading-kernel occupancy in {1, 2} waves per SIMD x the
  * traversal queue whole or split into 8 per-XCD segments — keeps the
  * fastest (each candidate is warmed once and timed twice, the faster run
- * counting), and resets the accumulation. */
+ * counting); then, at that configuration, times the occlusion kernel's
+ * refill threshold at 32 idle lanes against the engine's 48 (unless
+ * OXR_REFILL_ANY fixes it), and resets the accumulation. */
 typedef struct oxr_tune_result {
   uint32_t path_budget;     /* chosen paths per batch                  */
   uint32_t shade_waves;     /* chosen shading occupancy                */
@@ -403,6 +405,8 @@ typedef struct oxr_tune_result {
   uint32_t xcd_segments;    /* chosen traversal queue split (1 or 8)   */
   double paths_per_s;       /* of the chosen configuration             */
   double worst_paths_per_s; /* of the slowest candidate                */
+  uint32_t refill_closest;  /* idle lanes at which a traversal wave    */
+  uint32_t refill_any;      /* takes new rays (closest hit, occlusion) */
 } oxr_tune_result;
 int oxr_pt_autotune(oxr_pt* h, uint32_t spp, oxr_tune_result* out);
 

@@ -282,7 +282,8 @@ class PtStats(C.Structure):
 class TuneResult(C.Structure):
     """oxr_tune_result."""
     _fields_ = [("path_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("trials", C.c_uint32),
-                ("xcd_segments", C.c_uint32), ("paths_per_s", C.c_double), ("worst_paths_per_s", C.c_double)]
+                ("xcd_segments", C.c_uint32), ("paths_per_s", C.c_double), ("worst_paths_per_s", C.c_double),
+                ("refill_closest", C.c_uint32), ("refill_any", C.c_uint32)]
 
 
 class BvhInfo(C.Structure):
